@@ -1,0 +1,83 @@
+"""ctypes binding of ``libflorence_amd.so`` (declarations in include/florence_amd.h).
+
+The library is the only compute path: if it is missing or fails to load, every
+entry point raises -- there is no CPU fallback.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libflorence_amd.so")
+
+FL_MAT_NEOHOOKEAN = 1
+FL_MAT_MOONEYRIVLIN = 2
+FL_S_NONPOSITIVE_J = 1
+FL_S_NONFINITE = 2
+FL_S_PATTERN = 4
+
+# every symbol declared in include/florence_amd.h -> (restype, argtypes)
+_vp, _i32, _i64, _sz, _dp = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_size_t, ctypes.c_void_p
+_P64 = ctypes.POINTER(ctypes.c_int64)
+_P32 = ctypes.POINTER(ctypes.c_int32)
+_PD = ctypes.POINTER(ctypes.c_double)
+
+SIGNATURES = {
+    "fl_version": (ctypes.c_int, []),
+    "fl_last_error": (ctypes.c_char_p, []),
+    "fl_pattern_workspace_size": (_sz, [_i64]),
+    "fl_pattern_graph": (ctypes.c_int, [_vp, _i64, _i32, _i64, _vp, _vp, _vp, _P64, _P32, _vp, _sz, _vp]),
+    "fl_pattern_fill": (ctypes.c_int, [_vp, _i64, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "fl_df_workspace_size": (_sz, [_i32, _i64, _i32]),
+    "fl_df_assemble": (ctypes.c_int, [_i32, _PD, _vp, _vp, _vp, _vp, _vp, _i32, _i64, _i32, _i64, _i32,
+                                      _vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _sz, _vp, _vp]),
+    "fl_df_residual_workspace_size": (_sz, [_i32, _i64, _i32]),
+    "fl_df_residual": (ctypes.c_int, [_i32, _PD, _vp, _vp, _vp, _vp, _vp, _i32, _i64, _i32, _i64,
+                                      _vp, _vp, _vp, _vp, _sz, _vp, _vp]),
+}
+
+_lib = None
+
+
+class FlorenceAMDError(RuntimeError):
+    pass
+
+
+def load():
+    """Load the library (once) and attach signatures.  Raises if it is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FlorenceAMDError(
+            "libflorence_amd.so is not built (expected at {}); run `python -m florence_amd.build`".format(LIB_PATH))
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc, what):
+    if rc != 0:
+        msg = load().fl_last_error().decode(errors="replace")
+        raise FlorenceAMDError("{} failed (code {}): {}".format(what, rc, msg))
+
+
+def ptr(t):
+    """Device pointer of a torch tensor (None -> NULL)."""
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def stream_ptr(stream=None):
+    import torch
+    s = torch.cuda.current_stream() if stream is None else stream
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def double_array(values):
+    arr = (ctypes.c_double * len(values))(*[float(v) for v in values])
+    return ctypes.cast(arr, _PD), arr

@@ -1,0 +1,199 @@
+"""Assemble: the drop-in replacement of the reference's low-level DF assembly.
+
+Python surface kept from the reference:
+
+* ``Assemble(fem_solver, function_space, formulation, mesh, material, Eulerx, Eulerp)
+  -> (stiffness, T[:, None], F, mass)`` (Florence/FiniteElements/Assembly/Assembly.py:25-34)
+* ``LowLevelAssembly`` / ``_LowLevelAssembly_`` dispatch on the material class name
+  and refusal of materials without a low-level dispatcher (Assembly.py:37-95,
+  _LowLevelAssembly_.py:45-83).
+
+Everything between the Python call and the returned CSR runs in
+``libflorence_amd.so`` on the GPU: the sparsity pattern is built on the device once
+per mesh (cached on ``fem_solver``), and every call assembles K and T in HBM.  The
+returned ``stiffness`` is a ``DeviceCSR`` whose ``indptr``/``indices`` equal the
+reference pattern and whose data stay on the GPU; ``.to_scipy()`` gives the
+reference's scipy view.
+"""
+import time
+
+import numpy as np
+
+from . import _lib
+from .materials import MATERIAL_IDS
+from .sparse import DeviceCSR
+
+__all__ = ["Assemble", "LowLevelAssembly", "DeviceMeshData", "device_mesh_data", "AssembleResidual"]
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise _lib.FlorenceAMDError("florence_amd assembles on the GPU; no HIP device is visible")
+    return torch
+
+
+class DeviceMeshData(object):
+    """Device-resident, per-(mesh, function space) inputs of the kernels: coordinates,
+    int32 connectivity, basis tables, the node graph and the reference CSR pattern."""
+
+    def __init__(self, mesh, function_space, device="cuda", build_indices=True):
+        torch = _torch()
+        lib = _lib.load()
+        self.device = torch.device(device)
+        el = np.asarray(mesh.elements)
+        if el.max() >= 2 ** 31 - 1:
+            raise _lib.FlorenceAMDError("node ids exceed int32")
+        self.nelem, self.npe = int(el.shape[0]), int(el.shape[1])
+        self.nnode = int(np.asarray(mesh.points).shape[0])
+        self.ngauss = int(function_space.AllGauss.shape[0])
+        self.points = torch.as_tensor(np.ascontiguousarray(mesh.points, dtype=np.float64), device=self.device)
+        self.elements = torch.as_tensor(np.ascontiguousarray(el, dtype=np.int32), device=self.device)
+        self.Jm = torch.as_tensor(np.ascontiguousarray(function_space.Jm, dtype=np.float64), device=self.device)
+        self.AllGauss = torch.as_tensor(np.ascontiguousarray(function_space.AllGauss, dtype=np.float64).ravel(),
+                                        device=self.device)
+        i32 = torch.int32
+        self.node_elem_ptr = torch.empty(self.nnode + 1, dtype=i32, device=self.device)
+        self.node_elem = torch.empty(self.nelem * self.npe, dtype=i32, device=self.device)
+        self.node_nbr_ptr = torch.empty(self.nnode + 1, dtype=i32, device=self.device)
+        ws = torch.empty(int(lib.fl_pattern_workspace_size(self.nnode)), dtype=torch.uint8, device=self.device)
+        node_nnz = _lib.ctypes.c_int64(0)
+        maxdeg = _lib.ctypes.c_int32(0)
+        _lib.check(lib.fl_pattern_graph(_lib.ptr(self.elements), self.nelem, self.npe, self.nnode,
+                                        _lib.ptr(self.node_elem_ptr), _lib.ptr(self.node_elem),
+                                        _lib.ptr(self.node_nbr_ptr), _lib.ctypes.byref(node_nnz),
+                                        _lib.ctypes.byref(maxdeg), _lib.ptr(ws), ws.numel(), _lib.stream_ptr()),
+                   "fl_pattern_graph")
+        del ws
+        self.node_nnz = int(node_nnz.value)
+        self.max_degree = int(maxdeg.value)
+        self.nnz = 9 * self.node_nnz
+        self.node_nbrs = torch.empty(self.node_nnz, dtype=i32, device=self.device)
+        self.indptr = torch.empty(3 * self.nnode + 1, dtype=i32, device=self.device)
+        self.indices = torch.empty(self.nnz, dtype=i32, device=self.device) if build_indices else None
+        _lib.check(lib.fl_pattern_fill(_lib.ptr(self.elements), self.nelem, self.npe, self.nnode,
+                                       _lib.ptr(self.node_elem_ptr), _lib.ptr(self.node_elem),
+                                       _lib.ptr(self.node_nbr_ptr), _lib.ptr(self.node_nbrs),
+                                       _lib.ptr(self.indptr), _lib.ptr(self.indices), _lib.stream_ptr()),
+                   "fl_pattern_fill")
+        self.status = torch.zeros(2, dtype=i32, device=self.device)
+        self._ws = None
+        self._ws_res = None
+
+    def workspace(self, residual_only=False):
+        torch = _torch()
+        lib = _lib.load()
+        if residual_only:
+            if self._ws_res is None:
+                n = int(lib.fl_df_residual_workspace_size(self.ngauss, self.nelem, self.npe))
+                self._ws_res = torch.empty(n, dtype=torch.uint8, device=self.device)
+            return self._ws_res
+        if self._ws is None:
+            n = int(lib.fl_df_workspace_size(self.ngauss, self.nelem, self.npe))
+            if n == 0:
+                raise _lib.FlorenceAMDError("unsupported element size npe={}".format(self.npe))
+            self._ws = torch.empty(n, dtype=torch.uint8, device=self.device)
+        return self._ws
+
+    def release_workspace(self):
+        self._ws = None
+        self._ws_res = None
+
+
+def device_mesh_data(fem_solver, mesh, function_space):
+    """Per-mesh device data cached on the solver object (the reference caches its
+    pattern on fem_solver too, FEMSolver.py:1659-1684)."""
+    key = (id(mesh), id(function_space), np.asarray(mesh.elements).shape)
+    cache = getattr(fem_solver, "_fl_device_cache", None)
+    if cache is None or cache[0] != key:
+        dm = DeviceMeshData(mesh, function_space)
+        setattr(fem_solver, "_fl_device_cache", (key, dm))
+        return dm
+    return cache[1]
+
+
+def _as_device_coords(Eulerx, device):
+    torch = _torch()
+    if isinstance(Eulerx, torch.Tensor):
+        x = Eulerx.to(device=device, dtype=torch.float64)
+        return x.contiguous()
+    return torch.as_tensor(np.ascontiguousarray(Eulerx, dtype=np.float64), device=device)
+
+
+def assemble_device(dm, material, Eulerx, V=None, T=None, requires_geometry_update=1, stream=None):
+    """Raw device assembly on DeviceMeshData: returns (V, T, status) torch tensors."""
+    torch = _torch()
+    lib = _lib.load()
+    x = _as_device_coords(Eulerx, dm.device)
+    if V is None:
+        V = torch.empty(dm.nnz, dtype=torch.float64, device=dm.device)
+    if T is None:
+        T = torch.empty(3 * dm.nnode, dtype=torch.float64, device=dm.device)
+    ws = dm.workspace()
+    prm, _keep = _lib.double_array(material.device_parameters())
+    mid = MATERIAL_IDS.get(type(material).__name__)
+    if mid is None:
+        raise NotImplementedError("no low-level assembler for material {}".format(type(material).__name__))
+    _lib.check(lib.fl_df_assemble(mid, prm, _lib.ptr(dm.points), _lib.ptr(dm.elements), _lib.ptr(x),
+                                  _lib.ptr(dm.Jm), _lib.ptr(dm.AllGauss), dm.ngauss, dm.nelem, dm.npe, dm.nnode,
+                                  int(requires_geometry_update), _lib.ptr(dm.node_elem_ptr), _lib.ptr(dm.node_elem),
+                                  _lib.ptr(dm.node_nbr_ptr), _lib.ptr(dm.node_nbrs), dm.max_degree,
+                                  _lib.ptr(dm.indptr), _lib.ptr(V), _lib.ptr(T), _lib.ptr(ws), ws.numel(),
+                                  _lib.ptr(dm.status), _lib.stream_ptr(stream)),
+               "fl_df_assemble")
+    return V, T, dm.status
+
+
+def residual_device(dm, material, Eulerx, T=None, stream=None):
+    """Matrix-free internal force on DeviceMeshData (explicit residual-only pass)."""
+    torch = _torch()
+    lib = _lib.load()
+    x = _as_device_coords(Eulerx, dm.device)
+    if T is None:
+        T = torch.empty(3 * dm.nnode, dtype=torch.float64, device=dm.device)
+    ws = dm.workspace(residual_only=True)
+    prm, _keep = _lib.double_array(material.device_parameters())
+    mid = MATERIAL_IDS.get(type(material).__name__)
+    if mid is None:
+        raise NotImplementedError("no low-level assembler for material {}".format(type(material).__name__))
+    _lib.check(lib.fl_df_residual(mid, prm, _lib.ptr(dm.points), _lib.ptr(dm.elements), _lib.ptr(x),
+                                  _lib.ptr(dm.Jm), _lib.ptr(dm.AllGauss), dm.ngauss, dm.nelem, dm.npe, dm.nnode,
+                                  _lib.ptr(dm.node_elem_ptr), _lib.ptr(dm.node_elem), _lib.ptr(T), _lib.ptr(ws),
+                                  ws.numel(), _lib.ptr(dm.status), _lib.stream_ptr(stream)),
+               "fl_df_residual")
+    return T, dm.status
+
+
+def LowLevelAssembly(fem_solver, function_space, formulation, mesh, material, Eulerx, Eulerp):
+    """Assembly.py:37-95 (static branch): requires a low-level dispatcher."""
+    if not getattr(material, "has_low_level_dispatcher", False):
+        raise RuntimeError("Cannot dispatch to low level module, since material {} does not support it".format(
+            type(material).__name__))
+    if getattr(formulation, "fields", "mechanics") != "mechanics":
+        raise NotImplementedError("only the displacement formulation is assembled on the GPU")
+    if type(material).__name__ not in MATERIAL_IDS:
+        raise NotImplementedError("Turning optimise option on for {} material is not supported yet. "
+                                  "Consider 'optimise=False' for now".format(type(material).__name__))
+    dm = device_mesh_data(fem_solver, mesh, function_space)
+    update = int(getattr(fem_solver, "requires_geometry_update", True))
+    V, T, status = assemble_device(dm, material, Eulerx, requires_geometry_update=update)
+    stiffness = DeviceCSR(dm.indptr, dm.indices, V, (3 * dm.nnode, 3 * dm.nnode))
+    fem_solver.last_assembly_status = status
+    return stiffness, T[:, None], [], []
+
+
+def Assemble(fem_solver, function_space, formulation, mesh, material, Eulerx, Eulerp):
+    """(stiffness, T[:, None], F, mass) -- Assembly.py:25-34; records fem_solver.assembly_time."""
+    t0 = time.time()
+    out = LowLevelAssembly(fem_solver, function_space, formulation, mesh, material, Eulerx, Eulerp)
+    fem_solver.assembly_time = time.time() - t0
+    return out
+
+
+def AssembleResidual(fem_solver, function_space, formulation, mesh, material, Eulerx, Eulerp):
+    """Residual-only (explicit) internal force vector, (3*nnode, 1) on the GPU --
+    the mechanics part of AssembleExplicit (Assembly.py:664-716)."""
+    dm = device_mesh_data(fem_solver, mesh, function_space)
+    T, status = residual_device(dm, material, Eulerx)
+    fem_solver.last_assembly_status = status
+    return T[:, None]

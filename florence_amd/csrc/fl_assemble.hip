@@ -1,0 +1,622 @@
+// Displacement-formulation K + residual assembly for high-order tets/hexes, gfx950.
+//
+// Restates the element loop of _GlobalAssemblyDF_ (Florence/FiniteElements/Assembly/
+// _Assembly_/_LowLevelAssemblyDF_.h:69-161) as four device phases:
+//
+//  1. qp_kernel       one thread per (element, quadrature point): gather X, x; the
+//                     kinematics of _KinematicMeasures_.h:64-119 (parent Jacobians,
+//                     adjugate inverses, detJ = w |det dx/dxi|, F); the Neo-Hookean /
+//                     Mooney-Rivlin laws of _NeoHookean_.h:27-51 / _MooneyRivlin_.h:30-70.
+//                     Emits per point: dxi/dx (9), the six coupling blocks
+//                     D^{ik}_{jl} = detJ (c_ijkl + delta_ik sigma_jl) (54) and detJ*sigma (6).
+//  2. stiffness_kernel one workgroup per (element, block pair i<=k): the constitutive
+//                     B.H.B^T of _ConstitutiveStiffnessDF_.h:83-134 plus the geometric term
+//                     of _GeometricStiffness_.h:61-100, written per 3x3 displacement block
+//                       K^{ik}_{ab} = sum_{g,j} G_{a,(g,j)} W^{ik}_{b,(g,j)},
+//                       W^{ik}_{b,(g,j)} = sum_l D^{ik}_{jl}(g) G_{b,(g,l)},
+//                     G_{a,(g,j)} = dN_a/dx_j -- a dense GEMM over the 3G inner index run
+//                     on FP64 MFMA (v_mfma_f64_16x16x4f64) with the spatial gradients
+//                     and W rebuilt per 8-point chunk in LDS.  K^{ki} = (K^{ik})^T.
+//  3. traction_kernel one workgroup per element: t_(a,i) = sum_g detJ sigma_ij dN_a/dx_j
+//                     (_ConstitutiveStiffnessDF_.h:136-148).
+//  4. gather_kernel   one workgroup per node: its three CSR rows accumulated in LDS from
+//                     the incident elements in ascending element order (the order of the
+//                     reference's serial CSR accumulation, SparseAssemblyNative.h:32-45),
+//                     then written once, coalesced; same for T (_LowLevelAssemblyDF_.h:153-161).
+//                     No floating-point atomics: the result is bitwise reproducible.
+#include <cmath>
+
+#include "fl_common.hpp"
+
+namespace fl {
+
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+// per quadrature point record
+constexpr int QS = 72;        // stride (doubles)
+constexpr int Q_IX = 0;       // dxi/dx: ix[j*3+m] = d xi_m / d x_j
+constexpr int Q_D = 9;        // 6 pairs x 9: D^{ik}[j*3+l]
+constexpr int Q_S = 63;       // detJ*sigma: s00 s11 s22 s01 s02 s12
+constexpr int Q_DETJ = 69;
+constexpr int QS_RES = 16;    // residual-only record: ix (9), S (6 at offset 9), detJ (15)
+constexpr int QR_S = 9;
+constexpr int QR_DETJ = 15;
+
+__constant__ int c_pair_i[6] = {0, 1, 2, 0, 0, 1};
+__constant__ int c_pair_k[6] = {0, 1, 2, 1, 2, 2};
+
+struct MatParams {
+    int material;
+    double mu, mu1, mu2, lamb;
+};
+
+__device__ inline double invdet3x3(const double *s, double *d) {
+    d[0] = s[4] * s[8] - s[5] * s[7];
+    d[1] = -s[1] * s[8] + s[2] * s[7];
+    d[2] = s[1] * s[5] - s[2] * s[4];
+    d[3] = -s[3] * s[8] + s[5] * s[6];
+    d[4] = s[0] * s[8] - s[2] * s[6];
+    d[5] = -s[0] * s[5] + s[2] * s[3];
+    d[6] = s[3] * s[7] - s[4] * s[6];
+    d[7] = -s[0] * s[7] + s[1] * s[6];
+    d[8] = s[0] * s[4] - s[1] * s[3];
+    const double det = s[0] * d[0] + s[1] * d[3] + s[2] * d[6];
+    const double r = 1.0 / det;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) d[i] *= r;
+    return det;
+}
+
+// Jm^T padded: JmT[(g*3+m)*NP + a] = dN_a/dxi_m at point g (0 for a >= npe)
+__global__ void transpose_jm_kernel(const double *__restrict__ Jm, int npe, int G, int NP, double *__restrict__ JmT) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= G * 3 * NP) return;
+    const int a = t % NP, m = (t / NP) % 3, g = t / (3 * NP);
+    JmT[t] = a < npe ? Jm[((int64_t)m * npe + a) * G + g] : 0.0;
+}
+
+// ------------------------------------------------------------------ phase 1
+template <bool TANGENT>
+__global__ __launch_bounds__(256) void qp_kernel(MatParams mp, const double *__restrict__ points,
+                                                 const int32_t *__restrict__ elements,
+                                                 const double *__restrict__ Eulerx, const double *__restrict__ Jm,
+                                                 const double *__restrict__ w, int G, int64_t nelem, int npe,
+                                                 int update, double *__restrict__ qp, int32_t *__restrict__ status) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nelem * G) return;
+    const int64_t e = t / G;
+    const int g = (int)(t - e * G);
+    double PX[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, Px[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const int32_t *conn = elements + e * npe;
+    for (int a = 0; a < npe; ++a) {
+        const int64_t nd = conn[a];
+        const double X0 = points[3 * nd], X1 = points[3 * nd + 1], X2 = points[3 * nd + 2];
+        const double x0 = Eulerx[3 * nd], x1 = Eulerx[3 * nd + 1], x2 = Eulerx[3 * nd + 2];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const double jm = Jm[((int64_t)i * npe + a) * G + g];
+            PX[3 * i + 0] += jm * X0;
+            PX[3 * i + 1] += jm * X1;
+            PX[3 * i + 2] += jm * X2;
+            Px[3 * i + 0] += jm * x0;
+            Px[3 * i + 1] += jm * x1;
+            Px[3 * i + 2] += jm * x2;
+        }
+    }
+    double iX[9], ix[9];
+    const double detX = invdet3x3(PX, iX);
+    const double detx = invdet3x3(Px, ix);
+    const double detJ = w[g] * fabs(update == 1 ? detx : detX);
+    // F_lj = sum_m (dx_l/dxi_m)(dxi_m/dX_j) = sum_m Px[m][l] iX[j][m]
+    double F[9];
+#pragma unroll
+    for (int l = 0; l < 3; ++l)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) F[3 * l + j] = Px[l] * iX[3 * j] + Px[3 + l] * iX[3 * j + 1] + Px[6 + l] * iX[3 * j + 2];
+    const double J = F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6]) +
+                     F[2] * (F[3] * F[7] - F[4] * F[6]);
+    double b[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) b[3 * i + j] = F[3 * i] * F[3 * j] + F[3 * i + 1] * F[3 * j + 1] + F[3 * i + 2] * F[3 * j + 2];
+    double sig[9], c1, a1, a2;
+    if (mp.material == FL_MAT_NEOHOOKEAN) {
+        // sigma = mu/J (b - I) + lamb (J-1) I; c = a1 (d_ik d_jl + d_il d_jk) + a2 d_ij d_kl
+#pragma unroll
+        for (int i = 0; i < 9; ++i) sig[i] = mp.mu / J * (b[i] - ((i % 4) == 0 ? 1.0 : 0.0));
+        sig[0] += mp.lamb * (J - 1.0);
+        sig[4] += mp.lamb * (J - 1.0);
+        sig[8] += mp.lamb * (J - 1.0);
+        c1 = 0.0;
+        a1 = mp.mu / J - mp.lamb * (J - 1.0);
+        a2 = mp.lamb * (2.0 * J - 1.0);
+    } else {
+        // sigma = 2mu1/J b + 2mu2/J (tr b b - b b) - 2(mu1+2mu2)/J I + lamb (J-1) I
+        const double trb = b[0] + b[4] + b[8];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const double bb = b[3 * i] * b[j] + b[3 * i + 1] * b[3 + j] + b[3 * i + 2] * b[6 + j];
+                sig[3 * i + j] = 2.0 * mp.mu1 / J * b[3 * i + j] + 2.0 * mp.mu2 / J * (trb * b[3 * i + j] - bb);
+            }
+        const double diag = -2.0 * (mp.mu1 + 2.0 * mp.mu2) / J + mp.lamb * (J - 1.0);
+        sig[0] += diag;
+        sig[4] += diag;
+        sig[8] += diag;
+        c1 = 2.0 * mp.mu2 / J;
+        a1 = 2.0 * (mp.mu1 + 2.0 * mp.mu2) / J - mp.lamb * (J - 1.0);
+        a2 = mp.lamb * (2.0 * J - 1.0);
+    }
+    // status: det F <= 0 or non-finite values (wave-reduced, one atomic per wave)
+    bool nonpos = !(J > 0.0);
+    bool nonfin = !isfinite(J) || !isfinite(detJ) || !isfinite(sig[0] + sig[4] + sig[8] + sig[1] + sig[2] + sig[5]);
+    const unsigned long long m_np = __ballot(nonpos), m_nf = __ballot(nonfin);
+    if ((m_np | m_nf) && (threadIdx.x & 63) == __ffsll((long long)(m_np | m_nf)) - 1) {
+        atomicOr(status, (m_np ? FL_S_NONPOSITIVE_J : 0) | (m_nf ? FL_S_NONFINITE : 0));
+        atomicAdd(status + 1, (int)__popcll(m_np));
+    }
+    if (TANGENT) {
+        double *rec = qp + t * QS;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) rec[Q_IX + i] = ix[i];
+#pragma unroll
+        for (int p = 0; p < 6; ++p) {
+            const int i = p < 3 ? p : (p == 5 ? 1 : 0);
+            const int k = p < 3 ? p : (p == 3 ? 1 : 2);
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int l = 0; l < 3; ++l) {
+                    const double dik = i == k ? 1.0 : 0.0, djl = j == l ? 1.0 : 0.0;
+                    const double dil = i == l ? 1.0 : 0.0, djk = j == k ? 1.0 : 0.0;
+                    const double dij = i == j ? 1.0 : 0.0, dkl = k == l ? 1.0 : 0.0;
+                    const double cijkl = c1 * (2.0 * b[3 * i + j] * b[3 * k + l] - b[3 * i + k] * b[3 * j + l] -
+                                               b[3 * i + l] * b[3 * j + k]) +
+                                         a1 * (dik * djl + dil * djk) + a2 * dij * dkl;
+                    rec[Q_D + 9 * p + 3 * j + l] = detJ * (cijkl + dik * sig[3 * j + l]);
+                }
+        }
+        rec[Q_S + 0] = detJ * sig[0];
+        rec[Q_S + 1] = detJ * sig[4];
+        rec[Q_S + 2] = detJ * sig[8];
+        rec[Q_S + 3] = detJ * sig[1];
+        rec[Q_S + 4] = detJ * sig[2];
+        rec[Q_S + 5] = detJ * sig[5];
+        rec[Q_DETJ] = detJ;
+    } else {
+        double *rec = qp + t * QS_RES;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) rec[i] = ix[i];
+        rec[QR_S + 0] = detJ * sig[0];
+        rec[QR_S + 1] = detJ * sig[4];
+        rec[QR_S + 2] = detJ * sig[8];
+        rec[QR_S + 3] = detJ * sig[1];
+        rec[QR_S + 4] = detJ * sig[2];
+        rec[QR_S + 5] = detJ * sig[5];
+        rec[QR_DETJ] = detJ;
+    }
+}
+
+// ------------------------------------------------------------------ phase 2
+// NP: node count padded to a multiple of 16; each wave owns a (16*MT) x (16*NT) tile of
+// one 3x3 displacement block, the workgroup (WGM x WGN waves) covers NP x NP.
+template <int NP, int MT, int NT, int WGM, int WGN>
+__global__ __launch_bounds__(64 * WGM *WGN) void stiffness_kernel(const double *__restrict__ JmT,
+                                                                   const double *__restrict__ qp, int npe, int G,
+                                                                   int64_t nelem, double *__restrict__ Ke) {
+    static_assert(NP == 16 * MT * WGM && NP == 16 * NT * WGN, "tile shape must cover NP");
+    constexpr int NTH = 64 * WGM * WGN;
+    constexpr int QC = 8;        // quadrature points per chunk
+    constexpr int KC = 3 * QC;   // inner (g, j) indices per chunk
+    constexpr int LDA = NP + 4;
+    constexpr int TT = 16 * NT * 17;   // per-wave transpose tile (doubles)
+    static_assert(WGM * WGN * TT <= 2 * KC * LDA, "transpose tile must fit the staging buffers");
+    __shared__ double sAB[2 * KC * LDA];
+    __shared__ double sQ[QC * 18];
+    double *sA = sAB;
+    double *sB = sAB + KC * LDA;
+
+    // XCD-aware: workgroup ids are dealt round-robin over the 8 XCDs; keep the six block
+    // pairs of one element on the same XCD so its point records are read from one L2.
+    const int64_t wid = blockIdx.x;
+    const int xcd = (int)(wid & 7);
+    const int64_t slot = wid >> 3;
+    const int pair = (int)(slot % 6);
+    const int64_t e = (slot / 6) * 8 + xcd;
+    if (e >= nelem) return;
+    const int pi = c_pair_i[pair], pk = c_pair_k[pair];
+
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int wm = wave / WGN, wn = wave % WGN;
+    const int li = lane & 15, lk = lane >> 4;
+
+    double4_t acc[MT][NT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = double4_t{0.0, 0.0, 0.0, 0.0};
+
+    const double *qe = qp + e * (int64_t)G * QS;
+    for (int g0 = 0; g0 < G; g0 += QC) {
+        for (int idx = tid; idx < QC * 18; idx += NTH) {
+            const int gq = idx / 18, f = idx - gq * 18, g = g0 + gq;
+            double v = 0.0;
+            if (g < G) v = f < 9 ? qe[(int64_t)g * QS + Q_IX + f] : qe[(int64_t)g * QS + Q_D + 9 * pair + (f - 9)];
+            sQ[idx] = v;
+        }
+        __syncthreads();
+        // spatial gradients G_{a,(g,j)} = sum_m ix[j][m] dN_a/dxi_m
+        for (int idx = tid; idx < QC * NP; idx += NTH) {
+            const int gq = idx / NP, a = idx - gq * NP, g = g0 + gq;
+            double j0 = 0.0, j1 = 0.0, j2 = 0.0;
+            if (g < G) {
+                const double *jt = JmT + (int64_t)g * 3 * NP + a;
+                j0 = jt[0];
+                j1 = jt[NP];
+                j2 = jt[2 * NP];
+            }
+            const double *ix = sQ + gq * 18;
+            sA[(3 * gq + 0) * LDA + a] = ix[0] * j0 + ix[1] * j1 + ix[2] * j2;
+            sA[(3 * gq + 1) * LDA + a] = ix[3] * j0 + ix[4] * j1 + ix[5] * j2;
+            sA[(3 * gq + 2) * LDA + a] = ix[6] * j0 + ix[7] * j1 + ix[8] * j2;
+        }
+        __syncthreads();
+        // W^{ik}_{b,(g,j)} = sum_l D^{ik}_{jl} G_{b,(g,l)}
+        for (int idx = tid; idx < QC * NP; idx += NTH) {
+            const int gq = idx / NP, b = idx - gq * NP;
+            const double *D = sQ + gq * 18 + 9;
+            const double s0 = sA[(3 * gq + 0) * LDA + b];
+            const double s1 = sA[(3 * gq + 1) * LDA + b];
+            const double s2 = sA[(3 * gq + 2) * LDA + b];
+            sB[(3 * gq + 0) * LDA + b] = D[0] * s0 + D[1] * s1 + D[2] * s2;
+            sB[(3 * gq + 1) * LDA + b] = D[3] * s0 + D[4] * s1 + D[5] * s2;
+            sB[(3 * gq + 2) * LDA + b] = D[6] * s0 + D[7] * s1 + D[8] * s2;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < KC / 4; ++kk) {
+            const int q = kk * 4 + lk;
+            double af[MT], bf[NT];
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) af[mt] = sA[q * LDA + (wm * MT + mt) * 16 + li];
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) bf[nt] = sB[q * LDA + (wn * NT + nt) * 16 + li];
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt)
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[mt], bf[nt], acc[mt][nt], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+
+    // epilogue: K^{ik} rows a, cols b.  MFMA f64 16x16 layout: lane holds rows lk+4r, col li.
+    const int64_t npe2 = (int64_t)npe * npe;
+    double *Kd = Ke + (e * 9 + pi * 3 + pk) * npe2;
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int a = (wm * MT + mt) * 16 + lk + 4 * r;
+                const int b = (wn * NT + nt) * 16 + li;
+                if (a < npe && b < npe) Kd[(int64_t)a * npe + b] = acc[mt][nt][r];
+            }
+    if (pi != pk) {
+        // mirror block K^{ki}_{ba} through a per-wave LDS transpose (coalesced stores)
+        double *Kt = Ke + (e * 9 + pk * 3 + pi) * npe2;
+        double *tile = sAB + wave * TT;
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) tile[(nt * 16 + li) * 17 + lk + 4 * r] = acc[mt][nt][r];
+            __syncthreads();
+            for (int idx = lane; idx < 16 * NT * 16; idx += 64) {
+                const int bl = idx >> 4, al = idx & 15;
+                const int b = wn * NT * 16 + bl, a = (wm * MT + mt) * 16 + al;
+                if (a < npe && b < npe) Kt[(int64_t)b * npe + a] = tile[bl * 17 + al];
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// ------------------------------------------------------------------ phase 3
+// t_(a,i) = sum_g sum_m dN_a/dxi_m(g) P_g[m][i],  P_g[m][i] = sum_j ix[j][m] S_g[i][j]
+__global__ __launch_bounds__(256) void traction_kernel(const double *__restrict__ JmT, int NP,
+                                                       const double *__restrict__ qp, int qs, int soff, int npe,
+                                                       int G, double *__restrict__ te) {
+    extern __shared__ double P[];   // G*9
+    const int64_t e = blockIdx.x;
+    const double *qe = qp + e * (int64_t)G * qs;
+    for (int idx = threadIdx.x; idx < G * 9; idx += blockDim.x) {
+        const int g = idx / 9, m = (idx / 3) % 3, i = idx % 3;
+        const double *rec = qe + (int64_t)g * qs;
+        const double *S = rec + soff;   // s00 s11 s22 s01 s02 s12
+        const double Si[3][3] = {{S[0], S[3], S[4]}, {S[3], S[1], S[5]}, {S[4], S[5], S[2]}};
+        P[idx] = rec[0 * 3 + m] * Si[i][0] + rec[1 * 3 + m] * Si[i][1] + rec[2 * 3 + m] * Si[i][2];
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < 3 * npe; idx += blockDim.x) {
+        const int a = idx / 3, i = idx % 3;
+        double acc = 0.0;
+        for (int g = 0; g < G; ++g) {
+            const double *jt = JmT + (int64_t)g * 3 * NP + a;
+            acc += jt[0] * P[g * 9 + 0 + i] + jt[NP] * P[g * 9 + 3 + i] + jt[2 * NP] * P[g * 9 + 6 + i];
+        }
+        te[e * 3 * npe + idx] = acc;
+    }
+}
+
+// ------------------------------------------------------------------ phase 4
+__device__ inline int find_rank(const int32_t *s, int n, int32_t v) {
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (v < s[mid]) hi = mid; else lo = mid;
+    }
+    return s[lo] == v ? lo : -1;
+}
+
+__global__ __launch_bounds__(256) void gather_kernel(const int32_t *__restrict__ elements, int npe,
+                                                     const int32_t *__restrict__ node_elem_ptr,
+                                                     const int32_t *__restrict__ node_elem,
+                                                     const int32_t *__restrict__ nbr_ptr,
+                                                     const int32_t *__restrict__ nbrs, const int32_t *__restrict__ indptr,
+                                                     const double *__restrict__ Ke, const double *__restrict__ te,
+                                                     int max_degree, double *__restrict__ V, double *__restrict__ T,
+                                                     int32_t *__restrict__ status) {
+    extern __shared__ double smem[];
+    const int64_t a = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int deg = nbr_ptr[a + 1] - nbr_ptr[a];
+    const int32_t p0 = nbr_ptr[a];
+    double *acc = smem;                                              // 9*max_degree
+    int32_t *snb = reinterpret_cast<int32_t *>(smem + 9 * max_degree); // max_degree
+    int32_t *rr = snb + max_degree;                                  // npe
+    const int rowlen = 3 * deg;
+    for (int i = tid; i < 9 * deg; i += 256) acc[i] = 0.0;
+    for (int i = tid; i < deg; i += 256) snb[i] = nbrs[p0 + i];
+    __syncthreads();
+    const int32_t q0 = node_elem_ptr[a], q1 = node_elem_ptr[a + 1];
+    const int64_t npe2 = (int64_t)npe * npe;
+    bool bad = false;
+    for (int32_t q = q0; q < q1; ++q) {
+        const int32_t ent = node_elem[q];
+        const int64_t e = ent / npe;
+        const int la = ent - (int)(e * npe);
+        for (int lb = tid; lb < npe; lb += 256) {
+            const int r = find_rank(snb, deg, elements[e * npe + lb]);
+            bad |= r < 0;
+            rr[lb] = r < 0 ? 0 : r;
+        }
+        __syncthreads();
+        const double *Kb = Ke + e * 9 * npe2 + (int64_t)la * npe;
+        for (int idx = tid; idx < 9 * npe; idx += 256) {
+            const int ik = idx / npe, lb = idx - ik * npe;
+            const int i = ik / 3, k = ik - 3 * i;
+            acc[i * rowlen + 3 * rr[lb] + k] += Kb[ik * npe2 + lb];
+        }
+        __syncthreads();
+    }
+    for (int idx = tid; idx < 9 * deg; idx += 256) {
+        const int i = idx / rowlen, c = idx - i * rowlen;
+        V[(int64_t)indptr[3 * a + i] + c] = acc[idx];
+    }
+    if (tid < 3) {
+        double s = 0.0;
+        for (int32_t q = q0; q < q1; ++q) {
+            const int32_t ent = node_elem[q];
+            const int64_t e = ent / npe;
+            const int la = ent - (int)(e * npe);
+            s += te[e * 3 * npe + 3 * la + tid];
+        }
+        T[3 * a + tid] = s;
+    }
+    if (bad) atomicOr(status, 4);
+}
+
+__global__ void gather_traction_kernel(int npe, const int32_t *__restrict__ node_elem_ptr,
+                                       const int32_t *__restrict__ node_elem, const double *__restrict__ te,
+                                       int64_t nnode, double *__restrict__ T) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3 * nnode) return;
+    const int64_t a = t / 3;
+    const int i = (int)(t - 3 * a);
+    double s = 0.0;
+    for (int32_t q = node_elem_ptr[a]; q < node_elem_ptr[a + 1]; ++q) {
+        const int32_t ent = node_elem[q];
+        const int64_t e = ent / npe;
+        const int la = ent - (int)(e * npe);
+        s += te[e * 3 * npe + 3 * la + i];
+    }
+    T[t] = s;
+}
+
+// ------------------------------------------------------------------ host side
+static int padded_np(int npe) {
+    if (npe <= 16) return 16;
+    if (npe <= 32) return 32;
+    if (npe <= 48) return 48;
+    if (npe <= 64) return 64;
+    if (npe <= 128) return 128;
+    return -1;
+}
+
+static int launch_stiffness(int NP, const double *JmT, const double *qp, int npe, int G, int64_t nelem, double *Ke,
+                            hipStream_t st) {
+    const int64_t groups = (nelem + 7) / 8 * 8 * 6;
+    if (groups > (int64_t)INT32_MAX) {
+        set_error("too many elements for one launch");
+        return FL_E_LIMIT;
+    }
+    const dim3 grid((unsigned)groups);
+    switch (NP) {
+        case 16: stiffness_kernel<16, 1, 1, 1, 1><<<grid, 64, 0, st>>>(JmT, qp, npe, G, nelem, Ke); break;
+        case 32: stiffness_kernel<32, 2, 2, 1, 1><<<grid, 64, 0, st>>>(JmT, qp, npe, G, nelem, Ke); break;
+        case 48: stiffness_kernel<48, 3, 3, 1, 1><<<grid, 64, 0, st>>>(JmT, qp, npe, G, nelem, Ke); break;
+        case 64: stiffness_kernel<64, 2, 2, 2, 2><<<grid, 256, 0, st>>>(JmT, qp, npe, G, nelem, Ke); break;
+        case 128: stiffness_kernel<128, 4, 4, 2, 2><<<grid, 256, 0, st>>>(JmT, qp, npe, G, nelem, Ke); break;
+        default: set_error("unsupported node count"); return FL_E_ARG;
+    }
+    FL_LAUNCH_CHECK();
+    return 0;
+}
+
+struct DFWork {
+    double *JmT, *qp, *te, *Ke;
+};
+
+static DFWork carve_df(void *ws, size_t bytes, int G, int64_t nelem, int npe, bool tangent, size_t *need) {
+    const int NP = padded_np(npe);
+    Carve cv(ws, bytes);
+    DFWork w;
+    w.JmT = cv.take<double>((size_t)G * 3 * NP);
+    w.qp = cv.take<double>((size_t)nelem * G * (tangent ? QS : QS_RES));
+    w.te = cv.take<double>((size_t)nelem * 3 * npe);
+    w.Ke = tangent ? cv.take<double>((size_t)nelem * 9 * npe * npe) : nullptr;
+    *need = cv.off + 256;
+    if (!cv.ok) w.JmT = nullptr;
+    return w;
+}
+
+static bool check_common(int material, const double *params, int G, int64_t nelem, int npe, int64_t nnode) {
+    if (material != FL_MAT_NEOHOOKEAN && material != FL_MAT_MOONEYRIVLIN) {
+        set_error("unknown material id %d", material);
+        return false;
+    }
+    if (!params || G <= 0 || nelem <= 0 || nnode <= 0 || padded_np(npe) < 0) {
+        set_error("invalid sizes (ngauss=%d nelem=%lld npe=%d nnode=%lld); npe must be in [1,128]", G,
+                  (long long)nelem, npe, (long long)nnode);
+        return false;
+    }
+    if (nelem * (int64_t)npe >= INT32_MAX) {
+        set_error("nelem*npe exceeds int32");
+        return false;
+    }
+    return true;
+}
+
+static MatParams mat_params(int material, const double *p) {
+    MatParams m;
+    m.material = material;
+    m.mu = p[0];
+    m.mu1 = p[1];
+    m.mu2 = p[2];
+    m.lamb = p[3];
+    return m;
+}
+
+}  // namespace fl
+
+using namespace fl;
+
+extern "C" {
+
+size_t fl_df_workspace_size(int32_t ngauss, int64_t nelem, int32_t npe) {
+    if (padded_np(npe) < 0) return 0;
+    size_t need = 0;
+    carve_df(nullptr, 0, ngauss, nelem, npe, true, &need);
+    return need;
+}
+
+size_t fl_df_residual_workspace_size(int32_t ngauss, int64_t nelem, int32_t npe) {
+    if (padded_np(npe) < 0) return 0;
+    size_t need = 0;
+    carve_df(nullptr, 0, ngauss, nelem, npe, false, &need);
+    return need;
+}
+
+int fl_df_assemble(int32_t material, const double *params, const double *points, const int32_t *elements,
+                   const double *Eulerx, const double *Jm, const double *AllGauss, int32_t G, int64_t nelem,
+                   int32_t npe, int64_t nnode, int32_t update, const int32_t *node_elem_ptr,
+                   const int32_t *node_elem, const int32_t *node_nbr_ptr, const int32_t *node_nbrs,
+                   int32_t max_degree, const int32_t *indptr, double *V, double *T, void *workspace,
+                   size_t workspace_bytes, int32_t *status, void *stream) {
+    hipStream_t st = as_stream(stream);
+    if (!check_common(material, params, G, nelem, npe, nnode)) return FL_E_ARG;
+    if (!points || !elements || !Eulerx || !Jm || !AllGauss || !node_elem_ptr || !node_elem || !node_nbr_ptr ||
+        !node_nbrs || !indptr || !V || !T || !status || max_degree <= 0) {
+        set_error("fl_df_assemble: null pointer or max_degree <= 0");
+        return FL_E_ARG;
+    }
+    size_t need = 0;
+    DFWork w = carve_df(workspace, workspace_bytes, G, nelem, npe, true, &need);
+    if (!w.JmT) {
+        set_error("fl_df_assemble: workspace too small (%zu < %zu)", workspace_bytes, need);
+        return FL_E_SPACE;
+    }
+    const size_t lds_gather = (size_t)max_degree * 9 * 8 + (size_t)max_degree * 4 + (size_t)npe * 4 + 16;
+    if (lds_gather > 160 * 1024) {
+        set_error("fl_df_assemble: node degree %d needs %zu B of LDS", max_degree, lds_gather);
+        return FL_E_LIMIT;
+    }
+    const int NP = padded_np(npe);
+    const MatParams mp = mat_params(material, params);
+    FL_HIP_CHECK(hipMemsetAsync(status, 0, 2 * sizeof(int32_t), st));
+    transpose_jm_kernel<<<(G * 3 * NP + 255) / 256, 256, 0, st>>>(Jm, npe, G, NP, w.JmT);
+    FL_LAUNCH_CHECK();
+    const int64_t nq = nelem * G;
+    qp_kernel<true><<<(unsigned)((nq + 255) / 256), 256, 0, st>>>(mp, points, elements, Eulerx, Jm, AllGauss, G, nelem,
+                                                                   npe, update, w.qp, status);
+    FL_LAUNCH_CHECK();
+    int rc = launch_stiffness(NP, w.JmT, w.qp, npe, G, nelem, w.Ke, st);
+    if (rc) return rc;
+    const size_t lds_tr = (size_t)G * 9 * 8;
+    if (lds_tr > 64 * 1024) FL_HIP_CHECK(hipFuncSetAttribute((const void *)traction_kernel,
+                                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_tr));
+    traction_kernel<<<(unsigned)nelem, 256, lds_tr, st>>>(w.JmT, NP, w.qp, QS, Q_S, npe, G, w.te);
+    FL_LAUNCH_CHECK();
+    if (update != 1) FL_HIP_CHECK(hipMemsetAsync(w.te, 0, sizeof(double) * nelem * 3 * npe, st));
+    if (lds_gather > 64 * 1024)
+        FL_HIP_CHECK(hipFuncSetAttribute((const void *)gather_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)lds_gather));
+    gather_kernel<<<(unsigned)nnode, 256, lds_gather, st>>>(elements, npe, node_elem_ptr, node_elem, node_nbr_ptr,
+                                                            node_nbrs, indptr, w.Ke, w.te, max_degree, V, T, status);
+    FL_LAUNCH_CHECK();
+    return 0;
+}
+
+int fl_df_residual(int32_t material, const double *params, const double *points, const int32_t *elements,
+                   const double *Eulerx, const double *Jm, const double *AllGauss, int32_t G, int64_t nelem,
+                   int32_t npe, int64_t nnode, const int32_t *node_elem_ptr, const int32_t *node_elem, double *T,
+                   void *workspace, size_t workspace_bytes, int32_t *status, void *stream) {
+    hipStream_t st = as_stream(stream);
+    if (!check_common(material, params, G, nelem, npe, nnode)) return FL_E_ARG;
+    if (!points || !elements || !Eulerx || !Jm || !AllGauss || !node_elem_ptr || !node_elem || !T || !status) {
+        set_error("fl_df_residual: null pointer");
+        return FL_E_ARG;
+    }
+    size_t need = 0;
+    DFWork w = carve_df(workspace, workspace_bytes, G, nelem, npe, false, &need);
+    if (!w.JmT) {
+        set_error("fl_df_residual: workspace too small (%zu < %zu)", workspace_bytes, need);
+        return FL_E_SPACE;
+    }
+    const int NP = padded_np(npe);
+    const MatParams mp = mat_params(material, params);
+    FL_HIP_CHECK(hipMemsetAsync(status, 0, 2 * sizeof(int32_t), st));
+    transpose_jm_kernel<<<(G * 3 * NP + 255) / 256, 256, 0, st>>>(Jm, npe, G, NP, w.JmT);
+    FL_LAUNCH_CHECK();
+    const int64_t nq = nelem * G;
+    qp_kernel<false><<<(unsigned)((nq + 255) / 256), 256, 0, st>>>(mp, points, elements, Eulerx, Jm, AllGauss, G,
+                                                                    nelem, npe, 1, w.qp, status);
+    FL_LAUNCH_CHECK();
+    const size_t lds_tr = (size_t)G * 9 * 8;
+    if (lds_tr > 64 * 1024) FL_HIP_CHECK(hipFuncSetAttribute((const void *)traction_kernel,
+                                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_tr));
+    traction_kernel<<<(unsigned)nelem, 256, lds_tr, st>>>(w.JmT, NP, w.qp, QS_RES, QR_S, npe, G, w.te);
+    FL_LAUNCH_CHECK();
+    gather_traction_kernel<<<(unsigned)((3 * nnode + 255) / 256), 256, 0, st>>>(npe, node_elem_ptr, node_elem, w.te,
+                                                                                 nnode, T);
+    FL_LAUNCH_CHECK();
+    return 0;
+}
+
+}  // extern "C"
