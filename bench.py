@@ -1,0 +1,172 @@
+"""Benchmark: DF K+residual assembly elements/s on one MI355X.
+
+Workload (one "step" = one full K + residual assembly into the reference CSR pattern):
+unit cube of 24^3 p=4 Lagrange hexes (13,824 elements, 125 nodes, 375x375 local K,
+912,673 nodes, 1,728,900,297 CSR non-zeros), 6x6x6 Gauss-Lobatto points, interior
+nodes jittered +-0.03h (seed 0), Mooney-Rivlin (mu1=0.5, mu2=0.5, lamb=20), evaluated
+at a twisted current configuration.  Inputs are resident in HBM before timing.
+
+Printed: one JSON line with value = elements/s (whole job), the stiffness kernel's
+roofline (FP64 MFMA), and the CPU baseline (the C restatement of the reference
+assembly timed on a bounded sample of the same mesh, one core).
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 1]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (matrix and vector)
+HBM_PEAK_GBS = 8000.0
+
+
+def stiffness_flops_per_element(npe, G):
+    """FP64 flops of the block contraction actually executed for one element: six
+    3x3 displacement blocks (i <= k), each an npe x npe x 3G GEMM (2 flops per FMA)."""
+    return 6 * 2 * npe * npe * 3 * G
+
+
+def dense_b_flops_per_element(npe, G):
+    """Work of the reference's dense-B algorithm per element (dgemm B.(H.B^T) per point
+    plus the geometric, kinematic, residual and material terms)."""
+    n, V = 3 * npe, 6
+    return G * (2 * V * n * n + 2 * V * V * n + 2 * n * n + 6 * npe * npe + 108 * npe + 13 * n + 300)
+
+
+def cpu_baseline(mesh, fs, mat, x, nsample):
+    """Time oracle/liboracle_df.so (C restatement of _GlobalAssemblyDF_) on the first
+    ``nsample`` elements of the mesh, scattering into their own CSR pattern."""
+    from oracle import df_numpy as O
+    so = os.path.join(ROOT, "oracle", "liboracle_df.so")
+    if not os.path.exists(so):
+        return None
+    lib = ctypes.CDLL(so)
+    lib.oracle_df_assemble.restype = ctypes.c_int
+    el = np.asarray(mesh.elements, dtype=np.int64)[:nsample]
+    nodes, inv = np.unique(el, return_inverse=True)
+    sub = np.ascontiguousarray(inv.reshape(el.shape).astype(np.int64))
+    pts = np.ascontiguousarray(np.asarray(mesh.points)[nodes])
+    xs = np.ascontiguousarray(np.asarray(x)[nodes])
+    indptr, indices = O.sparsity_pattern(sub, nodes.size)
+    V = np.zeros(indices.size)
+    T = np.zeros(3 * nodes.size)
+    prm = np.ascontiguousarray(mat.device_parameters())
+    Jm, w = np.ascontiguousarray(fs.Jm), np.ascontiguousarray(fs.AllGauss.ravel())
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    t0 = time.perf_counter()
+    rc = lib.oracle_df_assemble(P(pts), P(sub), P(xs), P(Jm), P(w), ctypes.c_int64(w.size),
+                                ctypes.c_int64(sub.shape[1]), ctypes.c_int(mat.material_id), P(prm), ctypes.c_int(1),
+                                P(indptr), P(indices), P(V), P(T), ctypes.c_int64(0), ctypes.c_int64(sub.shape[0]))
+    dt = time.perf_counter() - t0
+    if rc != 0:
+        return None
+    return {"value": nsample / dt, "unit": "elements/s", "cores": 1, "kind": "port",
+            "sample": "first {} elements of the same p=4 hex mesh, K+T into their CSR pattern, "
+                      "{:.1f} s; C restatement of _GlobalAssemblyDF_ (dense B(HB^T) per point), "
+                      "gcc -O3 -march=native".format(nsample, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=24, help="cells per cube edge")
+    ap.add_argument("--cpu-sample", type=int, default=256, help="elements in the CPU baseline sample (0 = skip)")
+    ap.add_argument("--stiffness-minw", type=int, default=0, help="stiffness kernel variant (waves/SIMD), 0 = default")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the assembly would shard by "
+              "node-row ownership with halo elements; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from florence_amd import _lib
+    from florence_amd.assembly import DeviceMeshData, assemble_device
+    from florence_amd.workloads import cube_hex_p4, twisted_state
+
+    if args.stiffness_minw:
+        _lib.check(_lib.load().fl_set_tuning(_lib.FL_TUNE_STIFF_MINW, args.stiffness_minw), "fl_set_tuning")
+    t_setup = time.perf_counter()
+    mesh, fs, mat = cube_hex_p4(args.n)
+    x_host = twisted_state(mesh, 15.0)
+    dm = DeviceMeshData(mesh, fs, build_indices=False)
+    x = torch.as_tensor(x_host, device="cuda")
+    V = torch.empty(dm.nnz, dtype=torch.float64, device="cuda")
+    T = torch.empty(3 * dm.nnode, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t_setup
+    print("setup {:.1f}s nelem={} nnode={} nnz={} max_degree={}".format(
+        t_setup, dm.nelem, dm.nnode, dm.nnz, dm.max_degree), file=sys.stderr, flush=True)
+
+    for _ in range(args.warmup):
+        assemble_device(dm, mat, x, V, T)
+    torch.cuda.synchronize()
+    status = dm.status.cpu().numpy().tolist()
+
+    _lib.profile_enable(True)
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record()
+    for _ in range(args.steps):
+        assemble_device(dm, mat, x, V, T)
+    stop.record()
+    torch.cuda.synchronize()
+    total_ms = start.elapsed_time(stop)
+    calls = _lib.profile_read()
+    _lib.profile_enable(False)
+    phases = np.array([c[1] for c in calls if c[0] == 1])
+    phase_ms = phases.mean(axis=0) if phases.size else np.full(4, np.nan)
+    ms_per_step = total_ms / args.steps
+    value = dm.nelem / (ms_per_step / 1e3)
+
+    flops_exec = stiffness_flops_per_element(dm.npe, dm.ngauss) * dm.nelem
+    achieved = flops_exec / (phase_ms[1] / 1e3) / 1e12
+    dense_b = dense_b_flops_per_element(dm.npe, dm.ngauss) * dm.nelem
+    print("phases ms: qp {:.3f} stiffness {:.3f} traction {:.3f} gather {:.3f}".format(*phase_ms),
+          file=sys.stderr, flush=True)
+
+    cpu = None
+    if args.cpu_sample > 0:
+        cpu = cpu_baseline(mesh, fs, mat, x_host, args.cpu_sample)
+
+    out = {
+        "metric": "K+residual assembly elements/s (DF, hex p=4, FP64)",
+        "value": round(value, 1),
+        "unit": "elements/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "unit cube 24^3 hex p=4 (13824 elements, 125 nodes/element), 6^3 Gauss-Lobatto "
+                               "points, Mooney-Rivlin(0.5,0.5,20), twisted state, K+T into reference CSR "
+                               "(nnz {})".format(dm.nnz),
+                   "nelem": dm.nelem, "nnode": dm.nnode, "nnz": dm.nnz, "ngauss": dm.ngauss},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                     "kernel": "stiffness_kernel<128,4,4,2,2>",
+                     "flops_per_launch": flops_exec, "kernel_ms": round(float(phase_ms[1]), 4),
+                     "dense_b_equivalent_tflops": round(dense_b / (phase_ms[1] / 1e3) / 1e12, 3)},
+        "phase_ms": {"qp": round(float(phase_ms[0]), 4), "stiffness": round(float(phase_ms[1]), 4),
+                     "traction": round(float(phase_ms[2]), 4), "gather": round(float(phase_ms[3]), 4)},
+        "cpu_baseline": cpu,
+        "status": status,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

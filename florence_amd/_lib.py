@@ -14,6 +14,8 @@ FL_MAT_MOONEYRIVLIN = 2
 FL_S_NONPOSITIVE_J = 1
 FL_S_NONFINITE = 2
 FL_S_PATTERN = 4
+FL_GATHER_BUCKETS = 5
+FL_TUNE_STIFF_MINW = 1
 
 # every symbol declared in include/florence_amd.h -> (restype, argtypes)
 _vp, _i32, _i64, _sz, _dp = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_size_t, ctypes.c_void_p
@@ -28,12 +30,36 @@ SIGNATURES = {
     "fl_pattern_graph": (ctypes.c_int, [_vp, _i64, _i32, _i64, _vp, _vp, _vp, _P64, _P32, _vp, _sz, _vp]),
     "fl_pattern_fill": (ctypes.c_int, [_vp, _i64, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "fl_df_workspace_size": (_sz, [_i32, _i64, _i32]),
+    "fl_pattern_gather_plan": (ctypes.c_int, [_vp, _i64, _vp, _P32, _P32, _vp]),
     "fl_df_assemble": (ctypes.c_int, [_i32, _PD, _vp, _vp, _vp, _vp, _vp, _i32, _i64, _i32, _i64, _i32,
-                                      _vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _sz, _vp, _vp]),
+                                      _vp, _vp, _vp, _vp, _vp, _P32, _P32, _vp, _vp, _vp, _vp, _sz, _vp, _vp]),
+    "fl_set_tuning": (ctypes.c_int, [_i32, _i32]),
     "fl_df_residual_workspace_size": (_sz, [_i32, _i64, _i32]),
     "fl_df_residual": (ctypes.c_int, [_i32, _PD, _vp, _vp, _vp, _vp, _vp, _i32, _i64, _i32, _i64,
                                       _vp, _vp, _vp, _vp, _sz, _vp, _vp]),
+    "fl_csr_spmv": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp]),
+    "fl_pcg_workspace_size": (_sz, [_i64]),
+    "fl_pcg": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, ctypes.c_double, _i32, _i32, _vp, _sz,
+                              _P32, ctypes.POINTER(ctypes.c_double), _vp]),
+    "fl_profile_enable": (ctypes.c_int, [ctypes.c_int]),
+    "fl_profile_read": (ctypes.c_int, [_vp, _vp, _i32, _i32]),
 }
+
+
+def profile_enable(on=True):
+    load().fl_profile_enable(1 if on else 0)
+
+
+def profile_read(max_calls=4096, max_phases=4):
+    """[(kind, [phase ms ...]), ...] for the calls recorded since the last read."""
+    import numpy as np
+    kinds = np.zeros(max_calls, dtype=np.int32)
+    ms = np.zeros((max_calls, max_phases), dtype=np.float64)
+    n = load().fl_profile_read(kinds.ctypes.data_as(ctypes.c_void_p), ms.ctypes.data_as(ctypes.c_void_p),
+                               max_calls, max_phases)
+    if n < 0:
+        check(n, "fl_profile_read")
+    return [(int(kinds[i]), ms[i].tolist()) for i in range(n)]
 
 _lib = None
 

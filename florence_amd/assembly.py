@@ -76,6 +76,13 @@ class DeviceMeshData(object):
                                        _lib.ptr(self.node_nbr_ptr), _lib.ptr(self.node_nbrs),
                                        _lib.ptr(self.indptr), _lib.ptr(self.indices), _lib.stream_ptr()),
                    "fl_pattern_fill")
+        nb = _lib.FL_GATHER_BUCKETS
+        self.node_order = torch.empty(self.nnode, dtype=i32, device=self.device)
+        self.bucket_offsets = (_lib.ctypes.c_int32 * (nb + 1))()
+        self.bucket_degree = (_lib.ctypes.c_int32 * nb)()
+        _lib.check(lib.fl_pattern_gather_plan(_lib.ptr(self.node_nbr_ptr), self.nnode, _lib.ptr(self.node_order),
+                                              self.bucket_offsets, self.bucket_degree, _lib.stream_ptr()),
+                   "fl_pattern_gather_plan")
         self.status = torch.zeros(2, dtype=i32, device=self.device)
         self._ws = None
         self._ws_res = None
@@ -137,8 +144,8 @@ def assemble_device(dm, material, Eulerx, V=None, T=None, requires_geometry_upda
     _lib.check(lib.fl_df_assemble(mid, prm, _lib.ptr(dm.points), _lib.ptr(dm.elements), _lib.ptr(x),
                                   _lib.ptr(dm.Jm), _lib.ptr(dm.AllGauss), dm.ngauss, dm.nelem, dm.npe, dm.nnode,
                                   int(requires_geometry_update), _lib.ptr(dm.node_elem_ptr), _lib.ptr(dm.node_elem),
-                                  _lib.ptr(dm.node_nbr_ptr), _lib.ptr(dm.node_nbrs), dm.max_degree,
-                                  _lib.ptr(dm.indptr), _lib.ptr(V), _lib.ptr(T), _lib.ptr(ws), ws.numel(),
+                                  _lib.ptr(dm.node_nbr_ptr), _lib.ptr(dm.node_nbrs), _lib.ptr(dm.node_order),
+                                  dm.bucket_offsets, dm.bucket_degree, _lib.ptr(dm.indptr), _lib.ptr(V), _lib.ptr(T), _lib.ptr(ws), ws.numel(),
                                   _lib.ptr(dm.status), _lib.stream_ptr(stream)),
                "fl_df_assemble")
     return V, T, dm.status

@@ -27,6 +27,7 @@
 #include <cmath>
 
 #include "fl_common.hpp"
+#include "fl_profile.hpp"
 
 namespace fl {
 
@@ -200,132 +201,9 @@ __global__ __launch_bounds__(256) void qp_kernel(MatParams mp, const double *__r
 }
 
 // ------------------------------------------------------------------ phase 2
-// NP: node count padded to a multiple of 16; each wave owns a (16*MT) x (16*NT) tile of
-// one 3x3 displacement block, the workgroup (WGM x WGN waves) covers NP x NP.
-template <int NP, int MT, int NT, int WGM, int WGN>
-__global__ __launch_bounds__(64 * WGM *WGN) void stiffness_kernel(const double *__restrict__ JmT,
-                                                                   const double *__restrict__ qp, int npe, int G,
-                                                                   int64_t nelem, double *__restrict__ Ke) {
-    static_assert(NP == 16 * MT * WGM && NP == 16 * NT * WGN, "tile shape must cover NP");
-    constexpr int NTH = 64 * WGM * WGN;
-    constexpr int QC = 8;        // quadrature points per chunk
-    constexpr int KC = 3 * QC;   // inner (g, j) indices per chunk
-    constexpr int LDA = NP + 4;
-    constexpr int TT = 16 * NT * 17;   // per-wave transpose tile (doubles)
-    static_assert(WGM * WGN * TT <= 2 * KC * LDA, "transpose tile must fit the staging buffers");
-    __shared__ double sAB[2 * KC * LDA];
-    __shared__ double sQ[QC * 18];
-    double *sA = sAB;
-    double *sB = sAB + KC * LDA;
-
-    // XCD-aware: workgroup ids are dealt round-robin over the 8 XCDs; keep the six block
-    // pairs of one element on the same XCD so its point records are read from one L2.
-    const int64_t wid = blockIdx.x;
-    const int xcd = (int)(wid & 7);
-    const int64_t slot = wid >> 3;
-    const int pair = (int)(slot % 6);
-    const int64_t e = (slot / 6) * 8 + xcd;
-    if (e >= nelem) return;
-    const int pi = c_pair_i[pair], pk = c_pair_k[pair];
-
-    const int tid = threadIdx.x;
-    const int wave = tid >> 6, lane = tid & 63;
-    const int wm = wave / WGN, wn = wave % WGN;
-    const int li = lane & 15, lk = lane >> 4;
-
-    double4_t acc[MT][NT];
-#pragma unroll
-    for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-        for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = double4_t{0.0, 0.0, 0.0, 0.0};
-
-    const double *qe = qp + e * (int64_t)G * QS;
-    for (int g0 = 0; g0 < G; g0 += QC) {
-        for (int idx = tid; idx < QC * 18; idx += NTH) {
-            const int gq = idx / 18, f = idx - gq * 18, g = g0 + gq;
-            double v = 0.0;
-            if (g < G) v = f < 9 ? qe[(int64_t)g * QS + Q_IX + f] : qe[(int64_t)g * QS + Q_D + 9 * pair + (f - 9)];
-            sQ[idx] = v;
-        }
-        __syncthreads();
-        // spatial gradients G_{a,(g,j)} = sum_m ix[j][m] dN_a/dxi_m
-        for (int idx = tid; idx < QC * NP; idx += NTH) {
-            const int gq = idx / NP, a = idx - gq * NP, g = g0 + gq;
-            double j0 = 0.0, j1 = 0.0, j2 = 0.0;
-            if (g < G) {
-                const double *jt = JmT + (int64_t)g * 3 * NP + a;
-                j0 = jt[0];
-                j1 = jt[NP];
-                j2 = jt[2 * NP];
-            }
-            const double *ix = sQ + gq * 18;
-            sA[(3 * gq + 0) * LDA + a] = ix[0] * j0 + ix[1] * j1 + ix[2] * j2;
-            sA[(3 * gq + 1) * LDA + a] = ix[3] * j0 + ix[4] * j1 + ix[5] * j2;
-            sA[(3 * gq + 2) * LDA + a] = ix[6] * j0 + ix[7] * j1 + ix[8] * j2;
-        }
-        __syncthreads();
-        // W^{ik}_{b,(g,j)} = sum_l D^{ik}_{jl} G_{b,(g,l)}
-        for (int idx = tid; idx < QC * NP; idx += NTH) {
-            const int gq = idx / NP, b = idx - gq * NP;
-            const double *D = sQ + gq * 18 + 9;
-            const double s0 = sA[(3 * gq + 0) * LDA + b];
-            const double s1 = sA[(3 * gq + 1) * LDA + b];
-            const double s2 = sA[(3 * gq + 2) * LDA + b];
-            sB[(3 * gq + 0) * LDA + b] = D[0] * s0 + D[1] * s1 + D[2] * s2;
-            sB[(3 * gq + 1) * LDA + b] = D[3] * s0 + D[4] * s1 + D[5] * s2;
-            sB[(3 * gq + 2) * LDA + b] = D[6] * s0 + D[7] * s1 + D[8] * s2;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < KC / 4; ++kk) {
-            const int q = kk * 4 + lk;
-            double af[MT], bf[NT];
-#pragma unroll
-            for (int mt = 0; mt < MT; ++mt) af[mt] = sA[q * LDA + (wm * MT + mt) * 16 + li];
-#pragma unroll
-            for (int nt = 0; nt < NT; ++nt) bf[nt] = sB[q * LDA + (wn * NT + nt) * 16 + li];
-#pragma unroll
-            for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-                for (int nt = 0; nt < NT; ++nt)
-                    acc[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[mt], bf[nt], acc[mt][nt], 0, 0, 0);
-        }
-        __syncthreads();
-    }
-
-    // epilogue: K^{ik} rows a, cols b.  MFMA f64 16x16 layout: lane holds rows lk+4r, col li.
-    const int64_t npe2 = (int64_t)npe * npe;
-    double *Kd = Ke + (e * 9 + pi * 3 + pk) * npe2;
-#pragma unroll
-    for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-        for (int nt = 0; nt < NT; ++nt)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int a = (wm * MT + mt) * 16 + lk + 4 * r;
-                const int b = (wn * NT + nt) * 16 + li;
-                if (a < npe && b < npe) Kd[(int64_t)a * npe + b] = acc[mt][nt][r];
-            }
-    if (pi != pk) {
-        // mirror block K^{ki}_{ba} through a per-wave LDS transpose (coalesced stores)
-        double *Kt = Ke + (e * 9 + pk * 3 + pi) * npe2;
-        double *tile = sAB + wave * TT;
-#pragma unroll
-        for (int mt = 0; mt < MT; ++mt) {
-#pragma unroll
-            for (int nt = 0; nt < NT; ++nt)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) tile[(nt * 16 + li) * 17 + lk + 4 * r] = acc[mt][nt][r];
-            __syncthreads();
-            for (int idx = lane; idx < 16 * NT * 16; idx += 64) {
-                const int bl = idx >> 4, al = idx & 15;
-                const int b = wn * NT * 16 + bl, a = (wm * MT + mt) * 16 + al;
-                if (a < npe && b < npe) Kt[(int64_t)b * npe + a] = tile[bl * 17 + al];
-            }
-            __syncthreads();
-        }
-    }
-}
+}  // namespace fl
+#include "fl_stiffness.hpp"
+namespace fl {
 
 // ------------------------------------------------------------------ phase 3
 // t_(a,i) = sum_g sum_m dN_a/dxi_m(g) P_g[m][i],  P_g[m][i] = sum_j ix[j][m] S_g[i][j]
@@ -364,25 +242,33 @@ __device__ inline int find_rank(const int32_t *s, int n, int32_t v) {
     return s[lo] == v ? lo : -1;
 }
 
-__global__ __launch_bounds__(256) void gather_kernel(const int32_t *__restrict__ elements, int npe,
-                                                     const int32_t *__restrict__ node_elem_ptr,
-                                                     const int32_t *__restrict__ node_elem,
-                                                     const int32_t *__restrict__ nbr_ptr,
-                                                     const int32_t *__restrict__ nbrs, const int32_t *__restrict__ indptr,
-                                                     const double *__restrict__ Ke, const double *__restrict__ te,
-                                                     int max_degree, double *__restrict__ V, double *__restrict__ T,
-                                                     int32_t *__restrict__ status) {
+// One wave per node.  Nodes are visited through node_order (grouped by neighbour count
+// so each launch sizes its LDS to its own degree class); lds_deg is that class's max.
+constexpr int GT = 64;
+__global__ __launch_bounds__(GT) void gather_kernel(const int32_t *__restrict__ elements, int npe,
+                                                    const int32_t *__restrict__ node_order,
+                                                    const int32_t *__restrict__ node_elem_ptr,
+                                                    const int32_t *__restrict__ node_elem,
+                                                    const int32_t *__restrict__ nbr_ptr,
+                                                    const int32_t *__restrict__ nbrs, const int32_t *__restrict__ indptr,
+                                                    const double *__restrict__ Ke, const double *__restrict__ te,
+                                                    int lds_deg, double *__restrict__ V, double *__restrict__ T,
+                                                    int32_t *__restrict__ status) {
     extern __shared__ double smem[];
-    const int64_t a = blockIdx.x;
+    const int64_t a = node_order[blockIdx.x];
     const int tid = threadIdx.x;
     const int deg = nbr_ptr[a + 1] - nbr_ptr[a];
     const int32_t p0 = nbr_ptr[a];
-    double *acc = smem;                                              // 9*max_degree
-    int32_t *snb = reinterpret_cast<int32_t *>(smem + 9 * max_degree); // max_degree
-    int32_t *rr = snb + max_degree;                                  // npe
+    double *acc = smem;                                              // 9*lds_deg
+    int32_t *snb = reinterpret_cast<int32_t *>(smem + 9 * lds_deg);  // lds_deg
+    int32_t *rr = snb + lds_deg;                                     // npe
     const int rowlen = 3 * deg;
-    for (int i = tid; i < 9 * deg; i += 256) acc[i] = 0.0;
-    for (int i = tid; i < deg; i += 256) snb[i] = nbrs[p0 + i];
+    if (deg > lds_deg) {
+        if (tid == 0) atomicOr(status, FL_S_PATTERN);
+        return;
+    }
+    for (int i = tid; i < 9 * deg; i += GT) acc[i] = 0.0;
+    for (int i = tid; i < deg; i += GT) snb[i] = nbrs[p0 + i];
     __syncthreads();
     const int32_t q0 = node_elem_ptr[a], q1 = node_elem_ptr[a + 1];
     const int64_t npe2 = (int64_t)npe * npe;
@@ -391,21 +277,38 @@ __global__ __launch_bounds__(256) void gather_kernel(const int32_t *__restrict__
         const int32_t ent = node_elem[q];
         const int64_t e = ent / npe;
         const int la = ent - (int)(e * npe);
-        for (int lb = tid; lb < npe; lb += 256) {
+        for (int lb = tid; lb < npe; lb += GT) {
             const int r = find_rank(snb, deg, elements[e * npe + lb]);
             bad |= r < 0;
             rr[lb] = r < 0 ? 0 : r;
         }
         __syncthreads();
         const double *Kb = Ke + e * 9 * npe2 + (int64_t)la * npe;
-        for (int idx = tid; idx < 9 * npe; idx += 256) {
-            const int ik = idx / npe, lb = idx - ik * npe;
-            const int i = ik / 3, k = ik - 3 * i;
-            acc[i * rowlen + 3 * rr[lb] + k] += Kb[ik * npe2 + lb];
+        // batches of UB independent loads per lane keep enough bytes in flight; the LDS
+        // updates follow (within one element every target entry is distinct)
+        constexpr int UB = 8;
+        for (int base = 0; base < 9 * npe; base += GT * UB) {
+            double v[UB];
+            int pos[UB];
+#pragma unroll
+            for (int u = 0; u < UB; ++u) {
+                const int idx = base + u * GT + tid;
+                pos[u] = -1;
+                v[u] = 0.0;
+                if (idx < 9 * npe) {
+                    const int ik = idx / npe, lb = idx - ik * npe;
+                    const int i = ik / 3, k = ik - 3 * i;
+                    v[u] = Kb[ik * npe2 + lb];
+                    pos[u] = i * rowlen + 3 * rr[lb] + k;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < UB; ++u)
+                if (pos[u] >= 0) acc[pos[u]] += v[u];
         }
         __syncthreads();
     }
-    for (int idx = tid; idx < 9 * deg; idx += 256) {
+    for (int idx = tid; idx < 9 * deg; idx += GT) {
         const int i = idx / rowlen, c = idx - i * rowlen;
         V[(int64_t)indptr[3 * a + i] + c] = acc[idx];
     }
@@ -440,6 +343,10 @@ __global__ void gather_traction_kernel(int npe, const int32_t *__restrict__ node
 }
 
 // ------------------------------------------------------------------ host side
+static size_t gather_lds_bytes(int deg, int npe) {
+    return (size_t)deg * 9 * 8 + (size_t)deg * 4 + (size_t)npe * 4 + 16;
+}
+
 static int padded_np(int npe) {
     if (npe <= 16) return 16;
     if (npe <= 32) return 32;
@@ -462,7 +369,13 @@ static int launch_stiffness(int NP, const double *JmT, const double *qp, int npe
         case 32: stiffness_kernel<32, 2, 2, 1, 1><<<grid, 64, 0, st>>>(JmT, qp, npe, G, nelem, Ke); break;
         case 48: stiffness_kernel<48, 3, 3, 1, 1><<<grid, 64, 0, st>>>(JmT, qp, npe, G, nelem, Ke); break;
         case 64: stiffness_kernel<64, 2, 2, 2, 2><<<grid, 256, 0, st>>>(JmT, qp, npe, G, nelem, Ke); break;
-        case 128: stiffness_kernel<128, 4, 4, 2, 2><<<grid, 256, 0, st>>>(JmT, qp, npe, G, nelem, Ke); break;
+        case 128:
+            // default: registers capped for two waves per SIMD (measured faster on MI355X)
+            if (tuning_value(TUNE_STIFF_MINW, 2) == 1)
+                stiffness_kernel<128, 4, 4, 2, 2, 1><<<grid, 256, 0, st>>>(JmT, qp, npe, G, nelem, Ke);
+            else
+                stiffness_kernel<128, 4, 4, 2, 2, 2><<<grid, 256, 0, st>>>(JmT, qp, npe, G, nelem, Ke);
+            break;
         default: set_error("unsupported node count"); return FL_E_ARG;
     }
     FL_LAUNCH_CHECK();
@@ -537,13 +450,18 @@ int fl_df_assemble(int32_t material, const double *params, const double *points,
                    const double *Eulerx, const double *Jm, const double *AllGauss, int32_t G, int64_t nelem,
                    int32_t npe, int64_t nnode, int32_t update, const int32_t *node_elem_ptr,
                    const int32_t *node_elem, const int32_t *node_nbr_ptr, const int32_t *node_nbrs,
-                   int32_t max_degree, const int32_t *indptr, double *V, double *T, void *workspace,
-                   size_t workspace_bytes, int32_t *status, void *stream) {
+                   const int32_t *node_order, const int32_t *bucket_offsets, const int32_t *bucket_degree,
+                   const int32_t *indptr, double *V, double *T, void *workspace, size_t workspace_bytes,
+                   int32_t *status, void *stream) {
     hipStream_t st = as_stream(stream);
     if (!check_common(material, params, G, nelem, npe, nnode)) return FL_E_ARG;
     if (!points || !elements || !Eulerx || !Jm || !AllGauss || !node_elem_ptr || !node_elem || !node_nbr_ptr ||
-        !node_nbrs || !indptr || !V || !T || !status || max_degree <= 0) {
-        set_error("fl_df_assemble: null pointer or max_degree <= 0");
+        !node_nbrs || !node_order || !bucket_offsets || !bucket_degree || !indptr || !V || !T || !status) {
+        set_error("fl_df_assemble: null pointer");
+        return FL_E_ARG;
+    }
+    if (bucket_offsets[0] != 0 || bucket_offsets[FL_GATHER_BUCKETS] != nnode) {
+        set_error("fl_df_assemble: gather plan does not cover the %lld nodes", (long long)nnode);
         return FL_E_ARG;
     }
     size_t need = 0;
@@ -552,13 +470,15 @@ int fl_df_assemble(int32_t material, const double *params, const double *points,
         set_error("fl_df_assemble: workspace too small (%zu < %zu)", workspace_bytes, need);
         return FL_E_SPACE;
     }
-    const size_t lds_gather = (size_t)max_degree * 9 * 8 + (size_t)max_degree * 4 + (size_t)npe * 4 + 16;
-    if (lds_gather > 160 * 1024) {
-        set_error("fl_df_assemble: node degree %d needs %zu B of LDS", max_degree, lds_gather);
-        return FL_E_LIMIT;
+    for (int b = 0; b < FL_GATHER_BUCKETS; ++b) {
+        if (bucket_offsets[b + 1] > bucket_offsets[b] && gather_lds_bytes(bucket_degree[b], npe) > 160 * 1024) {
+            set_error("fl_df_assemble: node degree %d needs more LDS than a CU has", bucket_degree[b]);
+            return FL_E_LIMIT;
+        }
     }
     const int NP = padded_np(npe);
     const MatParams mp = mat_params(material, params);
+    prof_begin(st, PROF_DF_ASSEMBLE);
     FL_HIP_CHECK(hipMemsetAsync(status, 0, 2 * sizeof(int32_t), st));
     transpose_jm_kernel<<<(G * 3 * NP + 255) / 256, 256, 0, st>>>(Jm, npe, G, NP, w.JmT);
     FL_LAUNCH_CHECK();
@@ -566,20 +486,30 @@ int fl_df_assemble(int32_t material, const double *params, const double *points,
     qp_kernel<true><<<(unsigned)((nq + 255) / 256), 256, 0, st>>>(mp, points, elements, Eulerx, Jm, AllGauss, G, nelem,
                                                                    npe, update, w.qp, status);
     FL_LAUNCH_CHECK();
+    prof_mark(st);
     int rc = launch_stiffness(NP, w.JmT, w.qp, npe, G, nelem, w.Ke, st);
     if (rc) return rc;
+    prof_mark(st);
     const size_t lds_tr = (size_t)G * 9 * 8;
     if (lds_tr > 64 * 1024) FL_HIP_CHECK(hipFuncSetAttribute((const void *)traction_kernel,
                                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_tr));
     traction_kernel<<<(unsigned)nelem, 256, lds_tr, st>>>(w.JmT, NP, w.qp, QS, Q_S, npe, G, w.te);
     FL_LAUNCH_CHECK();
     if (update != 1) FL_HIP_CHECK(hipMemsetAsync(w.te, 0, sizeof(double) * nelem * 3 * npe, st));
-    if (lds_gather > 64 * 1024)
-        FL_HIP_CHECK(hipFuncSetAttribute((const void *)gather_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         (int)lds_gather));
-    gather_kernel<<<(unsigned)nnode, 256, lds_gather, st>>>(elements, npe, node_elem_ptr, node_elem, node_nbr_ptr,
-                                                            node_nbrs, indptr, w.Ke, w.te, max_degree, V, T, status);
-    FL_LAUNCH_CHECK();
+    prof_mark(st);
+    for (int b = 0; b < FL_GATHER_BUCKETS; ++b) {
+        const int32_t nb = bucket_offsets[b + 1] - bucket_offsets[b];
+        if (nb <= 0) continue;
+        const size_t lds = gather_lds_bytes(bucket_degree[b], npe);
+        if (lds > 64 * 1024)
+            FL_HIP_CHECK(hipFuncSetAttribute((const void *)gather_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             (int)lds));
+        gather_kernel<<<(unsigned)nb, GT, lds, st>>>(elements, npe, node_order + bucket_offsets[b], node_elem_ptr,
+                                                     node_elem, node_nbr_ptr, node_nbrs, indptr, w.Ke, w.te,
+                                                     bucket_degree[b], V, T, status);
+        FL_LAUNCH_CHECK();
+    }
+    prof_mark(st);
     return 0;
 }
 
@@ -601,6 +531,7 @@ int fl_df_residual(int32_t material, const double *params, const double *points,
     }
     const int NP = padded_np(npe);
     const MatParams mp = mat_params(material, params);
+    prof_begin(st, PROF_DF_RESIDUAL);
     FL_HIP_CHECK(hipMemsetAsync(status, 0, 2 * sizeof(int32_t), st));
     transpose_jm_kernel<<<(G * 3 * NP + 255) / 256, 256, 0, st>>>(Jm, npe, G, NP, w.JmT);
     FL_LAUNCH_CHECK();
@@ -608,14 +539,17 @@ int fl_df_residual(int32_t material, const double *params, const double *points,
     qp_kernel<false><<<(unsigned)((nq + 255) / 256), 256, 0, st>>>(mp, points, elements, Eulerx, Jm, AllGauss, G,
                                                                     nelem, npe, 1, w.qp, status);
     FL_LAUNCH_CHECK();
+    prof_mark(st);
     const size_t lds_tr = (size_t)G * 9 * 8;
     if (lds_tr > 64 * 1024) FL_HIP_CHECK(hipFuncSetAttribute((const void *)traction_kernel,
                                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_tr));
     traction_kernel<<<(unsigned)nelem, 256, lds_tr, st>>>(w.JmT, NP, w.qp, QS_RES, QR_S, npe, G, w.te);
     FL_LAUNCH_CHECK();
+    prof_mark(st);
     gather_traction_kernel<<<(unsigned)((3 * nnode + 255) / 256), 256, 0, st>>>(npe, node_elem_ptr, node_elem, w.te,
                                                                                  nnode, T);
     FL_LAUNCH_CHECK();
+    prof_mark(st);
     return 0;
 }
 

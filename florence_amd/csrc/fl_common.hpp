@@ -49,6 +49,10 @@ struct Carve {
     }
 };
 
+// Kernel-variant knobs (set through fl_set_tuning; 0 = library default).
+enum TuneKey { TUNE_STIFF_MINW = 1, TUNE_KEYS = 8 };
+int tuning_value(int key, int dflt);
+
 // Exclusive prefix sum of int64 values in place (n entries, last entry = total), device.
 int exclusive_scan_i64(int64_t *data, int64_t n, int64_t *scratch, size_t scratch_count, hipStream_t st);
 size_t scan_scratch_count(int64_t n);

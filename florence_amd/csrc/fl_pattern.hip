@@ -8,6 +8,7 @@
 // candidate nodes of its incident elements into LDS, bitonic-sorts them and compacts.
 #include <cstdarg>
 #include <climits>
+#include <vector>
 
 #include "fl_common.hpp"
 
@@ -369,6 +370,44 @@ int fl_pattern_fill(const int32_t *elements, int64_t nelem, int32_t npe, int64_t
         set_error("fl_pattern_fill: neighbour lists inconsistent with fl_pattern_graph (flag %d)", h);
         return FL_E_ARG;
     }
+    return 0;
+}
+
+int fl_pattern_gather_plan(const int32_t *node_nbr_ptr, int64_t nnode, int32_t *node_order, int32_t *bucket_offsets,
+                           int32_t *bucket_degree, void *stream) {
+    hipStream_t st = as_stream(stream);
+    if (!node_nbr_ptr || !node_order || !bucket_offsets || !bucket_degree || nnode <= 0) {
+        set_error("fl_pattern_gather_plan: invalid argument");
+        return FL_E_ARG;
+    }
+    static const int32_t limits[FL_GATHER_BUCKETS] = {128, 256, 512, 1024, 2048};
+    std::vector<int32_t> ptr((size_t)nnode + 1);
+    FL_HIP_CHECK(hipMemcpyAsync(ptr.data(), node_nbr_ptr, sizeof(int32_t) * (nnode + 1), hipMemcpyDeviceToHost, st));
+    FL_HIP_CHECK(hipStreamSynchronize(st));
+    std::vector<int32_t> cls((size_t)nnode);
+    int64_t count[FL_GATHER_BUCKETS] = {0, 0, 0, 0, 0};
+    for (int b = 0; b < FL_GATHER_BUCKETS; ++b) bucket_degree[b] = 0;
+    for (int64_t a = 0; a < nnode; ++a) {
+        const int32_t d = ptr[a + 1] - ptr[a];
+        int b = 0;
+        while (b < FL_GATHER_BUCKETS && d > limits[b]) ++b;
+        if (b == FL_GATHER_BUCKETS) {
+            set_error("fl_pattern_gather_plan: node %lld has %d neighbours (limit %d)", (long long)a, d,
+                      limits[FL_GATHER_BUCKETS - 1]);
+            return FL_E_LIMIT;
+        }
+        cls[a] = b;
+        ++count[b];
+        if (d > bucket_degree[b]) bucket_degree[b] = d;
+    }
+    bucket_offsets[0] = 0;
+    for (int b = 0; b < FL_GATHER_BUCKETS; ++b) bucket_offsets[b + 1] = bucket_offsets[b] + (int32_t)count[b];
+    std::vector<int32_t> order((size_t)nnode);
+    int64_t cur[FL_GATHER_BUCKETS];
+    for (int b = 0; b < FL_GATHER_BUCKETS; ++b) cur[b] = bucket_offsets[b];
+    for (int64_t a = 0; a < nnode; ++a) order[cur[cls[a]]++] = (int32_t)a;
+    FL_HIP_CHECK(hipMemcpyAsync(node_order, order.data(), sizeof(int32_t) * nnode, hipMemcpyHostToDevice, st));
+    FL_HIP_CHECK(hipStreamSynchronize(st));
     return 0;
 }
 

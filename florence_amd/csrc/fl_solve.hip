@@ -1,0 +1,351 @@
+// Linear algebra of the Newton loop on the device: SpMV on the reference CSR pattern
+// and a block-Jacobi preconditioned conjugate gradient for the Dirichlet-reduced
+// tangent.
+//
+// The reference reduces K to the free dofs by scipy fancy indexing
+// (BoundaryCondition.py:842-857) and calls a sparse direct solver
+// (LinearSolver.py:208-300).  Here the reduction is a mask: the operator is
+// A = M K M + (I - M) with M = diag(free), which keeps the CSR untouched; every
+// iterate vanishes on fixed dofs.  All reductions are two-pass with a fixed order,
+// so a solve is bitwise reproducible.
+#include <cmath>
+#include <vector>
+
+#include "fl_common.hpp"
+
+namespace fl {
+
+constexpr int SV_T = 256;   // threads per block for vector kernels
+constexpr int SP_W = 4;     // waves (nodes) per SpMV block
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// y = A x (masked when free != nullptr).  One wave per node a, its three rows share the
+// neighbour list; lanes stride over neighbours.  Optional: partial of dot(x, y).
+__global__ __launch_bounds__(64 * SP_W) void spmv_kernel(const int32_t *__restrict__ indptr,
+                                                         const int32_t *__restrict__ nbr_ptr,
+                                                         const int32_t *__restrict__ nbrs,
+                                                         const double *__restrict__ V, int64_t nnode,
+                                                         const uint8_t *__restrict__ free_mask,
+                                                         const double *__restrict__ x, double *__restrict__ y,
+                                                         double *__restrict__ dot_partial) {
+    __shared__ double sdot[SP_W];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t a = (int64_t)blockIdx.x * SP_W + wave;
+    double d = 0.0;
+    if (a < nnode) {
+        const int32_t p0 = nbr_ptr[a], deg = nbr_ptr[a + 1] - p0;
+        const double *r0 = V + indptr[3 * a];
+        const double *r1 = V + indptr[3 * a + 1];
+        const double *r2 = V + indptr[3 * a + 2];
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        for (int r = lane; r < deg; r += 64) {
+            const int64_t b = nbrs[p0 + r];
+            const double x0 = x[3 * b], x1 = x[3 * b + 1], x2 = x[3 * b + 2];
+            s0 += r0[3 * r] * x0 + r0[3 * r + 1] * x1 + r0[3 * r + 2] * x2;
+            s1 += r1[3 * r] * x0 + r1[3 * r + 1] * x1 + r1[3 * r + 2] * x2;
+            s2 += r2[3 * r] * x0 + r2[3 * r + 1] * x1 + r2[3 * r + 2] * x2;
+        }
+        s0 = wave_sum(s0);
+        s1 = wave_sum(s1);
+        s2 = wave_sum(s2);
+        if (lane == 0) {
+            double out[3] = {s0, s1, s2};
+            for (int i = 0; i < 3; ++i) {
+                const int64_t row = 3 * a + i;
+                const double v = (free_mask && !free_mask[row]) ? x[row] : out[i];
+                y[row] = v;
+                d += v * x[row];
+            }
+        }
+    }
+    if (dot_partial) {
+        if (lane == 0) sdot[wave] = d;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double s = 0.0;
+            for (int w = 0; w < SP_W; ++w) s += sdot[w];
+            dot_partial[blockIdx.x] = s;
+        }
+    }
+}
+
+// Per-node inverse of the masked 3x3 diagonal block (identity rows/cols on fixed dofs).
+__global__ void block_jacobi_kernel(const int32_t *__restrict__ indptr, const int32_t *__restrict__ nbr_ptr,
+                                    const int32_t *__restrict__ nbrs, const double *__restrict__ V, int64_t nnode,
+                                    const uint8_t *__restrict__ free_mask, double *__restrict__ P,
+                                    int32_t *__restrict__ bad) {
+    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nnode) return;
+    const int32_t p0 = nbr_ptr[a], deg = nbr_ptr[a + 1] - p0;
+    int lo = 0, hi = deg;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (a < nbrs[p0 + mid]) hi = mid; else lo = mid;
+    }
+    double B[9];
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) {
+            const bool fi = free_mask ? free_mask[3 * a + i] != 0 : true;
+            const bool fk = free_mask ? free_mask[3 * a + k] != 0 : true;
+            B[3 * i + k] = (fi && fk) ? V[indptr[3 * a + i] + 3 * lo + k] : (i == k ? 1.0 : 0.0);
+        }
+    double inv[9];
+    inv[0] = B[4] * B[8] - B[5] * B[7];
+    inv[1] = -B[1] * B[8] + B[2] * B[7];
+    inv[2] = B[1] * B[5] - B[2] * B[4];
+    inv[3] = -B[3] * B[8] + B[5] * B[6];
+    inv[4] = B[0] * B[8] - B[2] * B[6];
+    inv[5] = -B[0] * B[5] + B[2] * B[3];
+    inv[6] = B[3] * B[7] - B[4] * B[6];
+    inv[7] = -B[0] * B[7] + B[1] * B[6];
+    inv[8] = B[0] * B[4] - B[1] * B[3];
+    const double det = B[0] * inv[0] + B[1] * inv[3] + B[2] * inv[6];
+    if (!(fabs(det) > 0.0) || nbrs[p0 + lo] != a) {
+        atomicOr(bad, 1);
+        for (int i = 0; i < 9; ++i) P[9 * a + i] = (i % 4) == 0 ? 1.0 : 0.0;
+        return;
+    }
+    for (int i = 0; i < 9; ++i) P[9 * a + i] = inv[i] / det;
+}
+
+// scalars s[]: 0 rz, 1 pq, 2 alpha, 3 rz_new, 4 beta, 5 rr, 6 bb
+__global__ void reduce_kernel(const double *__restrict__ partial, int64_t n, double *__restrict__ out) {
+    __shared__ double sh[SV_T];
+    double s = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += SV_T) s += partial[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = SV_T / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sh[0];
+}
+
+__global__ void reduce2_kernel(const double *__restrict__ partial, int64_t n, double *__restrict__ out0,
+                               double *__restrict__ out1) {
+    __shared__ double sh0[SV_T], sh1[SV_T];
+    double s0 = 0.0, s1 = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += SV_T) {
+        s0 += partial[2 * i];
+        s1 += partial[2 * i + 1];
+    }
+    sh0[threadIdx.x] = s0;
+    sh1[threadIdx.x] = s1;
+    __syncthreads();
+    for (int off = SV_T / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            sh0[threadIdx.x] += sh0[threadIdx.x + off];
+            sh1[threadIdx.x] += sh1[threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        *out0 = sh0[0];
+        *out1 = sh1[0];
+    }
+}
+
+// x += alpha p; r -= alpha q; z = P r (node blocks); partials of (r.z, r.r)
+__global__ __launch_bounds__(SV_T) void cg_update_kernel(int64_t nnode, const double *__restrict__ s,
+                                                         const double *__restrict__ P, const double *__restrict__ p,
+                                                         const double *__restrict__ q, double *__restrict__ x,
+                                                         double *__restrict__ r, double *__restrict__ z,
+                                                         double *__restrict__ partial) {
+    __shared__ double sh0[SV_T], sh1[SV_T];
+    const int64_t a = (int64_t)blockIdx.x * SV_T + threadIdx.x;
+    double rz = 0.0, rr = 0.0;
+    if (a < nnode) {
+        const double alpha = s[2];
+        double rv[3];
+        for (int i = 0; i < 3; ++i) {
+            const int64_t k = 3 * a + i;
+            x[k] += alpha * p[k];
+            rv[i] = r[k] - alpha * q[k];
+            r[k] = rv[i];
+        }
+        for (int i = 0; i < 3; ++i) {
+            const double zi = P[9 * a + 3 * i] * rv[0] + P[9 * a + 3 * i + 1] * rv[1] + P[9 * a + 3 * i + 2] * rv[2];
+            z[3 * a + i] = zi;
+            rz += rv[i] * zi;
+            rr += rv[i] * rv[i];
+        }
+    }
+    sh0[threadIdx.x] = rz;
+    sh1[threadIdx.x] = rr;
+    __syncthreads();
+    for (int off = SV_T / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            sh0[threadIdx.x] += sh0[threadIdx.x + off];
+            sh1[threadIdx.x] += sh1[threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = sh0[0];
+        partial[2 * blockIdx.x + 1] = sh1[0];
+    }
+}
+
+// z = P r with partials of (r.z, r.r); used to start the iteration (x = 0, r = b).
+__global__ __launch_bounds__(SV_T) void cg_start_kernel(int64_t nnode, const double *__restrict__ P,
+                                                        const double *__restrict__ b, const uint8_t *__restrict__ free_mask,
+                                                        double *__restrict__ x, double *__restrict__ r,
+                                                        double *__restrict__ z, double *__restrict__ p,
+                                                        double *__restrict__ partial) {
+    __shared__ double sh0[SV_T], sh1[SV_T];
+    const int64_t a = (int64_t)blockIdx.x * SV_T + threadIdx.x;
+    double rz = 0.0, rr = 0.0;
+    if (a < nnode) {
+        double rv[3];
+        for (int i = 0; i < 3; ++i) {
+            const int64_t k = 3 * a + i;
+            rv[i] = (free_mask && !free_mask[k]) ? 0.0 : b[k];
+            r[k] = rv[i];
+            x[k] = 0.0;
+        }
+        for (int i = 0; i < 3; ++i) {
+            const double zi = P[9 * a + 3 * i] * rv[0] + P[9 * a + 3 * i + 1] * rv[1] + P[9 * a + 3 * i + 2] * rv[2];
+            z[3 * a + i] = zi;
+            p[3 * a + i] = zi;
+            rz += rv[i] * zi;
+            rr += rv[i] * rv[i];
+        }
+    }
+    sh0[threadIdx.x] = rz;
+    sh1[threadIdx.x] = rr;
+    __syncthreads();
+    for (int off = SV_T / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            sh0[threadIdx.x] += sh0[threadIdx.x + off];
+            sh1[threadIdx.x] += sh1[threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = sh0[0];
+        partial[2 * blockIdx.x + 1] = sh1[0];
+    }
+}
+
+__global__ void alpha_kernel(double *s) {
+    // s[1] = p.q ; alpha = rz / pq
+    s[2] = s[1] != 0.0 ? s[0] / s[1] : 0.0;
+}
+
+__global__ void beta_kernel(double *s) {
+    // s[3] = rz_new ; beta = rz_new / rz ; rz = rz_new
+    s[4] = s[0] != 0.0 ? s[3] / s[0] : 0.0;
+    s[0] = s[3];
+}
+
+__global__ void direction_kernel(int64_t n, const double *__restrict__ s, const double *__restrict__ z,
+                                 double *__restrict__ p) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = z[k] + s[4] * p[k];
+}
+
+}  // namespace fl
+
+using namespace fl;
+
+extern "C" {
+
+int fl_csr_spmv(const int32_t *indptr, const int32_t *node_nbr_ptr, const int32_t *node_nbrs, const double *V,
+                int64_t nnode, const uint8_t *free_mask, const double *x, double *y, void *stream) {
+    if (!indptr || !node_nbr_ptr || !node_nbrs || !V || !x || !y || nnode <= 0) {
+        set_error("fl_csr_spmv: invalid argument");
+        return FL_E_ARG;
+    }
+    hipStream_t st = as_stream(stream);
+    spmv_kernel<<<(unsigned)((nnode + SP_W - 1) / SP_W), 64 * SP_W, 0, st>>>(indptr, node_nbr_ptr, node_nbrs, V,
+                                                                            nnode, free_mask, x, y, nullptr);
+    FL_LAUNCH_CHECK();
+    return 0;
+}
+
+size_t fl_pcg_workspace_size(int64_t nnode) {
+    const int64_t n = 3 * nnode;
+    const int64_t nb_sp = (nnode + SP_W - 1) / SP_W, nb_v = (nnode + SV_T - 1) / SV_T;
+    return align_up(sizeof(double) * 9 * nnode, 256) + 4 * align_up(sizeof(double) * n, 256) +
+           align_up(sizeof(double) * (nb_sp + 2 * nb_v + 8), 256) + align_up(sizeof(double) * 8, 256) + 1024;
+}
+
+int fl_pcg(const int32_t *indptr, const int32_t *node_nbr_ptr, const int32_t *node_nbrs, const double *V,
+           int64_t nnode, const uint8_t *free_mask, const double *b, double *x, double rtol, int32_t maxit,
+           int32_t check_every, void *workspace, size_t workspace_bytes, int32_t *iterations, double *rel_residual,
+           void *stream) {
+    hipStream_t st = as_stream(stream);
+    if (!indptr || !node_nbr_ptr || !node_nbrs || !V || !b || !x || nnode <= 0 || maxit <= 0 || !iterations ||
+        !rel_residual) {
+        set_error("fl_pcg: invalid argument");
+        return FL_E_ARG;
+    }
+    if (check_every <= 0) check_every = 1;
+    const int64_t n = 3 * nnode;
+    const int64_t nb_sp = (nnode + SP_W - 1) / SP_W, nb_v = (nnode + SV_T - 1) / SV_T;
+    Carve cv(workspace, workspace_bytes);
+    double *P = cv.take<double>(9 * nnode);
+    double *r = cv.take<double>(n);
+    double *z = cv.take<double>(n);
+    double *p = cv.take<double>(n);
+    double *q = cv.take<double>(n);
+    double *part = cv.take<double>(nb_sp + 2 * nb_v + 8);
+    double *s = cv.take<double>(8);
+    if (!cv.ok) {
+        set_error("fl_pcg: workspace too small (%zu < %zu)", workspace_bytes, cv.off);
+        return FL_E_SPACE;
+    }
+    int32_t *bad = reinterpret_cast<int32_t *>(s + 7);
+    FL_HIP_CHECK(hipMemsetAsync(s, 0, sizeof(double) * 8, st));
+    block_jacobi_kernel<<<(unsigned)((nnode + 255) / 256), 256, 0, st>>>(indptr, node_nbr_ptr, node_nbrs, V, nnode,
+                                                                        free_mask, P, bad);
+    FL_LAUNCH_CHECK();
+    cg_start_kernel<<<(unsigned)nb_v, SV_T, 0, st>>>(nnode, P, b, free_mask, x, r, z, p, part);
+    FL_LAUNCH_CHECK();
+    reduce2_kernel<<<1, SV_T, 0, st>>>(part, nb_v, s + 0, s + 6);   // rz, bb (= r0.r0)
+    FL_LAUNCH_CHECK();
+    double hs[8];
+    FL_HIP_CHECK(hipMemcpyAsync(hs, s, sizeof(hs), hipMemcpyDeviceToHost, st));
+    FL_HIP_CHECK(hipStreamSynchronize(st));
+    if (*reinterpret_cast<int32_t *>(&hs[7])) {
+        set_error("fl_pcg: singular diagonal block or missing diagonal entry");
+        return FL_E_ARG;
+    }
+    const double bnorm = std::sqrt(hs[6]);
+    *iterations = 0;
+    *rel_residual = 0.0;
+    if (bnorm == 0.0) return 0;
+    double rel = 1.0;
+    int it = 0;
+    while (it < maxit) {
+        spmv_kernel<<<(unsigned)nb_sp, 64 * SP_W, 0, st>>>(indptr, node_nbr_ptr, node_nbrs, V, nnode, free_mask, p, q,
+                                                           part);
+        reduce_kernel<<<1, SV_T, 0, st>>>(part, nb_sp, s + 1);
+        alpha_kernel<<<1, 1, 0, st>>>(s);
+        cg_update_kernel<<<(unsigned)nb_v, SV_T, 0, st>>>(nnode, s, P, p, q, x, r, z, part);
+        reduce2_kernel<<<1, SV_T, 0, st>>>(part, nb_v, s + 3, s + 5);
+        beta_kernel<<<1, 1, 0, st>>>(s);
+        direction_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, s, z, p);
+        FL_LAUNCH_CHECK();
+        ++it;
+        if (it % check_every == 0 || it == maxit) {
+            double rr = 0.0;
+            FL_HIP_CHECK(hipMemcpyAsync(&rr, s + 5, sizeof(double), hipMemcpyDeviceToHost, st));
+            FL_HIP_CHECK(hipStreamSynchronize(st));
+            rel = std::sqrt(rr) / bnorm;
+            if (!(rel == rel)) break;
+            if (rel <= rtol) break;
+        }
+    }
+    *iterations = it;
+    *rel_residual = rel;
+    return 0;
+}
+
+}  // extern "C"

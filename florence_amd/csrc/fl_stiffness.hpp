@@ -1,0 +1,174 @@
+// Phase 2 of the DF assembly: element stiffness blocks on FP64 MFMA (included by
+// fl_assemble.hip).
+//
+// One workgroup per (element, displacement block pair i <= k).  With G_{a,(g,j)} =
+// dN_a/dx_j at point g and D^{ik}_{jl}(g) = detJ_g (c_ijkl + delta_ik sigma_jl), the
+// constitutive + geometric stiffness of the reference
+// (_ConstitutiveStiffnessDF_.h:83-134 and _GeometricStiffness_.h:61-100) is, per block,
+//
+//     K^{ik}_{ab} = sum_{(g,j)} G_{a,(g,j)} W^{ik}_{b,(g,j)},
+//     W^{ik}_{b,(g,j)} = sum_l D^{ik}_{jl}(g) G_{b,(g,l)},
+//
+// a dense npe x npe x 3G product.  The inner index runs in chunks of 8 points (24
+// indices).  Per chunk the workgroup builds G for all nodes in LDS (from the padded
+// parent gradients JmT and the per-point inverse Jacobians), and each lane forms its
+// W operand on the fly from three G values and one row of D while the MFMAs
+// (v_mfma_f64_16x16x4f64) run.  LDS is double buffered and the next chunk's global
+// loads are issued before the current chunk's MFMAs, so one barrier per chunk remains.
+#pragma once
+
+namespace fl {
+
+template <int NP, int MT, int NT, int WGM, int WGN, int MINW = 1>
+__global__ __launch_bounds__(64 * WGM *WGN, MINW) void stiffness_kernel(const double *__restrict__ JmT,
+                                                                   const double *__restrict__ qp, int npe, int G,
+                                                                   int64_t nelem, double *__restrict__ Ke) {
+    static_assert(NP == 16 * MT * WGM && NP == 16 * NT * WGN, "tile shape must cover NP");
+    constexpr int NTH = 64 * WGM * WGN;
+    constexpr int QC = 8;          // quadrature points per chunk
+    constexpr int KC = 3 * QC;     // inner (g, j) indices per chunk
+    constexpr int LDA = NP + 4;    // LDS row stride (doubles)
+    constexpr int TPG = NTH / QC;  // fill threads per point
+    constexpr int IPT = NP / TPG;  // nodes filled per thread
+    static_assert(NTH % QC == 0 && NP % TPG == 0, "fill mapping");
+    constexpr int PDN = (QC * 9 + NTH - 1) / NTH;   // D values staged per thread
+    constexpr int TT = 16 * NT * 17;   // per-wave transpose tile (doubles)
+    static_assert(WGM * WGN * TT <= 2 * KC * LDA, "transpose tile must fit the staging buffers");
+    __shared__ double sA[2 * KC * LDA];
+    __shared__ double sD[2 * QC * 9];
+
+    // XCD-aware order: workgroup ids are dealt round-robin over the 8 XCDs; the six
+    // block pairs of one element land on the same XCD and share its L2.
+    const int64_t wid = blockIdx.x;
+    const int xcd = (int)(wid & 7);
+    const int64_t slot = wid >> 3;
+    const int pair = (int)(slot % 6);
+    const int64_t e = (slot / 6) * 8 + xcd;
+    if (e >= nelem) return;
+    const int pi = c_pair_i[pair], pk = c_pair_k[pair];
+
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int wm = wave / WGN, wn = wave % WGN;
+    const int li = lane & 15, lk = lane >> 4;
+    const double *qe = qp + e * (int64_t)G * QS;
+
+    // fill role: point fg of the chunk, nodes fa0 + TPG*s
+    const int fg = tid / TPG, fa0 = tid % TPG;
+    double pj[IPT][3], pix[9], pd[PDN];
+    auto prefetch = [&](int g0) {
+        const int g = g0 + fg;
+        if (g < G) {
+            const double *jt = JmT + (int64_t)g * 3 * NP + fa0;
+#pragma unroll
+            for (int s = 0; s < IPT; ++s) {
+                pj[s][0] = jt[TPG * s];
+                pj[s][1] = jt[TPG * s + NP];
+                pj[s][2] = jt[TPG * s + 2 * NP];
+            }
+#pragma unroll
+            for (int m = 0; m < 9; ++m) pix[m] = qe[(int64_t)g * QS + Q_IX + m];
+        } else {
+#pragma unroll
+            for (int s = 0; s < IPT; ++s) pj[s][0] = pj[s][1] = pj[s][2] = 0.0;
+#pragma unroll
+            for (int m = 0; m < 9; ++m) pix[m] = 0.0;
+        }
+#pragma unroll
+        for (int s = 0; s < PDN; ++s) {
+            const int t = tid + s * NTH;
+            const int gq = t / 9, f = t - 9 * gq, gg = g0 + gq;
+            pd[s] = (t < QC * 9 && gg < G) ? qe[(int64_t)gg * QS + Q_D + 9 * pair + f] : 0.0;
+        }
+    };
+    auto commit = [&](int buf) {
+        double *A = sA + buf * KC * LDA;
+#pragma unroll
+        for (int s = 0; s < IPT; ++s) {
+            const int a = fa0 + TPG * s;
+            A[(3 * fg + 0) * LDA + a] = pix[0] * pj[s][0] + pix[1] * pj[s][1] + pix[2] * pj[s][2];
+            A[(3 * fg + 1) * LDA + a] = pix[3] * pj[s][0] + pix[4] * pj[s][1] + pix[5] * pj[s][2];
+            A[(3 * fg + 2) * LDA + a] = pix[6] * pj[s][0] + pix[7] * pj[s][1] + pix[8] * pj[s][2];
+        }
+#pragma unroll
+        for (int s = 0; s < PDN; ++s)
+            if (tid + s * NTH < QC * 9) sD[buf * QC * 9 + tid + s * NTH] = pd[s];
+    };
+
+    double4_t acc[MT][NT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = double4_t{0.0, 0.0, 0.0, 0.0};
+
+    prefetch(0);
+    commit(0);
+    __syncthreads();
+    int buf = 0;
+    for (int g0 = 0; g0 < G; g0 += QC) {
+        const bool more = g0 + QC < G;
+        if (more) prefetch(g0 + QC);
+        const double *A = sA + buf * KC * LDA;
+        const double *Dc = sD + buf * QC * 9;
+        // not unrolled: each step already issues MT*NT long MFMAs, and hoisting the LDS
+        // operands of several steps would cost the registers of a second wave per SIMD
+#pragma unroll 1
+        for (int kk = 0; kk < KC / 4; ++kk) {
+            const int q = kk * 4 + lk;
+            const int gq = q / 3, j = q - 3 * gq;
+            const double d0 = Dc[gq * 9 + 3 * j], d1 = Dc[gq * 9 + 3 * j + 1], d2 = Dc[gq * 9 + 3 * j + 2];
+            const double *Gb = A + 3 * gq * LDA;
+            double af[MT], bf[NT];
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) af[mt] = A[q * LDA + (wm * MT + mt) * 16 + li];
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                const int b = (wn * NT + nt) * 16 + li;
+                bf[nt] = d0 * Gb[b] + d1 * Gb[LDA + b] + d2 * Gb[2 * LDA + b];
+            }
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt)
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[mt], bf[nt], acc[mt][nt], 0, 0, 0);
+        }
+        if (more) commit(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+
+    // epilogue: K^{ik} rows a, cols b.  MFMA f64 16x16 layout: lane holds rows lk+4r, col li.
+    const int64_t npe2 = (int64_t)npe * npe;
+    double *Kd = Ke + (e * 9 + pi * 3 + pk) * npe2;
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int a = (wm * MT + mt) * 16 + lk + 4 * r;
+                const int b = (wn * NT + nt) * 16 + li;
+                if (a < npe && b < npe) Kd[(int64_t)a * npe + b] = acc[mt][nt][r];
+            }
+    if (pi != pk) {
+        // mirror block K^{ki}_{ba} through a per-wave LDS transpose (coalesced stores)
+        double *Kt = Ke + (e * 9 + pk * 3 + pi) * npe2;
+        double *tile = sA + wave * TT;
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) tile[(nt * 16 + li) * 17 + lk + 4 * r] = acc[mt][nt][r];
+            __syncthreads();
+            for (int idx = lane; idx < 16 * NT * 16; idx += 64) {
+                const int bl = idx >> 4, al = idx & 15;
+                const int b = wn * NT * 16 + bl, a = (wm * MT + mt) * 16 + al;
+                if (a < npe && b < npe) Kt[(int64_t)b * npe + a] = tile[bl * 17 + al];
+            }
+            __syncthreads();
+        }
+    }
+}
+
+}  // namespace fl

@@ -67,6 +67,16 @@ int fl_pattern_fill(const int32_t *elements_dev, int64_t nelem, int32_t npe, int
                     const int32_t *node_nbr_ptr_dev, int32_t *node_nbrs_dev, int32_t *indptr_dev,
                     int32_t *indices_dev, void *stream);
 
+/* Gather plan of the row-owner accumulation: node ids grouped by neighbour count
+ * into FL_GATHER_BUCKETS classes (at most 128, 256, 512, 1024, 2048 neighbours), so
+ * each class is accumulated with an LDS footprint sized to it.  node_order_dev:
+ * int32[nnode]; bucket_offsets (host): int32[FL_GATHER_BUCKETS+1]; bucket_degree
+ * (host): int32[FL_GATHER_BUCKETS], the largest neighbour count in each class.
+ * Synchronous.  No reference counterpart (part of the pattern set-up above). */
+#define FL_GATHER_BUCKETS 5
+int fl_pattern_gather_plan(const int32_t *node_nbr_ptr_dev, int64_t nnode, int32_t *node_order_dev,
+                           int32_t *bucket_offsets, int32_t *bucket_degree, void *stream);
+
 /* ------------------------------------------------- implicit DF assembly
  * Replaces _GlobalAssemblyDF_<Material> (FiniteElements/Assembly/_Assembly_/
  * _LowLevelAssemblyDF_.h:8-176) in its CSR mode (recompute_sparsity_pattern=0):
@@ -86,7 +96,8 @@ int fl_df_assemble(int32_t material, const double *params, const double *points_
                    const double *AllGauss_dev, int32_t ngauss, int64_t nelem, int32_t npe, int64_t nnode,
                    int32_t requires_geometry_update, const int32_t *node_elem_ptr_dev,
                    const int32_t *node_elem_dev, const int32_t *node_nbr_ptr_dev,
-                   const int32_t *node_nbrs_dev, int32_t max_degree, const int32_t *indptr_dev,
+                   const int32_t *node_nbrs_dev, const int32_t *node_order_dev,
+                   const int32_t *bucket_offsets, const int32_t *bucket_degree, const int32_t *indptr_dev,
                    double *V_dev, double *T_dev, void *workspace_dev, size_t workspace_bytes,
                    int32_t *status_dev, void *stream);
 
@@ -102,6 +113,47 @@ int fl_df_residual(int32_t material, const double *params, const double *points_
                    const double *AllGauss_dev, int32_t ngauss, int64_t nelem, int32_t npe, int64_t nnode,
                    const int32_t *node_elem_ptr_dev, const int32_t *node_elem_dev, double *T_dev,
                    void *workspace_dev, size_t workspace_bytes, int32_t *status_dev, void *stream);
+
+/* ------------------------------------------------------ Newton linear algebra
+ * The reference reduces the tangent to the free dofs with scipy fancy indexing
+ * (BoundaryCondition.py:842-857, GetReducedMatrices) and solves with a sparse direct
+ * solver (Solver/LinearSolver.py:208-300, LinearSolver.Solve).  Here the reduction is
+ * a mask free_mask[3*nnode] (1 = free dof; NULL = all free): the operator is
+ * A = M K M + (I - M), M = diag(free_mask), on the unmodified CSR of fl_df_assemble.
+ *
+ * fl_csr_spmv: y = A x (y = K x when free_mask is NULL).
+ * fl_pcg: block-Jacobi (3x3 node blocks) preconditioned conjugate gradient for
+ * A x = b, x starting from 0, stopping when ||r|| <= rtol ||b|| (tested every
+ * check_every iterations, which costs one host synchronisation) or after maxit
+ * iterations.  b must vanish on fixed dofs; x does.  Synchronous: returns the
+ * iteration count and final relative residual in host *iterations / *rel_residual.
+ * workspace: fl_pcg_workspace_size(nnode) bytes.
+ */
+int fl_csr_spmv(const int32_t *indptr_dev, const int32_t *node_nbr_ptr_dev, const int32_t *node_nbrs_dev,
+                const double *V_dev, int64_t nnode, const uint8_t *free_mask_dev, const double *x_dev,
+                double *y_dev, void *stream);
+size_t fl_pcg_workspace_size(int64_t nnode);
+int fl_pcg(const int32_t *indptr_dev, const int32_t *node_nbr_ptr_dev, const int32_t *node_nbrs_dev,
+           const double *V_dev, int64_t nnode, const uint8_t *free_mask_dev, const double *b_dev, double *x_dev,
+           double rtol, int32_t maxit, int32_t check_every, void *workspace_dev, size_t workspace_bytes,
+           int32_t *iterations, double *rel_residual, void *stream);
+
+/* ------------------------------------------------------------- profiling
+ * No reference counterpart (the reference only records wall time,
+ * Assembly.py:39,93).  When enabled, every fl_df_assemble / fl_df_residual call
+ * records HIP events on its stream at its phase boundaries.  fl_profile_read waits
+ * for them and returns, per recorded call, its kind (1 assemble, 2 residual) and the
+ * phase durations in ms -- assemble: qp, stiffness, traction, gather; residual: qp,
+ * traction, gather (unused phases -1) -- then forgets them.  Returns the call count.
+ */
+int fl_profile_enable(int on);
+
+/* Kernel-variant selection (no reference counterpart; the reference picks variants
+ * at build time, setup.py:247-251).  key 1 = minimum waves per SIMD the p=4 hex
+ * stiffness kernel is compiled for (1 or 2).  value 0 restores the default. */
+#define FL_TUNE_STIFF_MINW 1
+int fl_set_tuning(int32_t key, int32_t value);
+int fl_profile_read(int32_t *kinds, double *phase_ms, int32_t max_calls, int32_t max_phases);
 
 #ifdef __cplusplus
 }
