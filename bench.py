@@ -90,7 +90,7 @@ def main():
 
     import torch
     from florence_amd import _lib
-    from florence_amd.assembly import DeviceMeshData, assemble_device
+    from florence_amd.assembly import DeviceMeshData, assemble_device, residual_device
     from florence_amd.workloads import cube_hex_p4, twisted_state
 
     if args.stiffness_minw:
@@ -134,6 +134,31 @@ def main():
     print("phases ms: qp {:.3f} stiffness {:.3f} traction {:.3f} gather {:.3f}".format(*phase_ms),
           file=sys.stderr, flush=True)
 
+    # matrix-free residual-only pass on the same mesh (explicit DF internal force)
+    del V
+    dm.release_workspace()
+    Tres = torch.empty(3 * dm.nnode, dtype=torch.float64, device="cuda")
+    residual_device(dm, mat, x, Tres)
+    torch.cuda.synchronize()
+    _lib.profile_enable(True)
+    r0, r1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    r0.record()
+    for _ in range(args.steps):
+        residual_device(dm, mat, x, Tres)
+    r1.record()
+    torch.cuda.synchronize()
+    res_ms = r0.elapsed_time(r1) / args.steps
+    rcalls = _lib.profile_read()
+    _lib.profile_enable(False)
+    rph = np.array([c[1][:3] for c in rcalls if c[0] == 2]).mean(axis=0)
+    # residual-only algorithmic work: G * (108 npe + n + 300) flops per element (dense-B count)
+    res_flops = dm.ngauss * (108 * dm.npe + 3 * dm.npe + 300) * dm.nelem
+    residual_only = {"elements_per_s": round(dm.nelem / (res_ms / 1e3), 1), "ms_per_pass": round(res_ms, 4),
+                     "phase_ms": {"qp": round(float(rph[0]), 4), "traction": round(float(rph[1]), 4),
+                                  "gather": round(float(rph[2]), 4)},
+                     "tflops": round(res_flops / (res_ms / 1e3) / 1e12, 3),
+                     "max_abs_diff_vs_full_T": float((Tres - T).abs().max())}
+
     cpu = None
     if args.cpu_sample > 0:
         cpu = cpu_baseline(mesh, fs, mat, x_host, args.cpu_sample)
@@ -163,6 +188,7 @@ def main():
         "phase_ms": {"qp": round(float(phase_ms[0]), 4), "stiffness": round(float(phase_ms[1]), 4),
                      "traction": round(float(phase_ms[2]), 4), "gather": round(float(phase_ms[3]), 4)},
         "cpu_baseline": cpu,
+        "residual_only": residual_only,
         "status": status,
     }
     print(json.dumps(out), flush=True)

@@ -16,6 +16,10 @@ FL_S_NONFINITE = 2
 FL_S_PATTERN = 4
 FL_GATHER_BUCKETS = 5
 FL_TUNE_STIFF_MINW = 1
+FL_PREC_FP64 = 0
+FL_PREC_FP32 = 1
+FL_PREC_BF16 = 2
+PRECISIONS = {"fp64": FL_PREC_FP64, "fp32": FL_PREC_FP32, "bf16": FL_PREC_BF16}
 
 # every symbol declared in include/florence_amd.h -> (restype, argtypes)
 _vp, _i32, _i64, _sz, _dp = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_size_t, ctypes.c_void_p
@@ -34,6 +38,10 @@ SIGNATURES = {
     "fl_df_assemble": (ctypes.c_int, [_i32, _PD, _vp, _vp, _vp, _vp, _vp, _i32, _i64, _i32, _i64, _i32,
                                       _vp, _vp, _vp, _vp, _vp, _P32, _P32, _vp, _vp, _vp, _vp, _sz, _vp, _vp]),
     "fl_set_tuning": (ctypes.c_int, [_i32, _i32]),
+    "fl_df_workspace_size_ex": (_sz, [_i32, _i64, _i32, _i32]),
+    "fl_df_assemble_ex": (ctypes.c_int, [_i32, _PD, _vp, _vp, _vp, _vp, _vp, _i32, _i64, _i32, _i64, _i32,
+                                         _vp, _vp, _vp, _vp, _vp, _P32, _P32, _vp, _vp, _vp, _i32, _vp, _sz, _vp,
+                                         _vp]),
     "fl_df_residual_workspace_size": (_sz, [_i32, _i64, _i32]),
     "fl_df_residual": (ctypes.c_int, [_i32, _PD, _vp, _vp, _vp, _vp, _vp, _i32, _i64, _i32, _i64,
                                       _vp, _vp, _vp, _vp, _sz, _vp, _vp]),

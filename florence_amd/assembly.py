@@ -95,6 +95,7 @@ class DeviceMeshData(object):
                 n = int(lib.fl_df_residual_workspace_size(self.ngauss, self.nelem, self.npe))
                 self._ws_res = torch.empty(n, dtype=torch.uint8, device=self.device)
             return self._ws_res
+        # one workspace sized for the FP64 tangent serves every precision
         if self._ws is None:
             n = int(lib.fl_df_workspace_size(self.ngauss, self.nelem, self.npe))
             if n == 0:
@@ -127,8 +128,11 @@ def _as_device_coords(Eulerx, device):
     return torch.as_tensor(np.ascontiguousarray(Eulerx, dtype=np.float64), device=device)
 
 
-def assemble_device(dm, material, Eulerx, V=None, T=None, requires_geometry_update=1, stream=None):
-    """Raw device assembly on DeviceMeshData: returns (V, T, status) torch tensors."""
+def assemble_device(dm, material, Eulerx, V=None, T=None, requires_geometry_update=1, stream=None,
+                    precision="fp64"):
+    """Raw device assembly on DeviceMeshData: returns (V, T, status) torch tensors.
+    ``precision`` selects the element-block contraction: "fp64" (the reference's
+    arithmetic) or "fp32" (FP32 MFMA, FP32 accumulation; T and V sums stay FP64)."""
     torch = _torch()
     lib = _lib.load()
     x = _as_device_coords(Eulerx, dm.device)
@@ -141,13 +145,15 @@ def assemble_device(dm, material, Eulerx, V=None, T=None, requires_geometry_upda
     mid = MATERIAL_IDS.get(type(material).__name__)
     if mid is None:
         raise NotImplementedError("no low-level assembler for material {}".format(type(material).__name__))
-    _lib.check(lib.fl_df_assemble(mid, prm, _lib.ptr(dm.points), _lib.ptr(dm.elements), _lib.ptr(x),
-                                  _lib.ptr(dm.Jm), _lib.ptr(dm.AllGauss), dm.ngauss, dm.nelem, dm.npe, dm.nnode,
-                                  int(requires_geometry_update), _lib.ptr(dm.node_elem_ptr), _lib.ptr(dm.node_elem),
-                                  _lib.ptr(dm.node_nbr_ptr), _lib.ptr(dm.node_nbrs), _lib.ptr(dm.node_order),
-                                  dm.bucket_offsets, dm.bucket_degree, _lib.ptr(dm.indptr), _lib.ptr(V), _lib.ptr(T), _lib.ptr(ws), ws.numel(),
-                                  _lib.ptr(dm.status), _lib.stream_ptr(stream)),
-               "fl_df_assemble")
+    prec = _lib.PRECISIONS[precision]
+    _lib.check(lib.fl_df_assemble_ex(mid, prm, _lib.ptr(dm.points), _lib.ptr(dm.elements), _lib.ptr(x),
+                                     _lib.ptr(dm.Jm), _lib.ptr(dm.AllGauss), dm.ngauss, dm.nelem, dm.npe, dm.nnode,
+                                     int(requires_geometry_update), _lib.ptr(dm.node_elem_ptr),
+                                     _lib.ptr(dm.node_elem), _lib.ptr(dm.node_nbr_ptr), _lib.ptr(dm.node_nbrs),
+                                     _lib.ptr(dm.node_order), dm.bucket_offsets, dm.bucket_degree,
+                                     _lib.ptr(dm.indptr), _lib.ptr(V), _lib.ptr(T), prec, _lib.ptr(ws), ws.numel(),
+                                     _lib.ptr(dm.status), _lib.stream_ptr(stream)),
+               "fl_df_assemble_ex")
     return V, T, dm.status
 
 

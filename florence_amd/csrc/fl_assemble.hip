@@ -203,6 +203,7 @@ __global__ __launch_bounds__(256) void qp_kernel(MatParams mp, const double *__r
 // ------------------------------------------------------------------ phase 2
 }  // namespace fl
 #include "fl_stiffness.hpp"
+#include "fl_stiffness_bf16.hpp"
 namespace fl {
 
 // ------------------------------------------------------------------ phase 3
@@ -245,13 +246,14 @@ __device__ inline int find_rank(const int32_t *s, int n, int32_t v) {
 // One wave per node.  Nodes are visited through node_order (grouped by neighbour count
 // so each launch sizes its LDS to its own degree class); lds_deg is that class's max.
 constexpr int GT = 64;
+template <typename KT>
 __global__ __launch_bounds__(GT) void gather_kernel(const int32_t *__restrict__ elements, int npe,
                                                     const int32_t *__restrict__ node_order,
                                                     const int32_t *__restrict__ node_elem_ptr,
                                                     const int32_t *__restrict__ node_elem,
                                                     const int32_t *__restrict__ nbr_ptr,
                                                     const int32_t *__restrict__ nbrs, const int32_t *__restrict__ indptr,
-                                                    const double *__restrict__ Ke, const double *__restrict__ te,
+                                                    const KT *__restrict__ Ke, const double *__restrict__ te,
                                                     int lds_deg, double *__restrict__ V, double *__restrict__ T,
                                                     int32_t *__restrict__ status) {
     extern __shared__ double smem[];
@@ -283,7 +285,7 @@ __global__ __launch_bounds__(GT) void gather_kernel(const int32_t *__restrict__ 
             rr[lb] = r < 0 ? 0 : r;
         }
         __syncthreads();
-        const double *Kb = Ke + e * 9 * npe2 + (int64_t)la * npe;
+        const KT *Kb = Ke + e * 9 * npe2 + (int64_t)la * npe;
         // batches of UB independent loads per lane keep enough bytes in flight; the LDS
         // updates follow (within one element every target entry is distinct)
         constexpr int UB = 8;
@@ -356,8 +358,9 @@ static int padded_np(int npe) {
     return -1;
 }
 
-static int launch_stiffness(int NP, const double *JmT, const double *qp, int npe, int G, int64_t nelem, double *Ke,
-                            hipStream_t st) {
+template <typename CT, typename KT>
+static int launch_stiffness_t(int NP, const double *JmT, const double *qp, int npe, int G, int64_t nelem, KT *Ke,
+                              hipStream_t st) {
     const int64_t groups = (nelem + 7) / 8 * 8 * 6;
     if (groups > (int64_t)INT32_MAX) {
         set_error("too many elements for one launch");
@@ -365,16 +368,16 @@ static int launch_stiffness(int NP, const double *JmT, const double *qp, int npe
     }
     const dim3 grid((unsigned)groups);
     switch (NP) {
-        case 16: stiffness_kernel<16, 1, 1, 1, 1><<<grid, 64, 0, st>>>(JmT, qp, npe, G, nelem, Ke); break;
-        case 32: stiffness_kernel<32, 2, 2, 1, 1><<<grid, 64, 0, st>>>(JmT, qp, npe, G, nelem, Ke); break;
-        case 48: stiffness_kernel<48, 3, 3, 1, 1><<<grid, 64, 0, st>>>(JmT, qp, npe, G, nelem, Ke); break;
-        case 64: stiffness_kernel<64, 2, 2, 2, 2><<<grid, 256, 0, st>>>(JmT, qp, npe, G, nelem, Ke); break;
+        case 16: stiffness_kernel<CT, KT, 16, 1, 1, 1, 1><<<grid, 64, 0, st>>>(JmT, qp, npe, G, nelem, Ke); break;
+        case 32: stiffness_kernel<CT, KT, 32, 2, 2, 1, 1><<<grid, 64, 0, st>>>(JmT, qp, npe, G, nelem, Ke); break;
+        case 48: stiffness_kernel<CT, KT, 48, 3, 3, 1, 1><<<grid, 64, 0, st>>>(JmT, qp, npe, G, nelem, Ke); break;
+        case 64: stiffness_kernel<CT, KT, 64, 2, 2, 2, 2><<<grid, 256, 0, st>>>(JmT, qp, npe, G, nelem, Ke); break;
         case 128:
             // default: registers capped for two waves per SIMD (measured faster on MI355X)
             if (tuning_value(TUNE_STIFF_MINW, 2) == 1)
-                stiffness_kernel<128, 4, 4, 2, 2, 1><<<grid, 256, 0, st>>>(JmT, qp, npe, G, nelem, Ke);
+                stiffness_kernel<CT, KT, 128, 4, 4, 2, 2, 1><<<grid, 256, 0, st>>>(JmT, qp, npe, G, nelem, Ke);
             else
-                stiffness_kernel<128, 4, 4, 2, 2, 2><<<grid, 256, 0, st>>>(JmT, qp, npe, G, nelem, Ke);
+                stiffness_kernel<CT, KT, 128, 4, 4, 2, 2, 2><<<grid, 256, 0, st>>>(JmT, qp, npe, G, nelem, Ke);
             break;
         default: set_error("unsupported node count"); return FL_E_ARG;
     }
@@ -382,18 +385,43 @@ static int launch_stiffness(int NP, const double *JmT, const double *qp, int npe
     return 0;
 }
 
+static int launch_stiffness_bf16(int NP, const double *JmT, const double *qp, int npe, int G, int64_t nelem,
+                                 float *Ke, hipStream_t st) {
+    const int64_t groups = (nelem + 7) / 8 * 8 * 6;
+    if (groups > (int64_t)INT32_MAX) {
+        set_error("too many elements for one launch");
+        return FL_E_LIMIT;
+    }
+    const dim3 grid((unsigned)groups);
+    switch (NP) {
+        case 16: stiffness_bf16_kernel<16, 1, 1, 1, 1><<<grid, 64, 0, st>>>(JmT, qp, npe, G, nelem, Ke); break;
+        case 32: stiffness_bf16_kernel<32, 2, 2, 1, 1><<<grid, 64, 0, st>>>(JmT, qp, npe, G, nelem, Ke); break;
+        case 48: stiffness_bf16_kernel<48, 3, 3, 1, 1><<<grid, 64, 0, st>>>(JmT, qp, npe, G, nelem, Ke); break;
+        case 64: stiffness_bf16_kernel<64, 2, 2, 2, 2><<<grid, 256, 0, st>>>(JmT, qp, npe, G, nelem, Ke); break;
+        case 128: stiffness_bf16_kernel<128, 4, 4, 2, 2><<<grid, 256, 0, st>>>(JmT, qp, npe, G, nelem, Ke); break;
+        default: set_error("unsupported node count"); return FL_E_ARG;
+    }
+    FL_LAUNCH_CHECK();
+    return 0;
+}
+
 struct DFWork {
-    double *JmT, *qp, *te, *Ke;
+    double *JmT, *qp, *te;
+    void *Ke;
 };
 
-static DFWork carve_df(void *ws, size_t bytes, int G, int64_t nelem, int npe, bool tangent, size_t *need) {
+// bytes per element-block entry for each tangent precision
+static size_t ke_bytes(int precision) { return precision == FL_PREC_FP64 ? sizeof(double) : sizeof(float); }
+
+static DFWork carve_df(void *ws, size_t bytes, int G, int64_t nelem, int npe, bool tangent, size_t *need,
+                       int precision = FL_PREC_FP64) {
     const int NP = padded_np(npe);
     Carve cv(ws, bytes);
     DFWork w;
     w.JmT = cv.take<double>((size_t)G * 3 * NP);
     w.qp = cv.take<double>((size_t)nelem * G * (tangent ? QS : QS_RES));
     w.te = cv.take<double>((size_t)nelem * 3 * npe);
-    w.Ke = tangent ? cv.take<double>((size_t)nelem * 9 * npe * npe) : nullptr;
+    w.Ke = tangent ? (void *)cv.take<char>((size_t)nelem * 9 * npe * npe * ke_bytes(precision)) : nullptr;
     *need = cv.off + 256;
     if (!cv.ok) w.JmT = nullptr;
     return w;
@@ -446,6 +474,13 @@ size_t fl_df_residual_workspace_size(int32_t ngauss, int64_t nelem, int32_t npe)
     return need;
 }
 
+size_t fl_df_workspace_size_ex(int32_t ngauss, int64_t nelem, int32_t npe, int32_t precision) {
+    if (padded_np(npe) < 0 || precision < FL_PREC_FP64 || precision > FL_PREC_BF16) return 0;
+    size_t need = 0;
+    carve_df(nullptr, 0, ngauss, nelem, npe, true, &need, precision);
+    return need;
+}
+
 int fl_df_assemble(int32_t material, const double *params, const double *points, const int32_t *elements,
                    const double *Eulerx, const double *Jm, const double *AllGauss, int32_t G, int64_t nelem,
                    int32_t npe, int64_t nnode, int32_t update, const int32_t *node_elem_ptr,
@@ -453,8 +488,24 @@ int fl_df_assemble(int32_t material, const double *params, const double *points,
                    const int32_t *node_order, const int32_t *bucket_offsets, const int32_t *bucket_degree,
                    const int32_t *indptr, double *V, double *T, void *workspace, size_t workspace_bytes,
                    int32_t *status, void *stream) {
+    return fl_df_assemble_ex(material, params, points, elements, Eulerx, Jm, AllGauss, G, nelem, npe, nnode, update,
+                             node_elem_ptr, node_elem, node_nbr_ptr, node_nbrs, node_order, bucket_offsets,
+                             bucket_degree, indptr, V, T, FL_PREC_FP64, workspace, workspace_bytes, status, stream);
+}
+
+int fl_df_assemble_ex(int32_t material, const double *params, const double *points, const int32_t *elements,
+                      const double *Eulerx, const double *Jm, const double *AllGauss, int32_t G, int64_t nelem,
+                      int32_t npe, int64_t nnode, int32_t update, const int32_t *node_elem_ptr,
+                      const int32_t *node_elem, const int32_t *node_nbr_ptr, const int32_t *node_nbrs,
+                      const int32_t *node_order, const int32_t *bucket_offsets, const int32_t *bucket_degree,
+                      const int32_t *indptr, double *V, double *T, int32_t precision, void *workspace,
+                      size_t workspace_bytes, int32_t *status, void *stream) {
     hipStream_t st = as_stream(stream);
     if (!check_common(material, params, G, nelem, npe, nnode)) return FL_E_ARG;
+    if (precision < FL_PREC_FP64 || precision > FL_PREC_BF16) {
+        set_error("fl_df_assemble_ex: unknown tangent precision %d", precision);
+        return FL_E_ARG;
+    }
     if (!points || !elements || !Eulerx || !Jm || !AllGauss || !node_elem_ptr || !node_elem || !node_nbr_ptr ||
         !node_nbrs || !node_order || !bucket_offsets || !bucket_degree || !indptr || !V || !T || !status) {
         set_error("fl_df_assemble: null pointer");
@@ -465,7 +516,7 @@ int fl_df_assemble(int32_t material, const double *params, const double *points,
         return FL_E_ARG;
     }
     size_t need = 0;
-    DFWork w = carve_df(workspace, workspace_bytes, G, nelem, npe, true, &need);
+    DFWork w = carve_df(workspace, workspace_bytes, G, nelem, npe, true, &need, precision);
     if (!w.JmT) {
         set_error("fl_df_assemble: workspace too small (%zu < %zu)", workspace_bytes, need);
         return FL_E_SPACE;
@@ -487,7 +538,11 @@ int fl_df_assemble(int32_t material, const double *params, const double *points,
                                                                    npe, update, w.qp, status);
     FL_LAUNCH_CHECK();
     prof_mark(st);
-    int rc = launch_stiffness(NP, w.JmT, w.qp, npe, G, nelem, w.Ke, st);
+    int rc = precision == FL_PREC_FP64
+                 ? launch_stiffness_t<double, double>(NP, w.JmT, w.qp, npe, G, nelem, (double *)w.Ke, st)
+                 : precision == FL_PREC_FP32
+                       ? launch_stiffness_t<float, float>(NP, w.JmT, w.qp, npe, G, nelem, (float *)w.Ke, st)
+                       : launch_stiffness_bf16(NP, w.JmT, w.qp, npe, G, nelem, (float *)w.Ke, st);
     if (rc) return rc;
     prof_mark(st);
     const size_t lds_tr = (size_t)G * 9 * 8;
@@ -501,12 +556,21 @@ int fl_df_assemble(int32_t material, const double *params, const double *points,
         const int32_t nb = bucket_offsets[b + 1] - bucket_offsets[b];
         if (nb <= 0) continue;
         const size_t lds = gather_lds_bytes(bucket_degree[b], npe);
-        if (lds > 64 * 1024)
-            FL_HIP_CHECK(hipFuncSetAttribute((const void *)gather_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                             (int)lds));
-        gather_kernel<<<(unsigned)nb, GT, lds, st>>>(elements, npe, node_order + bucket_offsets[b], node_elem_ptr,
-                                                     node_elem, node_nbr_ptr, node_nbrs, indptr, w.Ke, w.te,
-                                                     bucket_degree[b], V, T, status);
+        if (precision == FL_PREC_FP64) {
+            if (lds > 64 * 1024)
+                FL_HIP_CHECK(hipFuncSetAttribute((const void *)gather_kernel<double>,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            gather_kernel<double><<<(unsigned)nb, GT, lds, st>>>(
+                elements, npe, node_order + bucket_offsets[b], node_elem_ptr, node_elem, node_nbr_ptr, node_nbrs,
+                indptr, (const double *)w.Ke, w.te, bucket_degree[b], V, T, status);
+        } else {
+            if (lds > 64 * 1024)
+                FL_HIP_CHECK(hipFuncSetAttribute((const void *)gather_kernel<float>,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            gather_kernel<float><<<(unsigned)nb, GT, lds, st>>>(
+                elements, npe, node_order + bucket_offsets[b], node_elem_ptr, node_elem, node_nbr_ptr, node_nbrs,
+                indptr, (const float *)w.Ke, w.te, bucket_degree[b], V, T, status);
+        }
         FL_LAUNCH_CHECK();
     }
     prof_mark(st);

@@ -45,7 +45,12 @@ __global__ __launch_bounds__(64 * SP_W) void spmv_kernel(const int32_t *__restri
         double s0 = 0.0, s1 = 0.0, s2 = 0.0;
         for (int r = lane; r < deg; r += 64) {
             const int64_t b = nbrs[p0 + r];
-            const double x0 = x[3 * b], x1 = x[3 * b + 1], x2 = x[3 * b + 2];
+            double x0 = x[3 * b], x1 = x[3 * b + 1], x2 = x[3 * b + 2];
+            if (free_mask) {
+                x0 = free_mask[3 * b] ? x0 : 0.0;
+                x1 = free_mask[3 * b + 1] ? x1 : 0.0;
+                x2 = free_mask[3 * b + 2] ? x2 : 0.0;
+            }
             s0 += r0[3 * r] * x0 + r0[3 * r + 1] * x1 + r0[3 * r + 2] * x2;
             s1 += r1[3 * r] * x0 + r1[3 * r + 1] * x1 + r1[3 * r + 2] * x2;
             s2 += r2[3 * r] * x0 + r2[3 * r + 1] * x1 + r2[3 * r + 2] * x2;

@@ -1,4 +1,4 @@
-// Phase 2 of the DF assembly: element stiffness blocks on FP64 MFMA (included by
+// Phase 2 of the DF assembly: element stiffness blocks on MFMA (included by
 // fl_assemble.hip).
 //
 // One workgroup per (element, displacement block pair i <= k).  With G_{a,(g,j)} =
@@ -11,31 +11,62 @@
 //
 // a dense npe x npe x 3G product.  The inner index runs in chunks of 8 points (24
 // indices).  Per chunk the workgroup builds G for all nodes in LDS (from the padded
-// parent gradients JmT and the per-point inverse Jacobians), and each lane forms its
-// W operand on the fly from three G values and one row of D while the MFMAs
-// (v_mfma_f64_16x16x4f64) run.  LDS is double buffered and the next chunk's global
-// loads are issued before the current chunk's MFMAs, so one barrier per chunk remains.
+// parent gradients JmT and the per-point inverse Jacobians, formed in FP64), and each
+// lane forms its W operand on the fly from three G values and one row of D while the
+// MFMAs run.  LDS is double buffered and the next chunk's global loads are issued
+// before the current chunk's MFMAs, so one barrier per chunk remains.
+//
+// CT is the contraction type: double -> v_mfma_f64_16x16x4f64 (the FP64 path);
+// float -> v_mfma_f32_16x16x4f32 (the FP32 tangent of the mixed-precision form, FP32
+// accumulation).  KT is the storage type of the element blocks.
 #pragma once
 
 namespace fl {
 
-template <int NP, int MT, int NT, int WGM, int WGN, int MINW = 1>
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+template <typename CT>
+struct Mfma16;
+
+template <>
+struct Mfma16<double> {
+    typedef double4_t acc_t;
+    __device__ static acc_t step(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    // output layout of the f64 16x16 MFMA: lane holds rows lk + 4r of column li
+    __device__ static int row(int lk, int r) { return lk + 4 * r; }
+};
+
+template <>
+struct Mfma16<float> {
+    typedef float4_t acc_t;
+    __device__ static acc_t step(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    // output layout of the f32 16x16 MFMA: lane holds rows 4 lk + r of column li
+    __device__ static int row(int lk, int r) { return 4 * lk + r; }
+};
+
+template <typename CT, typename KT, int NP, int MT, int NT, int WGM, int WGN, int MINW = 1>
 __global__ __launch_bounds__(64 * WGM *WGN, MINW) void stiffness_kernel(const double *__restrict__ JmT,
                                                                    const double *__restrict__ qp, int npe, int G,
-                                                                   int64_t nelem, double *__restrict__ Ke) {
+                                                                   int64_t nelem, KT *__restrict__ Ke) {
     static_assert(NP == 16 * MT * WGM && NP == 16 * NT * WGN, "tile shape must cover NP");
+    typedef Mfma16<CT> MF;
+    typedef typename MF::acc_t acc_t;
     constexpr int NTH = 64 * WGM * WGN;
     constexpr int QC = 8;          // quadrature points per chunk
     constexpr int KC = 3 * QC;     // inner (g, j) indices per chunk
-    constexpr int LDA = NP + 4;    // LDS row stride (doubles)
+    constexpr int LDA = NP + 4;    // LDS row stride (elements)
     constexpr int TPG = NTH / QC;  // fill threads per point
     constexpr int IPT = NP / TPG;  // nodes filled per thread
     static_assert(NTH % QC == 0 && NP % TPG == 0, "fill mapping");
     constexpr int PDN = (QC * 9 + NTH - 1) / NTH;   // D values staged per thread
-    constexpr int TT = 16 * NT * 17;   // per-wave transpose tile (doubles)
+    constexpr int TT = 16 * NT * 17;                // per-wave transpose tile (elements)
     static_assert(WGM * WGN * TT <= 2 * KC * LDA, "transpose tile must fit the staging buffers");
-    __shared__ double sA[2 * KC * LDA];
-    __shared__ double sD[2 * QC * 9];
+    __shared__ CT sA[2 * KC * LDA];
+    __shared__ CT sD[2 * QC * 9];
 
     // XCD-aware order: workgroup ids are dealt round-robin over the 8 XCDs; the six
     // block pairs of one element land on the same XCD and share its L2.
@@ -82,24 +113,24 @@ __global__ __launch_bounds__(64 * WGM *WGN, MINW) void stiffness_kernel(const do
         }
     };
     auto commit = [&](int buf) {
-        double *A = sA + buf * KC * LDA;
+        CT *A = sA + buf * KC * LDA;
 #pragma unroll
         for (int s = 0; s < IPT; ++s) {
             const int a = fa0 + TPG * s;
-            A[(3 * fg + 0) * LDA + a] = pix[0] * pj[s][0] + pix[1] * pj[s][1] + pix[2] * pj[s][2];
-            A[(3 * fg + 1) * LDA + a] = pix[3] * pj[s][0] + pix[4] * pj[s][1] + pix[5] * pj[s][2];
-            A[(3 * fg + 2) * LDA + a] = pix[6] * pj[s][0] + pix[7] * pj[s][1] + pix[8] * pj[s][2];
+            A[(3 * fg + 0) * LDA + a] = (CT)(pix[0] * pj[s][0] + pix[1] * pj[s][1] + pix[2] * pj[s][2]);
+            A[(3 * fg + 1) * LDA + a] = (CT)(pix[3] * pj[s][0] + pix[4] * pj[s][1] + pix[5] * pj[s][2]);
+            A[(3 * fg + 2) * LDA + a] = (CT)(pix[6] * pj[s][0] + pix[7] * pj[s][1] + pix[8] * pj[s][2]);
         }
 #pragma unroll
         for (int s = 0; s < PDN; ++s)
-            if (tid + s * NTH < QC * 9) sD[buf * QC * 9 + tid + s * NTH] = pd[s];
+            if (tid + s * NTH < QC * 9) sD[buf * QC * 9 + tid + s * NTH] = (CT)pd[s];
     };
 
-    double4_t acc[MT][NT];
+    acc_t acc[MT][NT];
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = double4_t{0.0, 0.0, 0.0, 0.0};
+        for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = acc_t{0, 0, 0, 0};
 
     prefetch(0);
     commit(0);
@@ -108,17 +139,17 @@ __global__ __launch_bounds__(64 * WGM *WGN, MINW) void stiffness_kernel(const do
     for (int g0 = 0; g0 < G; g0 += QC) {
         const bool more = g0 + QC < G;
         if (more) prefetch(g0 + QC);
-        const double *A = sA + buf * KC * LDA;
-        const double *Dc = sD + buf * QC * 9;
+        const CT *A = sA + buf * KC * LDA;
+        const CT *Dc = sD + buf * QC * 9;
         // not unrolled: each step already issues MT*NT long MFMAs, and hoisting the LDS
         // operands of several steps would cost the registers of a second wave per SIMD
 #pragma unroll 1
         for (int kk = 0; kk < KC / 4; ++kk) {
             const int q = kk * 4 + lk;
             const int gq = q / 3, j = q - 3 * gq;
-            const double d0 = Dc[gq * 9 + 3 * j], d1 = Dc[gq * 9 + 3 * j + 1], d2 = Dc[gq * 9 + 3 * j + 2];
-            const double *Gb = A + 3 * gq * LDA;
-            double af[MT], bf[NT];
+            const CT d0 = Dc[gq * 9 + 3 * j], d1 = Dc[gq * 9 + 3 * j + 1], d2 = Dc[gq * 9 + 3 * j + 2];
+            const CT *Gb = A + 3 * gq * LDA;
+            CT af[MT], bf[NT];
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) af[mt] = A[q * LDA + (wm * MT + mt) * 16 + li];
 #pragma unroll
@@ -129,42 +160,41 @@ __global__ __launch_bounds__(64 * WGM *WGN, MINW) void stiffness_kernel(const do
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-                for (int nt = 0; nt < NT; ++nt)
-                    acc[mt][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[mt], bf[nt], acc[mt][nt], 0, 0, 0);
+                for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = MF::step(af[mt], bf[nt], acc[mt][nt]);
         }
         if (more) commit(buf ^ 1);
         __syncthreads();
         buf ^= 1;
     }
 
-    // epilogue: K^{ik} rows a, cols b.  MFMA f64 16x16 layout: lane holds rows lk+4r, col li.
+    // epilogue: K^{ik} rows a, cols b.
     const int64_t npe2 = (int64_t)npe * npe;
-    double *Kd = Ke + (e * 9 + pi * 3 + pk) * npe2;
+    KT *Kd = Ke + (e * 9 + pi * 3 + pk) * npe2;
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int a = (wm * MT + mt) * 16 + lk + 4 * r;
+                const int a = (wm * MT + mt) * 16 + MF::row(lk, r);
                 const int b = (wn * NT + nt) * 16 + li;
-                if (a < npe && b < npe) Kd[(int64_t)a * npe + b] = acc[mt][nt][r];
+                if (a < npe && b < npe) Kd[(int64_t)a * npe + b] = (KT)acc[mt][nt][r];
             }
     if (pi != pk) {
         // mirror block K^{ki}_{ba} through a per-wave LDS transpose (coalesced stores)
-        double *Kt = Ke + (e * 9 + pk * 3 + pi) * npe2;
-        double *tile = sA + wave * TT;
+        KT *Kt = Ke + (e * 9 + pk * 3 + pi) * npe2;
+        CT *tile = sA + wave * TT;
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) {
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) tile[(nt * 16 + li) * 17 + lk + 4 * r] = acc[mt][nt][r];
+                for (int r = 0; r < 4; ++r) tile[(nt * 16 + li) * 17 + MF::row(lk, r)] = acc[mt][nt][r];
             __syncthreads();
             for (int idx = lane; idx < 16 * NT * 16; idx += 64) {
                 const int bl = idx >> 4, al = idx & 15;
                 const int b = wn * NT * 16 + bl, a = (wm * MT + mt) * 16 + al;
-                if (a < npe && b < npe) Kt[(int64_t)b * npe + a] = tile[bl * 17 + al];
+                if (a < npe && b < npe) Kt[(int64_t)b * npe + a] = (KT)tile[bl * 17 + al];
             }
             __syncthreads();
         }

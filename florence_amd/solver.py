@@ -90,7 +90,8 @@ class FEMSolver(object):
                  analysis_nature="nonlinear", requires_geometry_update=True, number_of_load_increments=1,
                  load_factor=None, newton_raphson_tolerance=1.0e-6, newton_raphson_solution_tolerance=None,
                  maximum_iteration_for_newton_raphson=50, break_at_stagnation=True, user_defined_break_func=None,
-                 user_defined_stop_func=None, report_log_level=0, linear_solver=None, warm_start=None, **kwargs):
+                 user_defined_stop_func=None, report_log_level=0, linear_solver=None, warm_start=None,
+                 tangent_precision="fp64", **kwargs):
         if has_low_level_dispatcher != optimise:
             has_low_level_dispatcher = True
         self.has_low_level_dispatcher = has_low_level_dispatcher
@@ -112,6 +113,9 @@ class FEMSolver(object):
         self.report_log_level = report_log_level
         self.linear_solver = linear_solver if linear_solver is not None else LinearSolver()
         self.warm_start = warm_start
+        # "fp64": the reference's arithmetic; "fp32": inexact-Jacobian Newton with the
+        # tangent contracted on FP32 MFMA (kinematics, residual and update stay FP64)
+        self.tangent_precision = tangent_precision
         self.assembly_time = 0.0
         self.newton_raphson_failed_to_converge = False
         for k, v in kwargs.items():
@@ -145,13 +149,14 @@ class FEMSolver(object):
         iters = []
         ta = 0.0
 
-        def assemble():
+        def assemble(x):
             nonlocal ta
             t = time.perf_counter()
-            assemble_device(dm, material, Eulerx, V, T, int(self.requires_geometry_update))
+            assemble_device(dm, material, x, V, T, int(self.requires_geometry_update),
+                            precision=self.tangent_precision)
             ta += time.perf_counter() - t
 
-        assemble()   # first assembly at the undeformed state (FEMSolver.py:641-642)
+        assemble(Eulerx)   # first assembly at the undeformed state (FEMSolver.py:641-642)
         Residual = torch.zeros(ndof, dtype=torch.float64, device=dev)
         dUD = torch.zeros(ndof, dtype=torch.float64, device=dev)
         KuD = torch.empty(ndof, dtype=torch.float64, device=dev)
@@ -201,7 +206,7 @@ class FEMSolver(object):
             self.linear_solver.Solve(dm, V, free, rhs, dU)
             Eulerx = Eulerx + dU
             self._Eulerx = Eulerx
-            assemble()
+            assemble(Eulerx)
             Residual = (T - NodalForces) * free_f
             self.abs_norm_residual = float(torch.linalg.norm(Residual))
             if Iter == 0:

@@ -101,6 +101,25 @@ int fl_df_assemble(int32_t material, const double *params, const double *points_
                    double *V_dev, double *T_dev, void *workspace_dev, size_t workspace_bytes,
                    int32_t *status_dev, void *stream);
 
+/* Mixed-precision form of fl_df_assemble (no reference counterpart: the reference
+ * is FP64 only).  precision = FL_PREC_FP64 is fl_df_assemble; FL_PREC_FP32 runs the
+ * element-block contraction on FP32 MFMA with FP32 accumulation and keeps the element
+ * blocks in FP32.  Kinematics, det F, the material law, the residual T and the CSR
+ * sums (V) stay FP64.  workspace: fl_df_workspace_size_ex() bytes. */
+#define FL_PREC_FP64 0
+#define FL_PREC_FP32 1
+#define FL_PREC_BF16 2   /* G and W rounded to BF16, v_mfma_f32_16x16x32_bf16, FP32 accumulation */
+size_t fl_df_workspace_size_ex(int32_t ngauss, int64_t nelem, int32_t npe, int32_t precision);
+int fl_df_assemble_ex(int32_t material, const double *params, const double *points_dev,
+                      const int32_t *elements_dev, const double *Eulerx_dev, const double *Jm_dev,
+                      const double *AllGauss_dev, int32_t ngauss, int64_t nelem, int32_t npe, int64_t nnode,
+                      int32_t requires_geometry_update, const int32_t *node_elem_ptr_dev,
+                      const int32_t *node_elem_dev, const int32_t *node_nbr_ptr_dev,
+                      const int32_t *node_nbrs_dev, const int32_t *node_order_dev,
+                      const int32_t *bucket_offsets, const int32_t *bucket_degree, const int32_t *indptr_dev,
+                      double *V_dev, double *T_dev, int32_t precision, void *workspace_dev,
+                      size_t workspace_bytes, int32_t *status_dev, void *stream);
+
 /* --------------------------------------------- residual-only (explicit) pass
  * Replaces the mechanics branch of _GlobalAssemblyExplicit_DF_DPF_
  * (FiniteElements/Assembly/_Assembly_/_LowLevelAssemblyExplicit_DF_DPF_.h:459-726):

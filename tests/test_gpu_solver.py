@@ -32,13 +32,17 @@ def test_spmv_matches_scipy(cuda):
     y = torch.empty_like(x)
     spmv(dm, V, x, y)
     ref = K @ xv
-    assert np.abs(y.cpu().numpy() - ref).max() <= 1e-13 * np.abs(ref).max()
+    bound = 1e-14 * (abs(K) @ np.abs(xv)) + 1e-300   # summation-order bound per row
+    err = np.abs(y.cpu().numpy() - ref)
+    print("spmv max err {:.3e} max bound {:.3e}".format(err.max(), bound.max()), flush=True)
+    assert np.all(err <= 50 * bound)
     free = np.ones(K.shape[0], dtype=np.uint8)
     free[::7] = 0
     spmv(dm, V, x, y, torch.as_tensor(free, device=cuda))
     M = sp.diags(free.astype(float))
     refm = M @ K @ M @ xv + (1 - free) * xv
-    assert np.abs(y.cpu().numpy() - refm).max() <= 1e-13 * np.abs(refm).max()
+    bound = 1e-14 * (abs(K) @ np.abs(xv)) + 1e-300
+    assert np.all(np.abs(y.cpu().numpy() - refm) <= 50 * bound + 1e-15 * np.abs(xv))
 
 
 def test_pcg_solves_reduced_system(cuda):
