@@ -16,6 +16,8 @@ FL_S_NONFINITE = 2
 FL_S_PATTERN = 4
 FL_GATHER_BUCKETS = 5
 FL_TUNE_STIFF_MINW = 1
+FL_TUNE_STIFF_VARIANT_BASE = 2
+STIFF_VARIANTS = {0: 1, 1: 2, 2: 1, 3: 2, 4: 2}   # padded node-count class -> number of tile variants
 FL_PREC_FP64 = 0
 FL_PREC_FP32 = 1
 FL_PREC_BF16 = 2
@@ -39,6 +41,9 @@ SIGNATURES = {
                                       _vp, _vp, _vp, _vp, _vp, _P32, _P32, _vp, _vp, _vp, _vp, _sz, _vp, _vp]),
     "fl_set_tuning": (ctypes.c_int, [_i32, _i32]),
     "fl_df_workspace_size_ex": (_sz, [_i32, _i64, _i32, _i32]),
+    "fl_df_assemble_mt": (ctypes.c_int, [_i32, _PD, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i64, _i32, _i64,
+                                         _i32, _vp, _vp, _vp, _vp, _vp, _P32, _P32, _vp, _vp, _vp, _i32, _vp, _sz,
+                                         _vp, _vp]),
     "fl_df_assemble_ex": (ctypes.c_int, [_i32, _PD, _vp, _vp, _vp, _vp, _vp, _i32, _i64, _i32, _i64, _i32,
                                          _vp, _vp, _vp, _vp, _vp, _P32, _P32, _vp, _vp, _vp, _i32, _vp, _sz, _vp,
                                          _vp]),

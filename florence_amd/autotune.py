@@ -1,0 +1,67 @@
+"""Autotuner of the stiffness kernel's tile shape per (element type, p, points, precision).
+
+For every group of a BatchedAssembly it times each tile-shape variant the library
+offers for the group's padded node count (FL_TUNE_STIFF_VARIANT_BASE keys), using the
+library's own per-phase HIP events, and keeps the fastest.  Results are cached as
+JSON keyed "tet-p2-G8-fp64" -> {"variant": v, "stiffness_ms": [...]}; the cache file is
+$FLORENCE_AMD_TUNING_CACHE or ~/.cache/florence_amd/tuning.json.
+"""
+import json
+import os
+
+import numpy as np
+
+from . import _lib
+
+__all__ = ["cache_path", "load_cache", "save_cache", "tune_batch"]
+
+
+def cache_path():
+    return os.environ.get("FLORENCE_AMD_TUNING_CACHE",
+                          os.path.join(os.path.expanduser("~"), ".cache", "florence_amd", "tuning.json"))
+
+
+def load_cache(path=None):
+    path = path or cache_path()
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return json.load(f)
+
+
+def save_cache(cache, path=None):
+    path = path or cache_path()
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    with open(path, "w") as f:
+        json.dump(cache, f, indent=1, sort_keys=True)
+
+
+def tune_batch(batched, repeats=3, cache=None, force=False):
+    """Pick each group's variant (mutates batched.groups[*]['variant']); returns the cache."""
+    import torch
+    from .batch import _np_class
+    cache = dict(cache or {})
+    for g in batched.groups:
+        key = batched.tuning_key(g["key"], g["fs"])
+        npc = _np_class(g["dm"].npe)
+        nvar = _lib.STIFF_VARIANTS[npc] if batched.precision != "bf16" else 1
+        if key in cache and not force:
+            g["variant"] = int(cache[key]["variant"])
+            continue
+        times = []
+        for v in range(nvar):
+            g["variant"] = v
+            batched._launch_group(g)            # warm-up (first-touch, kernel attributes)
+            torch.cuda.synchronize()
+            _lib.profile_enable(True)
+            for _ in range(repeats):
+                batched._launch_group(g)
+            torch.cuda.synchronize()
+            calls = _lib.profile_read()
+            _lib.profile_enable(False)
+            st = [c[1][1] for c in calls if c[0] == 1]
+            times.append(float(np.median(st)))
+        best = int(np.argmin(times))
+        g["variant"] = best
+        cache[key] = {"variant": best, "stiffness_ms": [round(t, 4) for t in times], "nelem": g["dm"].nelem}
+    return cache

@@ -68,6 +68,15 @@ __device__ inline double invdet3x3(const double *s, double *d) {
     return det;
 }
 
+__global__ void clear_status_kernel(int32_t *status) {
+    if (threadIdx.x < 2) status[threadIdx.x] = 0;
+}
+
+__global__ void zero_kernel(double *p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
 // Jm^T padded: JmT[(g*3+m)*NP + a] = dN_a/dxi_m at point g (0 for a >= npe)
 __global__ void transpose_jm_kernel(const double *__restrict__ Jm, int npe, int G, int NP, double *__restrict__ JmT) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -77,16 +86,27 @@ __global__ void transpose_jm_kernel(const double *__restrict__ Jm, int npe, int 
 }
 
 // ------------------------------------------------------------------ phase 1
+// elem_mat/mat_table (optional): per-element material parameters {mu, mu1, mu2, lamb}
 template <bool TANGENT>
-__global__ __launch_bounds__(256) void qp_kernel(MatParams mp, const double *__restrict__ points,
+__global__ __launch_bounds__(256) void qp_kernel(MatParams mp0, const double *__restrict__ points,
                                                  const int32_t *__restrict__ elements,
                                                  const double *__restrict__ Eulerx, const double *__restrict__ Jm,
                                                  const double *__restrict__ w, int G, int64_t nelem, int npe,
-                                                 int update, double *__restrict__ qp, int32_t *__restrict__ status) {
+                                                 int update, double *__restrict__ qp, int32_t *__restrict__ status,
+                                                 const int32_t *__restrict__ elem_mat,
+                                                 const double *__restrict__ mat_table) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nelem * G) return;
     const int64_t e = t / G;
     const int g = (int)(t - e * G);
+    MatParams mp = mp0;
+    if (elem_mat) {
+        const double *row = mat_table + 4 * (int64_t)elem_mat[e];
+        mp.mu = row[0];
+        mp.mu1 = row[1];
+        mp.mu2 = row[2];
+        mp.lamb = row[3];
+    }
     double PX[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, Px[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     const int32_t *conn = elements + e * npe;
     for (int a = 0; a < npe; ++a) {
@@ -245,8 +265,9 @@ __device__ inline int find_rank(const int32_t *s, int n, int32_t v) {
 
 // One wave per node.  Nodes are visited through node_order (grouped by neighbour count
 // so each launch sizes its LDS to its own degree class); lds_deg is that class's max.
-constexpr int GT = 64;
-template <typename KT>
+// GT = 64 (one wave) for nodes inside one element; GT = 256 for nodes shared by several
+// elements, whose per-element loop is the latency chain (rank search, loads, LDS adds).
+template <typename KT, int GT>
 __global__ __launch_bounds__(GT) void gather_kernel(const int32_t *__restrict__ elements, int npe,
                                                     const int32_t *__restrict__ node_order,
                                                     const int32_t *__restrict__ node_elem_ptr,
@@ -367,14 +388,26 @@ static int launch_stiffness_t(int NP, const double *JmT, const double *qp, int n
         return FL_E_LIMIT;
     }
     const dim3 grid((unsigned)groups);
+    // tile-shape variants per padded node count (fl_set_tuning key
+    // FL_TUNE_STIFF_VARIANT_BASE + class, value = variant + 1; 0 = default variant 0)
     switch (NP) {
         case 16: stiffness_kernel<CT, KT, 16, 1, 1, 1, 1><<<grid, 64, 0, st>>>(JmT, qp, npe, G, nelem, Ke); break;
-        case 32: stiffness_kernel<CT, KT, 32, 2, 2, 1, 1><<<grid, 64, 0, st>>>(JmT, qp, npe, G, nelem, Ke); break;
+        case 32:
+            if (tuning_value(TUNE_STIFF_VARIANT_BASE + 1, 1) == 2)
+                stiffness_kernel<CT, KT, 32, 1, 1, 2, 2><<<grid, 256, 0, st>>>(JmT, qp, npe, G, nelem, Ke);
+            else
+                stiffness_kernel<CT, KT, 32, 2, 2, 1, 1><<<grid, 64, 0, st>>>(JmT, qp, npe, G, nelem, Ke);
+            break;
         case 48: stiffness_kernel<CT, KT, 48, 3, 3, 1, 1><<<grid, 64, 0, st>>>(JmT, qp, npe, G, nelem, Ke); break;
-        case 64: stiffness_kernel<CT, KT, 64, 2, 2, 2, 2><<<grid, 256, 0, st>>>(JmT, qp, npe, G, nelem, Ke); break;
+        case 64:
+            if (tuning_value(TUNE_STIFF_VARIANT_BASE + 3, 1) == 2)
+                stiffness_kernel<CT, KT, 64, 4, 4, 1, 1><<<grid, 64, 0, st>>>(JmT, qp, npe, G, nelem, Ke);
+            else
+                stiffness_kernel<CT, KT, 64, 2, 2, 2, 2><<<grid, 256, 0, st>>>(JmT, qp, npe, G, nelem, Ke);
+            break;
         case 128:
             // default: registers capped for two waves per SIMD (measured faster on MI355X)
-            if (tuning_value(TUNE_STIFF_MINW, 2) == 1)
+            if (tuning_value(TUNE_STIFF_VARIANT_BASE + 4, 1) == 2 || tuning_value(TUNE_STIFF_MINW, 2) == 1)
                 stiffness_kernel<CT, KT, 128, 4, 4, 2, 2, 1><<<grid, 256, 0, st>>>(JmT, qp, npe, G, nelem, Ke);
             else
                 stiffness_kernel<CT, KT, 128, 4, 4, 2, 2, 2><<<grid, 256, 0, st>>>(JmT, qp, npe, G, nelem, Ke);
@@ -401,6 +434,20 @@ static int launch_stiffness_bf16(int NP, const double *JmT, const double *qp, in
         case 128: stiffness_bf16_kernel<128, 4, 4, 2, 2><<<grid, 256, 0, st>>>(JmT, qp, npe, G, nelem, Ke); break;
         default: set_error("unsupported node count"); return FL_E_ARG;
     }
+    FL_LAUNCH_CHECK();
+    return 0;
+}
+
+template <typename KT, int GTB>
+static int launch_gather(int32_t nb, size_t lds, const int32_t *elements, int npe, const int32_t *order,
+                         const int32_t *node_elem_ptr, const int32_t *node_elem, const int32_t *nbr_ptr,
+                         const int32_t *nbrs, const int32_t *indptr, const KT *Ke, const double *te, int deg,
+                         double *V, double *T, int32_t *status, hipStream_t st) {
+    if (lds > 64 * 1024)
+        FL_HIP_CHECK(hipFuncSetAttribute((const void *)gather_kernel<KT, GTB>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    gather_kernel<KT, GTB><<<(unsigned)nb, GTB, lds, st>>>(elements, npe, order, node_elem_ptr, node_elem, nbr_ptr,
+                                                           nbrs, indptr, Ke, te, deg, V, T, status);
     FL_LAUNCH_CHECK();
     return 0;
 }
@@ -500,8 +547,25 @@ int fl_df_assemble_ex(int32_t material, const double *params, const double *poin
                       const int32_t *node_order, const int32_t *bucket_offsets, const int32_t *bucket_degree,
                       const int32_t *indptr, double *V, double *T, int32_t precision, void *workspace,
                       size_t workspace_bytes, int32_t *status, void *stream) {
+    return fl_df_assemble_mt(material, params, nullptr, nullptr, points, elements, Eulerx, Jm, AllGauss, G, nelem,
+                             npe, nnode, update, node_elem_ptr, node_elem, node_nbr_ptr, node_nbrs, node_order,
+                             bucket_offsets, bucket_degree, indptr, V, T, precision, workspace, workspace_bytes,
+                             status, stream);
+}
+
+int fl_df_assemble_mt(int32_t material, const double *params, const int32_t *elem_mat, const double *mat_table,
+                      const double *points, const int32_t *elements, const double *Eulerx, const double *Jm,
+                      const double *AllGauss, int32_t G, int64_t nelem, int32_t npe, int64_t nnode, int32_t update,
+                      const int32_t *node_elem_ptr, const int32_t *node_elem, const int32_t *node_nbr_ptr,
+                      const int32_t *node_nbrs, const int32_t *node_order, const int32_t *bucket_offsets,
+                      const int32_t *bucket_degree, const int32_t *indptr, double *V, double *T, int32_t precision,
+                      void *workspace, size_t workspace_bytes, int32_t *status, void *stream) {
     hipStream_t st = as_stream(stream);
     if (!check_common(material, params, G, nelem, npe, nnode)) return FL_E_ARG;
+    if ((elem_mat == nullptr) != (mat_table == nullptr)) {
+        set_error("fl_df_assemble_mt: elem_material and material_table go together");
+        return FL_E_ARG;
+    }
     if (precision < FL_PREC_FP64 || precision > FL_PREC_BF16) {
         set_error("fl_df_assemble_ex: unknown tangent precision %d", precision);
         return FL_E_ARG;
@@ -530,12 +594,13 @@ int fl_df_assemble_ex(int32_t material, const double *params, const double *poin
     const int NP = padded_np(npe);
     const MatParams mp = mat_params(material, params);
     prof_begin(st, PROF_DF_ASSEMBLE);
-    FL_HIP_CHECK(hipMemsetAsync(status, 0, 2 * sizeof(int32_t), st));
+    // status cleared by a kernel, not a memset node, so the call can be graph-captured
+    clear_status_kernel<<<1, 64, 0, st>>>(status);
     transpose_jm_kernel<<<(G * 3 * NP + 255) / 256, 256, 0, st>>>(Jm, npe, G, NP, w.JmT);
     FL_LAUNCH_CHECK();
     const int64_t nq = nelem * G;
     qp_kernel<true><<<(unsigned)((nq + 255) / 256), 256, 0, st>>>(mp, points, elements, Eulerx, Jm, AllGauss, G, nelem,
-                                                                   npe, update, w.qp, status);
+                                                                   npe, update, w.qp, status, elem_mat, mat_table);
     FL_LAUNCH_CHECK();
     prof_mark(st);
     int rc = precision == FL_PREC_FP64
@@ -550,28 +615,34 @@ int fl_df_assemble_ex(int32_t material, const double *params, const double *poin
                                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_tr));
     traction_kernel<<<(unsigned)nelem, 256, lds_tr, st>>>(w.JmT, NP, w.qp, QS, Q_S, npe, G, w.te);
     FL_LAUNCH_CHECK();
-    if (update != 1) FL_HIP_CHECK(hipMemsetAsync(w.te, 0, sizeof(double) * nelem * 3 * npe, st));
+    if (update != 1) {
+        const int64_t nte = nelem * 3 * npe;
+        zero_kernel<<<(unsigned)((nte + 255) / 256), 256, 0, st>>>(w.te, nte);
+        FL_LAUNCH_CHECK();
+    }
     prof_mark(st);
     for (int b = 0; b < FL_GATHER_BUCKETS; ++b) {
         const int32_t nb = bucket_offsets[b + 1] - bucket_offsets[b];
         if (nb <= 0) continue;
         const size_t lds = gather_lds_bytes(bucket_degree[b], npe);
-        if (precision == FL_PREC_FP64) {
-            if (lds > 64 * 1024)
-                FL_HIP_CHECK(hipFuncSetAttribute((const void *)gather_kernel<double>,
-                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            gather_kernel<double><<<(unsigned)nb, GT, lds, st>>>(
-                elements, npe, node_order + bucket_offsets[b], node_elem_ptr, node_elem, node_nbr_ptr, node_nbrs,
-                indptr, (const double *)w.Ke, w.te, bucket_degree[b], V, T, status);
-        } else {
-            if (lds > 64 * 1024)
-                FL_HIP_CHECK(hipFuncSetAttribute((const void *)gather_kernel<float>,
-                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            gather_kernel<float><<<(unsigned)nb, GT, lds, st>>>(
-                elements, npe, node_order + bucket_offsets[b], node_elem_ptr, node_elem, node_nbr_ptr, node_nbrs,
-                indptr, (const float *)w.Ke, w.te, bucket_degree[b], V, T, status);
-        }
-        FL_LAUNCH_CHECK();
+        // the lowest class (nodes interior to one element) keeps one wave per node
+        const bool wide = b > 0;
+        int rc2 = 0;
+        if (precision == FL_PREC_FP64)
+            rc2 = wide ? launch_gather<double, 256>(nb, lds, elements, npe, node_order + bucket_offsets[b],
+                                                    node_elem_ptr, node_elem, node_nbr_ptr, node_nbrs, indptr,
+                                                    (const double *)w.Ke, w.te, bucket_degree[b], V, T, status, st)
+                       : launch_gather<double, 64>(nb, lds, elements, npe, node_order + bucket_offsets[b],
+                                                   node_elem_ptr, node_elem, node_nbr_ptr, node_nbrs, indptr,
+                                                   (const double *)w.Ke, w.te, bucket_degree[b], V, T, status, st);
+        else
+            rc2 = wide ? launch_gather<float, 256>(nb, lds, elements, npe, node_order + bucket_offsets[b],
+                                                   node_elem_ptr, node_elem, node_nbr_ptr, node_nbrs, indptr,
+                                                   (const float *)w.Ke, w.te, bucket_degree[b], V, T, status, st)
+                       : launch_gather<float, 64>(nb, lds, elements, npe, node_order + bucket_offsets[b],
+                                                  node_elem_ptr, node_elem, node_nbr_ptr, node_nbrs, indptr,
+                                                  (const float *)w.Ke, w.te, bucket_degree[b], V, T, status, st);
+        if (rc2) return rc2;
     }
     prof_mark(st);
     return 0;
@@ -596,12 +667,12 @@ int fl_df_residual(int32_t material, const double *params, const double *points,
     const int NP = padded_np(npe);
     const MatParams mp = mat_params(material, params);
     prof_begin(st, PROF_DF_RESIDUAL);
-    FL_HIP_CHECK(hipMemsetAsync(status, 0, 2 * sizeof(int32_t), st));
+    clear_status_kernel<<<1, 64, 0, st>>>(status);
     transpose_jm_kernel<<<(G * 3 * NP + 255) / 256, 256, 0, st>>>(Jm, npe, G, NP, w.JmT);
     FL_LAUNCH_CHECK();
     const int64_t nq = nelem * G;
     qp_kernel<false><<<(unsigned)((nq + 255) / 256), 256, 0, st>>>(mp, points, elements, Eulerx, Jm, AllGauss, G,
-                                                                    nelem, npe, 1, w.qp, status);
+                                                                    nelem, npe, 1, w.qp, status, nullptr, nullptr);
     FL_LAUNCH_CHECK();
     prof_mark(st);
     const size_t lds_tr = (size_t)G * 9 * 8;

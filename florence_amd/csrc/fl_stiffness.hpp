@@ -83,6 +83,11 @@ __global__ __launch_bounds__(64 * WGM *WGN, MINW) void stiffness_kernel(const do
     const int wm = wave / WGN, wn = wave % WGN;
     const int li = lane & 15, lk = lane >> 4;
     const double *qe = qp + e * (int64_t)G * QS;
+    // diagonal blocks K^{ii} are symmetric: a wave tile strictly below the diagonal is
+    // skipped and written as the transpose of its mirror tile
+    const bool diag = pi == pk;
+    const bool skip = WGM == WGN && diag && wm > wn;
+    const bool mirror = !diag || (WGM == WGN && wm < wn);
 
     // fill role: point fg of the chunk, nodes fa0 + TPG*s
     const int fg = tid / TPG, fa0 = tid % TPG;
@@ -144,7 +149,7 @@ __global__ __launch_bounds__(64 * WGM *WGN, MINW) void stiffness_kernel(const do
         // not unrolled: each step already issues MT*NT long MFMAs, and hoisting the LDS
         // operands of several steps would cost the registers of a second wave per SIMD
 #pragma unroll 1
-        for (int kk = 0; kk < KC / 4; ++kk) {
+        for (int kk = 0; kk < (skip ? 0 : KC / 4); ++kk) {
             const int q = kk * 4 + lk;
             const int gq = q / 3, j = q - 3 * gq;
             const CT d0 = Dc[gq * 9 + 3 * j], d1 = Dc[gq * 9 + 3 * j + 1], d2 = Dc[gq * 9 + 3 * j + 2];
@@ -178,23 +183,28 @@ __global__ __launch_bounds__(64 * WGM *WGN, MINW) void stiffness_kernel(const do
             for (int r = 0; r < 4; ++r) {
                 const int a = (wm * MT + mt) * 16 + MF::row(lk, r);
                 const int b = (wn * NT + nt) * 16 + li;
-                if (a < npe && b < npe) Kd[(int64_t)a * npe + b] = (KT)acc[mt][nt][r];
+                if (!skip && a < npe && b < npe) Kd[(int64_t)a * npe + b] = (KT)acc[mt][nt][r];
             }
-    if (pi != pk) {
-        // mirror block K^{ki}_{ba} through a per-wave LDS transpose (coalesced stores)
+    if (!diag || WGM * WGN > 1) {
+        // transposed tiles (K^{ki}_{ba}, or the skipped half of K^{ii}) through a per-wave
+        // LDS transpose for coalesced stores; barriers are workgroup-uniform
         KT *Kt = Ke + (e * 9 + pk * 3 + pi) * npe2;
         CT *tile = sA + wave * TT;
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) {
+            if (mirror) {
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt)
+                for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) tile[(nt * 16 + li) * 17 + MF::row(lk, r)] = acc[mt][nt][r];
+                    for (int r = 0; r < 4; ++r) tile[(nt * 16 + li) * 17 + MF::row(lk, r)] = acc[mt][nt][r];
+            }
             __syncthreads();
-            for (int idx = lane; idx < 16 * NT * 16; idx += 64) {
-                const int bl = idx >> 4, al = idx & 15;
-                const int b = wn * NT * 16 + bl, a = (wm * MT + mt) * 16 + al;
-                if (a < npe && b < npe) Kt[(int64_t)b * npe + a] = (KT)tile[bl * 17 + al];
+            if (mirror) {
+                for (int idx = lane; idx < 16 * NT * 16; idx += 64) {
+                    const int bl = idx >> 4, al = idx & 15;
+                    const int b = wn * NT * 16 + bl, a = (wm * MT + mt) * 16 + al;
+                    if (a < npe && b < npe) Kt[(int64_t)b * npe + a] = (KT)tile[bl * 17 + al];
+                }
             }
             __syncthreads();
         }

@@ -120,6 +120,21 @@ int fl_df_assemble_ex(int32_t material, const double *params, const double *poin
                       double *V_dev, double *T_dev, int32_t precision, void *workspace_dev,
                       size_t workspace_bytes, int32_t *status_dev, void *stream);
 
+/* Per-element materials (no reference counterpart: the reference assembles one
+ * material per call).  elem_material_dev: int32[nelem] rows into material_table_dev:
+ * f64[nmat][4] = {mu, mu1, mu2, lamb} for the law `material`; both NULL = params.
+ * Used to assemble many meshes as one (batched form).  Issues no memset, so the call
+ * can be captured into a HIP graph (the workspace must then stay fixed). */
+int fl_df_assemble_mt(int32_t material, const double *params, const int32_t *elem_material_dev,
+                      const double *material_table_dev, const double *points_dev, const int32_t *elements_dev,
+                      const double *Eulerx_dev, const double *Jm_dev, const double *AllGauss_dev, int32_t ngauss,
+                      int64_t nelem, int32_t npe, int64_t nnode, int32_t requires_geometry_update,
+                      const int32_t *node_elem_ptr_dev, const int32_t *node_elem_dev,
+                      const int32_t *node_nbr_ptr_dev, const int32_t *node_nbrs_dev, const int32_t *node_order_dev,
+                      const int32_t *bucket_offsets, const int32_t *bucket_degree, const int32_t *indptr_dev,
+                      double *V_dev, double *T_dev, int32_t precision, void *workspace_dev, size_t workspace_bytes,
+                      int32_t *status_dev, void *stream);
+
 /* --------------------------------------------- residual-only (explicit) pass
  * Replaces the mechanics branch of _GlobalAssemblyExplicit_DF_DPF_
  * (FiniteElements/Assembly/_Assembly_/_LowLevelAssemblyExplicit_DF_DPF_.h:459-726):
@@ -171,6 +186,11 @@ int fl_profile_enable(int on);
  * at build time, setup.py:247-251).  key 1 = minimum waves per SIMD the p=4 hex
  * stiffness kernel is compiled for (1 or 2).  value 0 restores the default. */
 #define FL_TUNE_STIFF_MINW 1
+/* keys FL_TUNE_STIFF_VARIANT_BASE + c, c = padded node-count class (16, 32, 48, 64, 128
+ * nodes -> 0..4): value v+1 selects tile-shape variant v of the FP64/FP32 stiffness
+ * kernel (32: v1 = 2x2 waves of 16x16 tiles; 64: v1 = one wave of 4x4 tiles;
+ * 128: v1 = one wave per SIMD).  Used by the autotuner. */
+#define FL_TUNE_STIFF_VARIANT_BASE 2
 int fl_set_tuning(int32_t key, int32_t value);
 int fl_profile_read(int32_t *kinds, double *phase_ms, int32_t max_calls, int32_t max_phases);
 
