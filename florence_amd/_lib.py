@@ -56,6 +56,14 @@ SIGNATURES = {
                               _P32, ctypes.POINTER(ctypes.c_double), _vp]),
     "fl_profile_enable": (ctypes.c_int, [ctypes.c_int]),
     "fl_profile_read": (ctypes.c_int, [_vp, _vp, _i32, _i32]),
+    "fl_mlp_param_count": (_i64, []),
+    "fl_mlp_workspace_size": (_sz, [_i64]),
+    "fl_mlp_forward": (ctypes.c_int, [_vp, _vp, _i64, _vp, _vp, _vp, _vp]),
+    "fl_mlp_backward": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i32, _vp, _vp, _vp, _sz, _vp]),
+    "fl_mlp_adam": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                   ctypes.c_float, _i32, _vp]),
+    "fl_warmstart_features": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, _i64, _vp, _vp, _vp]),
+    "fl_warmstart_apply": (ctypes.c_int, [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp]),
 }
 
 

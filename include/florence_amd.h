@@ -194,6 +194,39 @@ int fl_profile_enable(int on);
 int fl_set_tuning(int32_t key, int32_t value);
 int fl_profile_read(int32_t *kinds, double *phase_ms, int32_t max_calls, int32_t max_phases);
 
+/* ------------------------------------------------------------- warm-start MLP
+ * No reference counterpart (Florence starts each increment's Newton iteration from
+ * the previous solution plus the Dirichlet increment, FEMSolver.py:932-946).  A
+ * 3-layer MLP in -> 128 -> 128 -> out with tanh, FP32 master weights, BF16 MFMA
+ * products accumulated in FP32.  Parameters P (FP32, fl_mlp_param_count() floats) in
+ * the order W1[32][128] b1[128] W2[128][128] b2[128] W3[128][32] b3[32]; inputs X
+ * [N][32] and outputs Y [N][32] row-major FP32 (zero-padded columns).
+ *
+ * fl_mlp_forward: H1, H2 [N][128], Y [N][32].
+ * fl_mlp_backward: gradient Gr (FP32, param-count floats) of the loss
+ *   sum_{n, c < nout} (Y - Tgt)^2 / (nout N) at the activations of a forward call,
+ *   and the loss in device *loss.  N must be a multiple of 32.  Deterministic.
+ * fl_mlp_adam: one Adam step (bias-corrected, step >= 1) on P with moments m, v.
+ * fl_warmstart_features: F [nnode][32] = standardised (X, u1, u1 - u2, load_factor)
+ *   with norm = device {mean[10], inv_std[10]}.
+ * fl_warmstart_apply: u = free ? 2 u1 - u2 + out_norm[0..2] * Y + out_norm[3..5] : u_bc
+ *   per dof (free_mask uint8 [3 nnode]).
+ */
+int64_t fl_mlp_param_count(void);
+size_t fl_mlp_workspace_size(int64_t n);
+int fl_mlp_forward(const float *P_dev, const float *X_dev, int64_t n, float *H1_dev, float *H2_dev, float *Y_dev,
+                   void *stream);
+int fl_mlp_backward(const float *P_dev, const float *X_dev, const float *H1_dev, const float *H2_dev,
+                    const float *Y_dev, const float *Tgt_dev, int64_t n, int32_t nout, float *Gr_dev, float *loss_dev,
+                    void *workspace_dev, size_t workspace_bytes, void *stream);
+int fl_mlp_adam(float *P_dev, const float *Gr_dev, float *m_dev, float *v_dev, float lr, float beta1, float beta2,
+                float eps, int32_t step, void *stream);
+int fl_warmstart_features(const double *X_dev, const double *u1_dev, const double *u2_dev, double load_factor,
+                          int64_t nnode, const float *norm_dev, float *F_dev, void *stream);
+int fl_warmstart_apply(const double *u1_dev, const double *u2_dev, const float *Y_dev, int64_t nnode,
+                       const float *out_norm_dev, const uint8_t *free_mask_dev, const double *u_bc_dev,
+                       double *u_dev, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
