@@ -29,9 +29,13 @@ HBM_PEAK_GBS = 8000.0
 
 
 def stiffness_flops_per_element(npe, G):
-    """FP64 flops of the block contraction actually executed for one element: six
-    3x3 displacement blocks (i <= k), each an npe x npe x 3G GEMM (2 flops per FMA)."""
-    return 6 * 2 * npe * npe * 3 * G
+    """FP64 flops of the block contraction for one element: every entry of the upper
+    triangle of the symmetric 3npe x 3npe element matrix (diagonal included) is a
+    3G-term dot product (2 flops per term).  The p=4 kernels execute this: the three
+    off-diagonal npe x npe blocks in full, the three symmetric diagonal blocks as
+    upper triangles of 16 x 16 tiles."""
+    n = 3 * npe
+    return n * (n + 1) // 2 * 2 * 3 * G
 
 
 def dense_b_flops_per_element(npe, G):
@@ -39,6 +43,18 @@ def dense_b_flops_per_element(npe, G):
     plus the geometric, kinematic, residual and material terms)."""
     n, V = 3 * npe, 6
     return G * (2 * V * n * n + 2 * V * V * n + 2 * n * n + 6 * npe * npe + 108 * npe + 13 * n + 300)
+
+
+def measured_traffic(summary):
+    """HBM bytes per launch of the dominant kernel: FETCH_SIZE + WRITE_SIZE from the
+    committed rocprofv3 PMC summary (copy-calibrated), or None if there is none."""
+    import re
+    try:
+        with open(os.path.join(ROOT, "profiles", summary)) as f:
+            m = re.search(r"traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", f.read())
+    except OSError:
+        return None
+    return float(m.group(1)) if m else None
 
 
 def cpu_baseline(mesh, fs, mat, x, nsample):
@@ -89,9 +105,12 @@ def bench_batch(args):
     print("generated {} meshes, {} elements in {:.1f}s".format(len(batch), sum(m.nelem for m, _, _ in batch), t_mesh),
           file=sys.stderr, flush=True)
     ba = BatchedAssembly(batch, precision=args.precision)
-    ba.set_state(lambda X: 0.02 * np.sin(np.pi * X[:, [1, 2, 0]]) + 0.01 * X)
+    disp = lambda X: 0.02 * np.sin(np.pi * X[:, [1, 2, 0]]) + 0.01 * X
+    ba.set_state(disp)
+    print("{} groups, nnz {}".format(len(ba.groups), ba.nnz), file=sys.stderr, flush=True)
     cache = autotune.tune_batch(ba, cache=autotune.load_cache())
     autotune.save_cache(cache)
+    print("tuned: {}".format({k: v["variant"] for k, v in cache.items()}), file=sys.stderr, flush=True)
     ba.assemble()
     torch.cuda.synchronize()
     ba.capture()
@@ -115,7 +134,7 @@ def bench_batch(args):
         dm = DeviceMeshData(m, g["fs"])
         _lib.load().fl_set_tuning(_lib.FL_TUNE_STIFF_VARIANT_BASE + _np_class(dm.npe), variants[key] + 1)
         X = np.asarray(m.points)
-        x = X + 0.02 * np.sin(np.pi * X[:, [1, 2, 0]]) + 0.01 * X
+        x = X + disp(X)     # the same expression as set_state, so the same coordinates bitwise
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         assemble_device(dm, mat, x, precision=args.precision)
         e0.record()
@@ -130,6 +149,8 @@ def bench_batch(args):
         worst = max(worst, err)
         nbit += int(np.array_equal(data, Vh) and np.array_equal(Tb, Th))
         del dm, V, T
+        if i % 32 == 31:
+            print("checked {} meshes, worst rel err {:.3e}".format(i + 1, worst), file=sys.stderr, flush=True)
     t_check = time.perf_counter() - t_check
     out = {"metric": "batched K+residual assembly elements/s ({} meshes, one graph launch)".format(len(batch)),
            "value": round(ba.nelem / (ms / 1e3), 1), "unit": "elements/s", "n_gpus": 1, "steps": args.steps,
@@ -148,26 +169,121 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_learned(args):
+    """Learned warm start: record trajectories of the twist/stretch family with the
+    device Newton, train the MLP on the device, then solve held-out members with each
+    start mode and tangent precision and report wall time to FP64 residual 1e-8."""
+    import torch
+    from florence_amd.learned import TwistStretchFamily, LockstepNewton, WarmStartMLP, build_dataset, train
+
+    log = lambda s: print(s, file=sys.stderr, flush=True)
+    fam = TwistStretchFamily(n=args.learn_n, nincr=args.nincr)
+    t0 = time.perf_counter()
+    if args.weights:
+        net = WarmStartMLP.load(args.weights)
+        gen_info = {"weights": os.path.basename(args.weights)}
+    else:
+        runs, iters = [], []
+        for c in range(0, args.ntraj, args.traj_batch):
+            seeds = range(c, min(c + args.traj_batch, args.ntraj))
+            ln = LockstepNewton(fam, seeds)
+            r = ln.solve("extrap", record=True)
+            if r["failed"]:
+                raise RuntimeError("trajectory batch {} failed".format(c))
+            runs.append((ln.X.clone(), ln.free.clone(), r["trajectory"]))
+            iters.append(sum(r["lockstep_iterations"]))
+            log("trajectories {}..{}: {:.1f}s, {} lockstep Newton iterations".format(
+                seeds[0], seeds[-1], r["wall_s"], iters[-1]))
+            del ln
+        t_gen = time.perf_counter() - t0
+        net = WarmStartMLP(seed=args.seed)
+        F, T = build_dataset(net, fam, runs)
+        del runs
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        hist = train(net, F, T, epochs=args.epochs, batch=8192, lr=2e-3, seed=args.seed, log=log)
+        torch.cuda.synchronize()
+        t_train = time.perf_counter() - t1
+        gen_info = {"trajectories": args.ntraj, "increments": args.nincr, "rows": int(F.shape[0]),
+                    "generate_s": round(t_gen, 1), "train_s": round(t_train, 1), "epochs": args.epochs,
+                    "loss_first_epoch": hist[0], "loss_last_epoch": hist[-1]}
+        del F, T
+        if args.save_weights:
+            net.save(args.save_weights, **{k: v for k, v in gen_info.items() if not isinstance(v, str)})
+    held = range(10 ** 6, 10 ** 6 + args.heldout)
+    # (tangent precision, start, linear-solve forcing); the first is the plain FP64 path
+    modes = [("fp64", "previous", "fixed"), ("fp64", "extrap", "fixed"), ("fp64", "learned", "fixed"),
+             ("fp64", "extrap", "adaptive"), ("fp64", "learned", "adaptive"), ("fp32", "extrap", "adaptive"),
+             ("fp32", "learned", "adaptive"), ("bf16", "learned", "adaptive")]
+    results, ref = {}, None
+    for prec, start, forcing in modes:
+        ln = LockstepNewton(fam, held, precision=prec, forcing=forcing)
+        ln.solve(start, predictor=net)           # warm-up (first-touch, kernel attributes)
+        r = ln.solve(start, predictor=net, record=True)
+        final = r["trajectory"][-1]
+        if ref is None:
+            ref = final
+        key = "{}+{}+{}".format(prec, start, forcing)
+        results[key] = {"wall_s": round(r["wall_s"], 3), "linear_s": round(r["linear_s"], 3),
+                        "assembly_s": round(r["assembly_s"], 3), "residual_s": round(r["residual_s"], 3),
+                        "predict_s": round(r["predict_s"], 4),
+                        "lockstep_iterations": int(sum(r["lockstep_iterations"])),
+                        "member_iterations_mean": float(r["member_iterations"].sum(0).mean()),
+                        "pcg_iterations": int(sum(r["pcg_iterations"])),
+                        "start_residual_mean": float(np.mean(r["start_residual"])) if r["start_residual"] else None,
+                        "learned_chosen": int(sum(r["chose_learned"])), "failed": r["failed"],
+                        "lockstep_per_increment": r["lockstep_iterations"],
+                        "worst_residual_history_k10": [float("{:.3e}".format(v)) for v in r["residual_history"][9]]
+                        if len(r["residual_history"]) > 9 else None,
+                        "max_rel_diff_vs_fp64_previous": float(np.abs(final - ref).max() / np.abs(ref).max())}
+        log("{}: {}".format(key, results[key]))
+        del ln
+    plain = results["fp64+previous+fixed"]["wall_s"]
+    best_key = min((k for k in results if "learned" in k), key=lambda k: results[k]["wall_s"])
+    out = {"metric": "Newton wall time to FP64 residual 1e-8 (held-out twist/stretch members, lockstep)",
+           "value": results[best_key]["wall_s"], "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 1,
+           "higher_is_better": False, "vs_baseline": None, "dtype": best_key.split("+")[0], "data": "synthetic",
+           "config": {"workload": "hex p=4 unit cubes {0}^3 (jitter 0.03h), MR(0.5,0.5,20), 6^3 GLL, x=0 clamped, "
+                                  "x=1 twist U[0,60] deg + stretch U[0.8,1.5], {1} increments; {2} held-out members "
+                                  "solved in lockstep".format(args.learn_n, args.nincr, args.heldout)},
+           "best_mode": best_key, "speedup_vs_plain_fp64": round(plain / results[best_key]["wall_s"], 3),
+           "modes": results, "training": gen_info}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=24, help="cells per cube edge")
-    ap.add_argument("--cpu-sample", type=int, default=256, help="elements in the CPU baseline sample (0 = skip)")
+    ap.add_argument("--cpu-sample", type=int, default=1024, help="elements in the CPU baseline sample (0 = skip)")
     ap.add_argument("--stiffness-minw", type=int, default=0, help="stiffness kernel variant (waves/SIMD), 0 = default")
-    ap.add_argument("--workload", default="assembly", choices=["assembly", "batch"],
-                    help="assembly: the headline K+T metric; batch: the batched-meshes form")
+    ap.add_argument("--workload", default="assembly", choices=["assembly", "batch", "learned"],
+                    help="assembly: the headline K+T metric; batch: the batched-meshes form; "
+                         "learned: warm-start training and the Newton time comparison")
+    ap.add_argument("--learn-n", type=int, default=2, help="learned: cells per cube edge of the family")
+    ap.add_argument("--nincr", type=int, default=30)
+    ap.add_argument("--ntraj", type=int, default=2000)
+    ap.add_argument("--traj-batch", type=int, default=250)
+    ap.add_argument("--epochs", type=int, default=6)
+    ap.add_argument("--heldout", type=int, default=64)
+    ap.add_argument("--weights", default="", help="learned: load trained weights instead of training")
+    ap.add_argument("--save-weights", default="")
+    ap.add_argument("--copy-mib", type=int, default=0, help="assembly: run one D2D copy of this size first "
+                                                             "(calibrates PMC byte counters)")
     ap.add_argument("--nmesh", type=int, default=256)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16"])
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the assembly would shard by "
-              "node-row ownership with halo elements; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+              "node-row ownership with halo elements). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     if args.workload == "batch":
         return bench_batch(args)
+    if args.workload == "learned":
+        return bench_learned(args)
 
     import torch
     from florence_amd import _lib
@@ -188,6 +304,16 @@ def main():
     print("setup {:.1f}s nelem={} nnode={} nnz={} max_degree={}".format(
         t_setup, dm.nelem, dm.nnode, dm.nnz, dm.max_degree), file=sys.stderr, flush=True)
 
+    if args.copy_mib:
+        # a plain device-to-device copy of known size: the scale of the PMC byte counters
+        a = torch.ones(args.copy_mib << 17, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print("copy calibration: {} bytes read, {} bytes written".format(a.numel() * 8, a.numel() * 8),
+              file=sys.stderr, flush=True)
+        del a, b
     for _ in range(args.warmup):
         assemble_device(dm, mat, x, V, T)
     torch.cuda.synchronize()
@@ -262,8 +388,9 @@ def main():
                                "(nnz {})".format(dm.nnz),
                    "nelem": dm.nelem, "nnode": dm.nnode, "nnz": dm.nnz, "ngauss": dm.ngauss},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
-                     "kernel": "stiffness_kernel<128,4,4,2,2>",
+                     "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+                     "traffic": measured_traffic("stiffness_kernel_hex_p4_24cube_pmc.md"),
+                     "kernel": "stiffness_kernel<128,4,4,2,2> (off-diagonal + triangular diagonal launches)",
                      "flops_per_launch": flops_exec, "kernel_ms": round(float(phase_ms[1]), 4),
                      "dense_b_equivalent_tflops": round(dense_b / (phase_ms[1] / 1e3) / 1e12, 3)},
         "phase_ms": {"qp": round(float(phase_ms[0]), 4), "stiffness": round(float(phase_ms[1]), 4),

@@ -407,10 +407,15 @@ static int launch_stiffness_t(int NP, const double *JmT, const double *qp, int n
             break;
         case 128:
             // default: registers capped for two waves per SIMD (measured faster on MI355X)
-            if (tuning_value(TUNE_STIFF_VARIANT_BASE + 4, 1) == 2 || tuning_value(TUNE_STIFF_MINW, 2) == 1)
+            if (tuning_value(TUNE_STIFF_VARIANT_BASE + 4, 1) == 2 || tuning_value(TUNE_STIFF_MINW, 2) == 1) {
                 stiffness_kernel<CT, KT, 128, 4, 4, 2, 2, 1><<<grid, 256, 0, st>>>(JmT, qp, npe, G, nelem, Ke);
-            else
-                stiffness_kernel<CT, KT, 128, 4, 4, 2, 2, 2><<<grid, 256, 0, st>>>(JmT, qp, npe, G, nelem, Ke);
+            } else {
+                // off-diagonal pairs as full 128 x 128 blocks, then the diagonal pairs
+                // as balanced upper triangles
+                const dim3 half((unsigned)(groups / 2));
+                stiffness_kernel<CT, KT, 128, 4, 4, 2, 2, 2, 1><<<half, 256, 0, st>>>(JmT, qp, npe, G, nelem, Ke);
+                stiffness_kernel<CT, KT, 128, 4, 4, 2, 2, 2, 2><<<half, 256, 0, st>>>(JmT, qp, npe, G, nelem, Ke);
+            }
             break;
         default: set_error("unsupported node count"); return FL_E_ARG;
     }

@@ -48,7 +48,9 @@ struct Mfma16<float> {
     __device__ static int row(int lk, int r) { return 4 * lk + r; }
 };
 
-template <typename CT, typename KT, int NP, int MT, int NT, int WGM, int WGN, int MINW = 1>
+// PS selects the block pairs of the launch: 0 = all six, 1 = the three off-diagonal
+// pairs (i < k), 2 = the three diagonal pairs (i = k).
+template <typename CT, typename KT, int NP, int MT, int NT, int WGM, int WGN, int MINW = 1, int PS = 0>
 __global__ __launch_bounds__(64 * WGM *WGN, MINW) void stiffness_kernel(const double *__restrict__ JmT,
                                                                    const double *__restrict__ qp, int npe, int G,
                                                                    int64_t nelem, KT *__restrict__ Ke) {
@@ -70,11 +72,12 @@ __global__ __launch_bounds__(64 * WGM *WGN, MINW) void stiffness_kernel(const do
 
     // XCD-aware order: workgroup ids are dealt round-robin over the 8 XCDs; the six
     // block pairs of one element land on the same XCD and share its L2.
+    constexpr int NPAIR = PS == 0 ? 6 : 3;
     const int64_t wid = blockIdx.x;
     const int xcd = (int)(wid & 7);
     const int64_t slot = wid >> 3;
-    const int pair = (int)(slot % 6);
-    const int64_t e = (slot / 6) * 8 + xcd;
+    const int pair = (int)(slot % NPAIR) + (PS == 1 ? 3 : 0);
+    const int64_t e = (slot / NPAIR) * 8 + xcd;
     if (e >= nelem) return;
     const int pi = c_pair_i[pair], pk = c_pair_k[pair];
 
@@ -86,8 +89,14 @@ __global__ __launch_bounds__(64 * WGM *WGN, MINW) void stiffness_kernel(const do
     // diagonal blocks K^{ii} are symmetric: a wave tile strictly below the diagonal is
     // skipped and written as the transpose of its mirror tile
     const bool diag = pi == pk;
-    const bool skip = WGM == WGN && diag && wm > wn;
+    // 128-node diagonal blocks (PS = 2): the upper triangle of the 8 x 8 grid of 16 x 16
+    // tiles (36 tiles) is dealt to the four waves as row pairs {w, 7 - w} (8 - w and
+    // w + 1 tiles: 9 each), so these workgroups do 9/16 of an off-diagonal block's MFMAs
+    constexpr bool TRI = PS == 2 && NP == 128 && WGM == 2 && WGN == 2 && MT == 4 && NT == 4;
+    constexpr bool tri = TRI;
+    const bool skip = !tri && WGM == WGN && diag && wm > wn;
     const bool mirror = !diag || (WGM == WGN && wm < wn);
+    const int R0 = wave, R1 = 7 - wave;   // tile rows of the triangular split
 
     // fill role: point fg of the chunk, nodes fa0 + TPG*s
     const int fg = tid / TPG, fa0 = tid % TPG;
@@ -148,6 +157,25 @@ __global__ __launch_bounds__(64 * WGM *WGN, MINW) void stiffness_kernel(const do
         const CT *Dc = sD + buf * QC * 9;
         // not unrolled: each step already issues MT*NT long MFMAs, and hoisting the LDS
         // operands of several steps would cost the registers of a second wave per SIMD
+        if (tri) {
+#pragma unroll 1
+            for (int kk = 0; kk < KC / 4; ++kk) {
+                const int q = kk * 4 + lk;
+                const int gq = q / 3, j = q - 3 * gq;
+                const CT d0 = Dc[gq * 9 + 3 * j], d1 = Dc[gq * 9 + 3 * j + 1], d2 = Dc[gq * 9 + 3 * j + 2];
+                const CT *Gb = A + 3 * gq * LDA;
+                const CT a0 = A[q * LDA + R0 * 16 + li], a1 = A[q * LDA + R1 * 16 + li];
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    if (c >= R0) {
+                        const int b = c * 16 + li;
+                        const CT bf = d0 * Gb[b] + d1 * Gb[LDA + b] + d2 * Gb[2 * LDA + b];
+                        acc[c / 4][c % 4] = MF::step(a0, bf, acc[c / 4][c % 4]);
+                        if (c >= R1) acc[2 + c / 4][c % 4] = MF::step(a1, bf, acc[2 + c / 4][c % 4]);
+                    }
+                }
+            }
+        } else {
 #pragma unroll 1
         for (int kk = 0; kk < (skip ? 0 : KC / 4); ++kk) {
             const int q = kk * 4 + lk;
@@ -167,6 +195,7 @@ __global__ __launch_bounds__(64 * WGM *WGN, MINW) void stiffness_kernel(const do
 #pragma unroll
                 for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = MF::step(af[mt], bf[nt], acc[mt][nt]);
         }
+        }
         if (more) commit(buf ^ 1);
         __syncthreads();
         buf ^= 1;
@@ -175,6 +204,27 @@ __global__ __launch_bounds__(64 * WGM *WGN, MINW) void stiffness_kernel(const do
     // epilogue: K^{ik} rows a, cols b.
     const int64_t npe2 = (int64_t)npe * npe;
     KT *Kd = Ke + (e * 9 + pi * 3 + pk) * npe2;
+    if (tri) {
+        // tile (R, c), c >= R, and its mirror (c, R) inside the same symmetric block
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int R = h ? R1 : R0;
+                if (c >= R) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int a = R * 16 + MF::row(lk, r), b = c * 16 + li;
+                        const KT v = (KT)acc[2 * h + c / 4][c % 4][r];
+                        if (a < npe && b < npe) {
+                            Kd[(int64_t)a * npe + b] = v;
+                            if (c > R) Kd[(int64_t)b * npe + a] = v;
+                        }
+                    }
+                }
+            }
+        return;
+    }
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt)
 #pragma unroll

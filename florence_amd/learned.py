@@ -1,0 +1,474 @@
+"""Learned warm start of the incremental Newton iteration.
+
+No reference counterpart: Florence starts every load increment from the previous
+converged state plus the Dirichlet increment and takes its first step with the
+previous tangent (FEMSolver.py:857-862, 932-946).  This module adds
+
+* ``TwistStretchFamily`` -- the trajectories: hex p=4 unit cubes of n^3 cells (nodes
+  jittered by +-0.03 h per member seed), Mooney-Rivlin(0.5, 0.5, 20), 6^3
+  Gauss-Lobatto points, x=0 clamped, the x=1 face twisted by theta ~ U[0, 60] degrees
+  about the x axis through (y, z) = (0.5, 0.5) and moved axially by s - 1,
+  s ~ U[0.8, 1.5], applied over ``nincr`` proportional increments (the Dirichlet
+  values times k / nincr, as the reference's LoadFactor does);
+* ``LockstepNewton`` -- many members solved together: their meshes are concatenated
+  into one node-disjoint mesh (one assembly call, one block-diagonal CSR, one PCG
+  solve per Newton iteration) and every member is tested against its own FP64
+  residual: ||R_free,i|| <= tol * R0_i, R0_i = the member's predictor residual at the
+  first increment (the reference's NormForces before its per-increment rebase);
+* ``WarmStartMLP`` -- the predictor: per node, inputs (X, u_{k-1}, u_{k-1} - u_{k-2},
+  load factor) standardised, output the correction to the linear extrapolation
+  2 u_{k-1} - u_{k-2}; trained on the device (libflorence_amd fl_mlp_* kernels, BF16
+  MFMA products, FP32 master weights, Adam) on the trajectories the solver generates.
+
+Start modes of LockstepNewton.solve:
+  "previous"  the reference's: u_{k-1} + Dirichlet increment, first step with the
+              previous tangent on the predictor residual K[:, out] du_D;
+  "extrap"    2 u_{k-1} - u_{k-2} (u_0 = u_{-1} = 0), full Newton from there;
+  "learned"   the MLP's prediction, unless the FP64 residual at the extrapolated
+              start is smaller (decided per member): the fallback.
+Each Newton iteration of the warm-started modes assembles K and T at the current
+state, solves, and evaluates the residual-only FP64 kernel at the new state; K is
+assembled again only if another step is needed.
+
+Linear solves: ``forcing="fixed"`` solves every step to ``lin_rtol`` (the stand-in for
+the reference's direct solve); ``forcing="adaptive"`` is an inexact Newton method
+whose PCG tolerance per step follows the model |R_next| ~ C |R|^2 + eta |R| (C
+estimated from the previous steps): a step that can be the last is solved to
+0.5 tol / |R|, any other only to C |R|, capped at ``eta_max``.  Convergence is
+always judged on the FP64 residual.
+"""
+import time
+
+import numpy as np
+
+from . import _lib
+from .assembly import DeviceMeshData, assemble_device, residual_device
+from .batch import _Concat
+from .function_space import FunctionSpace
+from .materials import MooneyRivlin
+from .quadrature import QuadratureRule
+from .solver import LinearSolver, spmv
+from .workloads import cube_mesh, twist_field
+
+__all__ = ["TwistStretchFamily", "LockstepNewton", "WarmStartMLP", "build_dataset", "train"]
+
+NIN, IP, H, OP = 10, 32, 128, 32
+
+
+class TwistStretchFamily(object):
+
+    def __init__(self, n=2, p=4, nincr=30, theta_range=(0.0, 60.0), stretch_range=(0.8, 1.5), jitter=0.03):
+        self.n, self.p, self.nincr = n, p, nincr
+        self.theta_range, self.stretch_range, self.jitter = theta_range, stretch_range, jitter
+        self.material = MooneyRivlin(3, mu1=0.5, mu2=0.5, lamb=20.0)
+        self.rule = QuadratureRule(qtype="lobatto", norder=p + 2, mesh_type="hex")
+
+    def member(self, seed):
+        """(mesh, theta in degrees, stretch) of member ``seed``."""
+        rng = np.random.Generator(np.random.PCG64(seed))
+        theta = float(rng.uniform(*self.theta_range))
+        stretch = float(rng.uniform(*self.stretch_range))
+        return cube_mesh("hex", self.p, self.n, jitter=self.jitter, seed=seed), theta, stretch
+
+    def function_space(self, mesh):
+        return FunctionSpace(mesh, self.rule, p=self.p)
+
+    @staticmethod
+    def dirichlet(mesh, theta, stretch, tol=1e-10):
+        """(free mask uint8 [3 nnode], total Dirichlet displacement [3 nnode], 0 on free dofs)."""
+        X = np.asarray(mesh.points)
+        u = np.zeros_like(X)
+        fixed = np.zeros(X.shape, dtype=bool)
+        left = np.abs(X[:, 0]) < tol
+        right = np.abs(X[:, 0] - 1.0) < tol
+        fixed[left] = True
+        fixed[right] = True
+        u[right] = twist_field(X[right], np.deg2rad(theta))
+        u[right, 0] = stretch - 1.0
+        return (~fixed).reshape(-1).astype(np.uint8), u.reshape(-1)
+
+
+class LockstepNewton(object):
+    """Incremental Newton of many family members as one device problem."""
+
+    def __init__(self, family, seeds, precision="fp64", tol=1e-8, lin_rtol=1e-10, maxit=25, forcing="fixed",
+                 eta_max=1e-2, device="cuda"):
+        import torch
+        if forcing not in ("fixed", "adaptive"):
+            raise ValueError(forcing)
+        self.family, self.seeds = family, list(seeds)
+        self.precision, self.tol, self.maxit = precision, tol, maxit
+        self.forcing, self.lin_rtol, self.eta_max = forcing, lin_rtol, eta_max
+        members = [family.member(s) for s in self.seeds]
+        meshes = [m for m, _, _ in members]
+        npm = meshes[0].points.shape[0]
+        if any(m.points.shape[0] != npm for m in meshes):
+            raise ValueError("members must share one topology")
+        self.nmember, self.nnode_member = len(meshes), npm
+        cm = _Concat(meshes)
+        self.dm = DeviceMeshData(cm, family.function_space(meshes[0]), device=device, build_indices=False)
+        dev = self.dm.device
+        bc = [family.dirichlet(m, th, s) for m, th, s in members]
+        self.params = [(th, s) for _, th, s in members]
+        self.free = torch.as_tensor(np.concatenate([f for f, _ in bc]), device=dev)
+        self.free_f = self.free.to(torch.float64)
+        self.u_total = torch.as_tensor(np.concatenate([u for _, u in bc]), device=dev)
+        self.X = self.dm.points.reshape(-1)
+        self.V = torch.empty(self.dm.nnz, dtype=torch.float64, device=dev)
+        self.T = torch.empty(3 * self.dm.nnode, dtype=torch.float64, device=dev)
+        self.lin = LinearSolver(rtol=lin_rtol, maxit=50000, check_every=10)
+
+    def _norms(self, R):
+        import torch
+        return torch.linalg.norm(R.view(self.nmember, -1), dim=1)
+
+    def _per_dof(self, v):
+        return v.repeat_interleave(3 * self.nnode_member)
+
+    def _assemble(self, u):
+        assemble_device(self.dm, self.family.material, self.X + u, self.V, self.T, precision=self.precision)
+
+    def _residual(self, u, out):
+        residual_device(self.dm, self.family.material, self.X + u, out)
+        out.mul_(self.free_f)
+        return out
+
+    def _solve(self, rhs, du):
+        du.zero_()
+        self.lin.Solve(self.dm, self.V, self.free, rhs, du)
+
+    def _eta(self, r, C):
+        """Linear tolerance of the next step at (worst member's) relative residual r:
+        with the step's result modelled as C r^2 + eta r, a step that can be the last
+        solves to 0.5 tol / r, any other step only to C r (its Newton error)."""
+        if self.forcing == "fixed":
+            return self.lin_rtol
+        eta = 0.5 * self.tol / r if C * r * r <= 0.5 * self.tol else C * r
+        return float(min(self.eta_max, max(self.lin_rtol, eta)))
+
+    def solve(self, start="previous", predictor=None, record=False):
+        """Run every increment; returns a dict of iteration counts, timings, start
+        choices and (record=True) the converged displacements u_0..u_nincr on the host."""
+        import torch
+        if start not in ("previous", "extrap", "learned"):
+            raise ValueError(start)
+        if start == "learned" and predictor is None:
+            raise ValueError("start='learned' needs a predictor")
+        dev = self.dm.device
+        n3 = 3 * self.dm.nnode
+        nincr = self.family.nincr
+        fixed_f = 1.0 - self.free_f
+        z = lambda: torch.zeros(n3, dtype=torch.float64, device=dev)
+        u_prev, u_prev2, u, du, rhs, R, RL, uL = z(), z(), z(), z(), z(), z(), z(), z()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        t_lin = t_asm = t_res = t_pred = 0.0
+
+        def timed(fn, *a):
+            t = time.perf_counter()
+            fn(*a)
+            torch.cuda.synchronize()
+            return time.perf_counter() - t
+
+        t_asm += timed(self._assemble, u)
+        # R0_i: predictor residual norm of the first increment
+        spmv(self.dm, self.V, self.u_total * fixed_f / nincr, rhs)
+        R0 = self._norms(rhs * self.free_f)
+        R0 = torch.where(R0 > 0, R0, torch.full_like(R0, 1e-14))
+        thresh = self.tol * R0
+        iters = np.zeros((nincr, self.nmember), dtype=np.int32)      # solves until each member converged
+        lock_iters = np.zeros(nincr, dtype=np.int32)                  # solves of the lockstep batch
+        chose_learned = np.zeros(nincr, dtype=np.int32)
+        start_ratio = []                                              # |R(start)| / R0 per increment (mean)
+        history = []                                                  # worst |R| / R0 before/after each step
+        traj = [np.zeros(n3)] if record else None
+        failed = False
+        C = 1.0                                                       # quadratic-convergence constant estimate
+        for k in range(1, nincr + 1):
+            lam = k / float(nincr)
+            ubc = self.u_total * (lam * fixed_f)
+            first_linear = start == "previous" or k == 1
+            if first_linear:
+                # reference predictor: Eulerx += du_D; first rhs = -K[:, out] du_D (free rows)
+                torch.add(u_prev * self.free_f, ubc, out=u)
+                spmv(self.dm, self.V, u - u_prev, rhs)
+                rhs.mul_(-self.free_f)
+                conv = torch.zeros(self.nmember, dtype=torch.bool, device=dev)
+                r_cur = float((self._norms(rhs) / R0).max())
+            else:
+                uE = (2.0 * u_prev - u_prev2) * self.free_f + ubc
+                t_res += timed(self._residual, uE, R)
+                nE = self._norms(R)
+                if start == "learned":
+                    t_pred += timed(predictor.predict, self.X, u_prev, u_prev2, lam, self.free, ubc, uL)
+                    t_res += timed(self._residual, uL, RL)
+                    nL = self._norms(RL)
+                    use = nL < nE
+                    chose_learned[k - 1] = int(use.sum())
+                    u.copy_(torch.where(self._per_dof(use), uL, uE))
+                    R.copy_(torch.where(self._per_dof(use), RL, R))
+                    nS = torch.where(use, nL, nE)
+                else:
+                    u.copy_(uE)
+                    nS = nE
+                start_ratio.append(float((nS / R0).mean()))
+                conv = nS <= thresh
+                r_cur = float((nS / R0).max())
+                if bool(conv.all()):
+                    lock_iters[k - 1] = 0
+                    history.append([r_cur])
+                    u_prev2, u_prev = u_prev, u.clone()
+                    if record:
+                        traj.append(u.cpu().numpy())
+                    continue
+                t_asm += timed(self._assemble, u)
+                torch.neg(R, out=rhs)
+            it = 0
+            done = conv.cpu().numpy().copy()
+            hk = [r_cur]
+            while True:
+                eta = self._eta(r_cur, C)
+                self.lin.rtol = eta
+                t_lin += timed(self._solve, rhs, du)
+                u.add_(du)
+                it += 1
+                if first_linear:
+                    t_asm += timed(self._assemble, u)
+                    torch.mul(self.T, self.free_f, out=R)
+                else:
+                    t_res += timed(self._residual, u, R)
+                nr = self._norms(R)
+                live = torch.as_tensor(~done, device=dev)
+                r_new = float((nr / R0)[live].max())
+                if eta * r_cur < 0.5 * r_new and r_cur > 0:
+                    C = min(1e4, max(1e-2, r_new / (r_cur * r_cur)))
+                r_cur = float((nr / R0).max())
+                hk.append(r_cur)
+                c = (nr <= thresh).cpu().numpy()
+                newly = c & ~done
+                iters[k - 1, newly] = it
+                done |= c
+                if done.all():
+                    break
+                if it >= self.maxit or not bool(torch.isfinite(nr).all()):
+                    failed = True
+                    iters[k - 1, ~done] = -1
+                    break
+                if not first_linear:
+                    t_asm += timed(self._assemble, u)
+                torch.neg(R, out=rhs)
+            lock_iters[k - 1] = it
+            history.append(hk)
+            # in the reference's mode V now holds K at the converged state, which the
+            # next increment's predictor uses; the warm-started modes assemble at
+            # their start instead
+            u_prev2, u_prev = u_prev, u.clone()
+            if record:
+                traj.append(u.cpu().numpy())
+            if failed:
+                break
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        out = {"start": start, "precision": self.precision, "wall_s": wall, "linear_s": t_lin, "assembly_s": t_asm,
+               "residual_s": t_res, "predict_s": t_pred, "lockstep_iterations": lock_iters.tolist(),
+               "member_iterations": iters, "chose_learned": chose_learned.tolist(), "start_residual": start_ratio,
+               "pcg_iterations": list(self.lin.iterations), "residual_history": history,
+               "forcing": self.forcing, "failed": failed}
+        self.lin.iterations = []
+        self.lin.rtol = self.lin_rtol
+        if record:
+            out["trajectory"] = np.stack(traj)     # (nincr + 1, 3 nnode)
+        return out
+
+
+class WarmStartMLP(object):
+    """Device-resident predictor: parameters, Adam moments and normalisation."""
+
+    def __init__(self, device="cuda", seed=0):
+        import torch
+        lib = _lib.load()
+        self.device = torch.device(device)
+        self.nparam = int(lib.fl_mlp_param_count())
+        self.P = torch.as_tensor(self.initial_params(seed), device=self.device)
+        self.m = torch.zeros_like(self.P)
+        self.v = torch.zeros_like(self.P)
+        self.step = 0
+        self.norm_in = torch.as_tensor(np.concatenate([np.zeros(NIN), np.ones(NIN)]).astype(np.float32),
+                                       device=self.device)
+        self.norm_out = torch.as_tensor(np.array([1, 1, 1, 0, 0, 0], np.float32), device=self.device)
+        self._buf = {}
+
+    @staticmethod
+    def initial_params(seed):
+        """Weights U(-1/sqrt(fan_in), 1/sqrt(fan_in)), biases 0, in the library's order."""
+        rng = np.random.Generator(np.random.PCG64(seed))
+        parts = []
+        for fan_in, fan_out in ((IP, H), (H, H), (H, OP)):
+            b = 1.0 / np.sqrt(fan_in)
+            parts.append(rng.uniform(-b, b, size=fan_in * fan_out))
+            parts.append(np.zeros(fan_out))
+        return np.concatenate(parts).astype(np.float32)
+
+    def _buffers(self, n):
+        import torch
+        if self._buf.get("n", 0) < n:
+            f32 = dict(dtype=torch.float32, device=self.device)
+            self._buf = {"n": n, "H1": torch.empty((n, H), **f32), "H2": torch.empty((n, H), **f32),
+                         "Y": torch.empty((n, OP), **f32), "F": torch.empty((n, IP), **f32)}
+        return self._buf
+
+    def forward(self, F, n):
+        lib = _lib.load()
+        b = self._buffers(n)
+        _lib.check(lib.fl_mlp_forward(_lib.ptr(self.P), _lib.ptr(F), n, _lib.ptr(b["H1"]), _lib.ptr(b["H2"]),
+                                      _lib.ptr(b["Y"]), _lib.stream_ptr()), "fl_mlp_forward")
+        return b
+
+    def features(self, X, u1, u2, lam, F):
+        lib = _lib.load()
+        nn = X.numel() // 3
+        _lib.check(lib.fl_warmstart_features(_lib.ptr(X), _lib.ptr(u1), _lib.ptr(u2), float(lam), nn,
+                                             _lib.ptr(self.norm_in), _lib.ptr(F), _lib.stream_ptr()),
+                   "fl_warmstart_features")
+        return F
+
+    def predict(self, X, u1, u2, lam, free_u8, ubc, out):
+        """out = prediction of the next converged displacement (Dirichlet dofs = ubc)."""
+        lib = _lib.load()
+        nn = X.numel() // 3
+        b = self._buffers(nn)
+        self.features(X, u1, u2, lam, b["F"])
+        self.forward(b["F"], nn)
+        _lib.check(lib.fl_warmstart_apply(_lib.ptr(u1), _lib.ptr(u2), _lib.ptr(b["Y"]), nn, _lib.ptr(self.norm_out),
+                                          _lib.ptr(free_u8), _lib.ptr(ubc), _lib.ptr(out), _lib.stream_ptr()),
+                   "fl_warmstart_apply")
+        return out
+
+    def train_step(self, F, T, n, lr, nout=3, ws=None, G=None, loss=None):
+        """One Adam step on the minibatch (F, T) of n rows (n % 32 == 0); returns the
+        device loss tensor."""
+        import torch
+        lib = _lib.load()
+        b = self.forward(F, n)
+        if G is None:
+            G = torch.empty(self.nparam, dtype=torch.float32, device=self.device)
+        if loss is None:
+            loss = torch.empty(1, dtype=torch.float32, device=self.device)
+        if ws is None:
+            ws = torch.empty(int(lib.fl_mlp_workspace_size(n)), dtype=torch.uint8, device=self.device)
+        _lib.check(lib.fl_mlp_backward(_lib.ptr(self.P), _lib.ptr(F), _lib.ptr(b["H1"]), _lib.ptr(b["H2"]),
+                                       _lib.ptr(b["Y"]), _lib.ptr(T), n, nout, _lib.ptr(G), _lib.ptr(loss),
+                                       _lib.ptr(ws), ws.numel(), _lib.stream_ptr()), "fl_mlp_backward")
+        self.step += 1
+        _lib.check(lib.fl_mlp_adam(_lib.ptr(self.P), _lib.ptr(G), _lib.ptr(self.m), _lib.ptr(self.v), float(lr),
+                                   0.9, 0.999, 1e-8, self.step, _lib.stream_ptr()), "fl_mlp_adam")
+        return loss
+
+    def state(self):
+        return {"P": self.P.cpu().numpy(), "norm_in": self.norm_in.cpu().numpy(),
+                "norm_out": self.norm_out.cpu().numpy()}
+
+    def save(self, path, **meta):
+        np.savez(path, **self.state(), **{k: np.asarray(v) for k, v in meta.items()})
+
+    @classmethod
+    def load(cls, path, device="cuda"):
+        import torch
+        d = np.load(path, allow_pickle=False)
+        net = cls(device=device)
+        net.P.copy_(torch.as_tensor(d["P"]))
+        net.norm_in.copy_(torch.as_tensor(d["norm_in"]))
+        net.norm_out.copy_(torch.as_tensor(d["norm_out"]))
+        return net
+
+
+def _free_nodes(free_u8):
+    return free_u8.view(-1, 3).bool().all(dim=1)
+
+
+def build_dataset(net, family, runs):
+    """Training rows from recorded trajectories.
+
+    ``runs`` = [(X [3 nnode], free mask uint8 [3 nnode], trajectory (nincr + 1, 3 nnode)),
+    ...] (X and the mask as device tensors, e.g. a LockstepNewton's ``X``/``free``).  For every
+    increment k >= 2 and every node whose three dofs are free: inputs (X, u_{k-1},
+    u_{k-1} - u_{k-2}, k / nincr), target u_k - (2 u_{k-1} - u_{k-2}).  Sets the
+    network's input/output standardisation from these rows and returns device
+    (F [N, 32], T [N, 32]) with N rounded down to a multiple of 32."""
+    import torch
+    dev = net.device
+    nincr = family.nincr
+    raw_s = torch.zeros(NIN, dtype=torch.float64, device=dev)
+    raw_q = torch.zeros(NIN, dtype=torch.float64, device=dev)
+    tgt_s = torch.zeros(3, dtype=torch.float64, device=dev)
+    tgt_q = torch.zeros(3, dtype=torch.float64, device=dev)
+    count = 0
+    pieces = []
+    for Xf, free, traj in runs:
+        U = torch.as_tensor(traj, device=dev)
+        X = Xf.view(-1, 3)
+        sel = _free_nodes(free)
+        for k in range(2, nincr + 1):
+            u1, u2, uk = U[k - 1].view(-1, 3), U[k - 2].view(-1, 3), U[k].view(-1, 3)
+            raw = torch.cat([X, u1, u1 - u2, torch.full_like(X[:, :1], k / float(nincr))], dim=1)[sel]
+            tgt = (uk - (2 * u1 - u2))[sel]
+            raw_s += raw.sum(0)
+            raw_q += (raw * raw).sum(0)
+            tgt_s += tgt.sum(0)
+            tgt_q += (tgt * tgt).sum(0)
+            count += raw.shape[0]
+        pieces.append((Xf, U, sel))
+    mean = raw_s / count
+    std = torch.sqrt(torch.clamp(raw_q / count - mean * mean, min=0.0))
+    inv = torch.where(std > 1e-12, 1.0 / std, torch.ones_like(std))
+    tmean = tgt_s / count
+    tstd = torch.sqrt(torch.clamp(tgt_q / count - tmean * tmean, min=1e-30))
+    net.norm_in.copy_(torch.cat([mean, inv]).to(torch.float32))
+    net.norm_out.copy_(torch.cat([tstd, tmean]).to(torch.float32))
+    n = (count // 32) * 32
+    F = torch.empty((count, IP), dtype=torch.float32, device=dev)
+    T = torch.zeros((count, OP), dtype=torch.float32, device=dev)
+    o = 0
+    for Xf, U, sel in pieces:
+        nn = Xf.numel() // 3
+        Fk = torch.empty((nn, IP), dtype=torch.float32, device=dev)
+        for k in range(2, nincr + 1):
+            net.features(Xf, U[k - 1], U[k - 2], k / float(nincr), Fk)
+            u1, u2, uk = U[k - 1].view(-1, 3), U[k - 2].view(-1, 3), U[k].view(-1, 3)
+            tgt = ((uk - (2 * u1 - u2))[sel] - tmean) / tstd
+            m = tgt.shape[0]
+            F[o:o + m] = Fk[sel]
+            T[o:o + m, :3] = tgt.to(torch.float32)
+            o += m
+    return F[:n], T[:n]
+
+
+def train(net, F, T, epochs=8, batch=8192, lr=2e-3, lr_final=2e-5, seed=0, log=None):
+    """Adam over shuffled minibatches (cosine-decayed learning rate); returns the mean
+    loss of each epoch (standardised units)."""
+    import torch
+    lib = _lib.load()
+    N = F.shape[0]
+    batch = min(batch, (N // 32) * 32)
+    nb = N // batch
+    total = epochs * nb
+    G = torch.empty(net.nparam, dtype=torch.float32, device=net.device)
+    ws = torch.empty(int(lib.fl_mlp_workspace_size(batch)), dtype=torch.uint8, device=net.device)
+    losses = torch.zeros(nb, dtype=torch.float32, device=net.device)
+    gen = torch.Generator(device=net.device)
+    gen.manual_seed(seed)
+    hist = []
+    step = 0
+    for ep in range(epochs):
+        perm = torch.randperm(N, device=net.device, generator=gen)
+        for b in range(nb):
+            idx = perm[b * batch:(b + 1) * batch]
+            Fb = F.index_select(0, idx)
+            Tb = T.index_select(0, idx)
+            c = 0.5 * (1 + np.cos(np.pi * step / max(total - 1, 1)))
+            net.train_step(Fb, Tb, batch, lr_final + (lr - lr_final) * c, ws=ws, G=G, loss=losses[b:b + 1])
+            step += 1
+        hist.append(float(losses.mean()))
+        if log is not None:
+            log("epoch {} loss {:.4e}".format(ep, hist[-1]))
+    return hist

@@ -4,7 +4,8 @@ Checks: the FP32 tangent's error against the FP64 oracle is at FP32 level (state
 bar 1e-5 relative, infinity norm); the residual T, computed in FP64 by the same
 kernels, is bitwise the FP64 path's; the inexact-Jacobian Newton on the tet p=2 cube
 still reaches the 1e-8 relative residual and its displacement agrees with the FP64
-oracle's (tests/golden/c1_newton.npz) to 1e-7."""
+oracle's (tests/golden/c1_newton.npz) to 1e-7 (FP32 tangent) and 1e-6 (BF16 tangent:
+linear convergence, the last iterate's residual decides how close it lands)."""
 import os
 
 import numpy as np
@@ -48,7 +49,8 @@ def test_fp32_tangent_error(cuda, case):
     assert int(st[0].item()) == 0
 
 
-def test_fp32_newton_config1(cuda):
+@pytest.mark.parametrize("precision,bar", [("fp32", 1e-7), ("bf16", 1e-6)])
+def test_mixed_newton_config1(cuda, precision, bar):
     from florence_amd.boundary_condition import BoundaryCondition
     from florence_amd.solver import FEMSolver, LinearSolver
     from florence_amd.workloads import clamp_and_pull
@@ -63,13 +65,12 @@ def test_fp32_newton_config1(cuda):
     bc = BoundaryCondition()
     bc.SetDirichletCriteria(clamp_and_pull, m)
     solver = FEMSolver(optimise=True, number_of_load_increments=5, newton_raphson_tolerance=1e-8,
-                       linear_solver=LinearSolver(rtol=1e-12), tangent_precision="fp32")
+                       linear_solver=LinearSolver(rtol=1e-12), tangent_precision=precision)
     res = solver.Solve(formulation=Form(), mesh=m, material=mat, boundary_condition=bc)
     assert not res.newton_raphson_failed_to_converge
     disp = res.TotalDisp[:, :, -1].cpu().numpy()
     err = np.abs(disp - ref["disp_final"]).max() / np.abs(ref["disp_final"]).max()
-    print("fp32 tangent newton iterations", res.newton_iterations, "(fp64:", ref["iterations"].tolist(),
-          ") rel disp err {:.2e} wall {:.3f}s".format(err, res.wall_time), flush=True)
-    for v in res.NRConvergence.values():
-        assert v[-1] <= 1e-8 or True   # the loop exits on the same tests as the FP64 path
-    assert err <= 1e-7
+    print("{} tangent newton iterations {} (fp64: {}) rel disp err {:.2e} wall {:.3f}s final residuals {}".format(
+        precision, res.newton_iterations, ref["iterations"].tolist(), err, res.wall_time,
+        ["{:.1e}".format(v[-1]) for v in res.NRConvergence.values()]), flush=True)
+    assert err <= bar
