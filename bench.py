@@ -113,6 +113,18 @@ def bench_batch(args):
     print("tuned: {}".format({k: v["variant"] for k, v in cache.items()}), file=sys.stderr, flush=True)
     ba.assemble()
     torch.cuda.synchronize()
+    # per-group phase times of one eager pass (library HIP events)
+    _lib.profile_enable(True)
+    ba.assemble()
+    torch.cuda.synchronize()
+    calls = _lib.profile_read()
+    _lib.profile_enable(False)
+    group_phases = {}
+    for g, c in zip(ba.groups, [c for c in calls if c[0] == 1]):
+        k = "{}-p{}".format(*g["key"])
+        prev = group_phases.get(k, [0.0] * 4)
+        group_phases[k] = [round(a + b, 3) for a, b in zip(prev, c[1])]
+    print("group phases (qp, stiffness, traction, gather) ms: {}".format(group_phases), file=sys.stderr, flush=True)
     ba.capture()
     for _ in range(args.warmup):
         ba.replay()
@@ -162,6 +174,7 @@ def bench_batch(args):
            "per_mesh_eager_ms_sum": round(eager_ms, 3),
            "speedup_vs_per_mesh_calls": round(eager_ms / ms, 2),
            "tuned_variants": {"{}-p{}".format(*g["key"]): g["variant"] for g in ba.groups},
+           "group_phase_ms": group_phases,
            "tuning": cache,
            "per_mesh_check": {"max_rel_err": worst, "bitwise_equal_meshes": nbit, "meshes": len(batch),
                               "seconds": round(t_check, 1)},
@@ -251,6 +264,52 @@ def bench_learned(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_newton(args):
+    """The config-2 solve: incremental Newton on the device with the reference's Newton
+    semantics (FEMSolver), hex p=4 unit cube n^3, MR(0.5, 0.5, 20), 6^3 Gauss-Lobatto
+    points, x=0 clamped, x=1 twisted 30 degrees, 10 increments, tolerance 1e-8.  Runs
+    the plain FP64 path and, with --precision fp32/bf16, the mixed-precision tangent;
+    reports iterations, CG iterations, wall time and the final-displacement difference."""
+    import torch
+    from florence_amd.boundary_condition import BoundaryCondition
+    from florence_amd.solver import FEMSolver, LinearSolver
+    from florence_amd.workloads import clamp_and_twist, cube_hex_p4
+
+    mesh, fs, mat = cube_hex_p4(args.n)
+
+    class Form(object):
+        ndim = nvar = 3
+        fields = "mechanics"
+        function_spaces = (fs, None, None)
+
+    modes, ref = {}, None
+    for prec in ["fp64"] + ([args.precision] if args.precision != "fp64" else []):
+        bc = BoundaryCondition()
+        bc.SetDirichletCriteria(clamp_and_twist, mesh, 30.0)
+        solver = FEMSolver(optimise=True, number_of_load_increments=10, newton_raphson_tolerance=1e-8,
+                           linear_solver=LinearSolver(rtol=args.lin_rtol, maxit=500000, check_every=25),
+                           tangent_precision=prec, report_log_level=1)
+        res = solver.Solve(formulation=Form(), mesh=mesh, material=mat, boundary_condition=bc)
+        final = res.TotalDisp[:, :, -1].cpu().numpy()
+        ref = final if ref is None else ref
+        modes[prec] = {"wall_s": round(res.wall_time, 3), "assembly_s": round(res.assembly_time, 3),
+                       "newton_iterations": list(res.newton_iterations), "cg_iterations": res.cg_iterations,
+                       "failed": bool(res.newton_raphson_failed_to_converge),
+                       "final_residuals": [float("{:.3e}".format(v[-1])) for v in res.NRConvergence.values() if v],
+                       "max_rel_diff_vs_fp64": float(np.abs(final - ref).max() / np.abs(ref).max())}
+        print("{}: {}".format(prec, modes[prec]), file=sys.stderr, flush=True)
+        del solver, res
+        torch.cuda.empty_cache()
+    out = {"metric": "Newton wall time to 1e-8 (config-2 workload, reference Newton semantics)",
+           "value": modes["fp64"]["wall_s"], "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 0,
+           "higher_is_better": False, "vs_baseline": None, "dtype": "fp64", "data": "synthetic",
+           "config": {"workload": "unit cube {0}^3 hex p=4, 6^3 GLL, MR(0.5,0.5,20), x=0 clamped, x=1 twisted "
+                                  "30 deg, 10 increments, block-Jacobi PCG rtol {1}".format(args.n, args.lin_rtol),
+                      "nelem": int(mesh.nelem), "dofs": int(3 * mesh.nnode)},
+           "modes": modes}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -259,7 +318,7 @@ def main():
     ap.add_argument("--n", type=int, default=24, help="cells per cube edge")
     ap.add_argument("--cpu-sample", type=int, default=1024, help="elements in the CPU baseline sample (0 = skip)")
     ap.add_argument("--stiffness-minw", type=int, default=0, help="stiffness kernel variant (waves/SIMD), 0 = default")
-    ap.add_argument("--workload", default="assembly", choices=["assembly", "batch", "learned"],
+    ap.add_argument("--workload", default="assembly", choices=["assembly", "batch", "learned", "newton"],
                     help="assembly: the headline K+T metric; batch: the batched-meshes form; "
                          "learned: warm-start training and the Newton time comparison")
     ap.add_argument("--learn-n", type=int, default=2, help="learned: cells per cube edge of the family")
@@ -270,6 +329,7 @@ def main():
     ap.add_argument("--heldout", type=int, default=64)
     ap.add_argument("--weights", default="", help="learned: load trained weights instead of training")
     ap.add_argument("--save-weights", default="")
+    ap.add_argument("--lin-rtol", type=float, default=1e-10, help="newton: PCG relative tolerance")
     ap.add_argument("--copy-mib", type=int, default=0, help="assembly: run one D2D copy of this size first "
                                                              "(calibrates PMC byte counters)")
     ap.add_argument("--nmesh", type=int, default=256)
@@ -284,6 +344,8 @@ def main():
         return bench_batch(args)
     if args.workload == "learned":
         return bench_learned(args)
+    if args.workload == "newton":
+        return bench_newton(args)
 
     import torch
     from florence_amd import _lib

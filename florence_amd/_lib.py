@@ -42,8 +42,10 @@ SIGNATURES = {
     "fl_set_tuning": (ctypes.c_int, [_i32, _i32]),
     "fl_df_workspace_size_ex": (_sz, [_i32, _i64, _i32, _i32]),
     "fl_df_assemble_mt": (ctypes.c_int, [_i32, _PD, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i64, _i32, _i64,
-                                         _i32, _vp, _vp, _vp, _vp, _vp, _P32, _P32, _vp, _vp, _vp, _i32, _vp, _sz,
-                                         _vp, _vp]),
+                                         _i32, _vp, _vp, _vp, _vp, _vp, _P32, _P32, _vp, _vp, _vp, _i32, _vp, _vp,
+                                         _sz, _vp, _vp]),
+    "fl_pattern_rank_table_bytes": (_sz, [_i64, _i32]),
+    "fl_pattern_rank_table": (ctypes.c_int, [_vp, _i64, _i32, _vp, _vp, _vp, _vp, _vp]),
     "fl_df_assemble_ex": (ctypes.c_int, [_i32, _PD, _vp, _vp, _vp, _vp, _vp, _i32, _i64, _i32, _i64, _i32,
                                          _vp, _vp, _vp, _vp, _vp, _P32, _P32, _vp, _vp, _vp, _i32, _vp, _sz, _vp,
                                          _vp]),

@@ -37,7 +37,7 @@ class DeviceMeshData(object):
     """Device-resident, per-(mesh, function space) inputs of the kernels: coordinates,
     int32 connectivity, basis tables, the node graph and the reference CSR pattern."""
 
-    def __init__(self, mesh, function_space, device="cuda", build_indices=True):
+    def __init__(self, mesh, function_space, device="cuda", build_indices=True, build_ranks=True):
         torch = _torch()
         lib = _lib.load()
         self.device = torch.device(device)
@@ -84,6 +84,18 @@ class DeviceMeshData(object):
                                               self.bucket_offsets, self.bucket_degree, _lib.stream_ptr()),
                    "fl_pattern_gather_plan")
         self.status = torch.zeros(2, dtype=i32, device=self.device)
+        # column positions of every (element, row node, column node) entry: built once,
+        # they replace the gather's per-call neighbour-list searches
+        self.ranks = None
+        if build_ranks and self.max_degree <= 32767:
+            self.ranks = torch.empty(int(lib.fl_pattern_rank_table_bytes(self.nelem, self.npe)) // 2,
+                                     dtype=torch.int16, device=self.device)
+            _lib.check(lib.fl_pattern_rank_table(_lib.ptr(self.elements), self.nelem, self.npe,
+                                                 _lib.ptr(self.node_nbr_ptr), _lib.ptr(self.node_nbrs),
+                                                 _lib.ptr(self.ranks), _lib.ptr(self.status), _lib.stream_ptr()),
+                       "fl_pattern_rank_table")
+            if int(self.status[0].item()) & _lib.FL_S_PATTERN:
+                raise _lib.FlorenceAMDError("inconsistent sparsity pattern (rank table)")
         self._ws = None
         self._ws_res = None
 
@@ -146,14 +158,14 @@ def assemble_device(dm, material, Eulerx, V=None, T=None, requires_geometry_upda
     if mid is None:
         raise NotImplementedError("no low-level assembler for material {}".format(type(material).__name__))
     prec = _lib.PRECISIONS[precision]
-    _lib.check(lib.fl_df_assemble_ex(mid, prm, _lib.ptr(dm.points), _lib.ptr(dm.elements), _lib.ptr(x),
+    _lib.check(lib.fl_df_assemble_mt(mid, prm, None, None, _lib.ptr(dm.points), _lib.ptr(dm.elements), _lib.ptr(x),
                                      _lib.ptr(dm.Jm), _lib.ptr(dm.AllGauss), dm.ngauss, dm.nelem, dm.npe, dm.nnode,
                                      int(requires_geometry_update), _lib.ptr(dm.node_elem_ptr),
                                      _lib.ptr(dm.node_elem), _lib.ptr(dm.node_nbr_ptr), _lib.ptr(dm.node_nbrs),
                                      _lib.ptr(dm.node_order), dm.bucket_offsets, dm.bucket_degree,
-                                     _lib.ptr(dm.indptr), _lib.ptr(V), _lib.ptr(T), prec, _lib.ptr(ws), ws.numel(),
-                                     _lib.ptr(dm.status), _lib.stream_ptr(stream)),
-               "fl_df_assemble_ex")
+                                     _lib.ptr(dm.indptr), _lib.ptr(V), _lib.ptr(T), prec, _lib.ptr(dm.ranks),
+                                     _lib.ptr(ws), ws.numel(), _lib.ptr(dm.status), _lib.stream_ptr(stream)),
+               "fl_df_assemble_mt")
     return V, T, dm.status
 
 

@@ -177,7 +177,8 @@ class BatchedAssembly(object):
                                          _lib.ptr(dm.node_nbr_ptr), _lib.ptr(dm.node_nbrs), _lib.ptr(dm.node_order),
                                          dm.bucket_offsets, dm.bucket_degree, _lib.ptr(dm.indptr),
                                          _lib.ptr(g["V"]), _lib.ptr(g["T"]), _lib.PRECISIONS[self.precision],
-                                         _lib.ptr(ws), ws.numel(), _lib.ptr(dm.status), _lib.stream_ptr()),
+                                         _lib.ptr(dm.ranks), _lib.ptr(ws), ws.numel(), _lib.ptr(dm.status),
+                                         _lib.stream_ptr()),
                    "fl_df_assemble_mt")
 
     def assemble(self):

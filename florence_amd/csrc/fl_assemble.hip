@@ -275,7 +275,8 @@ __global__ __launch_bounds__(GT) void gather_kernel(const int32_t *__restrict__ 
                                                     const int32_t *__restrict__ nbr_ptr,
                                                     const int32_t *__restrict__ nbrs, const int32_t *__restrict__ indptr,
                                                     const KT *__restrict__ Ke, const double *__restrict__ te,
-                                                    int lds_deg, double *__restrict__ V, double *__restrict__ T,
+                                                    int lds_deg, const int16_t *__restrict__ ranks,
+                                                    double *__restrict__ V, double *__restrict__ T,
                                                     int32_t *__restrict__ status) {
     extern __shared__ double smem[];
     const int64_t a = node_order[blockIdx.x];
@@ -300,10 +301,16 @@ __global__ __launch_bounds__(GT) void gather_kernel(const int32_t *__restrict__ 
         const int32_t ent = node_elem[q];
         const int64_t e = ent / npe;
         const int la = ent - (int)(e * npe);
-        for (int lb = tid; lb < npe; lb += GT) {
-            const int r = find_rank(snb, deg, elements[e * npe + lb]);
-            bad |= r < 0;
-            rr[lb] = r < 0 ? 0 : r;
+        if (ranks) {
+            // column positions precomputed once per mesh (fl_pattern_rank_table)
+            const int16_t *rk = ranks + ((int64_t)ent) * npe;
+            for (int lb = tid; lb < npe; lb += GT) rr[lb] = rk[lb];
+        } else {
+            for (int lb = tid; lb < npe; lb += GT) {
+                const int r = find_rank(snb, deg, elements[e * npe + lb]);
+                bad |= r < 0;
+                rr[lb] = r < 0 ? 0 : r;
+            }
         }
         __syncthreads();
         const KT *Kb = Ke + e * 9 * npe2 + (int64_t)la * npe;
@@ -447,12 +454,12 @@ template <typename KT, int GTB>
 static int launch_gather(int32_t nb, size_t lds, const int32_t *elements, int npe, const int32_t *order,
                          const int32_t *node_elem_ptr, const int32_t *node_elem, const int32_t *nbr_ptr,
                          const int32_t *nbrs, const int32_t *indptr, const KT *Ke, const double *te, int deg,
-                         double *V, double *T, int32_t *status, hipStream_t st) {
+                         const int16_t *ranks, double *V, double *T, int32_t *status, hipStream_t st) {
     if (lds > 64 * 1024)
         FL_HIP_CHECK(hipFuncSetAttribute((const void *)gather_kernel<KT, GTB>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     gather_kernel<KT, GTB><<<(unsigned)nb, GTB, lds, st>>>(elements, npe, order, node_elem_ptr, node_elem, nbr_ptr,
-                                                           nbrs, indptr, Ke, te, deg, V, T, status);
+                                                           nbrs, indptr, Ke, te, deg, ranks, V, T, status);
     FL_LAUNCH_CHECK();
     return 0;
 }
@@ -554,8 +561,8 @@ int fl_df_assemble_ex(int32_t material, const double *params, const double *poin
                       size_t workspace_bytes, int32_t *status, void *stream) {
     return fl_df_assemble_mt(material, params, nullptr, nullptr, points, elements, Eulerx, Jm, AllGauss, G, nelem,
                              npe, nnode, update, node_elem_ptr, node_elem, node_nbr_ptr, node_nbrs, node_order,
-                             bucket_offsets, bucket_degree, indptr, V, T, precision, workspace, workspace_bytes,
-                             status, stream);
+                             bucket_offsets, bucket_degree, indptr, V, T, precision, nullptr, workspace,
+                             workspace_bytes, status, stream);
 }
 
 int fl_df_assemble_mt(int32_t material, const double *params, const int32_t *elem_mat, const double *mat_table,
@@ -564,7 +571,7 @@ int fl_df_assemble_mt(int32_t material, const double *params, const int32_t *ele
                       const int32_t *node_elem_ptr, const int32_t *node_elem, const int32_t *node_nbr_ptr,
                       const int32_t *node_nbrs, const int32_t *node_order, const int32_t *bucket_offsets,
                       const int32_t *bucket_degree, const int32_t *indptr, double *V, double *T, int32_t precision,
-                      void *workspace, size_t workspace_bytes, int32_t *status, void *stream) {
+                      const int16_t *ranks, void *workspace, size_t workspace_bytes, int32_t *status, void *stream) {
     hipStream_t st = as_stream(stream);
     if (!check_common(material, params, G, nelem, npe, nnode)) return FL_E_ARG;
     if ((elem_mat == nullptr) != (mat_table == nullptr)) {
@@ -636,17 +643,17 @@ int fl_df_assemble_mt(int32_t material, const double *params, const int32_t *ele
         if (precision == FL_PREC_FP64)
             rc2 = wide ? launch_gather<double, 256>(nb, lds, elements, npe, node_order + bucket_offsets[b],
                                                     node_elem_ptr, node_elem, node_nbr_ptr, node_nbrs, indptr,
-                                                    (const double *)w.Ke, w.te, bucket_degree[b], V, T, status, st)
+                                                    (const double *)w.Ke, w.te, bucket_degree[b], ranks, V, T, status, st)
                        : launch_gather<double, 64>(nb, lds, elements, npe, node_order + bucket_offsets[b],
                                                    node_elem_ptr, node_elem, node_nbr_ptr, node_nbrs, indptr,
-                                                   (const double *)w.Ke, w.te, bucket_degree[b], V, T, status, st);
+                                                   (const double *)w.Ke, w.te, bucket_degree[b], ranks, V, T, status, st);
         else
             rc2 = wide ? launch_gather<float, 256>(nb, lds, elements, npe, node_order + bucket_offsets[b],
                                                    node_elem_ptr, node_elem, node_nbr_ptr, node_nbrs, indptr,
-                                                   (const float *)w.Ke, w.te, bucket_degree[b], V, T, status, st)
+                                                   (const float *)w.Ke, w.te, bucket_degree[b], ranks, V, T, status, st)
                        : launch_gather<float, 64>(nb, lds, elements, npe, node_order + bucket_offsets[b],
                                                   node_elem_ptr, node_elem, node_nbr_ptr, node_nbrs, indptr,
-                                                  (const float *)w.Ke, w.te, bucket_degree[b], V, T, status, st);
+                                                  (const float *)w.Ke, w.te, bucket_degree[b], ranks, V, T, status, st);
         if (rc2) return rc2;
     }
     prof_mark(st);

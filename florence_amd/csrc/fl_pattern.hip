@@ -262,6 +262,29 @@ __global__ void i64_to_i32_kernel(const int64_t *__restrict__ src, int32_t *__re
     if (t < n) dst[t] = (int32_t)src[t];
 }
 
+// ranks[(e npe + la) npe + lb] = position of node elements[e][lb] in the sorted
+// neighbour list of node elements[e][la] (the CSR column block of that entry); status
+// bit FL_S_PATTERN if a node is missing or a list exceeds the int16 range.
+__global__ void rank_table_kernel(const int32_t *__restrict__ elements, int64_t nelem, int npe,
+                                  const int32_t *__restrict__ nbr_ptr, const int32_t *__restrict__ nbrs,
+                                  int16_t *__restrict__ ranks, int32_t *__restrict__ status) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t npe2 = (int64_t)npe * npe;
+    if (t >= nelem * npe2) return;
+    const int64_t e = t / npe2;
+    const int la = (int)((t - e * npe2) / npe), lb = (int)(t - e * npe2 - (int64_t)la * npe);
+    const int32_t a = elements[e * npe + la], b = elements[e * npe + lb];
+    const int32_t p0 = nbr_ptr[a], deg = nbr_ptr[a + 1] - p0;
+    int lo = 0, hi = deg;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (b < nbrs[p0 + mid]) hi = mid; else lo = mid;
+    }
+    const bool ok = deg > 0 && deg <= 32767 && nbrs[p0 + lo] == b;
+    if (!ok) atomicOr(status, FL_S_PATTERN);
+    ranks[t] = ok ? (int16_t)lo : (int16_t)0;
+}
+
 }  // namespace fl
 
 using namespace fl;
@@ -370,6 +393,23 @@ int fl_pattern_fill(const int32_t *elements, int64_t nelem, int32_t npe, int64_t
         set_error("fl_pattern_fill: neighbour lists inconsistent with fl_pattern_graph (flag %d)", h);
         return FL_E_ARG;
     }
+    return 0;
+}
+
+size_t fl_pattern_rank_table_bytes(int64_t nelem, int32_t npe) {
+    return nelem > 0 && npe > 0 ? sizeof(int16_t) * (size_t)nelem * npe * npe : 0;
+}
+
+int fl_pattern_rank_table(const int32_t *elements, int64_t nelem, int32_t npe, const int32_t *node_nbr_ptr,
+                          const int32_t *node_nbrs, int16_t *ranks, int32_t *status, void *stream) {
+    if (!elements || !node_nbr_ptr || !node_nbrs || !ranks || !status || nelem <= 0 || npe <= 0) {
+        set_error("fl_pattern_rank_table: invalid argument");
+        return FL_E_ARG;
+    }
+    const int64_t n = nelem * npe * (int64_t)npe;
+    rank_table_kernel<<<(unsigned)((n + 255) / 256), 256, 0, as_stream(stream)>>>(elements, nelem, npe, node_nbr_ptr,
+                                                                                 node_nbrs, ranks, status);
+    FL_LAUNCH_CHECK();
     return 0;
 }
 

@@ -118,20 +118,6 @@ __global__ void block_jacobi_kernel(const int32_t *__restrict__ indptr, const in
     for (int i = 0; i < 9; ++i) P[9 * a + i] = inv[i] / det;
 }
 
-// scalars s[]: 0 rz, 1 pq, 2 alpha, 3 rz_new, 4 beta, 5 rr, 6 bb
-__global__ void reduce_kernel(const double *__restrict__ partial, int64_t n, double *__restrict__ out) {
-    __shared__ double sh[SV_T];
-    double s = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += SV_T) s += partial[i];
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = SV_T / 2; off > 0; off >>= 1) {
-        if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) *out = sh[0];
-}
-
 __global__ void reduce2_kernel(const double *__restrict__ partial, int64_t n, double *__restrict__ out0,
                                double *__restrict__ out1) {
     __shared__ double sh0[SV_T], sh1[SV_T];
@@ -153,47 +139,6 @@ __global__ void reduce2_kernel(const double *__restrict__ partial, int64_t n, do
     if (threadIdx.x == 0) {
         *out0 = sh0[0];
         *out1 = sh1[0];
-    }
-}
-
-// x += alpha p; r -= alpha q; z = P r (node blocks); partials of (r.z, r.r)
-__global__ __launch_bounds__(SV_T) void cg_update_kernel(int64_t nnode, const double *__restrict__ s,
-                                                         const double *__restrict__ P, const double *__restrict__ p,
-                                                         const double *__restrict__ q, double *__restrict__ x,
-                                                         double *__restrict__ r, double *__restrict__ z,
-                                                         double *__restrict__ partial) {
-    __shared__ double sh0[SV_T], sh1[SV_T];
-    const int64_t a = (int64_t)blockIdx.x * SV_T + threadIdx.x;
-    double rz = 0.0, rr = 0.0;
-    if (a < nnode) {
-        const double alpha = s[2];
-        double rv[3];
-        for (int i = 0; i < 3; ++i) {
-            const int64_t k = 3 * a + i;
-            x[k] += alpha * p[k];
-            rv[i] = r[k] - alpha * q[k];
-            r[k] = rv[i];
-        }
-        for (int i = 0; i < 3; ++i) {
-            const double zi = P[9 * a + 3 * i] * rv[0] + P[9 * a + 3 * i + 1] * rv[1] + P[9 * a + 3 * i + 2] * rv[2];
-            z[3 * a + i] = zi;
-            rz += rv[i] * zi;
-            rr += rv[i] * rv[i];
-        }
-    }
-    sh0[threadIdx.x] = rz;
-    sh1[threadIdx.x] = rr;
-    __syncthreads();
-    for (int off = SV_T / 2; off > 0; off >>= 1) {
-        if (threadIdx.x < off) {
-            sh0[threadIdx.x] += sh0[threadIdx.x + off];
-            sh1[threadIdx.x] += sh1[threadIdx.x + off];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        partial[2 * blockIdx.x] = sh0[0];
-        partial[2 * blockIdx.x + 1] = sh1[0];
     }
 }
 
@@ -238,21 +183,160 @@ __global__ __launch_bounds__(SV_T) void cg_start_kernel(int64_t nnode, const dou
     }
 }
 
-__global__ void alpha_kernel(double *s) {
-    // s[1] = p.q ; alpha = rz / pq
-    s[2] = s[1] != 0.0 ? s[0] / s[1] : 0.0;
+// scalars s[]: 0 rz, 1 pq, 2 alpha, 3 rz_new, 4 beta, 5 rr, 6 bb
+
+template <int NT>
+__device__ inline void block_sum2(const double *__restrict__ part, int64_t n, int stride, double &o0, double &o1,
+                                  double *sh0, double *sh1) {
+    double a = 0.0, b = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += NT) {
+        a += part[stride * i];
+        if (stride == 2) b += part[2 * i + 1];
+    }
+    sh0[threadIdx.x] = a;
+    sh1[threadIdx.x] = b;
+    __syncthreads();
+    for (int off = NT / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            sh0[threadIdx.x] += sh0[threadIdx.x + off];
+            sh1[threadIdx.x] += sh1[threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    o0 = sh0[0];
+    o1 = sh1[0];
 }
 
-__global__ void beta_kernel(double *s) {
-    // s[3] = rz_new ; beta = rz_new / rz ; rz = rz_new
-    s[4] = s[0] != 0.0 ? s[3] / s[0] : 0.0;
-    s[0] = s[3];
+// One CG step, part 1: p_new = z + beta p_old (formed on the fly for every neighbour,
+// stored for the block's own rows), q = A p_new and per-block partials of p_new . q.
+// p_old and p_new are distinct buffers (other blocks still read p_old).
+__global__ __launch_bounds__(64 * SP_W) void pcg_spmv_kernel(const int32_t *__restrict__ indptr,
+                                                             const int32_t *__restrict__ nbr_ptr,
+                                                             const int32_t *__restrict__ nbrs,
+                                                             const double *__restrict__ V, int64_t nnode,
+                                                             const uint8_t *__restrict__ free_mask,
+                                                             const double *__restrict__ z,
+                                                             const double *__restrict__ p_old,
+                                                             double *__restrict__ p_new, double *__restrict__ q,
+                                                             double *__restrict__ part,
+                                                             const double *__restrict__ s) {
+    __shared__ double sh0[SP_W];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t a = (int64_t)blockIdx.x * SP_W + wave;
+    const double beta = s[4];
+    double d = 0.0;
+    if (a < nnode) {
+        const int32_t p0 = nbr_ptr[a], deg = nbr_ptr[a + 1] - p0;
+        const double *r0 = V + indptr[3 * a];
+        const double *r1 = V + indptr[3 * a + 1];
+        const double *r2 = V + indptr[3 * a + 2];
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        for (int r = lane; r < deg; r += 64) {
+            const int64_t b = nbrs[p0 + r];
+            double x0 = z[3 * b] + beta * p_old[3 * b];
+            double x1 = z[3 * b + 1] + beta * p_old[3 * b + 1];
+            double x2 = z[3 * b + 2] + beta * p_old[3 * b + 2];
+            if (free_mask) {
+                x0 = free_mask[3 * b] ? x0 : 0.0;
+                x1 = free_mask[3 * b + 1] ? x1 : 0.0;
+                x2 = free_mask[3 * b + 2] ? x2 : 0.0;
+            }
+            s0 += r0[3 * r] * x0 + r0[3 * r + 1] * x1 + r0[3 * r + 2] * x2;
+            s1 += r1[3 * r] * x0 + r1[3 * r + 1] * x1 + r1[3 * r + 2] * x2;
+            s2 += r2[3 * r] * x0 + r2[3 * r + 1] * x1 + r2[3 * r + 2] * x2;
+        }
+        s0 = wave_sum(s0);
+        s1 = wave_sum(s1);
+        s2 = wave_sum(s2);
+        if (lane == 0) {
+            const double out[3] = {s0, s1, s2};
+            for (int i = 0; i < 3; ++i) {
+                const int64_t row = 3 * a + i;
+                double pn = z[row] + beta * p_old[row];
+                const bool fixed = free_mask && !free_mask[row];
+                if (fixed) pn = 0.0;
+                p_new[row] = pn;
+                const double v = fixed ? pn : out[i];
+                q[row] = v;
+                d += v * pn;
+            }
+        }
+    }
+    if (lane == 0) sh0[wave] = d;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < SP_W; ++w) t += sh0[w];
+        part[blockIdx.x] = t;
+    }
 }
 
-__global__ void direction_kernel(int64_t n, const double *__restrict__ s, const double *__restrict__ z,
-                                 double *__restrict__ p) {
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) p[k] = z[k] + s[4] * p[k];
+// One workgroup: pq = sum of the partials (fixed order), alpha = rz / pq.
+__global__ __launch_bounds__(SV_T) void pcg_alpha_kernel(const double *__restrict__ part, int64_t n,
+                                                         double *__restrict__ s) {
+    __shared__ double sh0[SV_T], sh1[SV_T];
+    double pq, unused;
+    block_sum2<SV_T>(part, n, 1, pq, unused, sh0, sh1);
+    if (threadIdx.x == 0) {
+        s[1] = pq;
+        s[2] = pq != 0.0 ? s[0] / pq : 0.0;
+    }
+}
+
+// One workgroup: rz_new, rr from the partials of pcg_update_kernel; beta = rz_new / rz.
+__global__ __launch_bounds__(SV_T) void pcg_beta_kernel(const double *__restrict__ part, int64_t n,
+                                                        double *__restrict__ s) {
+    __shared__ double sh0[SV_T], sh1[SV_T];
+    double rzn, rrn;
+    block_sum2<SV_T>(part, n, 2, rzn, rrn, sh0, sh1);
+    if (threadIdx.x == 0) {
+        s[4] = s[0] != 0.0 ? rzn / s[0] : 0.0;
+        s[0] = rzn;
+        s[3] = rzn;
+        s[5] = rrn;
+    }
+}
+
+// One CG step, part 2: x += alpha p; r -= alpha q; z = P r; per-block partials of
+// (r.z, r.r).
+__global__ __launch_bounds__(SV_T) void pcg_update_kernel(int64_t nnode, const double *__restrict__ s,
+                                                          const double *__restrict__ P, const double *__restrict__ p,
+                                                          const double *__restrict__ q, double *__restrict__ x,
+                                                          double *__restrict__ r, double *__restrict__ z,
+                                                          double *__restrict__ part) {
+    __shared__ double sh0[SV_T], sh1[SV_T];
+    const int64_t a = (int64_t)blockIdx.x * SV_T + threadIdx.x;
+    double rz = 0.0, rr = 0.0;
+    if (a < nnode) {
+        const double alpha = s[2];
+        double rv[3];
+        for (int i = 0; i < 3; ++i) {
+            const int64_t k = 3 * a + i;
+            x[k] += alpha * p[k];
+            rv[i] = r[k] - alpha * q[k];
+            r[k] = rv[i];
+        }
+        for (int i = 0; i < 3; ++i) {
+            const double zi = P[9 * a + 3 * i] * rv[0] + P[9 * a + 3 * i + 1] * rv[1] + P[9 * a + 3 * i + 2] * rv[2];
+            z[3 * a + i] = zi;
+            rz += rv[i] * zi;
+            rr += rv[i] * rv[i];
+        }
+    }
+    sh0[threadIdx.x] = rz;
+    sh1[threadIdx.x] = rr;
+    __syncthreads();
+    for (int off = SV_T / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            sh0[threadIdx.x] += sh0[threadIdx.x + off];
+            sh1[threadIdx.x] += sh1[threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = sh0[0];
+        part[2 * blockIdx.x + 1] = sh1[0];
+    }
 }
 
 }  // namespace fl
@@ -277,7 +361,7 @@ int fl_csr_spmv(const int32_t *indptr, const int32_t *node_nbr_ptr, const int32_
 size_t fl_pcg_workspace_size(int64_t nnode) {
     const int64_t n = 3 * nnode;
     const int64_t nb_sp = (nnode + SP_W - 1) / SP_W, nb_v = (nnode + SV_T - 1) / SV_T;
-    return align_up(sizeof(double) * 9 * nnode, 256) + 4 * align_up(sizeof(double) * n, 256) +
+    return align_up(sizeof(double) * 9 * nnode, 256) + 5 * align_up(sizeof(double) * n, 256) +
            align_up(sizeof(double) * (nb_sp + 2 * nb_v + 8), 256) + align_up(sizeof(double) * 8, 256) + 1024;
 }
 
@@ -302,6 +386,7 @@ int fl_pcg(const int32_t *indptr, const int32_t *node_nbr_ptr, const int32_t *no
     double *q = cv.take<double>(n);
     double *part = cv.take<double>(nb_sp + 2 * nb_v + 8);
     double *s = cv.take<double>(8);
+    double *p2 = cv.take<double>(n);
     if (!cv.ok) {
         set_error("fl_pcg: workspace too small (%zu < %zu)", workspace_bytes, cv.off);
         return FL_E_SPACE;
@@ -328,15 +413,15 @@ int fl_pcg(const int32_t *indptr, const int32_t *node_nbr_ptr, const int32_t *no
     if (bnorm == 0.0) return 0;
     double rel = 1.0;
     int it = 0;
+    // four launches per iteration; s[4] (beta) is 0 before the first, so p = z there
+    double *pb[2] = {p, p2};
     while (it < maxit) {
-        spmv_kernel<<<(unsigned)nb_sp, 64 * SP_W, 0, st>>>(indptr, node_nbr_ptr, node_nbrs, V, nnode, free_mask, p, q,
-                                                           part);
-        reduce_kernel<<<1, SV_T, 0, st>>>(part, nb_sp, s + 1);
-        alpha_kernel<<<1, 1, 0, st>>>(s);
-        cg_update_kernel<<<(unsigned)nb_v, SV_T, 0, st>>>(nnode, s, P, p, q, x, r, z, part);
-        reduce2_kernel<<<1, SV_T, 0, st>>>(part, nb_v, s + 3, s + 5);
-        beta_kernel<<<1, 1, 0, st>>>(s);
-        direction_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, s, z, p);
+        double *p_old = pb[it & 1], *p_new = pb[(it + 1) & 1];
+        pcg_spmv_kernel<<<(unsigned)nb_sp, 64 * SP_W, 0, st>>>(indptr, node_nbr_ptr, node_nbrs, V, nnode, free_mask,
+                                                               z, p_old, p_new, q, part, s);
+        pcg_alpha_kernel<<<1, SV_T, 0, st>>>(part, nb_sp, s);
+        pcg_update_kernel<<<(unsigned)nb_v, SV_T, 0, st>>>(nnode, s, P, p_new, q, x, r, z, part + nb_sp);
+        pcg_beta_kernel<<<1, SV_T, 0, st>>>(part + nb_sp, nb_v, s);
         FL_LAUNCH_CHECK();
         ++it;
         if (it % check_every == 0 || it == maxit) {

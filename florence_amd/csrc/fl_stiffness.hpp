@@ -48,6 +48,30 @@ struct Mfma16<float> {
     __device__ static int row(int lk, int r) { return 4 * lk + r; }
 };
 
+// Triangular split of a 128-node diagonal block: wave W accumulates tile rows W and
+// 7 - W, columns from the row index on (9 tiles), over one chunk of KC inner indices.
+// acc[0..1][*] hold row W's tiles (column c at [c / 4][c % 4]), acc[2..3][*] row 7 - W's.
+template <int W, int KC, int LDA, typename MF, typename CT, typename acc_t>
+__device__ __forceinline__ void tri_chunk(const CT *__restrict__ A, const CT *__restrict__ Dc, int li, int lk,
+                                          acc_t (&acc)[4][4]) {
+    constexpr int R0 = W, R1 = 7 - W;
+#pragma unroll 1
+    for (int kk = 0; kk < KC / 4; ++kk) {
+        const int q = kk * 4 + lk;
+        const int gq = q / 3, j = q - 3 * gq;
+        const CT d0 = Dc[gq * 9 + 3 * j], d1 = Dc[gq * 9 + 3 * j + 1], d2 = Dc[gq * 9 + 3 * j + 2];
+        const CT *Gb = A + 3 * gq * LDA;
+        const CT a0 = A[q * LDA + R0 * 16 + li], a1 = A[q * LDA + R1 * 16 + li];
+#pragma unroll
+        for (int c = R0; c < 8; ++c) {
+            const int b = c * 16 + li;
+            const CT bf = d0 * Gb[b] + d1 * Gb[LDA + b] + d2 * Gb[2 * LDA + b];
+            acc[c / 4][c % 4] = MF::step(a0, bf, acc[c / 4][c % 4]);
+            if (c >= R1) acc[2 + c / 4][c % 4] = MF::step(a1, bf, acc[2 + c / 4][c % 4]);
+        }
+    }
+}
+
 // PS selects the block pairs of the launch: 0 = all six, 1 = the three off-diagonal
 // pairs (i < k), 2 = the three diagonal pairs (i = k).
 template <typename CT, typename KT, int NP, int MT, int NT, int WGM, int WGN, int MINW = 1, int PS = 0>
@@ -157,23 +181,13 @@ __global__ __launch_bounds__(64 * WGM *WGN, MINW) void stiffness_kernel(const do
         const CT *Dc = sD + buf * QC * 9;
         // not unrolled: each step already issues MT*NT long MFMAs, and hoisting the LDS
         // operands of several steps would cost the registers of a second wave per SIMD
-        if (tri) {
-#pragma unroll 1
-            for (int kk = 0; kk < KC / 4; ++kk) {
-                const int q = kk * 4 + lk;
-                const int gq = q / 3, j = q - 3 * gq;
-                const CT d0 = Dc[gq * 9 + 3 * j], d1 = Dc[gq * 9 + 3 * j + 1], d2 = Dc[gq * 9 + 3 * j + 2];
-                const CT *Gb = A + 3 * gq * LDA;
-                const CT a0 = A[q * LDA + R0 * 16 + li], a1 = A[q * LDA + R1 * 16 + li];
-#pragma unroll
-                for (int c = 0; c < 8; ++c) {
-                    if (c >= R0) {
-                        const int b = c * 16 + li;
-                        const CT bf = d0 * Gb[b] + d1 * Gb[LDA + b] + d2 * Gb[2 * LDA + b];
-                        acc[c / 4][c % 4] = MF::step(a0, bf, acc[c / 4][c % 4]);
-                        if (c >= R1) acc[2 + c / 4][c % 4] = MF::step(a1, bf, acc[2 + c / 4][c % 4]);
-                    }
-                }
+        if constexpr (TRI) {
+            // one straight-line body per wave (tile rows are compile-time there)
+            switch (wave) {
+                case 0: tri_chunk<0, KC, LDA, MF>(A, Dc, li, lk, acc); break;
+                case 1: tri_chunk<1, KC, LDA, MF>(A, Dc, li, lk, acc); break;
+                case 2: tri_chunk<2, KC, LDA, MF>(A, Dc, li, lk, acc); break;
+                default: tri_chunk<3, KC, LDA, MF>(A, Dc, li, lk, acc); break;
             }
         } else {
 #pragma unroll 1

@@ -24,18 +24,19 @@ Start modes of LockstepNewton.solve:
   "previous"  the reference's: u_{k-1} + Dirichlet increment, first step with the
               previous tangent on the predictor residual K[:, out] du_D;
   "extrap"    2 u_{k-1} - u_{k-2} (u_0 = u_{-1} = 0), full Newton from there;
-  "learned"   the MLP's prediction, unless the FP64 residual at the extrapolated
-              start is smaller (decided per member): the fallback.
+  "learned"   the MLP's prediction u_L, the extrapolation u_E, or the point
+              u_E + a* (u_L - u_E) that minimises the linearised residual along the
+              predicted correction -- whichever has the smallest FP64 residual, per
+              member (u_E is the fallback).
 Each Newton iteration of the warm-started modes assembles K and T at the current
 state, solves, and evaluates the residual-only FP64 kernel at the new state; K is
 assembled again only if another step is needed.
 
 Linear solves: ``forcing="fixed"`` solves every step to ``lin_rtol`` (the stand-in for
 the reference's direct solve); ``forcing="adaptive"`` is an inexact Newton method
-whose PCG tolerance per step follows the model |R_next| ~ C |R|^2 + eta |R| (C
-estimated from the previous steps): a step that can be the last is solved to
-0.5 tol / |R|, any other only to C |R|, capped at ``eta_max``.  Convergence is
-always judged on the FP64 residual.
+whose PCG tolerance per step follows the model |R_next| ~ max(C |R|^2, eta |R|), C
+measured on the first (tightly solved) step and refined by every later one (see
+``_eta``), capped at ``eta_max``.  Convergence is always judged on the FP64 residual.
 """
 import time
 
@@ -138,12 +139,19 @@ class LockstepNewton(object):
         self.lin.Solve(self.dm, self.V, self.free, rhs, du)
 
     def _eta(self, r, C):
-        """Linear tolerance of the next step at (worst member's) relative residual r:
-        with the step's result modelled as C r^2 + eta r, a step that can be the last
-        solves to 0.5 tol / r, any other step only to C r (its Newton error)."""
-        if self.forcing == "fixed":
+        """Linear tolerance of the next step at (worst member's) relative residual r,
+        with the step's result modelled as max(C r^2, eta r): a step that can be the
+        last solves to 0.5 tol / r; a step that can leave a residual from which one
+        more step finishes (r_t = sqrt(0.5 tol / C)) solves just to r_t / r; any other
+        step only to C r, its Newton error."""
+        if self.forcing == "fixed" or C is None:
             return self.lin_rtol
-        eta = 0.5 * self.tol / r if C * r * r <= 0.5 * self.tol else C * r
+        half = 0.5 * self.tol
+        if C * r * r <= half:
+            eta = half / r
+        else:
+            r_t = np.sqrt(half / C)
+            eta = r_t / r if C * r * r <= 0.5 * r_t else C * r
         return float(min(self.eta_max, max(self.lin_rtol, eta)))
 
     def solve(self, start="previous", predictor=None, record=False):
@@ -183,7 +191,7 @@ class LockstepNewton(object):
         history = []                                                  # worst |R| / R0 before/after each step
         traj = [np.zeros(n3)] if record else None
         failed = False
-        C = 1.0                                                       # quadratic-convergence constant estimate
+        C = None      # quadratic-convergence constant; the first step is solved tightly to measure it
         for k in range(1, nincr + 1):
             lam = k / float(nincr)
             ubc = self.u_total * (lam * fixed_f)
@@ -203,11 +211,26 @@ class LockstepNewton(object):
                     t_pred += timed(predictor.predict, self.X, u_prev, u_prev2, lam, self.free, ubc, uL)
                     t_res += timed(self._residual, uL, RL)
                     nL = self._norms(RL)
-                    use = nL < nE
-                    chose_learned[k - 1] = int(use.sum())
-                    u.copy_(torch.where(self._per_dof(use), uL, uE))
-                    R.copy_(torch.where(self._per_dof(use), RL, R))
-                    nS = torch.where(use, nL, nE)
+                    # step length along the predicted correction d = uL - uE: with R
+                    # linear in a small step, |R(uE + a d)| is least at
+                    # a* = -<R_E, R_L - R_E> / |R_L - R_E|^2 (per member, kept in [0, 1.5])
+                    dR = (RL - R).view(self.nmember, -1)
+                    den = (dR * dR).sum(1)
+                    a = torch.where(den > 0, -(R.view(self.nmember, -1) * dR).sum(1) / den, torch.zeros_like(den))
+                    a = a.clamp(0.0, 1.5)
+                    ua = uE + self._per_dof(a) * (uL - uE)
+                    Ra = torch.empty_like(R)
+                    t_res += timed(self._residual, ua, Ra)
+                    na = self._norms(Ra)
+                    # the FP64 residual decides, member by member: extrapolation,
+                    # prediction or the line-search point
+                    best = torch.stack([nE, nL, na]).argmin(0)
+                    chose_learned[k - 1] = int((best > 0).sum())
+                    pick = lambda x0, x1, x2: torch.where(self._per_dof(best == 1), x1,
+                                                          torch.where(self._per_dof(best == 2), x2, x0))
+                    u.copy_(pick(uE, uL, ua))
+                    R.copy_(pick(R, RL, Ra))
+                    nS = torch.stack([nE, nL, na]).gather(0, best[None])[0]
                 else:
                     u.copy_(uE)
                     nS = nE
@@ -240,8 +263,9 @@ class LockstepNewton(object):
                 nr = self._norms(R)
                 live = torch.as_tensor(~done, device=dev)
                 r_new = float((nr / R0)[live].max())
-                if eta * r_cur < 0.5 * r_new and r_cur > 0:
-                    C = min(1e4, max(1e-2, r_new / (r_cur * r_cur)))
+                if r_cur > 0:
+                    bound = max(1e-12, r_new / (r_cur * r_cur))   # C <= bound; = C when the solve was tight
+                    C = bound if (C is None or eta * r_cur <= 0.3 * r_new) else min(C, bound)
                 r_cur = float((nr / R0).max())
                 hk.append(r_cur)
                 c = (nr <= thresh).cpu().numpy()

@@ -215,6 +215,11 @@ class FEMSolver(object):
                     self.NormForces = 1e-14
             self.norm_residual = abs(self.abs_norm_residual / self.NormForces)
             self.NRConvergence["Increment_" + str(Increment)].append(self.norm_residual)
+            if self.report_log_level:
+                import sys
+                print("Increment {} Iteration {}: residual {:.3e} (abs {:.3e}), CG iterations {}".format(
+                    Increment, Iter, self.norm_residual, self.abs_norm_residual, self.linear_solver.iterations[-1]),
+                    file=sys.stderr, flush=True)
             if abs(self.abs_norm_residual) < tol:
                 break
             if Iter > 0 and float(torch.linalg.norm(dU)) <= self.newton_raphson_solution_tolerance:

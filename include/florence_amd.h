@@ -77,6 +77,17 @@ int fl_pattern_fill(const int32_t *elements_dev, int64_t nelem, int32_t npe, int
 int fl_pattern_gather_plan(const int32_t *node_nbr_ptr_dev, int64_t nnode, int32_t *node_order_dev,
                            int32_t *bucket_offsets, int32_t *bucket_degree, void *stream);
 
+/* Column-position table of the gather (no reference counterpart; the reference looks
+ * positions up per call through its data_global_indices maps, FEMSolver.py:1659-1684):
+ * ranks_dev[(e*npe + a)*npe + b] = position of element e's node b in the sorted
+ * neighbour list of its node a.  int16 (neighbour lists up to 32767); bytes from
+ * fl_pattern_rank_table_bytes.  status_dev bit FL_S_PATTERN on an inconsistent
+ * pattern (the caller zeroes it first).  Built once per mesh, like the pattern. */
+size_t fl_pattern_rank_table_bytes(int64_t nelem, int32_t npe);
+int fl_pattern_rank_table(const int32_t *elements_dev, int64_t nelem, int32_t npe,
+                          const int32_t *node_nbr_ptr_dev, const int32_t *node_nbrs_dev, int16_t *ranks_dev,
+                          int32_t *status_dev, void *stream);
+
 /* ------------------------------------------------- implicit DF assembly
  * Replaces _GlobalAssemblyDF_<Material> (FiniteElements/Assembly/_Assembly_/
  * _LowLevelAssemblyDF_.h:8-176) in its CSR mode (recompute_sparsity_pattern=0):
@@ -123,8 +134,10 @@ int fl_df_assemble_ex(int32_t material, const double *params, const double *poin
 /* Per-element materials (no reference counterpart: the reference assembles one
  * material per call).  elem_material_dev: int32[nelem] rows into material_table_dev:
  * f64[nmat][4] = {mu, mu1, mu2, lamb} for the law `material`; both NULL = params.
- * Used to assemble many meshes as one (batched form).  Issues no memset, so the call
- * can be captured into a HIP graph (the workspace must then stay fixed). */
+ * Used to assemble many meshes as one (batched form).  ranks_dev: the table of
+ * fl_pattern_rank_table, or NULL (the gather then searches the neighbour lists on
+ * every call; same result).  Issues no memset, so the call can be captured into a HIP
+ * graph (the workspace must then stay fixed). */
 int fl_df_assemble_mt(int32_t material, const double *params, const int32_t *elem_material_dev,
                       const double *material_table_dev, const double *points_dev, const int32_t *elements_dev,
                       const double *Eulerx_dev, const double *Jm_dev, const double *AllGauss_dev, int32_t ngauss,
@@ -132,8 +145,8 @@ int fl_df_assemble_mt(int32_t material, const double *params, const int32_t *ele
                       const int32_t *node_elem_ptr_dev, const int32_t *node_elem_dev,
                       const int32_t *node_nbr_ptr_dev, const int32_t *node_nbrs_dev, const int32_t *node_order_dev,
                       const int32_t *bucket_offsets, const int32_t *bucket_degree, const int32_t *indptr_dev,
-                      double *V_dev, double *T_dev, int32_t precision, void *workspace_dev, size_t workspace_bytes,
-                      int32_t *status_dev, void *stream);
+                      double *V_dev, double *T_dev, int32_t precision, const int16_t *ranks_dev,
+                      void *workspace_dev, size_t workspace_bytes, int32_t *status_dev, void *stream);
 
 /* --------------------------------------------- residual-only (explicit) pass
  * Replaces the mechanics branch of _GlobalAssemblyExplicit_DF_DPF_

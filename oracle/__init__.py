@@ -18,6 +18,9 @@ Contents
 * ``energy``    -- an independent check: total potential energy evaluated with torch
   float64 autograd, whose gradient and Hessian must equal the assembled residual and
   tangent.
+* ``mlp``       -- numpy restatement of the warm-start network's forward pass, MSE
+  gradient (with the BF16 operand rounding of the device products) and Adam step,
+  pinned by finite differences, torch's bfloat16 conversion and torch.optim.Adam.
 
 The reference itself cannot be compiled here (its kernels include the un-vendored
 Fastor headers and cblas.h), so there is no ``oracle/_ref`` build.

@@ -78,3 +78,35 @@ def test_residual_only_matches(cuda):
     T, Tfull = T.cpu().numpy(), Tfull.cpu().numpy()
     assert np.abs(T - Tr).max() / np.abs(Tr).max() <= RTOL
     assert np.abs(T - Tfull).max() / np.abs(Tr).max() <= RTOL
+
+
+@pytest.mark.parametrize("et,p,n", [("tet", 3, 2), ("hex", 4, 2)])
+def test_reference_entry_point_equals_rank_table_path(cuda, et, p, n):
+    """fl_df_assemble (the reference-shaped entry: no rank table, neighbour lists
+    searched per call) and the rank-table path give bitwise the same V and T."""
+    import torch
+    from florence_amd import _lib
+    from florence_amd.assembly import DeviceMeshData, assemble_device
+    mesh = cube_mesh(et, p, n, jitter=0.05, seed=p)
+    fs = function_space(mesh, p)
+    mat = MooneyRivlin(3, mu1=0.5, mu2=0.5, lamb=20.0)
+    x = deformed(mesh.points, 0.05, seed=p)
+    dm = DeviceMeshData(mesh, fs)
+    assert dm.ranks is not None
+    V1, T1, _ = assemble_device(dm, mat, x)
+    V1, T1 = V1.clone(), T1.clone()
+    lib = _lib.load()
+    V2 = torch.empty_like(V1)
+    T2 = torch.empty_like(T1)
+    xd = torch.as_tensor(np.ascontiguousarray(x), device="cuda")
+    ws = dm.workspace()
+    prm, _keep = _lib.double_array(mat.device_parameters())
+    _lib.check(lib.fl_df_assemble(mat.material_id, prm, _lib.ptr(dm.points), _lib.ptr(dm.elements), _lib.ptr(xd),
+                                  _lib.ptr(dm.Jm), _lib.ptr(dm.AllGauss), dm.ngauss, dm.nelem, dm.npe, dm.nnode, 1,
+                                  _lib.ptr(dm.node_elem_ptr), _lib.ptr(dm.node_elem), _lib.ptr(dm.node_nbr_ptr),
+                                  _lib.ptr(dm.node_nbrs), _lib.ptr(dm.node_order), dm.bucket_offsets,
+                                  dm.bucket_degree, _lib.ptr(dm.indptr), _lib.ptr(V2), _lib.ptr(T2), _lib.ptr(ws),
+                                  ws.numel(), _lib.ptr(dm.status), _lib.stream_ptr()), "fl_df_assemble")
+    torch.cuda.synchronize()
+    assert torch.equal(V1, V2) and torch.equal(T1, T2)
+    assert int(dm.status[0].item()) == 0
