@@ -17,11 +17,13 @@ FL_S_PATTERN = 4
 FL_GATHER_BUCKETS = 5
 FL_TUNE_STIFF_MINW = 1
 FL_TUNE_STIFF_VARIANT_BASE = 2
-STIFF_VARIANTS = {0: 1, 1: 2, 2: 1, 3: 2, 4: 2}   # padded node-count class -> number of tile variants
+STIFF_VARIANTS = {0: 1, 1: 2, 2: 1, 3: 2, 4: 3}   # padded node-count class -> number of tile variants
 FL_PREC_FP64 = 0
 FL_PREC_FP32 = 1
 FL_PREC_BF16 = 2
 PRECISIONS = {"fp64": FL_PREC_FP64, "fp32": FL_PREC_FP32, "bf16": FL_PREC_BF16}
+FL_PREC_CSR32 = 16
+FL_TUNE_STIFF_SF = 7
 
 # every symbol declared in include/florence_amd.h -> (restype, argtypes)
 _vp, _i32, _i64, _sz, _dp = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_size_t, ctypes.c_void_p
@@ -43,7 +45,8 @@ SIGNATURES = {
     "fl_df_workspace_size_ex": (_sz, [_i32, _i64, _i32, _i32]),
     "fl_df_assemble_mt": (ctypes.c_int, [_i32, _PD, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i64, _i32, _i64,
                                          _i32, _vp, _vp, _vp, _vp, _vp, _P32, _P32, _vp, _vp, _vp, _i32, _vp, _vp,
-                                         _sz, _vp, _vp]),
+                                         _vp, _sz, _vp, _vp]),
+    "fl_tensor_basis_bytes": (_sz, [_i32, _i32]),
     "fl_pattern_rank_table_bytes": (_sz, [_i64, _i32]),
     "fl_pattern_rank_table": (ctypes.c_int, [_vp, _i64, _i32, _vp, _vp, _vp, _vp, _vp]),
     "fl_df_assemble_ex": (ctypes.c_int, [_i32, _PD, _vp, _vp, _vp, _vp, _vp, _i32, _i64, _i32, _i64, _i32,
@@ -56,6 +59,9 @@ SIGNATURES = {
     "fl_pcg_workspace_size": (_sz, [_i64]),
     "fl_pcg": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, ctypes.c_double, _i32, _i32, _vp, _sz,
                               _P32, ctypes.POINTER(ctypes.c_double), _vp]),
+    "fl_csr_spmv_v32": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp]),
+    "fl_pcg_v32": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, ctypes.c_double, _i32, _i32, _vp, _sz,
+                                  _P32, ctypes.POINTER(ctypes.c_double), _vp]),
     "fl_profile_enable": (ctypes.c_int, [ctypes.c_int]),
     "fl_profile_read": (ctypes.c_int, [_vp, _vp, _i32, _i32]),
     "fl_mlp_param_count": (_i64, []),

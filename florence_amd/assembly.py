@@ -33,11 +33,58 @@ def _torch():
     return torch
 
 
+def tensor_basis_table(function_space, npe):
+    """The fl_tensor_basis table of a Lagrange hex under a tensor-product rule, as a
+    uint8 numpy array, or None when the element or the rule is not of that form.
+    The factorisation is verified against the function space's Jm."""
+    from .bases import LagrangeGaussLobatto1D, NodeArrangementHex
+    if getattr(function_space, "element_type", None) != "hex":
+        return None
+    P1 = int(round(npe ** (1.0 / 3.0)))
+    if P1 ** 3 != npe:
+        return None
+    z = np.asarray(function_space.quadrature.points, dtype=np.float64)
+    G = z.shape[0]
+    u = [np.unique(np.round(z[:, d], 13)) for d in range(3)]
+    Q = len(u[0])
+    if any(len(v) != Q for v in u) or Q ** 3 != G:
+        return None
+    qi = np.stack([np.searchsorted(u[d], np.round(z[:, d], 13)) for d in range(3)], axis=1)
+    g_of_t = np.full(G, -1, dtype=np.int32)
+    g_of_t[(qi[:, 2] * Q + qi[:, 1]) * Q + qi[:, 0]] = np.arange(G, dtype=np.int32)
+    if (g_of_t < 0).any():
+        return None
+    C = P1 - 2
+    f1d = np.zeros((3, 2, P1, Q))
+    for d in range(3):
+        for q in range(Q):
+            pts = z[qi[:, d] == q, d]
+            N, dN = LagrangeGaussLobatto1D(C, float(pts[0]))
+            f1d[d, 0, :, q] = N
+            f1d[d, 1, :, q] = dN
+    perm = NodeArrangementHex(C)
+    t = [perm % P1, (perm // P1) % P1, perm // (P1 * P1)]
+    Jm = np.asarray(function_space.Jm)
+    for m in range(3):
+        rec = np.ones((npe, G))
+        for d in range(3):
+            rec *= f1d[d, int(d == m)][t[d]][:, qi[:, d]]
+        if np.abs(rec - Jm[m]).max() > 1e-12 * max(1.0, np.abs(Jm[m]).max()):
+            return None
+    local_of_t = np.empty(npe, dtype=np.int32)
+    local_of_t[perm] = np.arange(npe, dtype=np.int32)
+    ints = [np.array([P1, Q, 0, 0], dtype=np.int32), g_of_t, local_of_t]
+    if (G + npe) % 2:
+        ints.append(np.zeros(1, dtype=np.int32))
+    return np.concatenate([np.concatenate(ints).view(np.uint8), f1d.ravel().view(np.uint8)])
+
+
 class DeviceMeshData(object):
     """Device-resident, per-(mesh, function space) inputs of the kernels: coordinates,
     int32 connectivity, basis tables, the node graph and the reference CSR pattern."""
 
-    def __init__(self, mesh, function_space, device="cuda", build_indices=True, build_ranks=True):
+    def __init__(self, mesh, function_space, device="cuda", build_indices=True, build_ranks=True,
+                 tensor_basis=True):
         torch = _torch()
         lib = _lib.load()
         self.device = torch.device(device)
@@ -98,6 +145,14 @@ class DeviceMeshData(object):
                 raise _lib.FlorenceAMDError("inconsistent sparsity pattern (rank table)")
         self._ws = None
         self._ws_res = None
+        # sum-factorised hex stiffness: the 1D bases of a tensor-product rule
+        self.tensor_basis = None
+        tb = tensor_basis_table(function_space, self.npe) if tensor_basis else None
+        if tb is not None:
+            if tb.size != int(lib.fl_tensor_basis_bytes(int(round(self.npe ** (1 / 3.))),
+                                                        int(round(self.ngauss ** (1 / 3.))))):
+                raise _lib.FlorenceAMDError("tensor basis table size mismatch")
+            self.tensor_basis = torch.as_tensor(tb, device=self.device)
 
     def workspace(self, residual_only=False):
         torch = _torch()
@@ -141,15 +196,19 @@ def _as_device_coords(Eulerx, device):
 
 
 def assemble_device(dm, material, Eulerx, V=None, T=None, requires_geometry_update=1, stream=None,
-                    precision="fp64"):
+                    precision="fp64", csr="fp64"):
     """Raw device assembly on DeviceMeshData: returns (V, T, status) torch tensors.
     ``precision`` selects the element-block contraction: "fp64" (the reference's
-    arithmetic) or "fp32" (FP32 MFMA, FP32 accumulation; T and V sums stay FP64)."""
+    arithmetic), "fp32" (FP32 MFMA) or "bf16" (BF16 MFMA), FP32 accumulation; T and the
+    V sums stay FP64.  ``csr="fp32"`` (with fp32/bf16) stores V as float32."""
     torch = _torch()
     lib = _lib.load()
     x = _as_device_coords(Eulerx, dm.device)
+    vdtype = torch.float32 if csr == "fp32" else torch.float64
     if V is None:
-        V = torch.empty(dm.nnz, dtype=torch.float64, device=dm.device)
+        V = torch.empty(dm.nnz, dtype=vdtype, device=dm.device)
+    if V.dtype != vdtype:
+        raise ValueError("V dtype {} does not match csr={}".format(V.dtype, csr))
     if T is None:
         T = torch.empty(3 * dm.nnode, dtype=torch.float64, device=dm.device)
     ws = dm.workspace()
@@ -157,14 +216,15 @@ def assemble_device(dm, material, Eulerx, V=None, T=None, requires_geometry_upda
     mid = MATERIAL_IDS.get(type(material).__name__)
     if mid is None:
         raise NotImplementedError("no low-level assembler for material {}".format(type(material).__name__))
-    prec = _lib.PRECISIONS[precision]
+    prec = _lib.PRECISIONS[precision] | (_lib.FL_PREC_CSR32 if csr == "fp32" else 0)
     _lib.check(lib.fl_df_assemble_mt(mid, prm, None, None, _lib.ptr(dm.points), _lib.ptr(dm.elements), _lib.ptr(x),
                                      _lib.ptr(dm.Jm), _lib.ptr(dm.AllGauss), dm.ngauss, dm.nelem, dm.npe, dm.nnode,
                                      int(requires_geometry_update), _lib.ptr(dm.node_elem_ptr),
                                      _lib.ptr(dm.node_elem), _lib.ptr(dm.node_nbr_ptr), _lib.ptr(dm.node_nbrs),
                                      _lib.ptr(dm.node_order), dm.bucket_offsets, dm.bucket_degree,
                                      _lib.ptr(dm.indptr), _lib.ptr(V), _lib.ptr(T), prec, _lib.ptr(dm.ranks),
-                                     _lib.ptr(ws), ws.numel(), _lib.ptr(dm.status), _lib.stream_ptr(stream)),
+                                     _lib.ptr(dm.tensor_basis), _lib.ptr(ws), ws.numel(), _lib.ptr(dm.status),
+                                     _lib.stream_ptr(stream)),
                "fl_df_assemble_mt")
     return V, T, dm.status
 

@@ -1,10 +1,12 @@
-"""Autotuner of the stiffness kernel's tile shape per (element type, p, points, precision).
+"""Autotuner of the stiffness kernel per (element type, p, points, precision).
 
 For every group of a BatchedAssembly it times each tile-shape variant the library
-offers for the group's padded node count (FL_TUNE_STIFF_VARIANT_BASE keys), using the
+offers for the group's padded node count (FL_TUNE_STIFF_VARIANT_BASE keys) and, for
+FP64 tensor-product hexes, the sum-factorised kernel (FL_TUNE_STIFF_SF), using the
 library's own per-phase HIP events, and keeps the fastest.  Results are cached as
-JSON keyed "tet-p2-G8-fp64" -> {"variant": v, "stiffness_ms": [...]}; the cache file is
-$FLORENCE_AMD_TUNING_CACHE or ~/.cache/florence_amd/tuning.json.
+JSON keyed "hex-p2-G27-fp64" -> {"variant": v, "sum_factorised": b, "stiffness_ms":
+[...]}; the cache file is $FLORENCE_AMD_TUNING_CACHE or
+~/.cache/florence_amd/tuning.json.
 """
 import json
 import os
@@ -37,7 +39,9 @@ def save_cache(cache, path=None):
 
 
 def tune_batch(batched, repeats=3, cache=None, force=False):
-    """Pick each group's variant (mutates batched.groups[*]['variant']); returns the cache."""
+    """Pick each group's stiffness kernel (mutates batched.groups[*]['variant'] and
+    ['sum_factorised']); returns the cache.  Candidates: every tile variant of the
+    dense contraction and, for FP64 tensor-product hexes, the sum-factorised kernel."""
     import torch
     from .batch import _np_class
     cache = dict(cache or {})
@@ -45,12 +49,16 @@ def tune_batch(batched, repeats=3, cache=None, force=False):
         key = batched.tuning_key(g["key"], g["fs"])
         npc = _np_class(g["dm"].npe)
         nvar = _lib.STIFF_VARIANTS[npc] if batched.precision != "bf16" else 1
-        if key in cache and not force:
+        cands = [(False, v) for v in range(nvar)]
+        if g["dm"].tensor_basis is not None and batched.precision == "fp64":
+            cands.append((True, 0))
+        if key in cache and not force and "sum_factorised" in cache[key]:
             g["variant"] = int(cache[key]["variant"])
+            g["sum_factorised"] = bool(cache[key]["sum_factorised"])
             continue
         times = []
-        for v in range(nvar):
-            g["variant"] = v
+        for sf, v in cands:
+            g["variant"], g["sum_factorised"] = v, sf
             batched._launch_group(g)            # warm-up (first-touch, kernel attributes)
             torch.cuda.synchronize()
             _lib.profile_enable(True)
@@ -62,6 +70,8 @@ def tune_batch(batched, repeats=3, cache=None, force=False):
             st = [c[1][1] for c in calls if c[0] == 1]
             times.append(float(np.median(st)))
         best = int(np.argmin(times))
-        g["variant"] = best
-        cache[key] = {"variant": best, "stiffness_ms": [round(t, 4) for t in times], "nelem": g["dm"].nelem}
+        g["sum_factorised"], g["variant"] = cands[best]
+        cache[key] = {"variant": int(cands[best][1]), "sum_factorised": bool(cands[best][0]),
+                      "candidates": ["sum-factorised" if sf else "dense v{}".format(v) for sf, v in cands],
+                      "stiffness_ms": [round(t, 4) for t in times], "nelem": g["dm"].nelem}
     return cache

@@ -141,7 +141,9 @@ class BatchedAssembly(object):
                 "x": torch.as_tensor(np.ascontiguousarray(cm.points), device=self.device).reshape(-1).clone(),
                 "V": torch.zeros(dm.nnz, dtype=torch.float64, device=self.device),
                 "T": torch.zeros(3 * dm.nnode, dtype=torch.float64, device=self.device),
-                "variant": int(tuning[tkey]["variant"]) if tuning and tkey in tuning else 0}
+                "variant": int(tuning[tkey]["variant"]) if tuning and tkey in tuning else 0,
+                "sum_factorised": bool(tuning[tkey].get("sum_factorised", True)) if tuning and tkey in tuning
+                else True}
 
     def tuning_key(self, key, fs):
         et, p = key
@@ -168,6 +170,8 @@ class BatchedAssembly(object):
         npc = _np_class(dm.npe)
         _lib.check(lib.fl_set_tuning(_lib.FL_TUNE_STIFF_VARIANT_BASE + npc, int(g["variant"]) + 1),
                    "fl_set_tuning")
+        _lib.check(lib.fl_set_tuning(_lib.FL_TUNE_STIFF_SF, 2 if g.get("sum_factorised", True) else 1),
+                   "fl_set_tuning")
         prm, _keep = _lib.double_array(g["params"])
         ws = self.workspace
         _lib.check(lib.fl_df_assemble_mt(g["material"], prm, _lib.ptr(g["elem_mat"]), _lib.ptr(g["table"]),
@@ -177,8 +181,8 @@ class BatchedAssembly(object):
                                          _lib.ptr(dm.node_nbr_ptr), _lib.ptr(dm.node_nbrs), _lib.ptr(dm.node_order),
                                          dm.bucket_offsets, dm.bucket_degree, _lib.ptr(dm.indptr),
                                          _lib.ptr(g["V"]), _lib.ptr(g["T"]), _lib.PRECISIONS[self.precision],
-                                         _lib.ptr(dm.ranks), _lib.ptr(ws), ws.numel(), _lib.ptr(dm.status),
-                                         _lib.stream_ptr()),
+                                         _lib.ptr(dm.ranks), _lib.ptr(dm.tensor_basis), _lib.ptr(ws), ws.numel(),
+                                         _lib.ptr(dm.status), _lib.stream_ptr()),
                    "fl_df_assemble_mt")
 
     def assemble(self):

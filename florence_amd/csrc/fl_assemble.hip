@@ -224,6 +224,7 @@ __global__ __launch_bounds__(256) void qp_kernel(MatParams mp0, const double *__
 }  // namespace fl
 #include "fl_stiffness.hpp"
 #include "fl_stiffness_bf16.hpp"
+#include "fl_stiffness_sf.hpp"
 namespace fl {
 
 // ------------------------------------------------------------------ phase 3
@@ -267,7 +268,7 @@ __device__ inline int find_rank(const int32_t *s, int n, int32_t v) {
 // so each launch sizes its LDS to its own degree class); lds_deg is that class's max.
 // GT = 64 (one wave) for nodes inside one element; GT = 256 for nodes shared by several
 // elements, whose per-element loop is the latency chain (rank search, loads, LDS adds).
-template <typename KT, int GT>
+template <typename KT, int GT, typename VT = double>
 __global__ __launch_bounds__(GT) void gather_kernel(const int32_t *__restrict__ elements, int npe,
                                                     const int32_t *__restrict__ node_order,
                                                     const int32_t *__restrict__ node_elem_ptr,
@@ -276,7 +277,7 @@ __global__ __launch_bounds__(GT) void gather_kernel(const int32_t *__restrict__ 
                                                     const int32_t *__restrict__ nbrs, const int32_t *__restrict__ indptr,
                                                     const KT *__restrict__ Ke, const double *__restrict__ te,
                                                     int lds_deg, const int16_t *__restrict__ ranks,
-                                                    double *__restrict__ V, double *__restrict__ T,
+                                                    VT *__restrict__ V, double *__restrict__ T,
                                                     int32_t *__restrict__ status) {
     extern __shared__ double smem[];
     const int64_t a = node_order[blockIdx.x];
@@ -291,11 +292,45 @@ __global__ __launch_bounds__(GT) void gather_kernel(const int32_t *__restrict__ 
         if (tid == 0) atomicOr(status, FL_S_PATTERN);
         return;
     }
+    const int32_t q0 = node_elem_ptr[a], q1 = node_elem_ptr[a + 1];
+    const int64_t npe2 = (int64_t)npe * npe;
+    if (q1 - q0 == 1 && deg == npe) {
+        // a node inside one element: its rows are that element's rows with the columns
+        // permuted -- no accumulation, each output is read once and written coalesced
+        // (as 0.0 + value, the accumulated form, so signed zeros match the sum path)
+        const int32_t ent = node_elem[q0];
+        const int64_t e = ent / npe;
+        const int la = ent - (int)(e * npe);
+        int32_t *inv = rr;   // position -> local column node (0 where a lookup failed)
+        bool bad1 = false;
+        for (int r = tid; r < npe; r += GT) inv[r] = 0;
+        __syncthreads();
+        if (ranks) {
+            const int16_t *rk = ranks + ((int64_t)ent) * npe;
+            for (int lb = tid; lb < npe; lb += GT) inv[rk[lb]] = lb;
+        } else {
+            for (int i = tid; i < deg; i += GT) snb[i] = nbrs[p0 + i];
+            __syncthreads();
+            for (int lb = tid; lb < npe; lb += GT) {
+                const int r = find_rank(snb, deg, elements[e * npe + lb]);
+                bad1 |= r < 0;
+                if (r >= 0) inv[r] = lb;
+            }
+        }
+        __syncthreads();
+        const KT *Kb = Ke + e * 9 * npe2 + (int64_t)la * npe;
+        for (int idx = tid; idx < 9 * deg; idx += GT) {
+            const int i = idx / rowlen, c = idx - i * rowlen;
+            const int r = c / 3, k = c - 3 * r;
+            V[(int64_t)indptr[3 * a + i] + c] = (VT)(0.0 + (double)Kb[(3 * i + k) * npe2 + inv[r]]);
+        }
+        if (tid < 3) T[3 * a + tid] = 0.0 + te[e * 3 * npe + 3 * la + tid];
+        if (bad1) atomicOr(status, 4);
+        return;
+    }
     for (int i = tid; i < 9 * deg; i += GT) acc[i] = 0.0;
     for (int i = tid; i < deg; i += GT) snb[i] = nbrs[p0 + i];
     __syncthreads();
-    const int32_t q0 = node_elem_ptr[a], q1 = node_elem_ptr[a + 1];
-    const int64_t npe2 = (int64_t)npe * npe;
     bool bad = false;
     for (int32_t q = q0; q < q1; ++q) {
         const int32_t ent = node_elem[q];
@@ -340,7 +375,7 @@ __global__ __launch_bounds__(GT) void gather_kernel(const int32_t *__restrict__ 
     }
     for (int idx = tid; idx < 9 * deg; idx += GT) {
         const int i = idx / rowlen, c = idx - i * rowlen;
-        V[(int64_t)indptr[3 * a + i] + c] = acc[idx];
+        V[(int64_t)indptr[3 * a + i] + c] = (VT)acc[idx];
     }
     if (tid < 3) {
         double s = 0.0;
@@ -417,10 +452,14 @@ static int launch_stiffness_t(int NP, const double *JmT, const double *qp, int n
             if (tuning_value(TUNE_STIFF_VARIANT_BASE + 4, 1) == 2 || tuning_value(TUNE_STIFF_MINW, 2) == 1) {
                 stiffness_kernel<CT, KT, 128, 4, 4, 2, 2, 1><<<grid, 256, 0, st>>>(JmT, qp, npe, G, nelem, Ke);
             } else {
-                // off-diagonal pairs as full 128 x 128 blocks, then the diagonal pairs
-                // as balanced upper triangles
+                // off-diagonal pairs as full 128 x 128 blocks (v0: 2 x 2 waves of 4 x 4
+                // tiles; v2: 1 x 4 waves of 8 x 2 tiles, fewer W operands per MFMA), then
+                // the diagonal pairs as balanced upper triangles
                 const dim3 half((unsigned)(groups / 2));
-                stiffness_kernel<CT, KT, 128, 4, 4, 2, 2, 2, 1><<<half, 256, 0, st>>>(JmT, qp, npe, G, nelem, Ke);
+                if (tuning_value(TUNE_STIFF_VARIANT_BASE + 4, 1) == 3)
+                    stiffness_kernel<CT, KT, 128, 8, 2, 1, 4, 2, 1><<<half, 256, 0, st>>>(JmT, qp, npe, G, nelem, Ke);
+                else
+                    stiffness_kernel<CT, KT, 128, 4, 4, 2, 2, 2, 1><<<half, 256, 0, st>>>(JmT, qp, npe, G, nelem, Ke);
                 stiffness_kernel<CT, KT, 128, 4, 4, 2, 2, 2, 2><<<half, 256, 0, st>>>(JmT, qp, npe, G, nelem, Ke);
             }
             break;
@@ -428,6 +467,37 @@ static int launch_stiffness_t(int NP, const double *JmT, const double *qp, int n
     }
     FL_LAUNCH_CHECK();
     return 0;
+}
+
+// Sum-factorised FP64 stiffness of tensor-product hexes (P1^3 nodes, Q^3 points).
+static int launch_stiffness_sf(int npe, int G, const double *qp, const void *basis, int64_t nelem, double *Ke,
+                               hipStream_t st) {
+    const int64_t groups = (nelem + 7) / 8 * 8 * 9;
+    if (groups > (int64_t)INT32_MAX) {
+        set_error("too many elements for one launch");
+        return FL_E_LIMIT;
+    }
+    const dim3 grid((unsigned)groups);
+    const int32_t *b = static_cast<const int32_t *>(basis);
+#define FL_SF_CASE(P, QQ)                                                                  \
+    if (npe == P * P * P && G == QQ * QQ * QQ) {                                           \
+        stiffness_sf_kernel<P, QQ><<<grid, 256, 0, st>>>(qp, b, nelem, Ke);                \
+        FL_LAUNCH_CHECK();                                                                 \
+        return 0;                                                                          \
+    }
+    FL_SF_CASE(2, 2)
+    FL_SF_CASE(3, 3)
+    FL_SF_CASE(4, 4)
+    FL_SF_CASE(5, 5)
+    FL_SF_CASE(5, 6)
+#undef FL_SF_CASE
+    set_error("no sum-factorised kernel for %d nodes and %d points", npe, G);
+    return FL_E_ARG;
+}
+
+static bool sf_shape(int npe, int G) {
+    return (npe == 8 && G == 8) || (npe == 27 && G == 27) || (npe == 64 && G == 64) || (npe == 125 && G == 125) ||
+           (npe == 125 && G == 216);
 }
 
 static int launch_stiffness_bf16(int NP, const double *JmT, const double *qp, int npe, int G, int64_t nelem,
@@ -450,16 +520,17 @@ static int launch_stiffness_bf16(int NP, const double *JmT, const double *qp, in
     return 0;
 }
 
-template <typename KT, int GTB>
+template <typename KT, int GTB, typename VT>
 static int launch_gather(int32_t nb, size_t lds, const int32_t *elements, int npe, const int32_t *order,
                          const int32_t *node_elem_ptr, const int32_t *node_elem, const int32_t *nbr_ptr,
                          const int32_t *nbrs, const int32_t *indptr, const KT *Ke, const double *te, int deg,
-                         const int16_t *ranks, double *V, double *T, int32_t *status, hipStream_t st) {
+                         const int16_t *ranks, VT *V, double *T, int32_t *status, hipStream_t st) {
     if (lds > 64 * 1024)
-        FL_HIP_CHECK(hipFuncSetAttribute((const void *)gather_kernel<KT, GTB>,
+        FL_HIP_CHECK(hipFuncSetAttribute((const void *)gather_kernel<KT, GTB, VT>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    gather_kernel<KT, GTB><<<(unsigned)nb, GTB, lds, st>>>(elements, npe, order, node_elem_ptr, node_elem, nbr_ptr,
-                                                           nbrs, indptr, Ke, te, deg, ranks, V, T, status);
+    gather_kernel<KT, GTB, VT><<<(unsigned)nb, GTB, lds, st>>>(elements, npe, order, node_elem_ptr, node_elem,
+                                                               nbr_ptr, nbrs, indptr, Ke, te, deg, ranks, V, T,
+                                                               status);
     FL_LAUNCH_CHECK();
     return 0;
 }
@@ -533,6 +604,13 @@ size_t fl_df_residual_workspace_size(int32_t ngauss, int64_t nelem, int32_t npe)
     return need;
 }
 
+size_t fl_tensor_basis_bytes(int32_t p1, int32_t q) {
+    if (p1 <= 0 || q <= 0) return 0;
+    const int64_t q3 = (int64_t)q * q * q, n3 = (int64_t)p1 * p1 * p1;
+    const int64_t ints = 4 + q3 + n3 + ((q3 + n3) % 2);
+    return (size_t)(4 * ints + 8 * 3 * 2 * p1 * q);
+}
+
 size_t fl_df_workspace_size_ex(int32_t ngauss, int64_t nelem, int32_t npe, int32_t precision) {
     if (padded_np(npe) < 0 || precision < FL_PREC_FP64 || precision > FL_PREC_BF16) return 0;
     size_t need = 0;
@@ -561,7 +639,7 @@ int fl_df_assemble_ex(int32_t material, const double *params, const double *poin
                       size_t workspace_bytes, int32_t *status, void *stream) {
     return fl_df_assemble_mt(material, params, nullptr, nullptr, points, elements, Eulerx, Jm, AllGauss, G, nelem,
                              npe, nnode, update, node_elem_ptr, node_elem, node_nbr_ptr, node_nbrs, node_order,
-                             bucket_offsets, bucket_degree, indptr, V, T, precision, nullptr, workspace,
+                             bucket_offsets, bucket_degree, indptr, V, T, precision, nullptr, nullptr, workspace,
                              workspace_bytes, status, stream);
 }
 
@@ -571,15 +649,19 @@ int fl_df_assemble_mt(int32_t material, const double *params, const int32_t *ele
                       const int32_t *node_elem_ptr, const int32_t *node_elem, const int32_t *node_nbr_ptr,
                       const int32_t *node_nbrs, const int32_t *node_order, const int32_t *bucket_offsets,
                       const int32_t *bucket_degree, const int32_t *indptr, double *V, double *T, int32_t precision,
-                      const int16_t *ranks, void *workspace, size_t workspace_bytes, int32_t *status, void *stream) {
+                      const int16_t *ranks, const void *tensor_basis, void *workspace, size_t workspace_bytes,
+                      int32_t *status, void *stream) {
     hipStream_t st = as_stream(stream);
     if (!check_common(material, params, G, nelem, npe, nnode)) return FL_E_ARG;
     if ((elem_mat == nullptr) != (mat_table == nullptr)) {
         set_error("fl_df_assemble_mt: elem_material and material_table go together");
         return FL_E_ARG;
     }
-    if (precision < FL_PREC_FP64 || precision > FL_PREC_BF16) {
-        set_error("fl_df_assemble_ex: unknown tangent precision %d", precision);
+    // FL_PREC_CSR32 (with an FP32/BF16 contraction): V is written as float[nnz]
+    const bool v32 = (precision & FL_PREC_CSR32) != 0;
+    precision &= ~FL_PREC_CSR32;
+    if (precision < FL_PREC_FP64 || precision > FL_PREC_BF16 || (v32 && precision == FL_PREC_FP64)) {
+        set_error("fl_df_assemble_mt: unsupported precision %d%s", precision, v32 ? " with FL_PREC_CSR32" : "");
         return FL_E_ARG;
     }
     if (!points || !elements || !Eulerx || !Jm || !AllGauss || !node_elem_ptr || !node_elem || !node_nbr_ptr ||
@@ -615,7 +697,10 @@ int fl_df_assemble_mt(int32_t material, const double *params, const int32_t *ele
                                                                    npe, update, w.qp, status, elem_mat, mat_table);
     FL_LAUNCH_CHECK();
     prof_mark(st);
-    int rc = precision == FL_PREC_FP64
+    const bool use_sf = precision == FL_PREC_FP64 && tensor_basis != nullptr && sf_shape(npe, G) &&
+                        tuning_value(TUNE_STIFF_SF, 0) != 1;
+    int rc = use_sf ? launch_stiffness_sf(npe, G, w.qp, tensor_basis, nelem, (double *)w.Ke, st)
+           : precision == FL_PREC_FP64
                  ? launch_stiffness_t<double, double>(NP, w.JmT, w.qp, npe, G, nelem, (double *)w.Ke, st)
                  : precision == FL_PREC_FP32
                        ? launch_stiffness_t<float, float>(NP, w.JmT, w.qp, npe, G, nelem, (float *)w.Ke, st)
@@ -640,20 +725,29 @@ int fl_df_assemble_mt(int32_t material, const double *params, const int32_t *ele
         // the lowest class (nodes interior to one element) keeps one wave per node
         const bool wide = b > 0;
         int rc2 = 0;
+        const int32_t *ord = node_order + bucket_offsets[b];
+        const int dg = bucket_degree[b];
         if (precision == FL_PREC_FP64)
-            rc2 = wide ? launch_gather<double, 256>(nb, lds, elements, npe, node_order + bucket_offsets[b],
-                                                    node_elem_ptr, node_elem, node_nbr_ptr, node_nbrs, indptr,
-                                                    (const double *)w.Ke, w.te, bucket_degree[b], ranks, V, T, status, st)
-                       : launch_gather<double, 64>(nb, lds, elements, npe, node_order + bucket_offsets[b],
-                                                   node_elem_ptr, node_elem, node_nbr_ptr, node_nbrs, indptr,
-                                                   (const double *)w.Ke, w.te, bucket_degree[b], ranks, V, T, status, st);
+            rc2 = wide ? launch_gather<double, 256>(nb, lds, elements, npe, ord, node_elem_ptr, node_elem, node_nbr_ptr,
+                                                    node_nbrs, indptr, (const double *)w.Ke, w.te, dg, ranks, V, T,
+                                                    status, st)
+                       : launch_gather<double, 64>(nb, lds, elements, npe, ord, node_elem_ptr, node_elem, node_nbr_ptr,
+                                                   node_nbrs, indptr, (const double *)w.Ke, w.te, dg, ranks, V, T,
+                                                   status, st);
+        else if (!v32)
+            rc2 = wide ? launch_gather<float, 256>(nb, lds, elements, npe, ord, node_elem_ptr, node_elem, node_nbr_ptr,
+                                                   node_nbrs, indptr, (const float *)w.Ke, w.te, dg, ranks, V, T,
+                                                   status, st)
+                       : launch_gather<float, 64>(nb, lds, elements, npe, ord, node_elem_ptr, node_elem, node_nbr_ptr,
+                                                  node_nbrs, indptr, (const float *)w.Ke, w.te, dg, ranks, V, T,
+                                                  status, st);
         else
-            rc2 = wide ? launch_gather<float, 256>(nb, lds, elements, npe, node_order + bucket_offsets[b],
-                                                   node_elem_ptr, node_elem, node_nbr_ptr, node_nbrs, indptr,
-                                                   (const float *)w.Ke, w.te, bucket_degree[b], ranks, V, T, status, st)
-                       : launch_gather<float, 64>(nb, lds, elements, npe, node_order + bucket_offsets[b],
-                                                  node_elem_ptr, node_elem, node_nbr_ptr, node_nbrs, indptr,
-                                                  (const float *)w.Ke, w.te, bucket_degree[b], ranks, V, T, status, st);
+            rc2 = wide ? launch_gather<float, 256>(nb, lds, elements, npe, ord, node_elem_ptr, node_elem, node_nbr_ptr,
+                                                   node_nbrs, indptr, (const float *)w.Ke, w.te, dg, ranks,
+                                                   reinterpret_cast<float *>(V), T, status, st)
+                       : launch_gather<float, 64>(nb, lds, elements, npe, ord, node_elem_ptr, node_elem, node_nbr_ptr,
+                                                  node_nbrs, indptr, (const float *)w.Ke, w.te, dg, ranks,
+                                                  reinterpret_cast<float *>(V), T, status, st);
         if (rc2) return rc2;
     }
     prof_mark(st);

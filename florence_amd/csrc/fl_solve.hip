@@ -26,10 +26,11 @@ __device__ inline double wave_sum(double v) {
 
 // y = A x (masked when free != nullptr).  One wave per node a, its three rows share the
 // neighbour list; lanes stride over neighbours.  Optional: partial of dot(x, y).
+template <typename VT>
 __global__ __launch_bounds__(64 * SP_W) void spmv_kernel(const int32_t *__restrict__ indptr,
                                                          const int32_t *__restrict__ nbr_ptr,
                                                          const int32_t *__restrict__ nbrs,
-                                                         const double *__restrict__ V, int64_t nnode,
+                                                         const VT *__restrict__ V, int64_t nnode,
                                                          const uint8_t *__restrict__ free_mask,
                                                          const double *__restrict__ x, double *__restrict__ y,
                                                          double *__restrict__ dot_partial) {
@@ -39,9 +40,9 @@ __global__ __launch_bounds__(64 * SP_W) void spmv_kernel(const int32_t *__restri
     double d = 0.0;
     if (a < nnode) {
         const int32_t p0 = nbr_ptr[a], deg = nbr_ptr[a + 1] - p0;
-        const double *r0 = V + indptr[3 * a];
-        const double *r1 = V + indptr[3 * a + 1];
-        const double *r2 = V + indptr[3 * a + 2];
+        const VT *r0 = V + indptr[3 * a];
+        const VT *r1 = V + indptr[3 * a + 1];
+        const VT *r2 = V + indptr[3 * a + 2];
         double s0 = 0.0, s1 = 0.0, s2 = 0.0;
         for (int r = lane; r < deg; r += 64) {
             const int64_t b = nbrs[p0 + r];
@@ -51,9 +52,9 @@ __global__ __launch_bounds__(64 * SP_W) void spmv_kernel(const int32_t *__restri
                 x1 = free_mask[3 * b + 1] ? x1 : 0.0;
                 x2 = free_mask[3 * b + 2] ? x2 : 0.0;
             }
-            s0 += r0[3 * r] * x0 + r0[3 * r + 1] * x1 + r0[3 * r + 2] * x2;
-            s1 += r1[3 * r] * x0 + r1[3 * r + 1] * x1 + r1[3 * r + 2] * x2;
-            s2 += r2[3 * r] * x0 + r2[3 * r + 1] * x1 + r2[3 * r + 2] * x2;
+            s0 += (double)r0[3 * r] * x0 + (double)r0[3 * r + 1] * x1 + (double)r0[3 * r + 2] * x2;
+            s1 += (double)r1[3 * r] * x0 + (double)r1[3 * r + 1] * x1 + (double)r1[3 * r + 2] * x2;
+            s2 += (double)r2[3 * r] * x0 + (double)r2[3 * r + 1] * x1 + (double)r2[3 * r + 2] * x2;
         }
         s0 = wave_sum(s0);
         s1 = wave_sum(s1);
@@ -80,8 +81,9 @@ __global__ __launch_bounds__(64 * SP_W) void spmv_kernel(const int32_t *__restri
 }
 
 // Per-node inverse of the masked 3x3 diagonal block (identity rows/cols on fixed dofs).
+template <typename VT>
 __global__ void block_jacobi_kernel(const int32_t *__restrict__ indptr, const int32_t *__restrict__ nbr_ptr,
-                                    const int32_t *__restrict__ nbrs, const double *__restrict__ V, int64_t nnode,
+                                    const int32_t *__restrict__ nbrs, const VT *__restrict__ V, int64_t nnode,
                                     const uint8_t *__restrict__ free_mask, double *__restrict__ P,
                                     int32_t *__restrict__ bad) {
     const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -97,7 +99,7 @@ __global__ void block_jacobi_kernel(const int32_t *__restrict__ indptr, const in
         for (int k = 0; k < 3; ++k) {
             const bool fi = free_mask ? free_mask[3 * a + i] != 0 : true;
             const bool fk = free_mask ? free_mask[3 * a + k] != 0 : true;
-            B[3 * i + k] = (fi && fk) ? V[indptr[3 * a + i] + 3 * lo + k] : (i == k ? 1.0 : 0.0);
+            B[3 * i + k] = (fi && fk) ? (double)V[indptr[3 * a + i] + 3 * lo + k] : (i == k ? 1.0 : 0.0);
         }
     double inv[9];
     inv[0] = B[4] * B[8] - B[5] * B[7];
@@ -210,10 +212,11 @@ __device__ inline void block_sum2(const double *__restrict__ part, int64_t n, in
 // One CG step, part 1: p_new = z + beta p_old (formed on the fly for every neighbour,
 // stored for the block's own rows), q = A p_new and per-block partials of p_new . q.
 // p_old and p_new are distinct buffers (other blocks still read p_old).
+template <typename VT>
 __global__ __launch_bounds__(64 * SP_W) void pcg_spmv_kernel(const int32_t *__restrict__ indptr,
                                                              const int32_t *__restrict__ nbr_ptr,
                                                              const int32_t *__restrict__ nbrs,
-                                                             const double *__restrict__ V, int64_t nnode,
+                                                             const VT *__restrict__ V, int64_t nnode,
                                                              const uint8_t *__restrict__ free_mask,
                                                              const double *__restrict__ z,
                                                              const double *__restrict__ p_old,
@@ -227,9 +230,9 @@ __global__ __launch_bounds__(64 * SP_W) void pcg_spmv_kernel(const int32_t *__re
     double d = 0.0;
     if (a < nnode) {
         const int32_t p0 = nbr_ptr[a], deg = nbr_ptr[a + 1] - p0;
-        const double *r0 = V + indptr[3 * a];
-        const double *r1 = V + indptr[3 * a + 1];
-        const double *r2 = V + indptr[3 * a + 2];
+        const VT *r0 = V + indptr[3 * a];
+        const VT *r1 = V + indptr[3 * a + 1];
+        const VT *r2 = V + indptr[3 * a + 2];
         double s0 = 0.0, s1 = 0.0, s2 = 0.0;
         for (int r = lane; r < deg; r += 64) {
             const int64_t b = nbrs[p0 + r];
@@ -241,9 +244,9 @@ __global__ __launch_bounds__(64 * SP_W) void pcg_spmv_kernel(const int32_t *__re
                 x1 = free_mask[3 * b + 1] ? x1 : 0.0;
                 x2 = free_mask[3 * b + 2] ? x2 : 0.0;
             }
-            s0 += r0[3 * r] * x0 + r0[3 * r + 1] * x1 + r0[3 * r + 2] * x2;
-            s1 += r1[3 * r] * x0 + r1[3 * r + 1] * x1 + r1[3 * r + 2] * x2;
-            s2 += r2[3 * r] * x0 + r2[3 * r + 1] * x1 + r2[3 * r + 2] * x2;
+            s0 += (double)r0[3 * r] * x0 + (double)r0[3 * r + 1] * x1 + (double)r0[3 * r + 2] * x2;
+            s1 += (double)r1[3 * r] * x0 + (double)r1[3 * r + 1] * x1 + (double)r1[3 * r + 2] * x2;
+            s2 += (double)r2[3 * r] * x0 + (double)r2[3 * r + 1] * x1 + (double)r2[3 * r + 2] * x2;
         }
         s0 = wave_sum(s0);
         s1 = wave_sum(s1);
@@ -339,40 +342,30 @@ __global__ __launch_bounds__(SV_T) void pcg_update_kernel(int64_t nnode, const d
     }
 }
 
-}  // namespace fl
-
-using namespace fl;
-
-extern "C" {
-
-int fl_csr_spmv(const int32_t *indptr, const int32_t *node_nbr_ptr, const int32_t *node_nbrs, const double *V,
-                int64_t nnode, const uint8_t *free_mask, const double *x, double *y, void *stream) {
+template <typename VT>
+static int csr_spmv(const int32_t *indptr, const int32_t *node_nbr_ptr, const int32_t *node_nbrs, const VT *V,
+                    int64_t nnode, const uint8_t *free_mask, const double *x, double *y, void *stream,
+                    const char *name) {
     if (!indptr || !node_nbr_ptr || !node_nbrs || !V || !x || !y || nnode <= 0) {
-        set_error("fl_csr_spmv: invalid argument");
+        set_error("%s: invalid argument", name);
         return FL_E_ARG;
     }
     hipStream_t st = as_stream(stream);
-    spmv_kernel<<<(unsigned)((nnode + SP_W - 1) / SP_W), 64 * SP_W, 0, st>>>(indptr, node_nbr_ptr, node_nbrs, V,
-                                                                            nnode, free_mask, x, y, nullptr);
+    spmv_kernel<VT><<<(unsigned)((nnode + SP_W - 1) / SP_W), 64 * SP_W, 0, st>>>(indptr, node_nbr_ptr, node_nbrs,
+                                                                                V, nnode, free_mask, x, y, nullptr);
     FL_LAUNCH_CHECK();
     return 0;
 }
 
-size_t fl_pcg_workspace_size(int64_t nnode) {
-    const int64_t n = 3 * nnode;
-    const int64_t nb_sp = (nnode + SP_W - 1) / SP_W, nb_v = (nnode + SV_T - 1) / SV_T;
-    return align_up(sizeof(double) * 9 * nnode, 256) + 5 * align_up(sizeof(double) * n, 256) +
-           align_up(sizeof(double) * (nb_sp + 2 * nb_v + 8), 256) + align_up(sizeof(double) * 8, 256) + 1024;
-}
-
-int fl_pcg(const int32_t *indptr, const int32_t *node_nbr_ptr, const int32_t *node_nbrs, const double *V,
-           int64_t nnode, const uint8_t *free_mask, const double *b, double *x, double rtol, int32_t maxit,
-           int32_t check_every, void *workspace, size_t workspace_bytes, int32_t *iterations, double *rel_residual,
-           void *stream) {
+template <typename VT>
+static int pcg(const int32_t *indptr, const int32_t *node_nbr_ptr, const int32_t *node_nbrs, const VT *V,
+               int64_t nnode, const uint8_t *free_mask, const double *b, double *x, double rtol, int32_t maxit,
+               int32_t check_every, void *workspace, size_t workspace_bytes, int32_t *iterations,
+               double *rel_residual, void *stream, const char *name) {
     hipStream_t st = as_stream(stream);
     if (!indptr || !node_nbr_ptr || !node_nbrs || !V || !b || !x || nnode <= 0 || maxit <= 0 || !iterations ||
         !rel_residual) {
-        set_error("fl_pcg: invalid argument");
+        set_error("%s: invalid argument", name);
         return FL_E_ARG;
     }
     if (check_every <= 0) check_every = 1;
@@ -388,12 +381,12 @@ int fl_pcg(const int32_t *indptr, const int32_t *node_nbr_ptr, const int32_t *no
     double *s = cv.take<double>(8);
     double *p2 = cv.take<double>(n);
     if (!cv.ok) {
-        set_error("fl_pcg: workspace too small (%zu < %zu)", workspace_bytes, cv.off);
+        set_error("%s: workspace too small (%zu < %zu)", name, workspace_bytes, cv.off);
         return FL_E_SPACE;
     }
     int32_t *bad = reinterpret_cast<int32_t *>(s + 7);
     FL_HIP_CHECK(hipMemsetAsync(s, 0, sizeof(double) * 8, st));
-    block_jacobi_kernel<<<(unsigned)((nnode + 255) / 256), 256, 0, st>>>(indptr, node_nbr_ptr, node_nbrs, V, nnode,
+    block_jacobi_kernel<VT><<<(unsigned)((nnode + 255) / 256), 256, 0, st>>>(indptr, node_nbr_ptr, node_nbrs, V, nnode,
                                                                         free_mask, P, bad);
     FL_LAUNCH_CHECK();
     cg_start_kernel<<<(unsigned)nb_v, SV_T, 0, st>>>(nnode, P, b, free_mask, x, r, z, p, part);
@@ -404,7 +397,7 @@ int fl_pcg(const int32_t *indptr, const int32_t *node_nbr_ptr, const int32_t *no
     FL_HIP_CHECK(hipMemcpyAsync(hs, s, sizeof(hs), hipMemcpyDeviceToHost, st));
     FL_HIP_CHECK(hipStreamSynchronize(st));
     if (*reinterpret_cast<int32_t *>(&hs[7])) {
-        set_error("fl_pcg: singular diagonal block or missing diagonal entry");
+        set_error("%s: singular diagonal block or missing diagonal entry", name);
         return FL_E_ARG;
     }
     const double bnorm = std::sqrt(hs[6]);
@@ -417,8 +410,8 @@ int fl_pcg(const int32_t *indptr, const int32_t *node_nbr_ptr, const int32_t *no
     double *pb[2] = {p, p2};
     while (it < maxit) {
         double *p_old = pb[it & 1], *p_new = pb[(it + 1) & 1];
-        pcg_spmv_kernel<<<(unsigned)nb_sp, 64 * SP_W, 0, st>>>(indptr, node_nbr_ptr, node_nbrs, V, nnode, free_mask,
-                                                               z, p_old, p_new, q, part, s);
+        pcg_spmv_kernel<VT><<<(unsigned)nb_sp, 64 * SP_W, 0, st>>>(indptr, node_nbr_ptr, node_nbrs, V, nnode,
+                                                                   free_mask, z, p_old, p_new, q, part, s);
         pcg_alpha_kernel<<<1, SV_T, 0, st>>>(part, nb_sp, s);
         pcg_update_kernel<<<(unsigned)nb_v, SV_T, 0, st>>>(nnode, s, P, p_new, q, x, r, z, part + nb_sp);
         pcg_beta_kernel<<<1, SV_T, 0, st>>>(part + nb_sp, nb_v, s);
@@ -436,6 +429,46 @@ int fl_pcg(const int32_t *indptr, const int32_t *node_nbr_ptr, const int32_t *no
     *iterations = it;
     *rel_residual = rel;
     return 0;
+}
+
+
+}  // namespace fl
+
+using namespace fl;
+
+extern "C" {
+
+size_t fl_pcg_workspace_size(int64_t nnode) {
+    const int64_t n = 3 * nnode;
+    const int64_t nb_sp = (nnode + SP_W - 1) / SP_W, nb_v = (nnode + SV_T - 1) / SV_T;
+    return align_up(sizeof(double) * 9 * nnode, 256) + 5 * align_up(sizeof(double) * n, 256) +
+           align_up(sizeof(double) * (nb_sp + 2 * nb_v + 8), 256) + align_up(sizeof(double) * 8, 256) + 1024;
+}
+
+int fl_csr_spmv(const int32_t *indptr, const int32_t *node_nbr_ptr, const int32_t *node_nbrs, const double *V,
+                int64_t nnode, const uint8_t *free_mask, const double *x, double *y, void *stream) {
+    return csr_spmv<double>(indptr, node_nbr_ptr, node_nbrs, V, nnode, free_mask, x, y, stream, "fl_csr_spmv");
+}
+
+int fl_csr_spmv_v32(const int32_t *indptr, const int32_t *node_nbr_ptr, const int32_t *node_nbrs, const float *V,
+                    int64_t nnode, const uint8_t *free_mask, const double *x, double *y, void *stream) {
+    return csr_spmv<float>(indptr, node_nbr_ptr, node_nbrs, V, nnode, free_mask, x, y, stream, "fl_csr_spmv_v32");
+}
+
+int fl_pcg(const int32_t *indptr, const int32_t *node_nbr_ptr, const int32_t *node_nbrs, const double *V,
+           int64_t nnode, const uint8_t *free_mask, const double *b, double *x, double rtol, int32_t maxit,
+           int32_t check_every, void *workspace, size_t workspace_bytes, int32_t *iterations, double *rel_residual,
+           void *stream) {
+    return pcg<double>(indptr, node_nbr_ptr, node_nbrs, V, nnode, free_mask, b, x, rtol, maxit, check_every,
+                       workspace, workspace_bytes, iterations, rel_residual, stream, "fl_pcg");
+}
+
+int fl_pcg_v32(const int32_t *indptr, const int32_t *node_nbr_ptr, const int32_t *node_nbrs, const float *V,
+               int64_t nnode, const uint8_t *free_mask, const double *b, double *x, double rtol, int32_t maxit,
+               int32_t check_every, void *workspace, size_t workspace_bytes, int32_t *iterations,
+               double *rel_residual, void *stream) {
+    return pcg<float>(indptr, node_nbr_ptr, node_nbrs, V, nnode, free_mask, b, x, rtol, maxit, check_every,
+                      workspace, workspace_bytes, iterations, rel_residual, stream, "fl_pcg_v32");
 }
 
 }  // extern "C"

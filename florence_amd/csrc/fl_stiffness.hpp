@@ -48,26 +48,30 @@ struct Mfma16<float> {
     __device__ static int row(int lk, int r) { return 4 * lk + r; }
 };
 
-// Triangular split of a 128-node diagonal block: wave W accumulates tile rows W and
-// 7 - W, columns from the row index on (9 tiles), over one chunk of KC inner indices.
-// acc[0..1][*] hold row W's tiles (column c at [c / 4][c % 4]), acc[2..3][*] row 7 - W's.
+// Triangular split of a 128-node diagonal block: wave W accumulates tile columns W and
+// 7 - W over the rows up to the column index (W + 1 and 8 - W tiles: 9), over one chunk
+// of KC inner indices.  Splitting by columns keeps the costly operand (W, formed from
+// three G values and a D row) at two fragments per step; the cheap one (G rows) is a
+// single LDS read per tile row.  acc[0..1][*] hold column W's tiles (row r at
+// [r / 4][r % 4]), acc[2..3][*] column 7 - W's.
 template <int W, int KC, int LDA, typename MF, typename CT, typename acc_t>
 __device__ __forceinline__ void tri_chunk(const CT *__restrict__ A, const CT *__restrict__ Dc, int li, int lk,
                                           acc_t (&acc)[4][4]) {
-    constexpr int R0 = W, R1 = 7 - W;
+    constexpr int C0 = W, C1 = 7 - W;
 #pragma unroll 1
     for (int kk = 0; kk < KC / 4; ++kk) {
         const int q = kk * 4 + lk;
         const int gq = q / 3, j = q - 3 * gq;
         const CT d0 = Dc[gq * 9 + 3 * j], d1 = Dc[gq * 9 + 3 * j + 1], d2 = Dc[gq * 9 + 3 * j + 2];
         const CT *Gb = A + 3 * gq * LDA;
-        const CT a0 = A[q * LDA + R0 * 16 + li], a1 = A[q * LDA + R1 * 16 + li];
+        const int b0 = C0 * 16 + li, b1 = C1 * 16 + li;
+        const CT w0 = d0 * Gb[b0] + d1 * Gb[LDA + b0] + d2 * Gb[2 * LDA + b0];
+        const CT w1 = d0 * Gb[b1] + d1 * Gb[LDA + b1] + d2 * Gb[2 * LDA + b1];
 #pragma unroll
-        for (int c = R0; c < 8; ++c) {
-            const int b = c * 16 + li;
-            const CT bf = d0 * Gb[b] + d1 * Gb[LDA + b] + d2 * Gb[2 * LDA + b];
-            acc[c / 4][c % 4] = MF::step(a0, bf, acc[c / 4][c % 4]);
-            if (c >= R1) acc[2 + c / 4][c % 4] = MF::step(a1, bf, acc[2 + c / 4][c % 4]);
+        for (int r = 0; r <= C1; ++r) {
+            const CT a = A[q * LDA + r * 16 + li];
+            if (r <= C0) acc[r / 4][r % 4] = MF::step(a, w0, acc[r / 4][r % 4]);
+            acc[2 + r / 4][r % 4] = MF::step(a, w1, acc[2 + r / 4][r % 4]);
         }
     }
 }
@@ -114,13 +118,13 @@ __global__ __launch_bounds__(64 * WGM *WGN, MINW) void stiffness_kernel(const do
     // skipped and written as the transpose of its mirror tile
     const bool diag = pi == pk;
     // 128-node diagonal blocks (PS = 2): the upper triangle of the 8 x 8 grid of 16 x 16
-    // tiles (36 tiles) is dealt to the four waves as row pairs {w, 7 - w} (8 - w and
-    // w + 1 tiles: 9 each), so these workgroups do 9/16 of an off-diagonal block's MFMAs
+    // tiles (36 tiles) is dealt to the four waves as column pairs {w, 7 - w} (w + 1 and
+    // 8 - w tiles: 9 each), so these workgroups do 9/16 of an off-diagonal block's MFMAs
     constexpr bool TRI = PS == 2 && NP == 128 && WGM == 2 && WGN == 2 && MT == 4 && NT == 4;
     constexpr bool tri = TRI;
     const bool skip = !tri && WGM == WGN && diag && wm > wn;
     const bool mirror = !diag || (WGM == WGN && wm < wn);
-    const int R0 = wave, R1 = 7 - wave;   // tile rows of the triangular split
+    const int R0 = wave, R1 = 7 - wave;   // tile columns of the triangular split
 
     // fill role: point fg of the chunk, nodes fa0 + TPG*s
     const int fg = tid / TPG, fa0 = tid % TPG;
@@ -219,20 +223,20 @@ __global__ __launch_bounds__(64 * WGM *WGN, MINW) void stiffness_kernel(const do
     const int64_t npe2 = (int64_t)npe * npe;
     KT *Kd = Ke + (e * 9 + pi * 3 + pk) * npe2;
     if (tri) {
-        // tile (R, c), c >= R, and its mirror (c, R) inside the same symmetric block
+        // tile (t, C), t <= C, and its mirror (C, t) inside the same symmetric block
 #pragma unroll
-        for (int c = 0; c < 8; ++c)
+        for (int t = 0; t < 8; ++t)
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
-                const int R = h ? R1 : R0;
-                if (c >= R) {
+                const int C = h ? R1 : R0;
+                if (t <= C) {
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
-                        const int a = R * 16 + MF::row(lk, r), b = c * 16 + li;
-                        const KT v = (KT)acc[2 * h + c / 4][c % 4][r];
+                        const int a = t * 16 + MF::row(lk, r), b = C * 16 + li;
+                        const KT v = (KT)acc[2 * h + t / 4][t % 4][r];
                         if (a < npe && b < npe) {
                             Kd[(int64_t)a * npe + b] = v;
-                            if (c > R) Kd[(int64_t)b * npe + a] = v;
+                            if (t < C) Kd[(int64_t)b * npe + a] = v;
                         }
                     }
                 }

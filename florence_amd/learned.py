@@ -93,13 +93,14 @@ class LockstepNewton(object):
     """Incremental Newton of many family members as one device problem."""
 
     def __init__(self, family, seeds, precision="fp64", tol=1e-8, lin_rtol=1e-10, maxit=25, forcing="fixed",
-                 eta_max=1e-2, device="cuda"):
+                 eta_max=1e-2, csr="fp64", device="cuda"):
         import torch
         if forcing not in ("fixed", "adaptive"):
             raise ValueError(forcing)
         self.family, self.seeds = family, list(seeds)
         self.precision, self.tol, self.maxit = precision, tol, maxit
         self.forcing, self.lin_rtol, self.eta_max = forcing, lin_rtol, eta_max
+        self.csr = csr
         members = [family.member(s) for s in self.seeds]
         meshes = [m for m, _, _ in members]
         npm = meshes[0].points.shape[0]
@@ -115,7 +116,7 @@ class LockstepNewton(object):
         self.free_f = self.free.to(torch.float64)
         self.u_total = torch.as_tensor(np.concatenate([u for _, u in bc]), device=dev)
         self.X = self.dm.points.reshape(-1)
-        self.V = torch.empty(self.dm.nnz, dtype=torch.float64, device=dev)
+        self.V = torch.empty(self.dm.nnz, dtype=torch.float32 if csr == "fp32" else torch.float64, device=dev)
         self.T = torch.empty(3 * self.dm.nnode, dtype=torch.float64, device=dev)
         self.lin = LinearSolver(rtol=lin_rtol, maxit=50000, check_every=10)
 
@@ -127,7 +128,8 @@ class LockstepNewton(object):
         return v.repeat_interleave(3 * self.nnode_member)
 
     def _assemble(self, u):
-        assemble_device(self.dm, self.family.material, self.X + u, self.V, self.T, precision=self.precision)
+        assemble_device(self.dm, self.family.material, self.X + u, self.V, self.T, precision=self.precision,
+                        csr=self.csr)
 
     def _residual(self, u, out):
         residual_device(self.dm, self.family.material, self.X + u, out)
@@ -154,9 +156,10 @@ class LockstepNewton(object):
             eta = r_t / r if C * r * r <= 0.5 * r_t else C * r
         return float(min(self.eta_max, max(self.lin_rtol, eta)))
 
-    def solve(self, start="previous", predictor=None, record=False):
+    def solve(self, start="previous", predictor=None, record=False, log=None):
         """Run every increment; returns a dict of iteration counts, timings, start
-        choices and (record=True) the converged displacements u_0..u_nincr on the host."""
+        choices and (record=True) the converged displacements u_0..u_nincr on the host.
+        ``log``: optional callable given one line per increment."""
         import torch
         if start not in ("previous", "extrap", "learned"):
             raise ValueError(start)
@@ -283,6 +286,9 @@ class LockstepNewton(object):
                 torch.neg(R, out=rhs)
             lock_iters[k - 1] = it
             history.append(hk)
+            if log is not None:
+                log("increment {}/{}: {} Newton iterations, worst |R|/R0 {}, {:.1f}s".format(
+                    k, nincr, it, ["{:.1e}".format(v) for v in hk], time.perf_counter() - t0))
             # in the reference's mode V now holds K at the converged state, which the
             # next increment's predictor uses; the warm-started modes assemble at
             # their start instead
@@ -293,7 +299,7 @@ class LockstepNewton(object):
                 break
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        out = {"start": start, "precision": self.precision, "wall_s": wall, "linear_s": t_lin, "assembly_s": t_asm,
+        out = {"start": start, "precision": self.precision, "csr": self.csr, "wall_s": wall, "linear_s": t_lin, "assembly_s": t_asm,
                "residual_s": t_res, "predict_s": t_pred, "lockstep_iterations": lock_iters.tolist(),
                "member_iterations": iters, "chose_learned": chose_learned.tolist(), "start_residual": start_ratio,
                "pcg_iterations": list(self.lin.iterations), "residual_history": history,

@@ -48,10 +48,11 @@ class LinearSolver(object):
             self._ws = torch.empty(int(lib.fl_pcg_workspace_size(dm.nnode)), dtype=torch.uint8, device=dm.device)
         it = _lib.ctypes.c_int32(0)
         rel = _lib.ctypes.c_double(0.0)
-        _lib.check(lib.fl_pcg(_lib.ptr(dm.indptr), _lib.ptr(dm.node_nbr_ptr), _lib.ptr(dm.node_nbrs), _lib.ptr(V),
-                              dm.nnode, _lib.ptr(free_mask), _lib.ptr(b), _lib.ptr(x), float(self.rtol),
-                              int(self.maxit), int(self.check_every), _lib.ptr(self._ws), self._ws.numel(),
-                              _lib.ctypes.byref(it), _lib.ctypes.byref(rel), _lib.stream_ptr()),
+        fn = lib.fl_pcg_v32 if V.dtype == torch.float32 else lib.fl_pcg
+        _lib.check(fn(_lib.ptr(dm.indptr), _lib.ptr(dm.node_nbr_ptr), _lib.ptr(dm.node_nbrs), _lib.ptr(V),
+                      dm.nnode, _lib.ptr(free_mask), _lib.ptr(b), _lib.ptr(x), float(self.rtol),
+                      int(self.maxit), int(self.check_every), _lib.ptr(self._ws), self._ws.numel(),
+                      _lib.ctypes.byref(it), _lib.ctypes.byref(rel), _lib.stream_ptr()),
                    "fl_pcg")
         self.iterations.append(int(it.value))
         if not (rel.value <= self.rtol) and it.value >= self.maxit:
@@ -61,9 +62,12 @@ class LinearSolver(object):
 
 
 def spmv(dm, V, x, y, free_mask=None):
+    """y = K x (masked operator when free_mask is given); V float64 or float32."""
+    import torch
     lib = _lib.load()
-    _lib.check(lib.fl_csr_spmv(_lib.ptr(dm.indptr), _lib.ptr(dm.node_nbr_ptr), _lib.ptr(dm.node_nbrs), _lib.ptr(V),
-                               dm.nnode, _lib.ptr(free_mask), _lib.ptr(x), _lib.ptr(y), _lib.stream_ptr()),
+    fn = lib.fl_csr_spmv_v32 if V.dtype == torch.float32 else lib.fl_csr_spmv
+    _lib.check(fn(_lib.ptr(dm.indptr), _lib.ptr(dm.node_nbr_ptr), _lib.ptr(dm.node_nbrs), _lib.ptr(V),
+                  dm.nnode, _lib.ptr(free_mask), _lib.ptr(x), _lib.ptr(y), _lib.stream_ptr()),
                "fl_csr_spmv")
     return y
 
@@ -91,7 +95,7 @@ class FEMSolver(object):
                  load_factor=None, newton_raphson_tolerance=1.0e-6, newton_raphson_solution_tolerance=None,
                  maximum_iteration_for_newton_raphson=50, break_at_stagnation=True, user_defined_break_func=None,
                  user_defined_stop_func=None, report_log_level=0, linear_solver=None, warm_start=None,
-                 tangent_precision="fp64", **kwargs):
+                 tangent_precision="fp64", csr_precision="fp64", **kwargs):
         if has_low_level_dispatcher != optimise:
             has_low_level_dispatcher = True
         self.has_low_level_dispatcher = has_low_level_dispatcher
@@ -116,6 +120,8 @@ class FEMSolver(object):
         # "fp64": the reference's arithmetic; "fp32": inexact-Jacobian Newton with the
         # tangent contracted on FP32 MFMA (kinematics, residual and update stay FP64)
         self.tangent_precision = tangent_precision
+        # "fp32" (with an fp32/bf16 tangent): CSR values stored as float32 for the solve
+        self.csr_precision = csr_precision
         self.assembly_time = 0.0
         self.newton_raphson_failed_to_converge = False
         for k, v in kwargs.items():
@@ -140,7 +146,7 @@ class FEMSolver(object):
         X = torch.as_tensor(np.ascontiguousarray(mesh.points, dtype=np.float64), device=dev)
         Eulerx = X.clone().reshape(-1)
         NodalForces = torch.zeros(ndof, dtype=torch.float64, device=dev)
-        V = torch.empty(dm.nnz, dtype=torch.float64, device=dev)
+        V = torch.empty(dm.nnz, dtype=torch.float32 if self.csr_precision == "fp32" else torch.float64, device=dev)
         T = torch.empty(ndof, dtype=torch.float64, device=dev)
         tol = self.newton_raphson_tolerance
         nincr = self.number_of_load_increments
@@ -153,7 +159,7 @@ class FEMSolver(object):
             nonlocal ta
             t = time.perf_counter()
             assemble_device(dm, material, x, V, T, int(self.requires_geometry_update),
-                            precision=self.tangent_precision)
+                            precision=self.tangent_precision, csr=self.csr_precision)
             ta += time.perf_counter() - t
 
         assemble(Eulerx)   # first assembly at the undeformed state (FEMSolver.py:641-642)

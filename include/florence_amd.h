@@ -120,6 +120,11 @@ int fl_df_assemble(int32_t material, const double *params, const double *points_
 #define FL_PREC_FP64 0
 #define FL_PREC_FP32 1
 #define FL_PREC_BF16 2   /* G and W rounded to BF16, v_mfma_f32_16x16x32_bf16, FP32 accumulation */
+/* OR'ed into the precision of fl_df_assemble_mt with FL_PREC_FP32 / FL_PREC_BF16: the
+ * CSR values are summed in FP64 as always but stored as FP32, so V_dev points to
+ * float[nnz] (the operand of fl_csr_spmv_v32 / fl_pcg_v32 in an inexact Newton
+ * method; half the bytes per Krylov iteration). */
+#define FL_PREC_CSR32 16
 size_t fl_df_workspace_size_ex(int32_t ngauss, int64_t nelem, int32_t npe, int32_t precision);
 int fl_df_assemble_ex(int32_t material, const double *params, const double *points_dev,
                       const int32_t *elements_dev, const double *Eulerx_dev, const double *Jm_dev,
@@ -146,7 +151,21 @@ int fl_df_assemble_mt(int32_t material, const double *params, const int32_t *ele
                       const int32_t *node_nbr_ptr_dev, const int32_t *node_nbrs_dev, const int32_t *node_order_dev,
                       const int32_t *bucket_offsets, const int32_t *bucket_degree, const int32_t *indptr_dev,
                       double *V_dev, double *T_dev, int32_t precision, const int16_t *ranks_dev,
-                      void *workspace_dev, size_t workspace_bytes, int32_t *status_dev, void *stream);
+                      const void *tensor_basis_dev, void *workspace_dev, size_t workspace_bytes,
+                      int32_t *status_dev, void *stream);
+
+/* Tensor basis of Lagrange hexes under a tensor-product rule (no reference
+ * counterpart): with it, fl_df_assemble_mt computes FP64 element blocks by sum
+ * factorisation (one direction of the point sum at a time) instead of the dense
+ * contraction; same K to rounding.  Layout (fl_tensor_basis_bytes(p1, q) bytes):
+ *   int32 p1, q, 0, 0; int32 g_of_t[q^3] (flattened point of tensor point
+ *   (qb*q + qe)*q + qz); int32 local_of_t[p1^3] (local node of tensor node
+ *   (ab*p1 + ae)*p1 + az); zero int32 padding to 8 bytes; double f1d[3][2][p1][q]
+ *   (the 1D bases and their derivatives of directions zeta, eta, beta at that
+ *   direction's points).  Supported (p1, q): (2,2) (3,3) (4,4) (5,5) (5,6).  NULL, an
+ *   FP32/BF16 precision or fl_set_tuning(FL_TUNE_STIFF_SF, 1) select the dense path. */
+#define FL_TUNE_STIFF_SF 7
+size_t fl_tensor_basis_bytes(int32_t p1, int32_t q);
 
 /* --------------------------------------------- residual-only (explicit) pass
  * Replaces the mechanics branch of _GlobalAssemblyExplicit_DF_DPF_
@@ -184,6 +203,15 @@ int fl_pcg(const int32_t *indptr_dev, const int32_t *node_nbr_ptr_dev, const int
            const double *V_dev, int64_t nnode, const uint8_t *free_mask_dev, const double *b_dev, double *x_dev,
            double rtol, int32_t maxit, int32_t check_every, void *workspace_dev, size_t workspace_bytes,
            int32_t *iterations, double *rel_residual, void *stream);
+/* The same two operations on FP32-stored CSR values (FL_PREC_CSR32); products and
+ * every vector stay FP64.  No reference counterpart (the mixed-precision form). */
+int fl_csr_spmv_v32(const int32_t *indptr_dev, const int32_t *node_nbr_ptr_dev, const int32_t *node_nbrs_dev,
+                    const float *V_dev, int64_t nnode, const uint8_t *free_mask_dev, const double *x_dev,
+                    double *y_dev, void *stream);
+int fl_pcg_v32(const int32_t *indptr_dev, const int32_t *node_nbr_ptr_dev, const int32_t *node_nbrs_dev,
+               const float *V_dev, int64_t nnode, const uint8_t *free_mask_dev, const double *b_dev, double *x_dev,
+               double rtol, int32_t maxit, int32_t check_every, void *workspace_dev, size_t workspace_bytes,
+               int32_t *iterations, double *rel_residual, void *stream);
 
 /* ------------------------------------------------------------- profiling
  * No reference counterpart (the reference only records wall time,
@@ -202,7 +230,8 @@ int fl_profile_enable(int on);
 /* keys FL_TUNE_STIFF_VARIANT_BASE + c, c = padded node-count class (16, 32, 48, 64, 128
  * nodes -> 0..4): value v+1 selects tile-shape variant v of the FP64/FP32 stiffness
  * kernel (32: v1 = 2x2 waves of 16x16 tiles; 64: v1 = one wave of 4x4 tiles;
- * 128: v1 = one wave per SIMD).  Used by the autotuner. */
+ * 128: v1 = one wave per SIMD, all pairs square; v2 = off-diagonal pairs on 1x4 waves
+ * of 8x2 tiles).  Used by the autotuner. */
 #define FL_TUNE_STIFF_VARIANT_BASE 2
 int fl_set_tuning(int32_t key, int32_t value);
 int fl_profile_read(int32_t *kinds, double *phase_ms, int32_t max_calls, int32_t max_phases);

@@ -83,7 +83,9 @@ def test_residual_only_matches(cuda):
 @pytest.mark.parametrize("et,p,n", [("tet", 3, 2), ("hex", 4, 2)])
 def test_reference_entry_point_equals_rank_table_path(cuda, et, p, n):
     """fl_df_assemble (the reference-shaped entry: no rank table, neighbour lists
-    searched per call) and the rank-table path give bitwise the same V and T."""
+    searched per call, dense contraction) and the rank-table path give bitwise the same
+    V and T (the rank-table side built without the tensor basis, so both use the same
+    stiffness kernel)."""
     import torch
     from florence_amd import _lib
     from florence_amd.assembly import DeviceMeshData, assemble_device
@@ -91,7 +93,7 @@ def test_reference_entry_point_equals_rank_table_path(cuda, et, p, n):
     fs = function_space(mesh, p)
     mat = MooneyRivlin(3, mu1=0.5, mu2=0.5, lamb=20.0)
     x = deformed(mesh.points, 0.05, seed=p)
-    dm = DeviceMeshData(mesh, fs)
+    dm = DeviceMeshData(mesh, fs, tensor_basis=False)
     assert dm.ranks is not None
     V1, T1, _ = assemble_device(dm, mat, x)
     V1, T1 = V1.clone(), T1.clone()
@@ -110,3 +112,49 @@ def test_reference_entry_point_equals_rank_table_path(cuda, et, p, n):
     torch.cuda.synchronize()
     assert torch.equal(V1, V2) and torch.equal(T1, T2)
     assert int(dm.status[0].item()) == 0
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_hex_p4_tile_variants(cuda, variant):
+    """Every tile-shape variant of the dense (MFMA) contraction the autotuner may pick
+    for 128-node elements meets the oracle bar (config 2 family, 2^3 cube); the
+    sum-factorised path is switched off for it."""
+    from florence_amd import _lib
+    lib = _lib.load()
+    key = _lib.FL_TUNE_STIFF_VARIANT_BASE + 4
+    _lib.check(lib.fl_set_tuning(key, variant + 1), "fl_set_tuning")
+    _lib.check(lib.fl_set_tuning(_lib.FL_TUNE_STIFF_SF, 1), "fl_set_tuning")
+    try:
+        m, fs, mat = config2(n=2)
+        _check(m, fs, mat, deformed(m.points, 0.05, seed=9), "hex p4 variant {}".format(variant))
+    finally:
+        lib.fl_set_tuning(key, 0)
+        lib.fl_set_tuning(_lib.FL_TUNE_STIFF_SF, 0)
+
+
+@pytest.mark.parametrize("p", [1, 2, 3, 4])
+def test_sum_factorised_equals_dense(cuda, p):
+    """Tensor-product hexes: the sum-factorised blocks (default when the tensor basis is
+    present) and the dense MFMA blocks both meet the oracle bar and agree to 1e-13."""
+    import torch
+    from florence_amd import _lib
+    from florence_amd.assembly import DeviceMeshData, assemble_device
+    if p == 4:
+        m, fs, mat = config2(n=2)
+    else:
+        m = cube_mesh("hex", p, 2, jitter=0.05, seed=p)
+        fs = function_space(m, p)
+        mat = MooneyRivlin(3, mu1=0.5, mu2=0.5, lamb=20.0)
+    x = deformed(m.points, 0.05, seed=20 + p)
+    dm, V, T = _check(m, fs, mat, x, "hex p{} sum-factorised".format(p))
+    assert dm.tensor_basis is not None
+    lib = _lib.load()
+    _lib.check(lib.fl_set_tuning(_lib.FL_TUNE_STIFF_SF, 1), "fl_set_tuning")
+    try:
+        V2, T2, _ = assemble_device(dm, mat, x)
+        torch.cuda.synchronize()
+    finally:
+        lib.fl_set_tuning(_lib.FL_TUNE_STIFF_SF, 0)
+    V2 = V2.cpu().numpy()
+    assert np.abs(V - V2).max() <= 1e-13 * np.abs(V2).max()
+    assert np.array_equal(T, T2.cpu().numpy())

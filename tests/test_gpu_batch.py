@@ -33,6 +33,7 @@ def test_batch_graph_matches_per_mesh(cuda):
         g = next(gg for gg in ba.groups if i in gg["mesh_ids"])
         dm = DeviceMeshData(m, g["fs"])
         _lib.load().fl_set_tuning(_lib.FL_TUNE_STIFF_VARIANT_BASE + _np_class(dm.npe), g["variant"] + 1)
+        _lib.load().fl_set_tuning(_lib.FL_TUNE_STIFF_SF, 2 if g.get("sum_factorised", True) else 1)
         X = np.asarray(m.points)
         V, T, st = assemble_device(dm, mat, X + disp(X))
         torch.cuda.synchronize()
@@ -42,3 +43,4 @@ def test_batch_graph_matches_per_mesh(cuda):
         assert np.array_equal(data, V.cpu().numpy()), (i, key)
         assert np.array_equal(Tb, T.cpu().numpy()), (i, key)
     assert all(s[0] == 0 for s in ba.statuses())
+    _lib.load().fl_set_tuning(_lib.FL_TUNE_STIFF_SF, 0)

@@ -74,3 +74,36 @@ def test_mixed_newton_config1(cuda, precision, bar):
         precision, res.newton_iterations, ref["iterations"].tolist(), err, res.wall_time,
         ["{:.1e}".format(v[-1]) for v in res.NRConvergence.values()]), flush=True)
     assert err <= bar
+
+
+def test_csr32_storage_and_solve(cuda):
+    """FL_PREC_CSR32: the FP32-stored values are the FP64 sums of the FP32-tangent path
+    rounded once (bitwise), T is unchanged, and block-Jacobi PCG on them solves the same
+    system to the FP32 level (solution within 1e-5 of the FP64-valued solve)."""
+    import torch
+    from florence_amd.assembly import DeviceMeshData, assemble_device
+    from florence_amd.solver import LinearSolver, spmv
+    m, fs, mat = cube_hex_p4(2)
+    x = _deformed(np.asarray(m.points))
+    dm = DeviceMeshData(m, fs)
+    V64, T64, _ = assemble_device(dm, mat, x, precision="fp32")
+    V64, T64 = V64.clone(), T64.clone()
+    V32, T32, st = assemble_device(dm, mat, x, precision="fp32", csr="fp32")
+    torch.cuda.synchronize()
+    assert V32.dtype == torch.float32
+    assert torch.equal(V32, V64.to(torch.float32)) and torch.equal(T32, T64)
+    assert int(st[0].item()) == 0
+    X = np.asarray(m.points)
+    free = torch.as_tensor((np.abs(X[:, 0]) > 1e-10).repeat(3).astype(np.uint8), device="cuda")
+    b = torch.as_tensor(np.sin(np.arange(3 * m.nnode) * 0.37), device="cuda") * free.to(torch.float64)
+    xs = {}
+    for name, V in (("v64", V64), ("v32", V32)):
+        sol = torch.zeros_like(b)
+        LinearSolver(rtol=1e-10).Solve(dm, V, free, b, sol)
+        r = torch.empty_like(b)
+        spmv(dm, V, sol, r, free)
+        assert float(torch.linalg.norm(r - b) / torch.linalg.norm(b)) < 1e-9, name
+        xs[name] = sol.cpu().numpy()
+    err = np.abs(xs["v32"] - xs["v64"]).max() / np.abs(xs["v64"]).max()
+    print("csr32 solve vs csr64 solve: rel diff {:.2e}".format(err))
+    assert err < 1e-5
