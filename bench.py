@@ -63,13 +63,13 @@ def dense_b_flops_per_element(npe, G):
     return G * (2 * V * n * n + 2 * V * V * n + 2 * n * n + 6 * npe * npe + 108 * npe + 13 * n + 300)
 
 
-def measured_traffic(summary):
-    """HBM bytes per launch of the dominant kernel: FETCH_SIZE + WRITE_SIZE from the
+def measured_traffic(summary, phase):
+    """HBM bytes per launch of one phase's kernels: FETCH_SIZE + WRITE_SIZE from the
     committed rocprofv3 PMC summary (copy-calibrated), or None if there is none."""
     import re
     try:
         with open(os.path.join(ROOT, "profiles", summary)) as f:
-            m = re.search(r"traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", f.read())
+            m = re.search(phase + r" traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", f.read())
     except OSError:
         return None
     return float(m.group(1)) if m else None
@@ -470,7 +470,7 @@ def main():
     gather_gbs = gbytes / (phase_ms[3] / 1e3) / 1e9
     stiff_roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                   "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-                  "traffic": measured_traffic("stiffness_kernel_hex_p4_24cube_pmc.md"),
+                  "traffic": measured_traffic("assembly_hex_p4_24cube_pmc.md", "stiffness"),
                   "kernel": ("stiffness_sf_kernel<5,6> (sum-factorised, FP64 VALU; MI355X's FP64 vector "
                              "peak equals its FP64 matrix peak)" if sf else
                              "stiffness_kernel<128,4,4,2,2> (off-diagonal + triangular diagonal launches)"),
@@ -480,7 +480,7 @@ def main():
                                                          / (phase_ms[1] / 1e3) / 1e12, 3)}
     gather_roof = {"bound": "hbm", "achieved": round(gather_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                    "frac": round(gather_gbs / HBM_PEAK_GBS, 4),
-                   "traffic": measured_traffic("gather_kernel_hex_p4_24cube_pmc.md"),
+                   "traffic": measured_traffic("assembly_hex_p4_24cube_pmc.md", "gather"),
                    "kernel": "gather_kernel (all degree classes of one assembly)", "bytes_per_launch": gbytes,
                    "kernel_ms": round(float(phase_ms[3]), 4)}
     dominant_is_stiffness = phase_ms[1] >= phase_ms[3]

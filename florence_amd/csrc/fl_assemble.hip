@@ -408,6 +408,7 @@ __global__ void gather_traction_kernel(int npe, const int32_t *__restrict__ node
 }
 
 // ------------------------------------------------------------------ host side
+// accumulator (9 deg doubles), neighbour list (deg ints), column positions (npe ints)
 static size_t gather_lds_bytes(int deg, int npe) {
     return (size_t)deg * 9 * 8 + (size_t)deg * 4 + (size_t)npe * 4 + 16;
 }
@@ -479,9 +480,14 @@ static int launch_stiffness_sf(int npe, int G, const double *qp, const void *bas
     }
     const dim3 grid((unsigned)groups);
     const int32_t *b = static_cast<const int32_t *>(basis);
+    // value 3 of FL_TUNE_STIFF_SF: the all-VALU form; default: eta and beta on MFMA
+    const bool valu = tuning_value(TUNE_STIFF_SF, 0) == 3;
 #define FL_SF_CASE(P, QQ)                                                                  \
     if (npe == P * P * P && G == QQ * QQ * QQ) {                                           \
-        stiffness_sf_kernel<P, QQ><<<grid, 256, 0, st>>>(qp, b, nelem, Ke);                \
+        if (valu)                                                                          \
+            stiffness_sf_kernel<P, QQ><<<grid, 256, 0, st>>>(qp, b, nelem, Ke);            \
+        else                                                                               \
+            stiffness_sf_mfma_kernel<P, QQ><<<grid, 256, 0, st>>>(qp, b, nelem, Ke);       \
         FL_LAUNCH_CHECK();                                                                 \
         return 0;                                                                          \
     }
@@ -492,6 +498,25 @@ static int launch_stiffness_sf(int npe, int G, const double *qp, const void *bas
     FL_SF_CASE(5, 6)
 #undef FL_SF_CASE
     set_error("no sum-factorised kernel for %d nodes and %d points", npe, G);
+    return FL_E_ARG;
+}
+
+static int launch_traction_sf(int npe, int G, const double *qp, int qs, int soff, const void *basis, int64_t nelem,
+                              double *te, hipStream_t st) {
+    const int32_t *b = static_cast<const int32_t *>(basis);
+#define FL_TSF_CASE(P, QQ)                                                                          \
+    if (npe == P * P * P && G == QQ * QQ * QQ) {                                                    \
+        traction_sf_kernel<P, QQ><<<(unsigned)nelem, 256, 0, st>>>(qp, qs, soff, b, nelem, te);     \
+        FL_LAUNCH_CHECK();                                                                          \
+        return 0;                                                                                   \
+    }
+    FL_TSF_CASE(2, 2)
+    FL_TSF_CASE(3, 3)
+    FL_TSF_CASE(4, 4)
+    FL_TSF_CASE(5, 5)
+    FL_TSF_CASE(5, 6)
+#undef FL_TSF_CASE
+    set_error("no sum-factorised traction kernel for %d nodes and %d points", npe, G);
     return FL_E_ARG;
 }
 
@@ -707,11 +732,19 @@ int fl_df_assemble_mt(int32_t material, const double *params, const int32_t *ele
                        : launch_stiffness_bf16(NP, w.JmT, w.qp, npe, G, nelem, (float *)w.Ke, st);
     if (rc) return rc;
     prof_mark(st);
-    const size_t lds_tr = (size_t)G * 9 * 8;
-    if (lds_tr > 64 * 1024) FL_HIP_CHECK(hipFuncSetAttribute((const void *)traction_kernel,
-                                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_tr));
-    traction_kernel<<<(unsigned)nelem, 256, lds_tr, st>>>(w.JmT, NP, w.qp, QS, Q_S, npe, G, w.te);
-    FL_LAUNCH_CHECK();
+    // tractions: factorised whenever the tensor basis is given (independent of the tangent
+    // precision, so T is the same for every precision)
+    if (tensor_basis != nullptr && sf_shape(npe, G) && tuning_value(TUNE_STIFF_SF, 0) != 1) {
+        rc = launch_traction_sf(npe, G, w.qp, QS, Q_S, tensor_basis, nelem, w.te, st);
+        if (rc) return rc;
+    } else {
+        const size_t lds_tr = (size_t)G * 9 * 8;
+        if (lds_tr > 64 * 1024)
+            FL_HIP_CHECK(hipFuncSetAttribute((const void *)traction_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             (int)lds_tr));
+        traction_kernel<<<(unsigned)nelem, 256, lds_tr, st>>>(w.JmT, NP, w.qp, QS, Q_S, npe, G, w.te);
+        FL_LAUNCH_CHECK();
+    }
     if (update != 1) {
         const int64_t nte = nelem * 3 * npe;
         zero_kernel<<<(unsigned)((nte + 255) / 256), 256, 0, st>>>(w.te, nte);
@@ -721,9 +754,9 @@ int fl_df_assemble_mt(int32_t material, const double *params, const int32_t *ele
     for (int b = 0; b < FL_GATHER_BUCKETS; ++b) {
         const int32_t nb = bucket_offsets[b + 1] - bucket_offsets[b];
         if (nb <= 0) continue;
-        const size_t lds = gather_lds_bytes(bucket_degree[b], npe);
         // the lowest class (nodes interior to one element) keeps one wave per node
         const bool wide = b > 0;
+        const size_t lds = gather_lds_bytes(bucket_degree[b], npe);
         int rc2 = 0;
         const int32_t *ord = node_order + bucket_offsets[b];
         const int dg = bucket_degree[b];

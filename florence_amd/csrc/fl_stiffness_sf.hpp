@@ -171,4 +171,240 @@ __global__ __launch_bounds__(256, 2) void stiffness_sf_kernel(const double *__re
     }
 }
 
+// The same factorisation with the eta and beta contractions on FP64 MFMA
+// (v_mfma_f64_16x16x4f64); the zeta contraction and the point tensor stay on the VALU.
+// Per zeta slice and per beta-type (the nine (m, n) grouped by their beta factor):
+//   eta:  T2 = sum_{mn in type} Fe^{st}[(ae,be)][qe] x T1[mn][qe][(bz,qb)]
+//         (P1^2 x Q x P1 Q, the inner index padded to a multiple of 4)
+//   beta: K[(ab,bb)][(ae,be,bz)] += Fb^{sb}Fb^{tb}[(ab,bb)][qb] x T2[qb][(ae,be,bz)]
+//         accumulated in the MFMA registers over the four types.
+// Pair factors are formed while the operands are loaded, and only one type's T2 is held,
+// so the workgroup needs ~37 KB of LDS and four of them share a CU.  Output tiles are
+// dealt round-robin to the four waves.
+template <int P1, int Q>
+__global__ __launch_bounds__(256, 4) void stiffness_sf_mfma_kernel(const double *__restrict__ qp,
+                                                                   const int32_t *__restrict__ basis, int64_t nelem,
+                                                                   double *__restrict__ Ke) {
+    typedef SfLayout<P1, Q> L;
+    typedef Mfma16<double> MF;
+    constexpr int Q3 = L::Q3, N3 = L::N3, PP = L::PP;
+    constexpr int NT = 256;
+    constexpr int MT2 = (PP + 15) / 16, NT2 = (P1 * Q + 15) / 16, KQ = (Q + 3) / 4;
+    constexpr int MT3 = (PP + 15) / 16, NT3 = (N3 + 15) / 16;
+    constexpr int TPW = (MT3 * NT3 + 3) / 4;   // beta tiles per wave
+    __shared__ double sF[3][2][P1][Q];
+    __shared__ double sC[9][Q3];
+    __shared__ double sT1[9][P1][Q * Q];
+    __shared__ double sT2[PP][P1][Q];
+    __shared__ int32_t sLoc[N3];
+
+    const int64_t wid = blockIdx.x;
+    const int xcd = (int)(wid & 7);
+    const int64_t slot = wid >> 3;
+    const int blk = (int)(slot % 9);
+    const int64_t e = (slot / 9) * 8 + xcd;
+    if (e >= nelem) return;
+    const int pi = blk / 3, pk = blk % 3;
+    const int sp = pi == pk ? pi : (pi + pk == 1 ? 3 : (pi + pk == 2 ? 4 : 5));
+    const bool tr = pi > pk;
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
+    const int32_t *g_of_t = basis + 4;
+    const int32_t *loc = basis + 4 + Q3;
+    const double *f1d = reinterpret_cast<const double *>(basis + L::INTS);
+
+    for (int t = tid; t < 3 * 2 * P1 * Q; t += NT) (&sF[0][0][0][0])[t] = f1d[t];
+    for (int t = tid; t < N3; t += NT) sLoc[t] = min(max(loc[t], 0), N3 - 1);
+    const double *qe_ = qp + e * (int64_t)(Q3)*QS;
+    for (int t = tid; t < Q3; t += NT) {
+        const double *rec = qe_ + (int64_t)min(max(g_of_t[t], 0), Q3 - 1) * QS;
+        double ix[9], D[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) ix[i] = rec[Q_IX + i];
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int l = 0; l < 3; ++l) D[3 * j + l] = rec[Q_D + 9 * sp + (tr ? 3 * l + j : 3 * j + l)];
+        double W[9];
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int n = 0; n < 3; ++n)
+                W[3 * j + n] = D[3 * j] * ix[n] + D[3 * j + 1] * ix[3 + n] + D[3 * j + 2] * ix[6 + n];
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+#pragma unroll
+            for (int n = 0; n < 3; ++n)
+                sC[3 * m + n][t] = ix[m] * W[n] + ix[3 + m] * W[3 + n] + ix[6 + m] * W[6 + n];
+    }
+    __syncthreads();
+
+    double *Kd = Ke + (e * 9 + pi * 3 + pk) * (int64_t)N3 * N3;
+    for (int az = 0; az < P1; ++az) {
+        // zeta (VALU): task (mn, bz, qe) forms its factor once and sweeps qb
+        for (int t = tid; t < 9 * P1 * Q; t += NT) {
+            const int mn = t / (P1 * Q), bz = (t / Q) % P1, qe = t % Q;
+            const int m = mn / 3, n = mn % 3;
+            double f[Q];
+#pragma unroll
+            for (int qz = 0; qz < Q; ++qz) f[qz] = sF[0][m == 0][az][qz] * sF[0][n == 0][bz][qz];
+#pragma unroll
+            for (int qb = 0; qb < Q; ++qb) {
+                const double *c = sC[mn] + (qb * Q + qe) * Q;
+                double s = 0.0;
+#pragma unroll
+                for (int qz = 0; qz < Q; ++qz) s += f[qz] * c[qz];
+                sT1[mn][bz][qe * Q + qb] = s;
+            }
+        }
+        __syncthreads();
+        typename MF::acc_t acc[TPW];
+#pragma unroll
+        for (int u = 0; u < TPW; ++u) acc[u] = typename MF::acc_t{0.0, 0.0, 0.0, 0.0};
+        for (int type = 0; type < 4; ++type) {
+            const int sb = type >> 1, tb = type & 1;
+            // eta (MFMA) for this type: tiles over rows (ae,be), columns (bz,qb)
+            for (int task = wave; task < MT2 * NT2; task += 4) {
+                const int tm = task / NT2, tn = task - tm * NT2;
+                const int arow = tm * 16 + li, bcol = tn * 16 + li;
+                const int ae_ = arow / P1, be_ = arow - ae_ * P1;
+                typename MF::acc_t a2 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int m = 0; m < 3; ++m) {
+                    if ((m == 2) != (sb == 1)) continue;
+#pragma unroll
+                    for (int n = 0; n < 3; ++n) {
+                        if ((n == 2) != (tb == 1)) continue;
+#pragma unroll
+                        for (int ks = 0; ks < KQ; ++ks) {
+                            const int k = ks * 4 + lk;   // qe
+                            const double a = (arow < PP && k < Q) ? sF[1][m == 1][ae_][k] * sF[1][n == 1][be_][k] : 0.0;
+                            const double b = (bcol < P1 * Q && k < Q) ? sT1[3 * m + n][bcol / Q][k * Q + bcol % Q] : 0.0;
+                            a2 = MF::step(a, b, a2);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = tm * 16 + MF::row(lk, r);
+                    if (row < PP && bcol < P1 * Q) sT2[row][bcol / Q][bcol % Q] = a2[r];
+                }
+            }
+            __syncthreads();
+            // beta (MFMA), this type's share of the inner index (qb, padded to 4)
+#pragma unroll
+            for (int u = 0; u < TPW; ++u) {
+                const int task = wave + 4 * u;
+                if (task < MT3 * NT3) {
+                    const int tm = task / NT3, tn = task - tm * NT3;
+                    const int arow = tm * 16 + li, bcol = tn * 16 + li;
+                    const int ab_ = arow / P1, bb_ = arow - ab_ * P1;
+                    const int ae_ = bcol / PP, be_ = (bcol / P1) % P1, bz_ = bcol % P1;
+#pragma unroll
+                    for (int ks = 0; ks < KQ; ++ks) {
+                        const int qb = ks * 4 + lk;
+                        const double a = (arow < PP && qb < Q) ? sF[2][sb][ab_][qb] * sF[2][tb][bb_][qb] : 0.0;
+                        const double b = (bcol < N3 && qb < Q) ? sT2[ae_ * P1 + be_][bz_][qb] : 0.0;
+                        acc[u] = MF::step(a, b, acc[u]);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int u = 0; u < TPW; ++u) {
+            const int task = wave + 4 * u;
+            if (task < MT3 * NT3) {
+                const int tm = task / NT3, tn = task - tm * NT3;
+                const int bcol = tn * 16 + li;
+                if (bcol < N3) {
+                    const int ae_ = bcol / PP, be_ = (bcol / P1) % P1, bz_ = bcol % P1;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int row = tm * 16 + MF::row(lk, r);
+                        if (row < PP) {
+                            const int ab = row / P1, bb = row - ab * P1;
+                            const int la = sLoc[(ab * P1 + ae_) * P1 + az];
+                            const int lb = sLoc[(bb * P1 + be_) * P1 + bz_];
+                            Kd[(int64_t)la * N3 + lb] = acc[u][r];
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Element tractions of tensor-product hexes by the same factorisation:
+// t_(a,i) = sum_q sum_m dN_a/dxi_m (q) P[q][m][i], P = (dxi/dx)^T (detJ sigma), contracted
+// zeta, eta, beta in turn for the nine (m, i).  One workgroup per element; te is written
+// in local node order like traction_kernel's.  qs / soff: record stride and offset of
+// detJ sigma (full or residual-only point records).
+template <int P1, int Q>
+__global__ __launch_bounds__(256) void traction_sf_kernel(const double *__restrict__ qp, int qs, int soff,
+                                                          const int32_t *__restrict__ basis, int64_t nelem,
+                                                          double *__restrict__ te) {
+    typedef SfLayout<P1, Q> L;
+    constexpr int Q3 = L::Q3, N3 = L::N3, PP = L::PP;
+    constexpr int NT = 256;
+    __shared__ double sF[3][2][P1][Q];
+    __shared__ double sP[9][Q3];
+    __shared__ double sU1[9][P1][Q * Q];
+    __shared__ double sU2[9][PP][Q];
+    __shared__ int32_t sLoc[N3];
+    const int64_t e = blockIdx.x;
+    if (e >= nelem) return;
+    const int tid = threadIdx.x;
+    const int32_t *g_of_t = basis + 4;
+    const int32_t *loc = basis + 4 + Q3;
+    const double *f1d = reinterpret_cast<const double *>(basis + L::INTS);
+    for (int t = tid; t < 3 * 2 * P1 * Q; t += NT) (&sF[0][0][0][0])[t] = f1d[t];
+    for (int t = tid; t < N3; t += NT) sLoc[t] = min(max(loc[t], 0), N3 - 1);
+    const double *qe_ = qp + e * (int64_t)Q3 * qs;
+    for (int t = tid; t < Q3; t += NT) {
+        const double *rec = qe_ + (int64_t)min(max(g_of_t[t], 0), Q3 - 1) * qs;
+        const double *S = rec + soff;   // s00 s11 s22 s01 s02 s12
+        const double Si[3][3] = {{S[0], S[3], S[4]}, {S[3], S[1], S[5]}, {S[4], S[5], S[2]}};
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                sP[3 * m + i][t] = rec[m] * Si[i][0] + rec[3 + m] * Si[i][1] + rec[6 + m] * Si[i][2];
+    }
+    __syncthreads();
+    for (int t = tid; t < 9 * P1 * Q * Q; t += NT) {
+        const int mi = t / (P1 * Q * Q), r = t - mi * P1 * Q * Q;
+        const int az = r / (Q * Q), qe = (r / Q) % Q, qb = r % Q;
+        const double *f = sF[0][mi / 3 == 0][az];
+        const double *p = sP[mi] + (qb * Q + qe) * Q;
+        double s = 0.0;
+#pragma unroll
+        for (int qz = 0; qz < Q; ++qz) s += f[qz] * p[qz];
+        sU1[mi][az][qe * Q + qb] = s;
+    }
+    __syncthreads();
+    for (int t = tid; t < 9 * PP * Q; t += NT) {
+        const int mi = t / (PP * Q), r = t - mi * PP * Q;
+        const int ae = r / (P1 * Q), az = (r / Q) % P1, qb = r % Q;
+        const double *f = sF[1][mi / 3 == 1][ae];
+        double s = 0.0;
+#pragma unroll
+        for (int qe = 0; qe < Q; ++qe) s += f[qe] * sU1[mi][az][qe * Q + qb];
+        sU2[mi][ae * P1 + az][qb] = s;
+    }
+    __syncthreads();
+    for (int t = tid; t < 3 * N3; t += NT) {
+        const int tn = t / 3, i = t - 3 * tn;
+        const int ab = tn / PP, aeaz = tn - ab * PP;
+        double s = 0.0;
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            const double *f = sF[2][m == 2][ab];
+#pragma unroll
+            for (int qb = 0; qb < Q; ++qb) s += f[qb] * sU2[3 * m + i][aeaz][qb];
+        }
+        te[e * 3 * N3 + 3 * sLoc[tn] + i] = s;
+    }
+}
+
 }  // namespace fl
