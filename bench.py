@@ -13,7 +13,9 @@ assembly timed on a bounded sample of the same mesh, one core).
     python bench.py [--gpus 1] [--steps 5] [--warmup 1]
 """
 import argparse
+import contextlib
 import ctypes
+import gc
 import json
 import os
 import sys
@@ -48,6 +50,29 @@ def sf_flops_per_element(p1, q):
     per_block = (q3 * 54 * 2 + p1 * (9 * p1 * q * q * q * 2) + p1 * (9 * p1 * p1 * p1 * q * q * 2)
                  + p1 * (p1 * n3 * (4 * q + p1 * 4 * q * 2)))
     return 9 * per_block
+
+
+SF_KERNELS = {
+    0: "stiffness_sf_mfma2_kernel<5,6> (sum-factorised: zeta on the FP64 VALU, eta and beta on FP64 MFMA, "
+       "two types per reduction)",
+    3: "stiffness_sf_kernel<5,6> (sum-factorised, all FP64 VALU; MI355X's FP64 vector peak equals its FP64 "
+       "matrix peak)",
+    5: "stiffness_sf_mfma_kernel<5,6> (sum-factorised: zeta on the FP64 VALU, eta and beta on FP64 MFMA, "
+       "one type per reduction)",
+}
+SF_KERNELS[2] = SF_KERNELS[4] = SF_KERNELS[0]
+
+
+@contextlib.contextmanager
+def no_gc():
+    """A timed region without Python's cyclic collector: a full collection with torch
+    loaded takes tens of milliseconds and would stall the launches of short loops."""
+    gc.collect()
+    gc.disable()
+    try:
+        yield
+    finally:
+        gc.enable()
 
 
 def gather_bytes(nelem, npe, nnode, nnz):
@@ -148,11 +173,12 @@ def bench_batch(args):
         ba.replay()
     torch.cuda.synchronize()
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    ev0.record()
-    for _ in range(args.steps):
-        ba.replay()
-    ev1.record()
-    torch.cuda.synchronize()
+    with no_gc():
+        ev0.record()
+        for _ in range(args.steps):
+            ba.replay()
+        ev1.record()
+        torch.cuda.synchronize()
     ms = ev0.elapsed_time(ev1) / args.steps
     print("batched graph: {:.3f} ms per batch".format(ms), file=sys.stderr, flush=True)
     # eager per-mesh reference: each mesh alone through the ordinary path, same variant
@@ -257,7 +283,8 @@ def bench_learned(args):
     for prec, start, forcing, csr in modes:
         ln = LockstepNewton(fam, held, precision=prec, forcing=forcing, csr=csr)
         ln.solve(start, predictor=net)           # warm-up (first-touch, kernel attributes)
-        r = ln.solve(start, predictor=net, record=True)
+        with no_gc():
+            r = ln.solve(start, predictor=net, record=True)
         final = r["trajectory"][-1]
         if ref is None:
             ref = final
@@ -326,7 +353,8 @@ def bench_newton(args):
                                linear_solver=LinearSolver(rtol=args.lin_rtol, maxit=500000, check_every=25),
                                tangent_precision=prec, csr_precision="fp32" if prec != "fp64" else "fp64",
                                report_log_level=1)
-            res = solver.Solve(formulation=Form(), mesh=mesh, material=mat, boundary_condition=bc)
+            with no_gc():
+                res = solver.Solve(formulation=Form(), mesh=mesh, material=mat, boundary_condition=bc)
             final = res.TotalDisp[:, :, -1].cpu().numpy()
             ref = final if ref is None else ref
             key = "femsolver+" + prec
@@ -343,7 +371,8 @@ def bench_newton(args):
     for prec, start, forcing, csr in (("fp64", "previous", "fixed", "fp64"), ("fp32", "extrap", "adaptive", "fp32")):
         ln = LockstepNewton(fam, [0], precision=prec, forcing=forcing, csr=csr, lin_rtol=args.lin_rtol)
         log("lockstep {} {} {} csr {}: {} dofs, nnz {}".format(prec, start, forcing, csr, 3 * ln.dm.nnode, ln.dm.nnz))
-        r = ln.solve(start, record=True, log=log)
+        with no_gc():
+            r = ln.solve(start, record=True, log=log)
         final = r["trajectory"][-1]
         ref = final if ref is None else ref
         key = "{}+{}+{}".format(prec, start, forcing) + ("+csr32" if csr == "fp32" else "")
@@ -376,6 +405,11 @@ def main():
     ap.add_argument("--stiffness-minw", type=int, default=0, help="stiffness kernel variant (waves/SIMD), 0 = default")
     ap.add_argument("--stiffness-variant", type=int, default=-1, help="p=4 stiffness tile variant (-1 = default)")
     ap.add_argument("--dense", action="store_true", help="dense MFMA stiffness even where sum factorisation applies")
+    ap.add_argument("--gather-form", type=int, default=0, choices=[0, 1, 2, 3],
+                    help="gather form bits: 1 one element at a time, 2 no single-element kernel")
+    ap.add_argument("--sf-form", type=int, default=0, choices=[0, 3, 5],
+                    help="sum-factorised stiffness form: 3 all-VALU, 5 MFMA one type per reduction (0: the "
+                         "library default, MFMA two types per reduction)")
     ap.add_argument("--workload", default="assembly", choices=["assembly", "batch", "learned", "newton"],
                     help="assembly: the headline K+T metric; batch: the batched-meshes form; "
                          "learned: warm-start training and the Newton time comparison")
@@ -415,6 +449,10 @@ def main():
         _lib.check(_lib.load().fl_set_tuning(_lib.FL_TUNE_STIFF_MINW, args.stiffness_minw), "fl_set_tuning")
     if args.dense:
         _lib.check(_lib.load().fl_set_tuning(_lib.FL_TUNE_STIFF_SF, 1), "fl_set_tuning")
+    elif args.sf_form:
+        _lib.check(_lib.load().fl_set_tuning(_lib.FL_TUNE_STIFF_SF, args.sf_form), "fl_set_tuning")
+    if args.gather_form:
+        _lib.check(_lib.load().fl_set_tuning(_lib.FL_TUNE_GATHER, args.gather_form), "fl_set_tuning")
     if args.stiffness_variant >= 0:
         _lib.check(_lib.load().fl_set_tuning(_lib.FL_TUNE_STIFF_VARIANT_BASE + 4, args.stiffness_variant + 1),
                    "fl_set_tuning")
@@ -448,11 +486,12 @@ def main():
     _lib.profile_enable(True)
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
-    start.record()
-    for _ in range(args.steps):
-        assemble_device(dm, mat, x, V, T)
-    stop.record()
-    torch.cuda.synchronize()
+    with no_gc():
+        start.record()
+        for _ in range(args.steps):
+            assemble_device(dm, mat, x, V, T)
+        stop.record()
+        torch.cuda.synchronize()
     total_ms = start.elapsed_time(stop)
     calls = _lib.profile_read()
     _lib.profile_enable(False)
@@ -471,8 +510,7 @@ def main():
     stiff_roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                   "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
                   "traffic": measured_traffic("assembly_hex_p4_24cube_pmc.md", "stiffness"),
-                  "kernel": ("stiffness_sf_kernel<5,6> (sum-factorised, FP64 VALU; MI355X's FP64 vector "
-                             "peak equals its FP64 matrix peak)" if sf else
+                  "kernel": (SF_KERNELS[args.sf_form] if sf else
                              "stiffness_kernel<128,4,4,2,2> (off-diagonal + triangular diagonal launches)"),
                   "flops_per_launch": flops_exec, "kernel_ms": round(float(phase_ms[1]), 4),
                   "dense_b_equivalent_tflops": round(dense_b / (phase_ms[1] / 1e3) / 1e12, 3),
@@ -495,11 +533,12 @@ def main():
     torch.cuda.synchronize()
     _lib.profile_enable(True)
     r0, r1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    r0.record()
-    for _ in range(args.steps):
-        residual_device(dm, mat, x, Tres)
-    r1.record()
-    torch.cuda.synchronize()
+    with no_gc():
+        r0.record()
+        for _ in range(args.steps):
+            residual_device(dm, mat, x, Tres)
+        r1.record()
+        torch.cuda.synchronize()
     res_ms = r0.elapsed_time(r1) / args.steps
     rcalls = _lib.profile_read()
     _lib.profile_enable(False)

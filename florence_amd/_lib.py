@@ -24,6 +24,7 @@ FL_PREC_BF16 = 2
 PRECISIONS = {"fp64": FL_PREC_FP64, "fp32": FL_PREC_FP32, "bf16": FL_PREC_BF16}
 FL_PREC_CSR32 = 16
 FL_TUNE_STIFF_SF = 7
+FL_TUNE_GATHER = 8
 
 # every symbol declared in include/florence_amd.h -> (restype, argtypes)
 _vp, _i32, _i64, _sz, _dp = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_size_t, ctypes.c_void_p

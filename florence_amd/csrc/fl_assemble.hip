@@ -24,6 +24,7 @@
 //                     reference's serial CSR accumulation, SparseAssemblyNative.h:32-45),
 //                     then written once, coalesced; same for T (_LowLevelAssemblyDF_.h:153-161).
 //                     No floating-point atomics: the result is bitwise reproducible.
+#include <algorithm>
 #include <cmath>
 
 #include "fl_common.hpp"
@@ -264,11 +265,108 @@ __device__ inline int find_rank(const int32_t *s, int n, int32_t v) {
     return s[lo] == v ? lo : -1;
 }
 
-// One wave per node.  Nodes are visited through node_order (grouped by neighbour count
-// so each launch sizes its LDS to its own degree class); lds_deg is that class's max.
-// GT = 64 (one wave) for nodes inside one element; GT = 256 for nodes shared by several
-// elements, whose per-element loop is the latency chain (rank search, loads, LDS adds).
-template <typename KT, int GT, typename VT = double>
+// A node inside one element: its three rows are that element's rows with the columns
+// permuted -- no accumulation.  The nine element rows (9 npe values) are read
+// coalesced into LDS (sv), then each output row is written coalesced, its values
+// picked from sv through the column map (as 0.0 + value, the accumulated form, so
+// signed zeros match the sum path).  LDS: inv (npe ints), snb (deg ints, used only
+// without a rank table), sv (9 npe doubles).
+template <typename KT, int GT, typename VT>
+__device__ __forceinline__ void gather_single_node(int64_t a, int32_t ent, int npe, int deg, int32_t p0,
+                                                   const int32_t *__restrict__ elements,
+                                                   const int32_t *__restrict__ nbrs,
+                                                   const int32_t *__restrict__ indptr, const KT *__restrict__ Ke,
+                                                   const double *__restrict__ te, const int16_t *__restrict__ ranks,
+                                                   int32_t *inv, int32_t *snb, double *sv, VT *__restrict__ V,
+                                                   double *__restrict__ T, int32_t *__restrict__ status) {
+    const int tid = threadIdx.x;
+    const int64_t npe2 = (int64_t)npe * npe;
+    const int64_t e = ent / npe;
+    const int la = ent - (int)(e * npe);
+    const KT *Kb = Ke + e * 9 * npe2 + (int64_t)la * npe;
+    // issue the row loads first (9 npe <= UF GT for npe <= 128)
+    constexpr int UF = (9 * 128 + GT - 1) / GT;
+    double v[UF];
+#pragma unroll
+    for (int u = 0; u < UF; ++u) {
+        const int idx = u * GT + tid;
+        v[u] = 0.0;
+        if (idx < 9 * npe) {
+            const int ik = idx / npe, lb = idx - ik * npe;
+            v[u] = (double)Kb[ik * npe2 + lb];
+        }
+    }
+    const double tv = tid < 3 ? te[e * 3 * npe + 3 * la + tid] : 0.0;
+    bool bad = false;
+    for (int r = tid; r < npe; r += GT) inv[r] = 0;   // position -> local column node
+    __syncthreads();
+    if (ranks) {
+        const int16_t *rk = ranks + ((int64_t)ent) * npe;
+        for (int lb = tid; lb < npe; lb += GT) inv[rk[lb]] = lb;
+    } else {
+        for (int i = tid; i < deg; i += GT) snb[i] = nbrs[p0 + i];
+        __syncthreads();
+        for (int lb = tid; lb < npe; lb += GT) {
+            const int r = find_rank(snb, deg, elements[e * npe + lb]);
+            bad |= r < 0;
+            if (r >= 0) inv[r] = lb;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < UF; ++u) {
+        const int idx = u * GT + tid;
+        if (idx < 9 * npe) sv[idx] = v[u];
+    }
+    __syncthreads();
+    const int rowlen = 3 * deg;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        VT *row = V + indptr[3 * a + i];
+        const double *si = sv + 3 * i * npe;
+        for (int c = tid; c < rowlen; c += GT) {
+            const int r = c / 3, k = c - 3 * r;
+            row[c] = (VT)(0.0 + si[k * npe + inv[r]]);
+        }
+    }
+    if (tid < 3) T[3 * a + tid] = 0.0 + tv;
+    if (bad) atomicOr(status, 4);
+}
+
+// The lowest degree class when every node in it lies inside one element: one wave per
+// node, LDS inv (npe ints) | snb (npe ints) | sv (9 npe doubles).
+template <typename KT, typename VT>
+__global__ __launch_bounds__(64) void gather_single_kernel(const int32_t *__restrict__ elements, int npe,
+                                                           const int32_t *__restrict__ node_order,
+                                                           const int32_t *__restrict__ node_elem_ptr,
+                                                           const int32_t *__restrict__ node_elem,
+                                                           const int32_t *__restrict__ nbr_ptr,
+                                                           const int32_t *__restrict__ nbrs,
+                                                           const int32_t *__restrict__ indptr,
+                                                           const KT *__restrict__ Ke, const double *__restrict__ te,
+                                                           const int16_t *__restrict__ ranks, VT *__restrict__ V,
+                                                           double *__restrict__ T, int32_t *__restrict__ status) {
+    extern __shared__ double dsm[];
+    int32_t *ism = reinterpret_cast<int32_t *>(dsm);
+    const int64_t a = node_order[blockIdx.x];
+    const int deg = nbr_ptr[a + 1] - nbr_ptr[a];
+    const int32_t q0 = node_elem_ptr[a], q1 = node_elem_ptr[a + 1];
+    if (q1 - q0 != 1 || deg != npe) {
+        if (threadIdx.x == 0) atomicOr(status, FL_S_PATTERN);
+        return;
+    }
+    gather_single_node<KT, 64, VT>(a, node_elem[q0], npe, deg, nbr_ptr[a], elements, nbrs, indptr, Ke, te, ranks,
+                                   ism, ism + npe, dsm + npe, V, T, status);
+}
+
+// One workgroup of GT threads per node.  Nodes are visited through node_order (grouped
+// by neighbour count so each launch sizes its LDS to its own degree class); lds_deg is
+// that class's max.  The rows of up to EB elements (and their tractions) are loaded
+// into registers (UB loads per lane per element, UB GT >= 9 npe) before any is added,
+// so a node's loads are in flight together; the adds then run element by element in
+// node_elem order with a barrier between elements, so the sums do not depend on timing.
+// LDS: rr (column positions, EB npe ints) | snb (lds_deg ints) | acc (9 lds_deg doubles;
+// the staging rows of a single-element node).
+template <typename KT, int GT, int EB, int UB, typename VT>
 __global__ __launch_bounds__(GT) void gather_kernel(const int32_t *__restrict__ elements, int npe,
                                                     const int32_t *__restrict__ node_order,
                                                     const int32_t *__restrict__ node_elem_ptr,
@@ -284,109 +382,92 @@ __global__ __launch_bounds__(GT) void gather_kernel(const int32_t *__restrict__ 
     const int tid = threadIdx.x;
     const int deg = nbr_ptr[a + 1] - nbr_ptr[a];
     const int32_t p0 = nbr_ptr[a];
-    double *acc = smem;                                              // 9*lds_deg
-    int32_t *snb = reinterpret_cast<int32_t *>(smem + 9 * lds_deg);  // lds_deg
-    int32_t *rr = snb + lds_deg;                                     // npe
-    const int rowlen = 3 * deg;
-    if (deg > lds_deg) {
+    int32_t *rr = reinterpret_cast<int32_t *>(smem);
+    int32_t *snb = rr + EB * npe;
+    double *acc = smem + (EB * npe + lds_deg + 1) / 2;
+    if (deg > lds_deg || 9 * npe > UB * GT) {
         if (tid == 0) atomicOr(status, FL_S_PATTERN);
         return;
     }
     const int32_t q0 = node_elem_ptr[a], q1 = node_elem_ptr[a + 1];
-    const int64_t npe2 = (int64_t)npe * npe;
     if (q1 - q0 == 1 && deg == npe) {
-        // a node inside one element: its rows are that element's rows with the columns
-        // permuted -- no accumulation, each output is read once and written coalesced
-        // (as 0.0 + value, the accumulated form, so signed zeros match the sum path)
-        const int32_t ent = node_elem[q0];
-        const int64_t e = ent / npe;
-        const int la = ent - (int)(e * npe);
-        int32_t *inv = rr;   // position -> local column node (0 where a lookup failed)
-        bool bad1 = false;
-        for (int r = tid; r < npe; r += GT) inv[r] = 0;
-        __syncthreads();
-        if (ranks) {
-            const int16_t *rk = ranks + ((int64_t)ent) * npe;
-            for (int lb = tid; lb < npe; lb += GT) inv[rk[lb]] = lb;
-        } else {
-            for (int i = tid; i < deg; i += GT) snb[i] = nbrs[p0 + i];
-            __syncthreads();
-            for (int lb = tid; lb < npe; lb += GT) {
-                const int r = find_rank(snb, deg, elements[e * npe + lb]);
-                bad1 |= r < 0;
-                if (r >= 0) inv[r] = lb;
-            }
-        }
-        __syncthreads();
-        const KT *Kb = Ke + e * 9 * npe2 + (int64_t)la * npe;
-        for (int idx = tid; idx < 9 * deg; idx += GT) {
-            const int i = idx / rowlen, c = idx - i * rowlen;
-            const int r = c / 3, k = c - 3 * r;
-            V[(int64_t)indptr[3 * a + i] + c] = (VT)(0.0 + (double)Kb[(3 * i + k) * npe2 + inv[r]]);
-        }
-        if (tid < 3) T[3 * a + tid] = 0.0 + te[e * 3 * npe + 3 * la + tid];
-        if (bad1) atomicOr(status, 4);
+        gather_single_node<KT, GT, VT>(a, node_elem[q0], npe, deg, p0, elements, nbrs, indptr, Ke, te, ranks, rr,
+                                       snb, acc, V, T, status);
         return;
     }
+    const int rowlen = 3 * deg;
+    const int64_t npe2 = (int64_t)npe * npe;
     for (int i = tid; i < 9 * deg; i += GT) acc[i] = 0.0;
-    for (int i = tid; i < deg; i += GT) snb[i] = nbrs[p0 + i];
+    if (!ranks)
+        for (int i = tid; i < deg; i += GT) snb[i] = nbrs[p0 + i];
+    // this lane's slots of an element's 9 npe values: source offset, target base, column node
+    int src[UB], tgt[UB], col[UB];
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+        const int idx = u * GT + tid;
+        src[u] = -1;
+        tgt[u] = 0;
+        col[u] = 0;
+        if (idx < 9 * npe) {
+            const int ik = idx / npe, lb = idx - ik * npe;
+            const int i = ik / 3, k = ik - 3 * i;
+            src[u] = ik * (int)npe2 + lb;
+            tgt[u] = i * rowlen + k;
+            col[u] = lb;
+        }
+    }
     __syncthreads();
     bool bad = false;
-    for (int32_t q = q0; q < q1; ++q) {
-        const int32_t ent = node_elem[q];
-        const int64_t e = ent / npe;
-        const int la = ent - (int)(e * npe);
-        if (ranks) {
-            // column positions precomputed once per mesh (fl_pattern_rank_table)
-            const int16_t *rk = ranks + ((int64_t)ent) * npe;
-            for (int lb = tid; lb < npe; lb += GT) rr[lb] = rk[lb];
-        } else {
-            for (int lb = tid; lb < npe; lb += GT) {
-                const int r = find_rank(snb, deg, elements[e * npe + lb]);
+    double tsum = 0.0;
+    for (int32_t c0 = q0; c0 < q1; c0 += EB) {
+        const int ne = min(EB, (int)(q1 - c0));
+        // column positions of the chunk's elements (precomputed per mesh, or searched)
+        for (int t = tid; t < ne * npe; t += GT) {
+            const int j = t / npe, lb = t - j * npe;
+            const int32_t ent = node_elem[c0 + j];
+            int r;
+            if (ranks) {
+                r = ranks[(int64_t)ent * npe + lb];
+            } else {
+                r = find_rank(snb, deg, elements[(int64_t)(ent / npe) * npe + lb]);
                 bad |= r < 0;
-                rr[lb] = r < 0 ? 0 : r;
+                r = r < 0 ? 0 : r;
+            }
+            rr[t] = r;
+        }
+        double v[EB][UB], tv[EB];
+#pragma unroll
+        for (int j = 0; j < EB; ++j) {
+            tv[j] = 0.0;
+            if (j < ne) {
+                const int32_t ent = node_elem[c0 + j];
+                const int64_t e = ent / npe;
+                const int la = ent - (int)(e * npe);
+                const KT *Kb = Ke + e * 9 * npe2 + (int64_t)la * npe;
+#pragma unroll
+                for (int u = 0; u < UB; ++u) v[j][u] = src[u] >= 0 ? (double)Kb[src[u]] : 0.0;
+                if (tid < 3) tv[j] = te[e * 3 * npe + 3 * la + tid];
             }
         }
         __syncthreads();
-        const KT *Kb = Ke + e * 9 * npe2 + (int64_t)la * npe;
-        // batches of UB independent loads per lane keep enough bytes in flight; the LDS
-        // updates follow (within one element every target entry is distinct)
-        constexpr int UB = 8;
-        for (int base = 0; base < 9 * npe; base += GT * UB) {
-            double v[UB];
-            int pos[UB];
 #pragma unroll
-            for (int u = 0; u < UB; ++u) {
-                const int idx = base + u * GT + tid;
-                pos[u] = -1;
-                v[u] = 0.0;
-                if (idx < 9 * npe) {
-                    const int ik = idx / npe, lb = idx - ik * npe;
-                    const int i = ik / 3, k = ik - 3 * i;
-                    v[u] = Kb[ik * npe2 + lb];
-                    pos[u] = i * rowlen + 3 * rr[lb] + k;
-                }
+        for (int j = 0; j < EB; ++j) {
+            if (j < ne) {
+#pragma unroll
+                for (int u = 0; u < UB; ++u)
+                    if (src[u] >= 0) acc[tgt[u] + 3 * rr[j * npe + col[u]]] += v[j][u];
+                tsum += tv[j];
+                __syncthreads();
             }
+        }
+    }
 #pragma unroll
-            for (int u = 0; u < UB; ++u)
-                if (pos[u] >= 0) acc[pos[u]] += v[u];
-        }
-        __syncthreads();
+    for (int i = 0; i < 3; ++i) {
+        VT *row = V + indptr[3 * a + i];
+        const double *ai = acc + i * rowlen;
+        for (int c = tid; c < rowlen; c += GT) row[c] = (VT)ai[c];
     }
-    for (int idx = tid; idx < 9 * deg; idx += GT) {
-        const int i = idx / rowlen, c = idx - i * rowlen;
-        V[(int64_t)indptr[3 * a + i] + c] = (VT)acc[idx];
-    }
-    if (tid < 3) {
-        double s = 0.0;
-        for (int32_t q = q0; q < q1; ++q) {
-            const int32_t ent = node_elem[q];
-            const int64_t e = ent / npe;
-            const int la = ent - (int)(e * npe);
-            s += te[e * 3 * npe + 3 * la + tid];
-        }
-        T[3 * a + tid] = s;
-    }
+    if (tid < 3) T[3 * a + tid] = tsum;
     if (bad) atomicOr(status, 4);
 }
 
@@ -408,9 +489,41 @@ __global__ void gather_traction_kernel(int npe, const int32_t *__restrict__ node
 }
 
 // ------------------------------------------------------------------ host side
-// accumulator (9 deg doubles), neighbour list (deg ints), column positions (npe ints)
-static size_t gather_lds_bytes(int deg, int npe) {
-    return (size_t)deg * 9 * 8 + (size_t)deg * 4 + (size_t)npe * 4 + 16;
+// Gather launch shapes (threads per node, elements per chunk, loads per lane per
+// element).  Classes above the lowest: <256, EB, 5> with EB = 2, 4 or 8 as the class's
+// degree suggests its nodes' element count (deg <= 2 npe, <= 4 npe, more): chunks
+// longer than a node's element count only cost registers.  The lowest class: one wave
+// per node, <64, 8, 2> for npe <= 14, <64, 4, 9> for npe <= 64 and <64, 2, 18> up to
+// 128; when every node in it lies inside one element (its degree equals npe: a node
+// of two distinct elements has more than npe neighbours) gather_single_kernel.
+enum GatherForm { G_SINGLE, G_W8X2, G_W4X9, G_W2X18, G_WIDE1, G_WIDE2, G_WIDE4, G_WIDE8 };
+
+// FL_TUNE_GATHER: bit 0 = the wide classes one element at a time (<256, 1, 5>), bit 1 =
+// the single-element class through gather_kernel<64, 2, 18>
+static GatherForm gather_form(int b, int deg, int npe) {
+    const int t = tuning_value(TUNE_GATHER, 0);
+    if (b > 0) {
+        if (t & 1) return G_WIDE1;
+        return deg <= 2 * npe ? G_WIDE2 : (deg <= 4 * npe ? G_WIDE4 : G_WIDE8);
+    }
+    if (deg == npe && !(t & 2)) return G_SINGLE;
+    return npe <= 14 ? G_W8X2 : (npe <= 64 ? G_W4X9 : G_W2X18);
+}
+
+static int gather_eb(GatherForm f) {
+    switch (f) {
+        case G_W8X2: case G_WIDE8: return 8;
+        case G_W4X9: case G_WIDE4: return 4;
+        case G_W2X18: case G_WIDE2: return 2;
+        default: return 1;
+    }
+}
+
+// LDS of one launch: gather_single_kernel's inv + snb + sv, or gather_kernel's
+// rr (eb npe ints) | snb (deg ints) | acc (9 deg doubles)
+static size_t gather_lds_bytes(GatherForm f, int deg, int npe) {
+    if (f == G_SINGLE) return (size_t)8 * npe + (size_t)72 * npe + 16;
+    return (size_t)(gather_eb(f) * npe + deg + 1) / 2 * 8 + (size_t)deg * 9 * 8 + 16;
 }
 
 static int padded_np(int npe) {
@@ -480,14 +593,17 @@ static int launch_stiffness_sf(int npe, int G, const double *qp, const void *bas
     }
     const dim3 grid((unsigned)groups);
     const int32_t *b = static_cast<const int32_t *>(basis);
-    // value 3 of FL_TUNE_STIFF_SF: the all-VALU form; default: eta and beta on MFMA
-    const bool valu = tuning_value(TUNE_STIFF_SF, 0) == 3;
+    // FL_TUNE_STIFF_SF value 3: the all-VALU form; 5: eta and beta on MFMA one type at a
+    // time; default: two types per MFMA reduction
+    const int form = tuning_value(TUNE_STIFF_SF, 0);
 #define FL_SF_CASE(P, QQ)                                                                  \
     if (npe == P * P * P && G == QQ * QQ * QQ) {                                           \
-        if (valu)                                                                          \
+        if (form == 3)                                                                     \
             stiffness_sf_kernel<P, QQ><<<grid, 256, 0, st>>>(qp, b, nelem, Ke);            \
-        else                                                                               \
+        else if (form == 5)                                                                \
             stiffness_sf_mfma_kernel<P, QQ><<<grid, 256, 0, st>>>(qp, b, nelem, Ke);       \
+        else                                                                               \
+            stiffness_sf_mfma2_kernel<P, QQ><<<grid, 256, 0, st>>>(qp, b, nelem, Ke);      \
         FL_LAUNCH_CHECK();                                                                 \
         return 0;                                                                          \
     }
@@ -545,19 +661,58 @@ static int launch_stiffness_bf16(int NP, const double *JmT, const double *qp, in
     return 0;
 }
 
-template <typename KT, int GTB, typename VT>
-static int launch_gather(int32_t nb, size_t lds, const int32_t *elements, int npe, const int32_t *order,
+template <typename KT, int GT, int EB, int UB, typename VT>
+static int launch_gather_t(int32_t nb, size_t lds, const int32_t *elements, int npe, const int32_t *order,
+                           const int32_t *node_elem_ptr, const int32_t *node_elem, const int32_t *nbr_ptr,
+                           const int32_t *nbrs, const int32_t *indptr, const KT *Ke, const double *te, int deg,
+                           const int16_t *ranks, VT *V, double *T, int32_t *status, hipStream_t st) {
+    if (lds > 64 * 1024)
+        FL_HIP_CHECK(hipFuncSetAttribute((const void *)gather_kernel<KT, GT, EB, UB, VT>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    gather_kernel<KT, GT, EB, UB, VT><<<(unsigned)nb, GT, lds, st>>>(elements, npe, order, node_elem_ptr,
+                                                                     node_elem, nbr_ptr, nbrs, indptr, Ke, te, deg,
+                                                                     ranks, V, T, status);
+    FL_LAUNCH_CHECK();
+    return 0;
+}
+
+// one degree class's gather launch
+template <typename KT, typename VT>
+static int launch_gather(int b, int32_t nb, const int32_t *elements, int npe, const int32_t *order,
                          const int32_t *node_elem_ptr, const int32_t *node_elem, const int32_t *nbr_ptr,
                          const int32_t *nbrs, const int32_t *indptr, const KT *Ke, const double *te, int deg,
                          const int16_t *ranks, VT *V, double *T, int32_t *status, hipStream_t st) {
-    if (lds > 64 * 1024)
-        FL_HIP_CHECK(hipFuncSetAttribute((const void *)gather_kernel<KT, GTB, VT>,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    gather_kernel<KT, GTB, VT><<<(unsigned)nb, GTB, lds, st>>>(elements, npe, order, node_elem_ptr, node_elem,
-                                                               nbr_ptr, nbrs, indptr, Ke, te, deg, ranks, V, T,
-                                                               status);
-    FL_LAUNCH_CHECK();
-    return 0;
+    const GatherForm f = gather_form(b, deg, npe);
+    const size_t lds = gather_lds_bytes(f, deg, npe);
+    switch (f) {
+        case G_SINGLE:
+            gather_single_kernel<KT, VT><<<(unsigned)nb, 64, lds, st>>>(elements, npe, order, node_elem_ptr,
+                                                                       node_elem, nbr_ptr, nbrs, indptr, Ke, te,
+                                                                       ranks, V, T, status);
+            FL_LAUNCH_CHECK();
+            return 0;
+        case G_W8X2:
+            return launch_gather_t<KT, 64, 8, 2, VT>(nb, lds, elements, npe, order, node_elem_ptr, node_elem, nbr_ptr,
+                                                     nbrs, indptr, Ke, te, deg, ranks, V, T, status, st);
+        case G_W4X9:
+            return launch_gather_t<KT, 64, 4, 9, VT>(nb, lds, elements, npe, order, node_elem_ptr, node_elem, nbr_ptr,
+                                                     nbrs, indptr, Ke, te, deg, ranks, V, T, status, st);
+        case G_W2X18:
+            return launch_gather_t<KT, 64, 2, 18, VT>(nb, lds, elements, npe, order, node_elem_ptr, node_elem,
+                                                      nbr_ptr, nbrs, indptr, Ke, te, deg, ranks, V, T, status, st);
+        case G_WIDE1:
+            return launch_gather_t<KT, 256, 1, 5, VT>(nb, lds, elements, npe, order, node_elem_ptr, node_elem,
+                                                      nbr_ptr, nbrs, indptr, Ke, te, deg, ranks, V, T, status, st);
+        case G_WIDE2:
+            return launch_gather_t<KT, 256, 2, 5, VT>(nb, lds, elements, npe, order, node_elem_ptr, node_elem,
+                                                      nbr_ptr, nbrs, indptr, Ke, te, deg, ranks, V, T, status, st);
+        case G_WIDE4:
+            return launch_gather_t<KT, 256, 4, 5, VT>(nb, lds, elements, npe, order, node_elem_ptr, node_elem,
+                                                      nbr_ptr, nbrs, indptr, Ke, te, deg, ranks, V, T, status, st);
+        default:
+            return launch_gather_t<KT, 256, 8, 5, VT>(nb, lds, elements, npe, order, node_elem_ptr, node_elem,
+                                                      nbr_ptr, nbrs, indptr, Ke, te, deg, ranks, V, T, status, st);
+    }
 }
 
 struct DFWork {
@@ -705,7 +860,8 @@ int fl_df_assemble_mt(int32_t material, const double *params, const int32_t *ele
         return FL_E_SPACE;
     }
     for (int b = 0; b < FL_GATHER_BUCKETS; ++b) {
-        if (bucket_offsets[b + 1] > bucket_offsets[b] && gather_lds_bytes(bucket_degree[b], npe) > 160 * 1024) {
+        if (bucket_offsets[b + 1] > bucket_offsets[b] &&
+            gather_lds_bytes(gather_form(b, bucket_degree[b], npe), bucket_degree[b], npe) > 160 * 1024) {
             set_error("fl_df_assemble: node degree %d needs more LDS than a CU has", bucket_degree[b]);
             return FL_E_LIMIT;
         }
@@ -754,33 +910,21 @@ int fl_df_assemble_mt(int32_t material, const double *params, const int32_t *ele
     for (int b = 0; b < FL_GATHER_BUCKETS; ++b) {
         const int32_t nb = bucket_offsets[b + 1] - bucket_offsets[b];
         if (nb <= 0) continue;
-        // the lowest class (nodes interior to one element) keeps one wave per node
-        const bool wide = b > 0;
-        const size_t lds = gather_lds_bytes(bucket_degree[b], npe);
         int rc2 = 0;
         const int32_t *ord = node_order + bucket_offsets[b];
         const int dg = bucket_degree[b];
         if (precision == FL_PREC_FP64)
-            rc2 = wide ? launch_gather<double, 256>(nb, lds, elements, npe, ord, node_elem_ptr, node_elem, node_nbr_ptr,
-                                                    node_nbrs, indptr, (const double *)w.Ke, w.te, dg, ranks, V, T,
-                                                    status, st)
-                       : launch_gather<double, 64>(nb, lds, elements, npe, ord, node_elem_ptr, node_elem, node_nbr_ptr,
-                                                   node_nbrs, indptr, (const double *)w.Ke, w.te, dg, ranks, V, T,
-                                                   status, st);
+            rc2 = launch_gather<double, double>(b, nb, elements, npe, ord, node_elem_ptr, node_elem, node_nbr_ptr,
+                                                node_nbrs, indptr, (const double *)w.Ke, w.te, dg, ranks, V, T,
+                                                status, st);
         else if (!v32)
-            rc2 = wide ? launch_gather<float, 256>(nb, lds, elements, npe, ord, node_elem_ptr, node_elem, node_nbr_ptr,
-                                                   node_nbrs, indptr, (const float *)w.Ke, w.te, dg, ranks, V, T,
-                                                   status, st)
-                       : launch_gather<float, 64>(nb, lds, elements, npe, ord, node_elem_ptr, node_elem, node_nbr_ptr,
-                                                  node_nbrs, indptr, (const float *)w.Ke, w.te, dg, ranks, V, T,
-                                                  status, st);
+            rc2 = launch_gather<float, double>(b, nb, elements, npe, ord, node_elem_ptr, node_elem, node_nbr_ptr,
+                                               node_nbrs, indptr, (const float *)w.Ke, w.te, dg, ranks, V, T,
+                                               status, st);
         else
-            rc2 = wide ? launch_gather<float, 256>(nb, lds, elements, npe, ord, node_elem_ptr, node_elem, node_nbr_ptr,
-                                                   node_nbrs, indptr, (const float *)w.Ke, w.te, dg, ranks,
-                                                   reinterpret_cast<float *>(V), T, status, st)
-                       : launch_gather<float, 64>(nb, lds, elements, npe, ord, node_elem_ptr, node_elem, node_nbr_ptr,
-                                                  node_nbrs, indptr, (const float *)w.Ke, w.te, dg, ranks,
-                                                  reinterpret_cast<float *>(V), T, status, st);
+            rc2 = launch_gather<float, float>(b, nb, elements, npe, ord, node_elem_ptr, node_elem, node_nbr_ptr,
+                                              node_nbrs, indptr, (const float *)w.Ke, w.te, dg, ranks,
+                                              reinterpret_cast<float *>(V), T, status, st);
         if (rc2) return rc2;
     }
     prof_mark(st);

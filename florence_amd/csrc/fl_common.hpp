@@ -50,7 +50,7 @@ struct Carve {
 };
 
 // Kernel-variant knobs (set through fl_set_tuning; 0 = library default).
-enum TuneKey { TUNE_STIFF_MINW = 1, TUNE_STIFF_VARIANT_BASE = 2, TUNE_STIFF_SF = 7, TUNE_KEYS = 8 };
+enum TuneKey { TUNE_STIFF_MINW = 1, TUNE_STIFF_VARIANT_BASE = 2, TUNE_STIFF_SF = 7, TUNE_GATHER = 8, TUNE_KEYS = 9 };
 int tuning_value(int key, int dflt);
 
 // Exclusive prefix sum of int64 values in place (n entries, last entry = total), device.

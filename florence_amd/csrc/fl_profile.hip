@@ -16,6 +16,18 @@ struct CallRecord {
 std::mutex g_mu;
 bool g_on = false;
 std::vector<CallRecord> g_calls;
+// events are reused: creating one per mark stalls the host for milliseconds once many
+// exist, which would show up as idle GPU time between short launches
+std::vector<hipEvent_t> g_pool;
+
+bool take_event(hipEvent_t *e) {
+    if (!g_pool.empty()) {
+        *e = g_pool.back();
+        g_pool.pop_back();
+        return true;
+    }
+    return hipEventCreate(e) == hipSuccess;
+}
 }  // namespace
 
 void prof_begin(hipStream_t st, int kind) {
@@ -24,7 +36,7 @@ void prof_begin(hipStream_t st, int kind) {
     CallRecord r;
     r.kind = kind;
     hipEvent_t e;
-    if (hipEventCreate(&e) != hipSuccess) return;
+    if (!take_event(&e)) return;
     hipEventRecord(e, st);
     r.ev.push_back(e);
     g_calls.push_back(r);
@@ -34,13 +46,13 @@ void prof_mark(hipStream_t st) {
     std::lock_guard<std::mutex> lk(g_mu);
     if (!g_on || g_calls.empty()) return;
     hipEvent_t e;
-    if (hipEventCreate(&e) != hipSuccess) return;
+    if (!take_event(&e)) return;
     hipEventRecord(e, st);
     g_calls.back().ev.push_back(e);
 }
 
 namespace {
-int g_tune[TUNE_KEYS] = {0, 0, 0, 0, 0, 0, 0, 0};
+int g_tune[TUNE_KEYS] = {};
 }
 
 int tuning_value(int key, int dflt) {
@@ -51,7 +63,7 @@ int tuning_value(int key, int dflt) {
 
 static void clear_locked() {
     for (auto &c : g_calls)
-        for (auto e : c.ev) hipEventDestroy(e);
+        for (auto e : c.ev) g_pool.push_back(e);
     g_calls.clear();
 }
 

@@ -18,12 +18,14 @@
 //
 // where F_d[a,b][q] = T_d^{s}[a][q] T_d^{t}[b][q] with (s, t) = ([m = d], [n = d]) and
 // "type" groups the nine (m, n) by their beta factor.  At p=4 with 6^3 points this is
-// ~1.1 MFLOP per block instead of 20 MFLOP for the dense contraction.  The FP64 vector
-// rate of MI355X equals its FP64 matrix rate, so the work runs on the VALU from LDS.
+// ~1.1 MFLOP per block instead of 20 MFLOP for the dense contraction.  Three forms:
+// stiffness_sf_kernel (all on the FP64 VALU, whose rate on MI355X equals the FP64
+// matrix rate), stiffness_sf_mfma_kernel (eta and beta on 16x16x4 FP64 MFMA, one type
+// per reduction) and stiffness_sf_mfma2_kernel (the default: two types per reduction).
 //
 // One workgroup per (element, block i,k) -- all nine blocks, so every workgroup writes
 // its own rows contiguously and no block is stored transposed.  The az slices are
-// processed one after another to keep T1 and T2 small (two workgroups per CU).
+// processed one after another to keep T1 and T2 small.
 //
 // Tensor basis table (fl_tensor_basis layout, built once on the host):
 //   int32  P1, Q, pad, pad
@@ -330,6 +332,181 @@ __global__ __launch_bounds__(256, 4) void stiffness_sf_mfma_kernel(const double 
                         }
                     }
                 }
+            }
+        }
+    }
+}
+
+// The same contraction with the inner indices of two types merged into one MFMA
+// reduction: the two types sharing the beta factor of m (a "phase") are contracted
+// together, K = (type, qb) of length 2Q for beta and K = (m, n, qe) of length
+// pairs x Q for eta, so Q = 6 fills K exactly (3 steps of 4 instead of 2 x 2 padded
+// ones) and an az slice needs four barriers instead of nine.  Each wave owns every
+// row tile of its column tiles, so an A fragment (a product of two basis values)
+// serves TNW tiles and a B fragment MT3.  LDS: T2 of the phase's two types (~42 KB at
+// p = 4, three workgroups per CU).
+template <int P1, int Q>
+__global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(const double *__restrict__ qp,
+                                                                    const int32_t *__restrict__ basis,
+                                                                    int64_t nelem, double *__restrict__ Ke) {
+    typedef SfLayout<P1, Q> L;
+    typedef Mfma16<double> MF;
+    constexpr int Q3 = L::Q3, N3 = L::N3, PP = L::PP;
+    constexpr int NT = 256;
+    constexpr int MT2 = (PP + 15) / 16, NT2 = (P1 * Q + 15) / 16;
+    constexpr int MT3 = (PP + 15) / 16, NT3 = (N3 + 15) / 16, TNW = (NT3 + 3) / 4;
+    constexpr int KE_MAX = (4 * Q + 3) / 4, KB = (2 * Q + 3) / 4;
+    __shared__ double sF[3][2][P1][Q];
+    __shared__ double sC[9][Q3];
+    __shared__ double sT1[9][P1][Q * Q];
+    __shared__ double sT2[2][PP][P1][Q];
+    __shared__ int32_t sLoc[N3];
+
+    const int64_t wid = blockIdx.x;
+    const int xcd = (int)(wid & 7);
+    const int64_t slot = wid >> 3;
+    const int blk = (int)(slot % 9);
+    const int64_t e = (slot / 9) * 8 + xcd;
+    if (e >= nelem) return;
+    const int pi = blk / 3, pk = blk % 3;
+    const int sp = pi == pk ? pi : (pi + pk == 1 ? 3 : (pi + pk == 2 ? 4 : 5));
+    const bool tr = pi > pk;
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
+    const int32_t *g_of_t = basis + 4;
+    const int32_t *loc = basis + 4 + Q3;
+    const double *f1d = reinterpret_cast<const double *>(basis + L::INTS);
+
+    for (int t = tid; t < 3 * 2 * P1 * Q; t += NT) (&sF[0][0][0][0])[t] = f1d[t];
+    for (int t = tid; t < N3; t += NT) sLoc[t] = min(max(loc[t], 0), N3 - 1);
+    const double *qe_ = qp + e * (int64_t)(Q3)*QS;
+    for (int t = tid; t < Q3; t += NT) {
+        const double *rec = qe_ + (int64_t)min(max(g_of_t[t], 0), Q3 - 1) * QS;
+        double ix[9], D[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) ix[i] = rec[Q_IX + i];
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int l = 0; l < 3; ++l) D[3 * j + l] = rec[Q_D + 9 * sp + (tr ? 3 * l + j : 3 * j + l)];
+        double W[9];
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int n = 0; n < 3; ++n)
+                W[3 * j + n] = D[3 * j] * ix[n] + D[3 * j + 1] * ix[3 + n] + D[3 * j + 2] * ix[6 + n];
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+#pragma unroll
+            for (int n = 0; n < 3; ++n)
+                sC[3 * m + n][t] = ix[m] * W[n] + ix[3 + m] * W[3 + n] + ix[6 + m] * W[6 + n];
+    }
+    __syncthreads();
+
+    double *Kd = Ke + (e * 9 + pi * 3 + pk) * (int64_t)N3 * N3;
+    for (int az = 0; az < P1; ++az) {
+        // zeta (VALU): task (mn, bz, qe) forms its factor once and sweeps qb
+        for (int t = tid; t < 9 * P1 * Q; t += NT) {
+            const int mn = t / (P1 * Q), bz = (t / Q) % P1, qe = t % Q;
+            const int m = mn / 3, n = mn % 3;
+            double f[Q];
+#pragma unroll
+            for (int qz = 0; qz < Q; ++qz) f[qz] = sF[0][m == 0][az][qz] * sF[0][n == 0][bz][qz];
+#pragma unroll
+            for (int qb = 0; qb < Q; ++qb) {
+                const double *c = sC[mn] + (qb * Q + qe) * Q;
+                double s = 0.0;
+#pragma unroll
+                for (int qz = 0; qz < Q; ++qz) s += f[qz] * c[qz];
+                sT1[mn][bz][qe * Q + qb] = s;
+            }
+        }
+        __syncthreads();
+        typename MF::acc_t acc[MT3][TNW];
+#pragma unroll
+        for (int tm = 0; tm < MT3; ++tm)
+#pragma unroll
+            for (int v = 0; v < TNW; ++v) acc[tm][v] = typename MF::acc_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            // eta (MFMA) for the phase's types (m's beta factor h, n's tl): rows (ae,be),
+            // columns (bz,qb), K = (m, n, qe) over the type's (m, n)
+            for (int task = wave; task < 2 * MT2 * NT2; task += 4) {
+                const int tl = task / (MT2 * NT2), tile = task - tl * (MT2 * NT2);
+                const int tm = tile / NT2, tn = tile - tm * NT2;
+                const int arow = tm * 16 + li, bcol = tn * 16 + li;
+                const int ae_ = min(arow / P1, P1 - 1), be_ = arow % P1;
+                const int bz_ = min(bcol / Q, P1 - 1), qb_ = bcol % Q;
+                const int nn = tl ? 1 : 2;
+                const int ke = (h ? 1 : 2) * nn * Q;
+                typename MF::acc_t a2 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int ks = 0; ks < KE_MAX; ++ks) {
+                    if (ks * 4 < ke) {
+                        const int kk = ks * 4 + lk;
+                        double a = 0.0, b = 0.0;
+                        if (kk < ke) {
+                            const int pr = kk / Q, qe = kk - pr * Q;
+                            const int m = h ? 2 : pr / nn, n = tl ? 2 : pr % nn;
+                            if (arow < PP) a = sF[1][m == 1][ae_][qe] * sF[1][n == 1][be_][qe];
+                            if (bcol < P1 * Q) b = sT1[3 * m + n][bz_][qe * Q + qb_];
+                        }
+                        a2 = MF::step(a, b, a2);
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = tm * 16 + MF::row(lk, r);
+                    if (row < PP && bcol < P1 * Q) sT2[tl][row][bz_][qb_] = a2[r];
+                }
+            }
+            __syncthreads();
+            // beta (MFMA): rows (ab,bb), columns (ae,be,bz), K = (tl, qb)
+#pragma unroll
+            for (int ks = 0; ks < KB; ++ks) {
+                const int kk = ks * 4 + lk;
+                const bool kv = kk < 2 * Q;
+                const int tl = kv ? kk / Q : 0, qb = kv ? kk - (kk / Q) * Q : 0;
+                double af[MT3], bf[TNW];
+#pragma unroll
+                for (int tm = 0; tm < MT3; ++tm) {
+                    const int arow = tm * 16 + li;
+                    const int ab_ = min(arow / P1, P1 - 1), bb_ = arow % P1;
+                    af[tm] = (kv && arow < PP) ? sF[2][h][ab_][qb] * sF[2][tl][bb_][qb] : 0.0;
+                }
+#pragma unroll
+                for (int v = 0; v < TNW; ++v) {
+                    const int bcol = (wave + 4 * v) * 16 + li;
+                    const int ae_ = min(bcol / PP, P1 - 1), be_ = (bcol / P1) % P1, bz_ = bcol % P1;
+                    bf[v] = (kv && bcol < N3) ? sT2[tl][ae_ * P1 + be_][bz_][qb] : 0.0;
+                }
+#pragma unroll
+                for (int tm = 0; tm < MT3; ++tm)
+#pragma unroll
+                    for (int v = 0; v < TNW; ++v)
+                        if (wave + 4 * v < NT3) acc[tm][v] = MF::step(af[tm], bf[v], acc[tm][v]);
+            }
+            // the next phase's eta rewrites sT2; the next slice's zeta only rewrites sT1,
+            // whose last reads precede the barrier after this phase's eta
+            if (h == 0) __syncthreads();
+        }
+#pragma unroll
+        for (int v = 0; v < TNW; ++v) {
+            const int bcol = (wave + 4 * v) * 16 + li;
+            if (wave + 4 * v < NT3 && bcol < N3) {
+                const int ae_ = bcol / PP, be_ = (bcol / P1) % P1, bz_ = bcol % P1;
+#pragma unroll
+                for (int tm = 0; tm < MT3; ++tm)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int row = tm * 16 + MF::row(lk, r);
+                        if (row < PP) {
+                            const int ab = row / P1, bb = row - ab * P1;
+                            const int la = sLoc[(ab * P1 + ae_) * P1 + az];
+                            const int lb = sLoc[(bb * P1 + be_) * P1 + bz_];
+                            Kd[(int64_t)la * N3 + lb] = acc[tm][v][r];
+                        }
+                    }
             }
         }
     }

@@ -163,7 +163,10 @@ int fl_df_assemble_mt(int32_t material, const double *params, const int32_t *ele
  *   (ab*p1 + ae)*p1 + az); zero int32 padding to 8 bytes; double f1d[3][2][p1][q]
  *   (the 1D bases and their derivatives of directions zeta, eta, beta at that
  *   direction's points).  Supported (p1, q): (2,2) (3,3) (4,4) (5,5) (5,6).  NULL, an
- *   FP32/BF16 precision or fl_set_tuning(FL_TUNE_STIFF_SF, 1) select the dense path. */
+ *   FP32/BF16 precision or fl_set_tuning(FL_TUNE_STIFF_SF, 1) select the dense path;
+ *   values 3 (all-VALU contraction) and 5 (MFMA, one basis-factor type per reduction)
+ *   select other forms of the factorised kernel (default: MFMA, two types per
+ *   reduction). */
 #define FL_TUNE_STIFF_SF 7
 size_t fl_tensor_basis_bytes(int32_t p1, int32_t q);
 
@@ -233,6 +236,10 @@ int fl_profile_enable(int on);
  * 128: v1 = one wave per SIMD, all pairs square; v2 = off-diagonal pairs on 1x4 waves
  * of 8x2 tiles).  Used by the autotuner. */
 #define FL_TUNE_STIFF_VARIANT_BASE 2
+/* key 8: gather form -- bit 0: nodes of several elements add one element's rows at a
+ * time instead of loading eight elements' rows first; bit 1: nodes inside one element
+ * go through the general kernel instead of the single-element one.  Same V and T. */
+#define FL_TUNE_GATHER 8
 int fl_set_tuning(int32_t key, int32_t value);
 int fl_profile_read(int32_t *kinds, double *phase_ms, int32_t max_calls, int32_t max_phases);
 

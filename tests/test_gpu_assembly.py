@@ -158,11 +158,15 @@ def test_sum_factorised_equals_dense(cuda, p):
     V2, T2 = V2.cpu().numpy(), T2.cpu().numpy()
     assert np.abs(V - V2).max() <= 1e-13 * np.abs(V2).max()
     assert np.abs(T - T2).max() <= 1e-13 * np.abs(T2).max()
-    # the all-VALU form of the factorised contraction (FL_TUNE_STIFF_SF = 3)
-    _lib.check(lib.fl_set_tuning(_lib.FL_TUNE_STIFF_SF, 3), "fl_set_tuning")
-    try:
-        V3, _, _ = assemble_device(dm, mat, x)
-        torch.cuda.synchronize()
-    finally:
-        lib.fl_set_tuning(_lib.FL_TUNE_STIFF_SF, 0)
-    assert np.abs(V - V3.cpu().numpy()).max() <= 1e-13 * np.abs(V2).max()
+    # the all-VALU form of the factorised contraction (FL_TUNE_STIFF_SF = 3) and the
+    # form with one type per MFMA reduction (5)
+    for form in (3, 5):
+        _lib.check(lib.fl_set_tuning(_lib.FL_TUNE_STIFF_SF, form), "fl_set_tuning")
+        try:
+            V3, T3, _ = assemble_device(dm, mat, x)
+            torch.cuda.synchronize()
+        finally:
+            lib.fl_set_tuning(_lib.FL_TUNE_STIFF_SF, 0)
+        print("form {}: |dV|/|V| = {:.3e}".format(form, np.abs(V - V3.cpu().numpy()).max() / np.abs(V2).max()))
+        assert np.abs(V - V3.cpu().numpy()).max() <= 1e-13 * np.abs(V2).max(), form
+        assert np.array_equal(T, T3.cpu().numpy()), form
