@@ -87,48 +87,18 @@ __global__ void transpose_jm_kernel(const double *__restrict__ Jm, int npe, int 
 }
 
 // ------------------------------------------------------------------ phase 1
-// elem_mat/mat_table (optional): per-element material parameters {mu, mu1, mu2, lamb}
+// Per-point kinematics, kinetics and record of one quadrature point from its parent
+// Jacobians PX[3m+c] = dX_c/dxi_m, Px = dx_c/dxi_m (_KinematicMeasures_.h:64-119,
+// _NeoHookean_.h:27-51, _MooneyRivlin_.h:30-70).  valid = false lanes only join the
+// wave's status ballot.
 template <bool TANGENT>
-__global__ __launch_bounds__(256) void qp_kernel(MatParams mp0, const double *__restrict__ points,
-                                                 const int32_t *__restrict__ elements,
-                                                 const double *__restrict__ Eulerx, const double *__restrict__ Jm,
-                                                 const double *__restrict__ w, int G, int64_t nelem, int npe,
-                                                 int update, double *__restrict__ qp, int32_t *__restrict__ status,
-                                                 const int32_t *__restrict__ elem_mat,
-                                                 const double *__restrict__ mat_table) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nelem * G) return;
-    const int64_t e = t / G;
-    const int g = (int)(t - e * G);
-    MatParams mp = mp0;
-    if (elem_mat) {
-        const double *row = mat_table + 4 * (int64_t)elem_mat[e];
-        mp.mu = row[0];
-        mp.mu1 = row[1];
-        mp.mu2 = row[2];
-        mp.lamb = row[3];
-    }
-    double PX[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, Px[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    const int32_t *conn = elements + e * npe;
-    for (int a = 0; a < npe; ++a) {
-        const int64_t nd = conn[a];
-        const double X0 = points[3 * nd], X1 = points[3 * nd + 1], X2 = points[3 * nd + 2];
-        const double x0 = Eulerx[3 * nd], x1 = Eulerx[3 * nd + 1], x2 = Eulerx[3 * nd + 2];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            const double jm = Jm[((int64_t)i * npe + a) * G + g];
-            PX[3 * i + 0] += jm * X0;
-            PX[3 * i + 1] += jm * X1;
-            PX[3 * i + 2] += jm * X2;
-            Px[3 * i + 0] += jm * x0;
-            Px[3 * i + 1] += jm * x1;
-            Px[3 * i + 2] += jm * x2;
-        }
-    }
+__device__ __forceinline__ void qp_point(const MatParams &mp, const double (&PX)[9], const double (&Px)[9], double wg,
+                                         int update, bool valid, double *__restrict__ rec,
+                                         int32_t *__restrict__ status) {
     double iX[9], ix[9];
     const double detX = invdet3x3(PX, iX);
     const double detx = invdet3x3(Px, ix);
-    const double detJ = w[g] * fabs(update == 1 ? detx : detX);
+    const double detJ = wg * fabs(update == 1 ? detx : detX);
     // F_lj = sum_m (dx_l/dxi_m)(dxi_m/dX_j) = sum_m Px[m][l] iX[j][m]
     double F[9];
 #pragma unroll
@@ -172,15 +142,16 @@ __global__ __launch_bounds__(256) void qp_kernel(MatParams mp0, const double *__
         a2 = mp.lamb * (2.0 * J - 1.0);
     }
     // status: det F <= 0 or non-finite values (wave-reduced, one atomic per wave)
-    bool nonpos = !(J > 0.0);
-    bool nonfin = !isfinite(J) || !isfinite(detJ) || !isfinite(sig[0] + sig[4] + sig[8] + sig[1] + sig[2] + sig[5]);
+    bool nonpos = valid && !(J > 0.0);
+    bool nonfin =
+        valid && (!isfinite(J) || !isfinite(detJ) || !isfinite(sig[0] + sig[4] + sig[8] + sig[1] + sig[2] + sig[5]));
     const unsigned long long m_np = __ballot(nonpos), m_nf = __ballot(nonfin);
     if ((m_np | m_nf) && (threadIdx.x & 63) == __ffsll((long long)(m_np | m_nf)) - 1) {
         atomicOr(status, (m_np ? FL_S_NONPOSITIVE_J : 0) | (m_nf ? FL_S_NONFINITE : 0));
         atomicAdd(status + 1, (int)__popcll(m_np));
     }
+    if (!valid) return;
     if (TANGENT) {
-        double *rec = qp + t * QS;
 #pragma unroll
         for (int i = 0; i < 9; ++i) rec[Q_IX + i] = ix[i];
 #pragma unroll
@@ -208,7 +179,6 @@ __global__ __launch_bounds__(256) void qp_kernel(MatParams mp0, const double *__
         rec[Q_S + 5] = detJ * sig[5];
         rec[Q_DETJ] = detJ;
     } else {
-        double *rec = qp + t * QS_RES;
 #pragma unroll
         for (int i = 0; i < 9; ++i) rec[i] = ix[i];
         rec[QR_S + 0] = detJ * sig[0];
@@ -221,6 +191,57 @@ __global__ __launch_bounds__(256) void qp_kernel(MatParams mp0, const double *__
     }
 }
 
+__device__ __forceinline__ MatParams element_material(const MatParams &mp0, int64_t e,
+                                                      const int32_t *__restrict__ elem_mat,
+                                                      const double *__restrict__ mat_table) {
+    MatParams mp = mp0;
+    if (elem_mat) {
+        const double *row = mat_table + 4 * (int64_t)elem_mat[e];
+        mp.mu = row[0];
+        mp.mu1 = row[1];
+        mp.mu2 = row[2];
+        mp.lamb = row[3];
+    }
+    return mp;
+}
+
+// One thread per point: the parent Jacobians summed over the element's nodes.
+// elem_mat/mat_table (optional): per-element material parameters {mu, mu1, mu2, lamb}
+template <bool TANGENT>
+__global__ __launch_bounds__(256) void qp_kernel(MatParams mp0, const double *__restrict__ points,
+                                                 const int32_t *__restrict__ elements,
+                                                 const double *__restrict__ Eulerx, const double *__restrict__ Jm,
+                                                 const double *__restrict__ w, int G, int64_t nelem, int npe,
+                                                 int update, double *__restrict__ qp, int32_t *__restrict__ status,
+                                                 const int32_t *__restrict__ elem_mat,
+                                                 const double *__restrict__ mat_table) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = t < nelem * G;
+    const int64_t e = valid ? t / G : 0;
+    const int g = valid ? (int)(t - e * G) : 0;
+    const MatParams mp = element_material(mp0, e, elem_mat, mat_table);
+    double PX[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, Px[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const int32_t *conn = elements + e * npe;
+    for (int a = 0; valid && a < npe; ++a) {
+        const int64_t nd = conn[a];
+        const double X0 = points[3 * nd], X1 = points[3 * nd + 1], X2 = points[3 * nd + 2];
+        const double x0 = Eulerx[3 * nd], x1 = Eulerx[3 * nd + 1], x2 = Eulerx[3 * nd + 2];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const double jm = Jm[((int64_t)i * npe + a) * G + g];
+            PX[3 * i + 0] += jm * X0;
+            PX[3 * i + 1] += jm * X1;
+            PX[3 * i + 2] += jm * X2;
+            Px[3 * i + 0] += jm * x0;
+            Px[3 * i + 1] += jm * x1;
+            Px[3 * i + 2] += jm * x2;
+        }
+    }
+    if (!valid) PX[0] = PX[4] = PX[8] = Px[0] = Px[4] = Px[8] = 1.0;
+    qp_point<TANGENT>(mp, PX, Px, valid ? w[g] : 1.0, update, valid, qp + t * (TANGENT ? QS : QS_RES), status);
+}
+
+// ------------------------------------------------------------------ phase 2
 // ------------------------------------------------------------------ phase 2
 }  // namespace fl
 #include "fl_stiffness.hpp"
@@ -636,6 +657,28 @@ static int launch_traction_sf(int npe, int G, const double *qp, int qs, int soff
     return FL_E_ARG;
 }
 
+template <bool TANGENT>
+static int launch_qp_sf(const MatParams &mp, const double *points, const int32_t *elements, const double *Eulerx,
+                        const void *basis, const double *w, int npe, int G, int64_t nelem, int update, double *qp,
+                        int32_t *status, const int32_t *elem_mat, const double *mat_table, hipStream_t st) {
+    const int32_t *b = static_cast<const int32_t *>(basis);
+#define FL_QSF_CASE(P, QQ)                                                                                       \
+    if (npe == P * P * P && G == QQ * QQ * QQ) {                                                                 \
+        qp_sf_kernel<P, QQ, TANGENT><<<(unsigned)nelem, 256, 0, st>>>(mp, points, elements, Eulerx, b, w, nelem, \
+                                                                       update, qp, status, elem_mat, mat_table);  \
+        FL_LAUNCH_CHECK();                                                                                       \
+        return 0;                                                                                                \
+    }
+    FL_QSF_CASE(2, 2)
+    FL_QSF_CASE(3, 3)
+    FL_QSF_CASE(4, 4)
+    FL_QSF_CASE(5, 5)
+    FL_QSF_CASE(5, 6)
+#undef FL_QSF_CASE
+    set_error("no sum-factorised point kernel for %d nodes and %d points", npe, G);
+    return FL_E_ARG;
+}
+
 static bool sf_shape(int npe, int G) {
     return (npe == 8 && G == 8) || (npe == 27 && G == 27) || (npe == 64 && G == 64) || (npe == 125 && G == 125) ||
            (npe == 125 && G == 216);
@@ -873,13 +916,22 @@ int fl_df_assemble_mt(int32_t material, const double *params, const int32_t *ele
     clear_status_kernel<<<1, 64, 0, st>>>(status);
     transpose_jm_kernel<<<(G * 3 * NP + 255) / 256, 256, 0, st>>>(Jm, npe, G, NP, w.JmT);
     FL_LAUNCH_CHECK();
+    // the factorised point, traction and (FP64) stiffness kernels whenever the tensor
+    // basis is given, so T is the same for every tangent precision
+    const bool sf_ok = tensor_basis != nullptr && sf_shape(npe, G) && tuning_value(TUNE_STIFF_SF, 0) != 1;
     const int64_t nq = nelem * G;
-    qp_kernel<true><<<(unsigned)((nq + 255) / 256), 256, 0, st>>>(mp, points, elements, Eulerx, Jm, AllGauss, G, nelem,
-                                                                   npe, update, w.qp, status, elem_mat, mat_table);
-    FL_LAUNCH_CHECK();
+    if (sf_ok) {
+        const int rq = launch_qp_sf<true>(mp, points, elements, Eulerx, tensor_basis, AllGauss, npe, G, nelem, update,
+                                          w.qp, status, elem_mat, mat_table, st);
+        if (rq) return rq;
+    } else {
+        qp_kernel<true><<<(unsigned)((nq + 255) / 256), 256, 0, st>>>(mp, points, elements, Eulerx, Jm, AllGauss, G,
+                                                                       nelem, npe, update, w.qp, status, elem_mat,
+                                                                       mat_table);
+        FL_LAUNCH_CHECK();
+    }
     prof_mark(st);
-    const bool use_sf = precision == FL_PREC_FP64 && tensor_basis != nullptr && sf_shape(npe, G) &&
-                        tuning_value(TUNE_STIFF_SF, 0) != 1;
+    const bool use_sf = precision == FL_PREC_FP64 && sf_ok;
     int rc = use_sf ? launch_stiffness_sf(npe, G, w.qp, tensor_basis, nelem, (double *)w.Ke, st)
            : precision == FL_PREC_FP64
                  ? launch_stiffness_t<double, double>(NP, w.JmT, w.qp, npe, G, nelem, (double *)w.Ke, st)
@@ -888,9 +940,7 @@ int fl_df_assemble_mt(int32_t material, const double *params, const int32_t *ele
                        : launch_stiffness_bf16(NP, w.JmT, w.qp, npe, G, nelem, (float *)w.Ke, st);
     if (rc) return rc;
     prof_mark(st);
-    // tractions: factorised whenever the tensor basis is given (independent of the tangent
-    // precision, so T is the same for every precision)
-    if (tensor_basis != nullptr && sf_shape(npe, G) && tuning_value(TUNE_STIFF_SF, 0) != 1) {
+    if (sf_ok) {
         rc = launch_traction_sf(npe, G, w.qp, QS, Q_S, tensor_basis, nelem, w.te, st);
         if (rc) return rc;
     } else {

@@ -134,8 +134,9 @@ def test_hex_p4_tile_variants(cuda, variant):
 
 @pytest.mark.parametrize("p", [1, 2, 3, 4])
 def test_sum_factorised_equals_dense(cuda, p):
-    """Tensor-product hexes: the sum-factorised blocks (default when the tensor basis is
-    present) and the dense MFMA blocks both meet the oracle bar and agree to 1e-13."""
+    """Tensor-product hexes: the sum-factorised point, block and traction kernels (default
+    when the tensor basis is present) and the dense ones both meet the oracle bar; their
+    blocks agree to 1e-13."""
     import torch
     from florence_amd import _lib
     from florence_amd.assembly import DeviceMeshData, assemble_device
@@ -157,7 +158,9 @@ def test_sum_factorised_equals_dense(cuda, p):
         lib.fl_set_tuning(_lib.FL_TUNE_STIFF_SF, 0)
     V2, T2 = V2.cpu().numpy(), T2.cpu().numpy()
     assert np.abs(V - V2).max() <= 1e-13 * np.abs(V2).max()
-    assert np.abs(T - T2).max() <= 1e-13 * np.abs(T2).max()
+    # T sums element forces that partly cancel, and the two paths also take the point
+    # Jacobians' node sums in different orders: the oracle bar
+    assert np.abs(T - T2).max() <= RTOL * np.abs(T2).max()
     # the all-VALU form of the factorised contraction (FL_TUNE_STIFF_SF = 3) and the
     # form with one type per MFMA reduction (5)
     for form in (3, 5):
