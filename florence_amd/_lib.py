@@ -56,6 +56,8 @@ SIGNATURES = {
     "fl_df_residual_workspace_size": (_sz, [_i32, _i64, _i32]),
     "fl_df_residual": (ctypes.c_int, [_i32, _PD, _vp, _vp, _vp, _vp, _vp, _i32, _i64, _i32, _i64,
                                       _vp, _vp, _vp, _vp, _sz, _vp, _vp]),
+    "fl_df_residual_tb": (ctypes.c_int, [_i32, _PD, _vp, _vp, _vp, _vp, _vp, _i32, _i64, _i32, _i64,
+                                         _vp, _vp, _vp, _vp, _vp, _sz, _vp, _vp]),
     "fl_csr_spmv": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp]),
     "fl_pcg_workspace_size": (_sz, [_i64]),
     "fl_pcg": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, ctypes.c_double, _i32, _i32, _vp, _sz,

@@ -241,11 +241,12 @@ def residual_device(dm, material, Eulerx, T=None, stream=None):
     mid = MATERIAL_IDS.get(type(material).__name__)
     if mid is None:
         raise NotImplementedError("no low-level assembler for material {}".format(type(material).__name__))
-    _lib.check(lib.fl_df_residual(mid, prm, _lib.ptr(dm.points), _lib.ptr(dm.elements), _lib.ptr(x),
-                                  _lib.ptr(dm.Jm), _lib.ptr(dm.AllGauss), dm.ngauss, dm.nelem, dm.npe, dm.nnode,
-                                  _lib.ptr(dm.node_elem_ptr), _lib.ptr(dm.node_elem), _lib.ptr(T), _lib.ptr(ws),
-                                  ws.numel(), _lib.ptr(dm.status), _lib.stream_ptr(stream)),
-               "fl_df_residual")
+    _lib.check(lib.fl_df_residual_tb(mid, prm, _lib.ptr(dm.points), _lib.ptr(dm.elements), _lib.ptr(x),
+                                     _lib.ptr(dm.Jm), _lib.ptr(dm.AllGauss), dm.ngauss, dm.nelem, dm.npe, dm.nnode,
+                                     _lib.ptr(dm.node_elem_ptr), _lib.ptr(dm.node_elem), _lib.ptr(T),
+                                     _lib.ptr(dm.tensor_basis), _lib.ptr(ws), ws.numel(), _lib.ptr(dm.status),
+                                     _lib.stream_ptr(stream)),
+               "fl_df_residual_tb")
     return T, dm.status
 
 

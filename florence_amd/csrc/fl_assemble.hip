@@ -981,10 +981,11 @@ int fl_df_assemble_mt(int32_t material, const double *params, const int32_t *ele
     return 0;
 }
 
-int fl_df_residual(int32_t material, const double *params, const double *points, const int32_t *elements,
-                   const double *Eulerx, const double *Jm, const double *AllGauss, int32_t G, int64_t nelem,
-                   int32_t npe, int64_t nnode, const int32_t *node_elem_ptr, const int32_t *node_elem, double *T,
-                   void *workspace, size_t workspace_bytes, int32_t *status, void *stream) {
+static int df_residual(int32_t material, const double *params, const double *points, const int32_t *elements,
+                       const double *Eulerx, const double *Jm, const double *AllGauss, int32_t G, int64_t nelem,
+                       int32_t npe, int64_t nnode, const int32_t *node_elem_ptr, const int32_t *node_elem, double *T,
+                       const void *tensor_basis, void *workspace, size_t workspace_bytes, int32_t *status,
+                       void *stream) {
     hipStream_t st = as_stream(stream);
     if (!check_common(material, params, G, nelem, npe, nnode)) return FL_E_ARG;
     if (!points || !elements || !Eulerx || !Jm || !AllGauss || !node_elem_ptr || !node_elem || !T || !status) {
@@ -999,26 +1000,56 @@ int fl_df_residual(int32_t material, const double *params, const double *points,
     }
     const int NP = padded_np(npe);
     const MatParams mp = mat_params(material, params);
+    const bool sf_ok = tensor_basis != nullptr && sf_shape(npe, G) && tuning_value(TUNE_STIFF_SF, 0) != 1;
     prof_begin(st, PROF_DF_RESIDUAL);
     clear_status_kernel<<<1, 64, 0, st>>>(status);
-    transpose_jm_kernel<<<(G * 3 * NP + 255) / 256, 256, 0, st>>>(Jm, npe, G, NP, w.JmT);
-    FL_LAUNCH_CHECK();
-    const int64_t nq = nelem * G;
-    qp_kernel<false><<<(unsigned)((nq + 255) / 256), 256, 0, st>>>(mp, points, elements, Eulerx, Jm, AllGauss, G,
-                                                                    nelem, npe, 1, w.qp, status, nullptr, nullptr);
-    FL_LAUNCH_CHECK();
+    if (sf_ok) {
+        const int rq = launch_qp_sf<false>(mp, points, elements, Eulerx, tensor_basis, AllGauss, npe, G, nelem, 1,
+                                           w.qp, status, nullptr, nullptr, st);
+        if (rq) return rq;
+    } else {
+        transpose_jm_kernel<<<(G * 3 * NP + 255) / 256, 256, 0, st>>>(Jm, npe, G, NP, w.JmT);
+        FL_LAUNCH_CHECK();
+        const int64_t nq = nelem * G;
+        qp_kernel<false><<<(unsigned)((nq + 255) / 256), 256, 0, st>>>(mp, points, elements, Eulerx, Jm, AllGauss, G,
+                                                                        nelem, npe, 1, w.qp, status, nullptr, nullptr);
+        FL_LAUNCH_CHECK();
+    }
     prof_mark(st);
-    const size_t lds_tr = (size_t)G * 9 * 8;
-    if (lds_tr > 64 * 1024) FL_HIP_CHECK(hipFuncSetAttribute((const void *)traction_kernel,
-                                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_tr));
-    traction_kernel<<<(unsigned)nelem, 256, lds_tr, st>>>(w.JmT, NP, w.qp, QS_RES, QR_S, npe, G, w.te);
-    FL_LAUNCH_CHECK();
+    if (sf_ok) {
+        const int rt = launch_traction_sf(npe, G, w.qp, QS_RES, QR_S, tensor_basis, nelem, w.te, st);
+        if (rt) return rt;
+    } else {
+        const size_t lds_tr = (size_t)G * 9 * 8;
+        if (lds_tr > 64 * 1024)
+            FL_HIP_CHECK(hipFuncSetAttribute((const void *)traction_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             (int)lds_tr));
+        traction_kernel<<<(unsigned)nelem, 256, lds_tr, st>>>(w.JmT, NP, w.qp, QS_RES, QR_S, npe, G, w.te);
+        FL_LAUNCH_CHECK();
+    }
     prof_mark(st);
     gather_traction_kernel<<<(unsigned)((3 * nnode + 255) / 256), 256, 0, st>>>(npe, node_elem_ptr, node_elem, w.te,
                                                                                  nnode, T);
     FL_LAUNCH_CHECK();
     prof_mark(st);
     return 0;
+}
+
+int fl_df_residual(int32_t material, const double *params, const double *points, const int32_t *elements,
+                   const double *Eulerx, const double *Jm, const double *AllGauss, int32_t G, int64_t nelem,
+                   int32_t npe, int64_t nnode, const int32_t *node_elem_ptr, const int32_t *node_elem, double *T,
+                   void *workspace, size_t workspace_bytes, int32_t *status, void *stream) {
+    return df_residual(material, params, points, elements, Eulerx, Jm, AllGauss, G, nelem, npe, nnode, node_elem_ptr,
+                       node_elem, T, nullptr, workspace, workspace_bytes, status, stream);
+}
+
+int fl_df_residual_tb(int32_t material, const double *params, const double *points, const int32_t *elements,
+                      const double *Eulerx, const double *Jm, const double *AllGauss, int32_t G, int64_t nelem,
+                      int32_t npe, int64_t nnode, const int32_t *node_elem_ptr, const int32_t *node_elem, double *T,
+                      const void *tensor_basis, void *workspace, size_t workspace_bytes, int32_t *status,
+                      void *stream) {
+    return df_residual(material, params, points, elements, Eulerx, Jm, AllGauss, G, nelem, npe, nnode, node_elem_ptr,
+                       node_elem, T, tensor_basis, workspace, workspace_bytes, status, stream);
 }
 
 }  // extern "C"

@@ -440,9 +440,12 @@ __global__ __launch_bounds__(256, 4) void stiffness_sf_mfma_kernel(const double 
 // together, K = (type, qb) of length 2Q for beta and K = (m, n, qe) of length
 // pairs x Q for eta, so Q = 6 fills K exactly (3 steps of 4 instead of 2 x 2 padded
 // ones) and an az slice needs four barriers instead of nine.  Each wave owns every
-// row tile of its column tiles, so an A fragment (a product of two basis values)
-// serves TNW tiles and a B fragment MT3.  LDS: T2 of the phase's two types (~42 KB at
-// p = 4, three workgroups per CU).
+// row tile of its column tiles, so an A fragment serves TNW tiles and a B fragment
+// MT3.  The products of two 1D basis values that form the zeta factors and the eta A
+// fragments are tabulated in LDS once per workgroup, and the beta A fragments, which
+// depend only on the lane, stay in registers: LDS traffic, not MFMA issue, bounds this
+// kernel.  LDS: T2 of the phase's two types and the tables (~50 KB at p = 4, three
+// workgroups per CU).
 template <int P1, int Q>
 __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(const double *__restrict__ qp,
                                                                     const int32_t *__restrict__ basis,
@@ -454,7 +457,10 @@ __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(const double
     constexpr int MT2 = (PP + 15) / 16, NT2 = (P1 * Q + 15) / 16;
     constexpr int MT3 = (PP + 15) / 16, NT3 = (N3 + 15) / 16, TNW = (NT3 + 3) / 4;
     constexpr int KE_MAX = (4 * Q + 3) / 4, KB = (2 * Q + 3) / 4;
-    __shared__ double sF[3][2][P1][Q];
+    // pair products of the 1D bases: zeta [s*2+t][az][bz][qz] = T0^s[az][qz] T0^t[bz][qz],
+    // eta [s*2+t][ae*P1+be][qe] likewise; the beta ones live in registers
+    __shared__ double sPz[4][P1][P1][Q];
+    __shared__ double sPe[4][PP][Q];
     __shared__ double sC[9][Q3];
     __shared__ double sT1[9][P1][Q * Q];
     __shared__ double sT2[2][PP][P1][Q];
@@ -475,8 +481,32 @@ __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(const double
     const int32_t *loc = basis + 4 + Q3;
     const double *f1d = reinterpret_cast<const double *>(basis + L::INTS);
 
-    for (int t = tid; t < 3 * 2 * P1 * Q; t += NT) (&sF[0][0][0][0])[t] = f1d[t];
+    for (int t = tid; t < 4 * PP * Q; t += NT) {
+        const int st = t / (PP * Q), r = t - st * (PP * Q);
+        const int a = r / (P1 * Q), b = (r / Q) % P1, q = r % Q;
+        const int s0 = st >> 1, t0 = st & 1;
+        (&sPz[0][0][0][0])[t] = f1d[(s0 * P1 + a) * Q + q] * f1d[(t0 * P1 + b) * Q + q];
+        (&sPe[0][0][0])[t] = f1d[((2 + s0) * P1 + a) * Q + q] * f1d[((2 + t0) * P1 + b) * Q + q];
+    }
     for (int t = tid; t < N3; t += NT) sLoc[t] = min(max(loc[t], 0), N3 - 1);
+    // this lane's beta A fragments: phase h, step ks, row tile tm -> T2^h[ab][qb] T2^tl[bb][qb]
+    double afr[2][KB][MT3];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int ks = 0; ks < KB; ++ks) {
+            const int kk = ks * 4 + lk;
+            const bool kv = kk < 2 * Q;
+            const int tl = kv ? kk / Q : 0, qb = kv ? kk - (kk / Q) * Q : 0;
+#pragma unroll
+            for (int tm = 0; tm < MT3; ++tm) {
+                const int arow = tm * 16 + li;
+                const int ab_ = min(arow / P1, P1 - 1), bb_ = arow % P1;
+                afr[h][ks][tm] = (kv && arow < PP)
+                                     ? f1d[((4 + h) * P1 + ab_) * Q + qb] * f1d[((4 + tl) * P1 + bb_) * Q + qb]
+                                     : 0.0;
+            }
+        }
     const double *qe_ = qp + e * (int64_t)(Q3)*QS;
     for (int t = tid; t < Q3; t += NT) {
         const double *rec = qe_ + (int64_t)min(max(g_of_t[t], 0), Q3 - 1) * QS;
@@ -507,9 +537,7 @@ __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(const double
         for (int t = tid; t < 9 * P1 * Q; t += NT) {
             const int mn = t / (P1 * Q), bz = (t / Q) % P1, qe = t % Q;
             const int m = mn / 3, n = mn % 3;
-            double f[Q];
-#pragma unroll
-            for (int qz = 0; qz < Q; ++qz) f[qz] = sF[0][m == 0][az][qz] * sF[0][n == 0][bz][qz];
+            const double *f = sPz[(m == 0) * 2 + (n == 0)][az][bz];
 #pragma unroll
             for (int qb = 0; qb < Q; ++qb) {
                 const double *c = sC[mn] + (qb * Q + qe) * Q;
@@ -546,7 +574,7 @@ __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(const double
                         if (kk < ke) {
                             const int pr = kk / Q, qe = kk - pr * Q;
                             const int m = h ? 2 : pr / nn, n = tl ? 2 : pr % nn;
-                            if (arow < PP) a = sF[1][m == 1][ae_][qe] * sF[1][n == 1][be_][qe];
+                            if (arow < PP) a = sPe[(m == 1) * 2 + (n == 1)][arow][qe];
                             if (bcol < P1 * Q) b = sT1[3 * m + n][bz_][qe * Q + qb_];
                         }
                         a2 = MF::step(a, b, a2);
@@ -565,13 +593,7 @@ __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(const double
                 const int kk = ks * 4 + lk;
                 const bool kv = kk < 2 * Q;
                 const int tl = kv ? kk / Q : 0, qb = kv ? kk - (kk / Q) * Q : 0;
-                double af[MT3], bf[TNW];
-#pragma unroll
-                for (int tm = 0; tm < MT3; ++tm) {
-                    const int arow = tm * 16 + li;
-                    const int ab_ = min(arow / P1, P1 - 1), bb_ = arow % P1;
-                    af[tm] = (kv && arow < PP) ? sF[2][h][ab_][qb] * sF[2][tl][bb_][qb] : 0.0;
-                }
+                double bf[TNW];
 #pragma unroll
                 for (int v = 0; v < TNW; ++v) {
                     const int bcol = (wave + 4 * v) * 16 + li;
@@ -582,7 +604,7 @@ __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(const double
                 for (int tm = 0; tm < MT3; ++tm)
 #pragma unroll
                     for (int v = 0; v < TNW; ++v)
-                        if (wave + 4 * v < NT3) acc[tm][v] = MF::step(af[tm], bf[v], acc[tm][v]);
+                        if (wave + 4 * v < NT3) acc[tm][v] = MF::step(afr[h][ks][tm], bf[v], acc[tm][v]);
             }
             // the next phase's eta rewrites sT2; the next slice's zeta only rewrites sT1,
             // whose last reads precede the barrier after this phase's eta

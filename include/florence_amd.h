@@ -182,6 +182,15 @@ int fl_df_residual(int32_t material, const double *params, const double *points_
                    const double *AllGauss_dev, int32_t ngauss, int64_t nelem, int32_t npe, int64_t nnode,
                    const int32_t *node_elem_ptr_dev, const int32_t *node_elem_dev, double *T_dev,
                    void *workspace_dev, size_t workspace_bytes, int32_t *status_dev, void *stream);
+/* The same pass given the tensor basis of fl_tensor_basis_bytes (no reference
+ * counterpart): tensor-product hexes then take the sum-factorised point and traction
+ * kernels (same T to rounding); NULL behaves as fl_df_residual. */
+int fl_df_residual_tb(int32_t material, const double *params, const double *points_dev,
+                      const int32_t *elements_dev, const double *Eulerx_dev, const double *Jm_dev,
+                      const double *AllGauss_dev, int32_t ngauss, int64_t nelem, int32_t npe, int64_t nnode,
+                      const int32_t *node_elem_ptr_dev, const int32_t *node_elem_dev, double *T_dev,
+                      const void *tensor_basis_dev, void *workspace_dev, size_t workspace_bytes,
+                      int32_t *status_dev, void *stream);
 
 /* ------------------------------------------------------ Newton linear algebra
  * The reference reduces the tangent to the free dofs with scipy fancy indexing

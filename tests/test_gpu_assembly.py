@@ -75,8 +75,24 @@ def test_residual_only_matches(cuda):
     _, Tfull, _ = assemble_device(dm, mat, x)
     torch.cuda.synchronize()
     Tr = O.residual_only(m.points, m.elements, x, fs.Jm, fs.AllGauss, mat)
-    T, Tfull = T.cpu().numpy(), Tfull.cpu().numpy()
-    assert np.abs(T - Tr).max() / np.abs(Tr).max() <= RTOL
+    # the reference-shaped entry (no tensor basis: dense point and traction kernels)
+    from florence_amd import _lib
+    lib = _lib.load()
+    Td = torch.empty_like(T)
+    xd = torch.as_tensor(np.ascontiguousarray(x), device="cuda")
+    ws = dm.workspace(residual_only=True)
+    prm, _keep = _lib.double_array(mat.device_parameters())
+    _lib.check(lib.fl_df_residual(mat.material_id, prm, _lib.ptr(dm.points), _lib.ptr(dm.elements), _lib.ptr(xd),
+                                  _lib.ptr(dm.Jm), _lib.ptr(dm.AllGauss), dm.ngauss, dm.nelem, dm.npe, dm.nnode,
+                                  _lib.ptr(dm.node_elem_ptr), _lib.ptr(dm.node_elem), _lib.ptr(Td), _lib.ptr(ws),
+                                  ws.numel(), _lib.ptr(dm.status), _lib.stream_ptr()), "fl_df_residual")
+    torch.cuda.synchronize()
+    assert dm.tensor_basis is not None
+    T, Tfull, Td = T.cpu().numpy(), Tfull.cpu().numpy(), Td.cpu().numpy()
+    for label, t in (("factorised", T), ("assembly", Tfull), ("dense", Td)):
+        err = np.abs(t - Tr).max() / np.abs(Tr).max()
+        print("residual-only {}: |dT|/|T| = {:.3e}".format(label, err))
+        assert err <= RTOL, label
     assert np.abs(T - Tfull).max() / np.abs(Tr).max() <= RTOL
 
 
