@@ -62,7 +62,9 @@ def tensor_basis_table(function_space, npe):
             N, dN = LagrangeGaussLobatto1D(C, float(pts[0]))
             f1d[d, 0, :, q] = N
             f1d[d, 1, :, q] = dN
-    perm = NodeArrangementHex(C)
+    perm = np.asarray(NodeArrangementHex(C))
+    if perm.shape != (npe,) or not np.array_equal(np.sort(perm), np.arange(npe)):
+        return None
     t = [perm % P1, (perm // P1) % P1, perm // (P1 * P1)]
     Jm = np.asarray(function_space.Jm)
     for m in range(3):
@@ -73,8 +75,8 @@ def tensor_basis_table(function_space, npe):
             return None
     local_of_t = np.empty(npe, dtype=np.int32)
     local_of_t[perm] = np.arange(npe, dtype=np.int32)
-    ints = [np.array([P1, Q, 0, 0], dtype=np.int32), g_of_t, local_of_t]
-    if (G + npe) % 2:
+    ints = [np.array([P1, Q, 0, 0], dtype=np.int32), g_of_t, local_of_t, np.asarray(perm, dtype=np.int32)]
+    if G % 2:
         ints.append(np.zeros(1, dtype=np.int32))
     return np.concatenate([np.concatenate(ints).view(np.uint8), f1d.ravel().view(np.uint8)])
 

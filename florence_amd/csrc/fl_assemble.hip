@@ -286,6 +286,20 @@ __device__ inline int find_rank(const int32_t *s, int n, int32_t v) {
     return s[lo] == v ? lo : -1;
 }
 
+// eslot[q] = stored row of node-element entry q's block (entry q = e npe + la)
+__global__ void entry_slot_kernel(const int32_t *__restrict__ node_elem, int64_t n, int npe,
+                                  const int32_t *__restrict__ kperm, int32_t *__restrict__ eslot) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) eslot[q] = min(max(kperm[node_elem[q] % npe], 0), npe - 1);
+}
+
+// Stored row/column of local node l in an element block: blocks of the sum-factorised
+// kernels are stored in tensor node order (kperm = tensor_of_l), the others in local
+// order (kperm = NULL).
+__device__ __forceinline__ int kslot(const int32_t *__restrict__ kperm, int l, int npe) {
+    return kperm ? min(max(kperm[l], 0), npe - 1) : l;
+}
+
 // A node inside one element: its three rows are that element's rows with the columns
 // permuted -- no accumulation.  The nine element rows (9 npe values) are read
 // coalesced into LDS (sv), then each output row is written coalesced, its values
@@ -293,18 +307,19 @@ __device__ inline int find_rank(const int32_t *s, int n, int32_t v) {
 // signed zeros match the sum path).  LDS: inv (npe ints), snb (deg ints, used only
 // without a rank table), sv (9 npe doubles).
 template <typename KT, int GT, typename VT>
-__device__ __forceinline__ void gather_single_node(int64_t a, int32_t ent, int npe, int deg, int32_t p0,
+__device__ __forceinline__ void gather_single_node(int64_t a, int32_t ent, int rslot, int npe, int deg, int32_t p0,
                                                    const int32_t *__restrict__ elements,
                                                    const int32_t *__restrict__ nbrs,
                                                    const int32_t *__restrict__ indptr, const KT *__restrict__ Ke,
                                                    const double *__restrict__ te, const int16_t *__restrict__ ranks,
-                                                   int32_t *inv, int32_t *snb, double *sv, VT *__restrict__ V,
-                                                   double *__restrict__ T, int32_t *__restrict__ status) {
+                                                   const int32_t *__restrict__ kperm, int32_t *inv, int32_t *snb,
+                                                   double *sv, VT *__restrict__ V, double *__restrict__ T,
+                                                   int32_t *__restrict__ status) {
     const int tid = threadIdx.x;
     const int64_t npe2 = (int64_t)npe * npe;
     const int64_t e = ent / npe;
     const int la = ent - (int)(e * npe);
-    const KT *Kb = Ke + e * 9 * npe2 + (int64_t)la * npe;
+    const KT *Kb = Ke + e * 9 * npe2 + (int64_t)rslot * npe;
     // issue the row loads first (9 npe <= UF GT for npe <= 128)
     constexpr int UF = (9 * 128 + GT - 1) / GT;
     double v[UF];
@@ -319,18 +334,18 @@ __device__ __forceinline__ void gather_single_node(int64_t a, int32_t ent, int n
     }
     const double tv = tid < 3 ? te[e * 3 * npe + 3 * la + tid] : 0.0;
     bool bad = false;
-    for (int r = tid; r < npe; r += GT) inv[r] = 0;   // position -> local column node
+    for (int r = tid; r < npe; r += GT) inv[r] = 0;   // position -> stored column
     __syncthreads();
     if (ranks) {
         const int16_t *rk = ranks + ((int64_t)ent) * npe;
-        for (int lb = tid; lb < npe; lb += GT) inv[rk[lb]] = lb;
+        for (int lb = tid; lb < npe; lb += GT) inv[rk[lb]] = kslot(kperm, lb, npe);
     } else {
         for (int i = tid; i < deg; i += GT) snb[i] = nbrs[p0 + i];
         __syncthreads();
         for (int lb = tid; lb < npe; lb += GT) {
             const int r = find_rank(snb, deg, elements[e * npe + lb]);
             bad |= r < 0;
-            if (r >= 0) inv[r] = lb;
+            if (r >= 0) inv[r] = kslot(kperm, lb, npe);
         }
     }
 #pragma unroll
@@ -364,7 +379,9 @@ __global__ __launch_bounds__(64) void gather_single_kernel(const int32_t *__rest
                                                            const int32_t *__restrict__ nbrs,
                                                            const int32_t *__restrict__ indptr,
                                                            const KT *__restrict__ Ke, const double *__restrict__ te,
-                                                           const int16_t *__restrict__ ranks, VT *__restrict__ V,
+                                                           const int16_t *__restrict__ ranks,
+                                                           const int32_t *__restrict__ kperm,
+                                                           const int32_t *__restrict__ eslot, VT *__restrict__ V,
                                                            double *__restrict__ T, int32_t *__restrict__ status) {
     extern __shared__ double dsm[];
     int32_t *ism = reinterpret_cast<int32_t *>(dsm);
@@ -375,7 +392,9 @@ __global__ __launch_bounds__(64) void gather_single_kernel(const int32_t *__rest
         if (threadIdx.x == 0) atomicOr(status, FL_S_PATTERN);
         return;
     }
-    gather_single_node<KT, 64, VT>(a, node_elem[q0], npe, deg, nbr_ptr[a], elements, nbrs, indptr, Ke, te, ranks,
+    const int32_t ent = node_elem[q0];
+    const int rs = eslot ? min(max(eslot[q0], 0), npe - 1) : ent % npe;
+    gather_single_node<KT, 64, VT>(a, ent, rs, npe, deg, nbr_ptr[a], elements, nbrs, indptr, Ke, te, ranks, kperm,
                                    ism, ism + npe, dsm + npe, V, T, status);
 }
 
@@ -385,8 +404,8 @@ __global__ __launch_bounds__(64) void gather_single_kernel(const int32_t *__rest
 // into registers (UB loads per lane per element, UB GT >= 9 npe) before any is added,
 // so a node's loads are in flight together; the adds then run element by element in
 // node_elem order with a barrier between elements, so the sums do not depend on timing.
-// LDS: rr (column positions, EB npe ints) | snb (lds_deg ints) | acc (9 lds_deg doubles;
-// the staging rows of a single-element node).
+// LDS: rr (column positions by stored column, EB npe ints) | snb (lds_deg ints) | acc
+// (9 lds_deg doubles; the staging rows of a single-element node).
 template <typename KT, int GT, int EB, int UB, typename VT>
 __global__ __launch_bounds__(GT) void gather_kernel(const int32_t *__restrict__ elements, int npe,
                                                     const int32_t *__restrict__ node_order,
@@ -396,8 +415,10 @@ __global__ __launch_bounds__(GT) void gather_kernel(const int32_t *__restrict__ 
                                                     const int32_t *__restrict__ nbrs, const int32_t *__restrict__ indptr,
                                                     const KT *__restrict__ Ke, const double *__restrict__ te,
                                                     int lds_deg, const int16_t *__restrict__ ranks,
-                                                    VT *__restrict__ V, double *__restrict__ T,
-                                                    int32_t *__restrict__ status) {
+                                                    const int32_t *__restrict__ kperm,
+                                                    const int32_t *__restrict__ kinv,
+                                                    const int32_t *__restrict__ eslot, VT *__restrict__ V,
+                                                    double *__restrict__ T, int32_t *__restrict__ status) {
     extern __shared__ double smem[];
     const int64_t a = node_order[blockIdx.x];
     const int tid = threadIdx.x;
@@ -412,7 +433,9 @@ __global__ __launch_bounds__(GT) void gather_kernel(const int32_t *__restrict__ 
     }
     const int32_t q0 = node_elem_ptr[a], q1 = node_elem_ptr[a + 1];
     if (q1 - q0 == 1 && deg == npe) {
-        gather_single_node<KT, GT, VT>(a, node_elem[q0], npe, deg, p0, elements, nbrs, indptr, Ke, te, ranks, rr,
+        const int32_t ent = node_elem[q0];
+        const int rs = eslot ? min(max(eslot[q0], 0), npe - 1) : ent % npe;
+        gather_single_node<KT, GT, VT>(a, ent, rs, npe, deg, p0, elements, nbrs, indptr, Ke, te, ranks, kperm, rr,
                                        snb, acc, V, T, status);
         return;
     }
@@ -421,7 +444,8 @@ __global__ __launch_bounds__(GT) void gather_kernel(const int32_t *__restrict__ 
     for (int i = tid; i < 9 * deg; i += GT) acc[i] = 0.0;
     if (!ranks)
         for (int i = tid; i < deg; i += GT) snb[i] = nbrs[p0 + i];
-    // this lane's slots of an element's 9 npe values: source offset, target base, column node
+    // this lane's slots of an element's 9 npe stored values (consecutive lanes read
+    // consecutive stored columns): source offset, target base, stored column
     int src[UB], tgt[UB], col[UB];
 #pragma unroll
     for (int u = 0; u < UB; ++u) {
@@ -430,11 +454,11 @@ __global__ __launch_bounds__(GT) void gather_kernel(const int32_t *__restrict__ 
         tgt[u] = 0;
         col[u] = 0;
         if (idx < 9 * npe) {
-            const int ik = idx / npe, lb = idx - ik * npe;
+            const int ik = idx / npe, ts = idx - ik * npe;
             const int i = ik / 3, k = ik - 3 * i;
-            src[u] = ik * (int)npe2 + lb;
+            src[u] = ik * (int)npe2 + ts;
             tgt[u] = i * rowlen + k;
-            col[u] = lb;
+            col[u] = ts;
         }
     }
     __syncthreads();
@@ -442,9 +466,10 @@ __global__ __launch_bounds__(GT) void gather_kernel(const int32_t *__restrict__ 
     double tsum = 0.0;
     for (int32_t c0 = q0; c0 < q1; c0 += EB) {
         const int ne = min(EB, (int)(q1 - c0));
-        // column positions of the chunk's elements (precomputed per mesh, or searched)
+        // column positions of the chunk's elements (precomputed per mesh, or searched), by
+        // stored column
         for (int t = tid; t < ne * npe; t += GT) {
-            const int j = t / npe, lb = t - j * npe;
+            const int j = t / npe, lb = kslot(kinv, t - j * npe, npe);
             const int32_t ent = node_elem[c0 + j];
             int r;
             if (ranks) {
@@ -464,7 +489,8 @@ __global__ __launch_bounds__(GT) void gather_kernel(const int32_t *__restrict__ 
                 const int32_t ent = node_elem[c0 + j];
                 const int64_t e = ent / npe;
                 const int la = ent - (int)(e * npe);
-                const KT *Kb = Ke + e * 9 * npe2 + (int64_t)la * npe;
+                const int rs = eslot ? min(max(eslot[c0 + j], 0), npe - 1) : la;
+                const KT *Kb = Ke + e * 9 * npe2 + (int64_t)rs * npe;
 #pragma unroll
                 for (int u = 0; u < UB; ++u) v[j][u] = src[u] >= 0 ? (double)Kb[src[u]] : 0.0;
                 if (tid < 3) tv[j] = te[e * 3 * npe + 3 * la + tid];
@@ -708,13 +734,14 @@ template <typename KT, int GT, int EB, int UB, typename VT>
 static int launch_gather_t(int32_t nb, size_t lds, const int32_t *elements, int npe, const int32_t *order,
                            const int32_t *node_elem_ptr, const int32_t *node_elem, const int32_t *nbr_ptr,
                            const int32_t *nbrs, const int32_t *indptr, const KT *Ke, const double *te, int deg,
-                           const int16_t *ranks, VT *V, double *T, int32_t *status, hipStream_t st) {
+                           const int16_t *ranks, const int32_t *kperm, const int32_t *kinv, const int32_t *eslot,
+                           VT *V, double *T, int32_t *status, hipStream_t st) {
     if (lds > 64 * 1024)
         FL_HIP_CHECK(hipFuncSetAttribute((const void *)gather_kernel<KT, GT, EB, UB, VT>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     gather_kernel<KT, GT, EB, UB, VT><<<(unsigned)nb, GT, lds, st>>>(elements, npe, order, node_elem_ptr,
                                                                      node_elem, nbr_ptr, nbrs, indptr, Ke, te, deg,
-                                                                     ranks, V, T, status);
+                                                                     ranks, kperm, kinv, eslot, V, T, status);
     FL_LAUNCH_CHECK();
     return 0;
 }
@@ -724,43 +751,50 @@ template <typename KT, typename VT>
 static int launch_gather(int b, int32_t nb, const int32_t *elements, int npe, const int32_t *order,
                          const int32_t *node_elem_ptr, const int32_t *node_elem, const int32_t *nbr_ptr,
                          const int32_t *nbrs, const int32_t *indptr, const KT *Ke, const double *te, int deg,
-                         const int16_t *ranks, VT *V, double *T, int32_t *status, hipStream_t st) {
+                         const int16_t *ranks, const int32_t *kperm, const int32_t *kinv, const int32_t *eslot,
+                         VT *V, double *T, int32_t *status, hipStream_t st) {
     const GatherForm f = gather_form(b, deg, npe);
     const size_t lds = gather_lds_bytes(f, deg, npe);
     switch (f) {
         case G_SINGLE:
             gather_single_kernel<KT, VT><<<(unsigned)nb, 64, lds, st>>>(elements, npe, order, node_elem_ptr,
                                                                        node_elem, nbr_ptr, nbrs, indptr, Ke, te,
-                                                                       ranks, V, T, status);
+                                                                       ranks, kperm, eslot, V, T, status);
             FL_LAUNCH_CHECK();
             return 0;
         case G_W8X2:
             return launch_gather_t<KT, 64, 8, 2, VT>(nb, lds, elements, npe, order, node_elem_ptr, node_elem, nbr_ptr,
-                                                     nbrs, indptr, Ke, te, deg, ranks, V, T, status, st);
+                                                     nbrs, indptr, Ke, te, deg, ranks, kperm, kinv, eslot, V, T, status, st);
         case G_W4X9:
             return launch_gather_t<KT, 64, 4, 9, VT>(nb, lds, elements, npe, order, node_elem_ptr, node_elem, nbr_ptr,
-                                                     nbrs, indptr, Ke, te, deg, ranks, V, T, status, st);
+                                                     nbrs, indptr, Ke, te, deg, ranks, kperm, kinv, eslot, V, T, status, st);
         case G_W2X18:
             return launch_gather_t<KT, 64, 2, 18, VT>(nb, lds, elements, npe, order, node_elem_ptr, node_elem,
-                                                      nbr_ptr, nbrs, indptr, Ke, te, deg, ranks, V, T, status, st);
+                                                      nbr_ptr, nbrs, indptr, Ke, te, deg, ranks, kperm, kinv, eslot, V,
+                                                      T, status, st);
         case G_WIDE1:
             return launch_gather_t<KT, 256, 1, 5, VT>(nb, lds, elements, npe, order, node_elem_ptr, node_elem,
-                                                      nbr_ptr, nbrs, indptr, Ke, te, deg, ranks, V, T, status, st);
+                                                      nbr_ptr, nbrs, indptr, Ke, te, deg, ranks, kperm, kinv, eslot, V,
+                                                      T, status, st);
         case G_WIDE2:
             return launch_gather_t<KT, 256, 2, 5, VT>(nb, lds, elements, npe, order, node_elem_ptr, node_elem,
-                                                      nbr_ptr, nbrs, indptr, Ke, te, deg, ranks, V, T, status, st);
+                                                      nbr_ptr, nbrs, indptr, Ke, te, deg, ranks, kperm, kinv, eslot, V,
+                                                      T, status, st);
         case G_WIDE4:
             return launch_gather_t<KT, 256, 4, 5, VT>(nb, lds, elements, npe, order, node_elem_ptr, node_elem,
-                                                      nbr_ptr, nbrs, indptr, Ke, te, deg, ranks, V, T, status, st);
+                                                      nbr_ptr, nbrs, indptr, Ke, te, deg, ranks, kperm, kinv, eslot, V,
+                                                      T, status, st);
         default:
             return launch_gather_t<KT, 256, 8, 5, VT>(nb, lds, elements, npe, order, node_elem_ptr, node_elem,
-                                                      nbr_ptr, nbrs, indptr, Ke, te, deg, ranks, V, T, status, st);
+                                                      nbr_ptr, nbrs, indptr, Ke, te, deg, ranks, kperm, kinv, eslot, V,
+                                                      T, status, st);
     }
 }
 
 struct DFWork {
     double *JmT, *qp, *te;
     void *Ke;
+    int32_t *slot;   // stored row of each node-element entry's block (tensor-ordered blocks)
 };
 
 // bytes per element-block entry for each tangent precision
@@ -775,6 +809,7 @@ static DFWork carve_df(void *ws, size_t bytes, int G, int64_t nelem, int npe, bo
     w.qp = cv.take<double>((size_t)nelem * G * (tangent ? QS : QS_RES));
     w.te = cv.take<double>((size_t)nelem * 3 * npe);
     w.Ke = tangent ? (void *)cv.take<char>((size_t)nelem * 9 * npe * npe * ke_bytes(precision)) : nullptr;
+    w.slot = tangent ? cv.take<int32_t>((size_t)nelem * npe) : nullptr;
     *need = cv.off + 256;
     if (!cv.ok) w.JmT = nullptr;
     return w;
@@ -830,7 +865,7 @@ size_t fl_df_residual_workspace_size(int32_t ngauss, int64_t nelem, int32_t npe)
 size_t fl_tensor_basis_bytes(int32_t p1, int32_t q) {
     if (p1 <= 0 || q <= 0) return 0;
     const int64_t q3 = (int64_t)q * q * q, n3 = (int64_t)p1 * p1 * p1;
-    const int64_t ints = 4 + q3 + n3 + ((q3 + n3) % 2);
+    const int64_t ints = 4 + q3 + 2 * n3 + (q3 % 2);
     return (size_t)(4 * ints + 8 * 3 * 2 * p1 * q);
 }
 
@@ -932,6 +967,17 @@ int fl_df_assemble_mt(int32_t material, const double *params, const int32_t *ele
     }
     prof_mark(st);
     const bool use_sf = precision == FL_PREC_FP64 && sf_ok;
+    // the factorised blocks are stored in tensor node order: tensor_of_l of the basis table,
+    // and each node-element entry's stored row read next to the entry itself
+    const int32_t *kperm = use_sf ? static_cast<const int32_t *>(tensor_basis) + 4 + G + npe : nullptr;
+    const int32_t *kinv = use_sf ? static_cast<const int32_t *>(tensor_basis) + 4 + G : nullptr;   // local_of_t
+    const int32_t *eslot = nullptr;
+    if (kperm) {
+        const int64_t ne = nelem * npe;
+        entry_slot_kernel<<<(unsigned)((ne + 255) / 256), 256, 0, st>>>(node_elem, ne, npe, kperm, w.slot);
+        FL_LAUNCH_CHECK();
+        eslot = w.slot;
+    }
     int rc = use_sf ? launch_stiffness_sf(npe, G, w.qp, tensor_basis, nelem, (double *)w.Ke, st)
            : precision == FL_PREC_FP64
                  ? launch_stiffness_t<double, double>(NP, w.JmT, w.qp, npe, G, nelem, (double *)w.Ke, st)
@@ -965,16 +1011,16 @@ int fl_df_assemble_mt(int32_t material, const double *params, const int32_t *ele
         const int dg = bucket_degree[b];
         if (precision == FL_PREC_FP64)
             rc2 = launch_gather<double, double>(b, nb, elements, npe, ord, node_elem_ptr, node_elem, node_nbr_ptr,
-                                                node_nbrs, indptr, (const double *)w.Ke, w.te, dg, ranks, V, T,
-                                                status, st);
+                                                node_nbrs, indptr, (const double *)w.Ke, w.te, dg, ranks, kperm,
+                                                kinv, eslot, V, T, status, st);
         else if (!v32)
             rc2 = launch_gather<float, double>(b, nb, elements, npe, ord, node_elem_ptr, node_elem, node_nbr_ptr,
-                                               node_nbrs, indptr, (const float *)w.Ke, w.te, dg, ranks, V, T,
-                                               status, st);
+                                               node_nbrs, indptr, (const float *)w.Ke, w.te, dg, ranks, kperm,
+                                               kinv, eslot, V, T, status, st);
         else
             rc2 = launch_gather<float, float>(b, nb, elements, npe, ord, node_elem_ptr, node_elem, node_nbr_ptr,
-                                              node_nbrs, indptr, (const float *)w.Ke, w.te, dg, ranks,
-                                              reinterpret_cast<float *>(V), T, status, st);
+                                              node_nbrs, indptr, (const float *)w.Ke, w.te, dg, ranks, kperm,
+                                              kinv, eslot, reinterpret_cast<float *>(V), T, status, st);
         if (rc2) return rc2;
     }
     prof_mark(st);

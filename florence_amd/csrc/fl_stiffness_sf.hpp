@@ -24,13 +24,16 @@
 // per reduction) and stiffness_sf_mfma2_kernel (the default: two types per reduction).
 //
 // One workgroup per (element, block i,k) -- all nine blocks, so every workgroup writes
-// its own rows contiguously and no block is stored transposed.  The az slices are
-// processed one after another to keep T1 and T2 small.
+// its own rows and no block is stored transposed.  Rows and columns of the stored
+// blocks are in tensor node order (the gather maps local nodes through tensor_of_l),
+// so the stores need no lookups.  The az slices are processed one after another to
+// keep T1 and T2 small.
 //
 // Tensor basis table (fl_tensor_basis layout, built once on the host):
 //   int32  P1, Q, pad, pad
 //   int32  g_of_t[Q^3]      flattened point index of tensor point (qb*Q + qe)*Q + qz
 //   int32  local_of_t[P1^3] local node of tensor node (ab*P1 + ae)*P1 + az
+//   int32  tensor_of_l[P1^3] its inverse
 //   (padded to 8 bytes)
 //   double f1d[3][2][P1][Q] l and l' of direction d (zeta, eta, beta) at its points
 #pragma once
@@ -40,7 +43,7 @@ namespace fl {
 template <int P1, int Q>
 struct SfLayout {
     static constexpr int Q3 = Q * Q * Q, N3 = P1 * P1 * P1, PP = P1 * P1;
-    static constexpr int INTS = 4 + Q3 + N3 + ((Q3 + N3) % 2);   // 8-byte aligned
+    static constexpr int INTS = 4 + Q3 + 2 * N3 + (Q3 % 2);   // 8-byte aligned
 };
 
 // Phase 1 for tensor-product hexes: one workgroup per element.  The element's X and x
@@ -153,7 +156,6 @@ __global__ __launch_bounds__(256, 2) void stiffness_sf_kernel(const double *__re
     __shared__ double sC[9][Q3];
     __shared__ double sT1[9][P1][Q * Q];
     __shared__ double sT2[4][PP][P1][Q];
-    __shared__ int32_t sLoc[N3];
 
     const int64_t wid = blockIdx.x;
     const int xcd = (int)(wid & 7);
@@ -167,7 +169,6 @@ __global__ __launch_bounds__(256, 2) void stiffness_sf_kernel(const double *__re
     const bool tr = pi > pk;
     const int tid = threadIdx.x;
     const int32_t *g_of_t = basis + 4;
-    const int32_t *loc = basis + 4 + Q3;
     const double *f1d = reinterpret_cast<const double *>(basis + L::INTS);   // [3][2][P1][Q]
 
     for (int t = tid; t < 4 * PP * Q; t += NT) {
@@ -180,7 +181,6 @@ __global__ __launch_bounds__(256, 2) void stiffness_sf_kernel(const double *__re
     for (int t = tid; t < 2 * P1 * Q; t += NT) sFb[t / (P1 * Q)][(t / Q) % P1][t % Q] = f1d[(4 * P1) * Q + t];
     // table entries are clamped into range: a table that does not match the launch
     // shape gives wrong blocks, never an access outside them
-    for (int t = tid; t < N3; t += NT) sLoc[t] = min(max(loc[t], 0), N3 - 1);
     // C[mn][t] = sum_jl ix[j][m] D[j][l] ix[l][n] at tensor point t
     const double *qe_ = qp + e * (int64_t)(Q3)*QS;
     for (int t = tid; t < Q3; t += NT) {
@@ -255,7 +255,6 @@ __global__ __launch_bounds__(256, 2) void stiffness_sf_kernel(const double *__re
 #pragma unroll
                 for (int qb = 0; qb < Q; ++qb)
                     u[type][qb] = sFb[type & 1][bb][qb] * sT2[type][ae * P1 + be][bz][qb];
-            const int lb = sLoc[tb];
 #pragma unroll
             for (int ab = 0; ab < P1; ++ab) {
                 double s = 0.0;
@@ -263,8 +262,7 @@ __global__ __launch_bounds__(256, 2) void stiffness_sf_kernel(const double *__re
                 for (int type = 0; type < 4; ++type)
 #pragma unroll
                     for (int qb = 0; qb < Q; ++qb) s += sFb[type >> 1][ab][qb] * u[type][qb];
-                const int la = sLoc[(ab * P1 + ae) * P1 + az];
-                Kd[(int64_t)la * N3 + lb] = s;
+                Kd[(int64_t)((ab * P1 + ae) * P1 + az) * N3 + tb] = s;
             }
         }
         __syncthreads();
@@ -296,7 +294,6 @@ __global__ __launch_bounds__(256, 4) void stiffness_sf_mfma_kernel(const double 
     __shared__ double sC[9][Q3];
     __shared__ double sT1[9][P1][Q * Q];
     __shared__ double sT2[PP][P1][Q];
-    __shared__ int32_t sLoc[N3];
 
     const int64_t wid = blockIdx.x;
     const int xcd = (int)(wid & 7);
@@ -310,11 +307,9 @@ __global__ __launch_bounds__(256, 4) void stiffness_sf_mfma_kernel(const double 
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
     const int32_t *g_of_t = basis + 4;
-    const int32_t *loc = basis + 4 + Q3;
     const double *f1d = reinterpret_cast<const double *>(basis + L::INTS);
 
     for (int t = tid; t < 3 * 2 * P1 * Q; t += NT) (&sF[0][0][0][0])[t] = f1d[t];
-    for (int t = tid; t < N3; t += NT) sLoc[t] = min(max(loc[t], 0), N3 - 1);
     const double *qe_ = qp + e * (int64_t)(Q3)*QS;
     for (int t = tid; t < Q3; t += NT) {
         const double *rec = qe_ + (int64_t)min(max(g_of_t[t], 0), Q3 - 1) * QS;
@@ -424,9 +419,8 @@ __global__ __launch_bounds__(256, 4) void stiffness_sf_mfma_kernel(const double 
                         const int row = tm * 16 + MF::row(lk, r);
                         if (row < PP) {
                             const int ab = row / P1, bb = row - ab * P1;
-                            const int la = sLoc[(ab * P1 + ae_) * P1 + az];
-                            const int lb = sLoc[(bb * P1 + be_) * P1 + bz_];
-                            Kd[(int64_t)la * N3 + lb] = acc[u][r];
+                            const int ta = (ab * P1 + ae_) * P1 + az, tb = (bb * P1 + be_) * P1 + bz_;
+                            Kd[(int64_t)ta * N3 + tb] = acc[u][r];
                         }
                     }
                 }
@@ -464,7 +458,6 @@ __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(const double
     __shared__ double sC[9][Q3];
     __shared__ double sT1[9][P1][Q * Q];
     __shared__ double sT2[2][PP][P1][Q];
-    __shared__ int32_t sLoc[N3];
 
     const int64_t wid = blockIdx.x;
     const int xcd = (int)(wid & 7);
@@ -478,7 +471,6 @@ __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(const double
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
     const int32_t *g_of_t = basis + 4;
-    const int32_t *loc = basis + 4 + Q3;
     const double *f1d = reinterpret_cast<const double *>(basis + L::INTS);
 
     for (int t = tid; t < 4 * PP * Q; t += NT) {
@@ -488,7 +480,6 @@ __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(const double
         (&sPz[0][0][0][0])[t] = f1d[(s0 * P1 + a) * Q + q] * f1d[(t0 * P1 + b) * Q + q];
         (&sPe[0][0][0])[t] = f1d[((2 + s0) * P1 + a) * Q + q] * f1d[((2 + t0) * P1 + b) * Q + q];
     }
-    for (int t = tid; t < N3; t += NT) sLoc[t] = min(max(loc[t], 0), N3 - 1);
     // this lane's beta A fragments: phase h, step ks, row tile tm -> T2^h[ab][qb] T2^tl[bb][qb]
     double afr[2][KB][MT3];
 #pragma unroll
@@ -622,9 +613,8 @@ __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(const double
                         const int row = tm * 16 + MF::row(lk, r);
                         if (row < PP) {
                             const int ab = row / P1, bb = row - ab * P1;
-                            const int la = sLoc[(ab * P1 + ae_) * P1 + az];
-                            const int lb = sLoc[(bb * P1 + be_) * P1 + bz_];
-                            Kd[(int64_t)la * N3 + lb] = acc[tm][v][r];
+                            const int ta = (ab * P1 + ae_) * P1 + az, tb = (bb * P1 + be_) * P1 + bz_;
+                            Kd[(int64_t)ta * N3 + tb] = acc[tm][v][r];
                         }
                     }
             }

@@ -160,7 +160,8 @@ int fl_df_assemble_mt(int32_t material, const double *params, const int32_t *ele
  * contraction; same K to rounding.  Layout (fl_tensor_basis_bytes(p1, q) bytes):
  *   int32 p1, q, 0, 0; int32 g_of_t[q^3] (flattened point of tensor point
  *   (qb*q + qe)*q + qz); int32 local_of_t[p1^3] (local node of tensor node
- *   (ab*p1 + ae)*p1 + az); zero int32 padding to 8 bytes; double f1d[3][2][p1][q]
+ *   (ab*p1 + ae)*p1 + az); int32 tensor_of_l[p1^3] (its inverse); zero int32
+ *   padding to 8 bytes; double f1d[3][2][p1][q]
  *   (the 1D bases and their derivatives of directions zeta, eta, beta at that
  *   direction's points).  Supported (p1, q): (2,2) (3,3) (4,4) (5,5) (5,6).  NULL, an
  *   FP32/BF16 precision or fl_set_tuning(FL_TUNE_STIFF_SF, 1) select the dense path;
