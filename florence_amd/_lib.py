@@ -14,7 +14,7 @@ FL_MAT_MOONEYRIVLIN = 2
 FL_S_NONPOSITIVE_J = 1
 FL_S_NONFINITE = 2
 FL_S_PATTERN = 4
-FL_GATHER_BUCKETS = 5
+FL_GATHER_BUCKETS = 6
 FL_TUNE_STIFF_MINW = 1
 FL_TUNE_STIFF_VARIANT_BASE = 2
 STIFF_VARIANTS = {0: 1, 1: 2, 2: 1, 3: 2, 4: 3}   # padded node-count class -> number of tile variants

@@ -537,24 +537,28 @@ __global__ void gather_traction_kernel(int npe, const int32_t *__restrict__ node
 
 // ------------------------------------------------------------------ host side
 // Gather launch shapes (threads per node, elements per chunk, loads per lane per
-// element).  Classes above the lowest: <256, EB, 5> with EB = 2, 4 or 8 as the class's
-// degree suggests its nodes' element count (deg <= 2 npe, <= 4 npe, more): chunks
-// longer than a node's element count only cost registers.  The lowest class: one wave
-// per node, <64, 8, 2> for npe <= 14, <64, 4, 9> for npe <= 64 and <64, 2, 18> up to
-// 128; when every node in it lies inside one element (its degree equals npe: a node
-// of two distinct elements has more than npe neighbours) gather_single_kernel.
+// element).  Class 0 (the smallest neighbour count) whose degree equals npe holds
+// element-interior nodes only (a node of two distinct elements has more than npe
+// neighbours): gather_single_kernel.  Elements of at most 64 nodes: one wave per node
+// in every class, <64, 8, 2> for npe <= 14 and <64, 4, 9> up to 64.  Larger elements:
+// the other classes use <256, EB, 5> with EB = 2, 4 or 8 as the class's degree suggests
+// its nodes' element count (deg <= 2 npe, <= 4 npe, more: chunks longer than that only
+// cost registers), class 0 one wave per node, <64, 2, 18>.
 enum GatherForm { G_SINGLE, G_W8X2, G_W4X9, G_W2X18, G_WIDE1, G_WIDE2, G_WIDE4, G_WIDE8 };
 
 // FL_TUNE_GATHER: bit 0 = the wide classes one element at a time (<256, 1, 5>), bit 1 =
 // the single-element class through gather_kernel<64, 2, 18>
 static GatherForm gather_form(int b, int deg, int npe) {
     const int t = tuning_value(TUNE_GATHER, 0);
+    if (b == 0 && deg == npe && !(t & 2)) return G_SINGLE;
+    // up to 64 nodes an element's 9 npe values fill at most 9 loads of one wave
+    if (npe <= 14) return G_W8X2;
+    if (npe <= 64) return G_W4X9;
     if (b > 0) {
         if (t & 1) return G_WIDE1;
         return deg <= 2 * npe ? G_WIDE2 : (deg <= 4 * npe ? G_WIDE4 : G_WIDE8);
     }
-    if (deg == npe && !(t & 2)) return G_SINGLE;
-    return npe <= 14 ? G_W8X2 : (npe <= 64 ? G_W4X9 : G_W2X18);
+    return G_W2X18;
 }
 
 static int gather_eb(GatherForm f) {

@@ -8,6 +8,7 @@
 // candidate nodes of its incident elements into LDS, bitonic-sorts them and compacts.
 #include <cstdarg>
 #include <climits>
+#include <algorithm>
 #include <vector>
 
 #include "fl_common.hpp"
@@ -420,17 +421,25 @@ int fl_pattern_gather_plan(const int32_t *node_nbr_ptr, int64_t nnode, int32_t *
         set_error("fl_pattern_gather_plan: invalid argument");
         return FL_E_ARG;
     }
-    static const int32_t limits[FL_GATHER_BUCKETS] = {128, 256, 512, 1024, 2048};
+    // class 0: the nodes of the smallest neighbour count (in meshes with element-interior
+    // nodes, exactly those: a node of two distinct elements has more neighbours than an
+    // element has nodes); classes 1.. by the limits
+    static const int32_t limits[FL_GATHER_BUCKETS] = {0, 128, 256, 512, 1024, 2048};
     std::vector<int32_t> ptr((size_t)nnode + 1);
     FL_HIP_CHECK(hipMemcpyAsync(ptr.data(), node_nbr_ptr, sizeof(int32_t) * (nnode + 1), hipMemcpyDeviceToHost, st));
     FL_HIP_CHECK(hipStreamSynchronize(st));
+    int32_t dmin = INT32_MAX;
+    for (int64_t a = 0; a < nnode; ++a) dmin = std::min(dmin, ptr[a + 1] - ptr[a]);
     std::vector<int32_t> cls((size_t)nnode);
-    int64_t count[FL_GATHER_BUCKETS] = {0, 0, 0, 0, 0};
+    int64_t count[FL_GATHER_BUCKETS] = {};
     for (int b = 0; b < FL_GATHER_BUCKETS; ++b) bucket_degree[b] = 0;
     for (int64_t a = 0; a < nnode; ++a) {
         const int32_t d = ptr[a + 1] - ptr[a];
         int b = 0;
-        while (b < FL_GATHER_BUCKETS && d > limits[b]) ++b;
+        if (d != dmin) {
+            b = 1;
+            while (b < FL_GATHER_BUCKETS && d > limits[b]) ++b;
+        }
         if (b == FL_GATHER_BUCKETS) {
             set_error("fl_pattern_gather_plan: node %lld has %d neighbours (limit %d)", (long long)a, d,
                       limits[FL_GATHER_BUCKETS - 1]);

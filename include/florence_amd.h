@@ -68,12 +68,14 @@ int fl_pattern_fill(const int32_t *elements_dev, int64_t nelem, int32_t npe, int
                     int32_t *indices_dev, void *stream);
 
 /* Gather plan of the row-owner accumulation: node ids grouped by neighbour count
- * into FL_GATHER_BUCKETS classes (at most 128, 256, 512, 1024, 2048 neighbours), so
- * each class is accumulated with an LDS footprint sized to it.  node_order_dev:
+ * into FL_GATHER_BUCKETS classes (the nodes of the smallest count -- element-interior
+ * nodes where the mesh has them -- then the others by at most 128, 256, 512, 1024,
+ * 2048 neighbours), so each class is accumulated with an LDS footprint and kernel
+ * shape sized to it.  node_order_dev:
  * int32[nnode]; bucket_offsets (host): int32[FL_GATHER_BUCKETS+1]; bucket_degree
  * (host): int32[FL_GATHER_BUCKETS], the largest neighbour count in each class.
  * Synchronous.  No reference counterpart (part of the pattern set-up above). */
-#define FL_GATHER_BUCKETS 5
+#define FL_GATHER_BUCKETS 6
 int fl_pattern_gather_plan(const int32_t *node_nbr_ptr_dev, int64_t nnode, int32_t *node_order_dev,
                            int32_t *bucket_offsets, int32_t *bucket_degree, void *stream);
 
