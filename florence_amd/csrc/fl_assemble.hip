@@ -635,8 +635,10 @@ static int launch_stiffness_t(int NP, const double *JmT, const double *qp, int n
 }
 
 // Sum-factorised FP64 stiffness of tensor-product hexes (P1^3 nodes, Q^3 points).
+// Vdirect (the default form only): rows of element-interior nodes go straight into V
 static int launch_stiffness_sf(int npe, int G, const double *qp, const void *basis, int64_t nelem, double *Ke,
-                               hipStream_t st) {
+                               const int32_t *elements, const int32_t *node_elem_ptr, const int32_t *indptr,
+                               int32_t *irow, double *Vdirect, hipStream_t st) {
     const int64_t groups = (nelem + 7) / 8 * 8 * 9;
     if (groups > (int64_t)INT32_MAX) {
         set_error("too many elements for one launch");
@@ -653,8 +655,16 @@ static int launch_stiffness_sf(int npe, int G, const double *qp, const void *bas
             stiffness_sf_kernel<P, QQ><<<grid, 256, 0, st>>>(qp, b, nelem, Ke);            \
         else if (form == 5)                                                                \
             stiffness_sf_mfma_kernel<P, QQ><<<grid, 256, 0, st>>>(qp, b, nelem, Ke);       \
-        else                                                                               \
-            stiffness_sf_mfma2_kernel<P, QQ><<<grid, 256, 0, st>>>(qp, b, nelem, Ke);      \
+        else {                                                                             \
+            if (Vdirect) {                                                                 \
+                const int64_t nt = nelem * P * P * P;                                      \
+                interior_rows_kernel<P, QQ><<<(unsigned)((nt + 255) / 256), 256, 0, st>>>( \
+                    b, elements, nelem, node_elem_ptr, indptr, irow);                      \
+                FL_LAUNCH_CHECK();                                                         \
+            }                                                                              \
+            stiffness_sf_mfma2_kernel<P, QQ><<<grid, 256, 0, st>>>(qp, b, nelem, Ke, irow, \
+                                                                   Vdirect);               \
+        }                                                                                  \
         FL_LAUNCH_CHECK();                                                                 \
         return 0;                                                                          \
     }
@@ -668,14 +678,17 @@ static int launch_stiffness_sf(int npe, int G, const double *qp, const void *bas
     return FL_E_ARG;
 }
 
+// Tdirect: T entries of element-interior nodes written here (their gather is skipped)
 static int launch_traction_sf(int npe, int G, const double *qp, int qs, int soff, const void *basis, int64_t nelem,
-                              double *te, hipStream_t st) {
+                              double *te, const int32_t *elements, const int32_t *node_elem_ptr, double *Tdirect,
+                              hipStream_t st) {
     const int32_t *b = static_cast<const int32_t *>(basis);
-#define FL_TSF_CASE(P, QQ)                                                                          \
-    if (npe == P * P * P && G == QQ * QQ * QQ) {                                                    \
-        traction_sf_kernel<P, QQ><<<(unsigned)nelem, 256, 0, st>>>(qp, qs, soff, b, nelem, te);     \
-        FL_LAUNCH_CHECK();                                                                          \
-        return 0;                                                                                   \
+#define FL_TSF_CASE(P, QQ)                                                                                \
+    if (npe == P * P * P && G == QQ * QQ * QQ) {                                                          \
+        traction_sf_kernel<P, QQ><<<(unsigned)nelem, 256, 0, st>>>(qp, qs, soff, b, nelem, te, elements,  \
+                                                                    node_elem_ptr, Tdirect);              \
+        FL_LAUNCH_CHECK();                                                                                \
+        return 0;                                                                                         \
     }
     FL_TSF_CASE(2, 2)
     FL_TSF_CASE(3, 3)
@@ -799,6 +812,7 @@ struct DFWork {
     double *JmT, *qp, *te;
     void *Ke;
     int32_t *slot;   // stored row of each node-element entry's block (tensor-ordered blocks)
+    int32_t *irow;   // interior_rows_kernel's table (4 ints per element node)
 };
 
 // bytes per element-block entry for each tangent precision
@@ -814,6 +828,7 @@ static DFWork carve_df(void *ws, size_t bytes, int G, int64_t nelem, int npe, bo
     w.te = cv.take<double>((size_t)nelem * 3 * npe);
     w.Ke = tangent ? (void *)cv.take<char>((size_t)nelem * 9 * npe * npe * ke_bytes(precision)) : nullptr;
     w.slot = tangent ? cv.take<int32_t>((size_t)nelem * npe) : nullptr;
+    w.irow = tangent ? cv.take<int32_t>((size_t)nelem * npe * 4) : nullptr;
     *need = cv.off + 256;
     if (!cv.ok) w.JmT = nullptr;
     return w;
@@ -982,7 +997,14 @@ int fl_df_assemble_mt(int32_t material, const double *params, const int32_t *ele
         FL_LAUNCH_CHECK();
         eslot = w.slot;
     }
-    int rc = use_sf ? launch_stiffness_sf(npe, G, w.qp, tensor_basis, nelem, (double *)w.Ke, st)
+    // element-interior rows straight from the default factorised kernel into V (and their
+    // T entries from the traction kernel) when the lowest gather class holds exactly
+    // those nodes (its degree equals npe); not with update != 1, whose T is zero
+    const bool fuse_interior = use_sf && update == 1 && tuning_value(TUNE_STIFF_SF, 0) != 3 &&
+                               tuning_value(TUNE_STIFF_SF, 0) != 5 && bucket_offsets[1] > bucket_offsets[0] &&
+                               bucket_degree[0] == npe;
+    int rc = use_sf ? launch_stiffness_sf(npe, G, w.qp, tensor_basis, nelem, (double *)w.Ke, elements, node_elem_ptr,
+                                          indptr, w.irow, fuse_interior ? V : nullptr, st)
            : precision == FL_PREC_FP64
                  ? launch_stiffness_t<double, double>(NP, w.JmT, w.qp, npe, G, nelem, (double *)w.Ke, st)
                  : precision == FL_PREC_FP32
@@ -991,7 +1013,8 @@ int fl_df_assemble_mt(int32_t material, const double *params, const int32_t *ele
     if (rc) return rc;
     prof_mark(st);
     if (sf_ok) {
-        rc = launch_traction_sf(npe, G, w.qp, QS, Q_S, tensor_basis, nelem, w.te, st);
+        rc = launch_traction_sf(npe, G, w.qp, QS, Q_S, tensor_basis, nelem, w.te, elements, node_elem_ptr,
+                                fuse_interior ? T : nullptr, st);
         if (rc) return rc;
     } else {
         const size_t lds_tr = (size_t)G * 9 * 8;
@@ -1007,7 +1030,7 @@ int fl_df_assemble_mt(int32_t material, const double *params, const int32_t *ele
         FL_LAUNCH_CHECK();
     }
     prof_mark(st);
-    for (int b = 0; b < FL_GATHER_BUCKETS; ++b) {
+    for (int b = fuse_interior ? 1 : 0; b < FL_GATHER_BUCKETS; ++b) {
         const int32_t nb = bucket_offsets[b + 1] - bucket_offsets[b];
         if (nb <= 0) continue;
         int rc2 = 0;
@@ -1067,7 +1090,8 @@ static int df_residual(int32_t material, const double *params, const double *poi
     }
     prof_mark(st);
     if (sf_ok) {
-        const int rt = launch_traction_sf(npe, G, w.qp, QS_RES, QR_S, tensor_basis, nelem, w.te, st);
+        const int rt = launch_traction_sf(npe, G, w.qp, QS_RES, QR_S, tensor_basis, nelem, w.te, nullptr, nullptr,
+                                          nullptr, st);
         if (rt) return rt;
     } else {
         const size_t lds_tr = (size_t)G * 9 * 8;

@@ -429,6 +429,34 @@ __global__ __launch_bounds__(256, 4) void stiffness_sf_mfma_kernel(const double 
     }
 }
 
+// Per element and tensor node t: the CSR offsets of rows 3a, 3a+1, 3a+2 when node a lies
+// in this element only (else -1) and the position of a among the element's nodes sorted
+// by id -- the column block of t in every such row, whose neighbours are exactly the
+// element's nodes.  irow[(e N3 + t) 4 + {0,1,2,3}].  One thread per (element, node).
+template <int P1, int Q>
+__global__ __launch_bounds__(256) void interior_rows_kernel(const int32_t *__restrict__ basis,
+                                                            const int32_t *__restrict__ elements, int64_t nelem,
+                                                            const int32_t *__restrict__ node_elem_ptr,
+                                                            const int32_t *__restrict__ indptr,
+                                                            int32_t *__restrict__ irow) {
+    typedef SfLayout<P1, Q> L;
+    constexpr int Q3 = L::Q3, N3 = L::N3;
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nelem * N3) return;
+    const int64_t e = g / N3;
+    const int t = (int)(g - e * N3);
+    const int32_t *loc = basis + 4 + Q3;
+    const int32_t *conn = elements + e * N3;
+    const int32_t a = conn[min(max(loc[t], 0), N3 - 1)];
+    int r = 0;
+    for (int u = 0; u < N3; ++u) r += conn[u] < a;
+    const bool interior = node_elem_ptr[a + 1] - node_elem_ptr[a] == 1;
+    int32_t *o = irow + g * 4;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) o[i] = interior ? indptr[3 * (int64_t)a + i] : -1;
+    o[3] = r;
+}
+
 // The same contraction with the inner indices of two types merged into one MFMA
 // reduction: the two types sharing the beta factor of m (a "phase") are contracted
 // together, K = (type, qb) of length 2Q for beta and K = (m, n, qe) of length
@@ -438,12 +466,15 @@ __global__ __launch_bounds__(256, 4) void stiffness_sf_mfma_kernel(const double 
 // MT3.  The products of two 1D basis values that form the zeta factors and the eta A
 // fragments are tabulated in LDS once per workgroup, and the beta A fragments, which
 // depend only on the lane, stay in registers: LDS traffic, not MFMA issue, bounds this
-// kernel.  LDS: T2 of the phase's two types and the tables (~50 KB at p = 4, three
-// workgroups per CU).
+// kernel.  With V given, rows of element-interior nodes go straight into the CSR
+// instead of the block buffer (the gather then skips those nodes).  LDS: T2 of the
+// phase's two types and the tables (~51 KB at p = 4, three workgroups per CU).
 template <int P1, int Q>
 __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(const double *__restrict__ qp,
                                                                     const int32_t *__restrict__ basis,
-                                                                    int64_t nelem, double *__restrict__ Ke) {
+                                                                    int64_t nelem, double *__restrict__ Ke,
+                                                                    const int32_t *__restrict__ irow,
+                                                                    double *__restrict__ V) {
     typedef SfLayout<P1, Q> L;
     typedef Mfma16<double> MF;
     constexpr int Q3 = L::Q3, N3 = L::N3, PP = L::PP;
@@ -458,6 +489,12 @@ __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(const double
     __shared__ double sC[9][Q3];
     __shared__ double sT1[9][P1][Q * Q];
     __shared__ double sT2[2][PP][P1][Q];
+    // interior rows written straight into the CSR (V != NULL): CSR offset of row 3a+pi of
+    // tensor node t when node a lies in this element only (else -1), and the position
+    // of tensor node t among the element's nodes sorted by id -- the column block of
+    // every such row, whose neighbours are exactly the element's nodes
+    __shared__ int32_t sRow[N3];
+    __shared__ int32_t sRank[N3];
 
     const int64_t wid = blockIdx.x;
     const int xcd = (int)(wid & 7);
@@ -473,6 +510,14 @@ __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(const double
     const int32_t *g_of_t = basis + 4;
     const double *f1d = reinterpret_cast<const double *>(basis + L::INTS);
 
+    if (V) {
+        // interior_rows_kernel's table: per tensor node three row offsets and the rank
+        const int32_t *ir = irow + e * (int64_t)N3 * 4;
+        for (int t = tid; t < N3; t += NT) {
+            sRow[t] = ir[4 * t + pi];
+            sRank[t] = min(max(ir[4 * t + 3], 0), N3 - 1);
+        }
+    }
     for (int t = tid; t < 4 * PP * Q; t += NT) {
         const int st = t / (PP * Q), r = t - st * (PP * Q);
         const int a = r / (P1 * Q), b = (r / Q) % P1, q = r % Q;
@@ -614,7 +659,11 @@ __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(const double
                         if (row < PP) {
                             const int ab = row / P1, bb = row - ab * P1;
                             const int ta = (ab * P1 + ae_) * P1 + az, tb = (bb * P1 + be_) * P1 + bz_;
-                            Kd[(int64_t)ta * N3 + tb] = acc[tm][v][r];
+                            const int32_t ro = V ? sRow[ta] : -1;
+                            if (ro >= 0)
+                                V[(int64_t)ro + 3 * sRank[tb] + pk] = 0.0 + acc[tm][v][r];
+                            else
+                                Kd[(int64_t)ta * N3 + tb] = acc[tm][v][r];
                         }
                     }
             }
@@ -630,7 +679,9 @@ __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(const double
 template <int P1, int Q>
 __global__ __launch_bounds__(256) void traction_sf_kernel(const double *__restrict__ qp, int qs, int soff,
                                                           const int32_t *__restrict__ basis, int64_t nelem,
-                                                          double *__restrict__ te) {
+                                                          double *__restrict__ te, const int32_t *__restrict__ elements,
+                                                          const int32_t *__restrict__ node_elem_ptr,
+                                                          double *__restrict__ T) {
     typedef SfLayout<P1, Q> L;
     constexpr int Q3 = L::Q3, N3 = L::N3, PP = L::PP;
     constexpr int NT = 256;
@@ -690,7 +741,14 @@ __global__ __launch_bounds__(256) void traction_sf_kernel(const double *__restri
 #pragma unroll
             for (int qb = 0; qb < Q; ++qb) s += f[qb] * sU2[3 * m + i][aeaz][qb];
         }
-        te[e * 3 * N3 + 3 * sLoc[tn] + i] = s;
+        const int lt = sLoc[tn];
+        te[e * 3 * N3 + 3 * lt + i] = s;
+        if (T) {
+            // a node of this element only: its T entry is this element's (as 0.0 + value,
+            // the accumulated form)
+            const int32_t a = elements[e * N3 + lt];
+            if (node_elem_ptr[a + 1] - node_elem_ptr[a] == 1) T[3 * (int64_t)a + i] = 0.0 + s;
+        }
     }
 }
 
