@@ -592,35 +592,44 @@ __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(const double
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             // eta (MFMA) for the phase's types (m's beta factor h, n's tl): rows (ae,be),
-            // columns (bz,qb), K = (m, n, qe) over the type's (m, n)
-            for (int task = wave; task < 2 * MT2 * NT2; task += 4) {
-                const int tl = task / (MT2 * NT2), tile = task - tl * (MT2 * NT2);
+            // columns (bz,qb), K = (m, n, qe) over the type's (m, n).  A wave takes one tile
+            // for both types, whose two accumulation chains interleave.
+            for (int tile = wave; tile < MT2 * NT2; tile += 4) {
                 const int tm = tile / NT2, tn = tile - tm * NT2;
                 const int arow = tm * 16 + li, bcol = tn * 16 + li;
-                const int ae_ = min(arow / P1, P1 - 1), be_ = arow % P1;
+                const bool av = arow < PP, bv = bcol < P1 * Q;
+                const int arc = av ? arow : 0;
                 const int bz_ = min(bcol / Q, P1 - 1), qb_ = bcol % Q;
-                const int nn = tl ? 1 : 2;
-                const int ke = (h ? 1 : 2) * nn * Q;
-                typename MF::acc_t a2 = {0.0, 0.0, 0.0, 0.0};
+                typename MF::acc_t a2[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
 #pragma unroll
                 for (int ks = 0; ks < KE_MAX; ++ks) {
-                    if (ks * 4 < ke) {
+                    double a[2], b[2];
+#pragma unroll
+                    for (int tl = 0; tl < 2; ++tl) {
+                        const int nn = tl ? 1 : 2, ke = (h ? 1 : 2) * nn * Q;
                         const int kk = ks * 4 + lk;
-                        double a = 0.0, b = 0.0;
-                        if (kk < ke) {
+                        a[tl] = 0.0;
+                        b[tl] = 0.0;
+                        if (ks * 4 < ke && kk < ke) {
                             const int pr = kk / Q, qe = kk - pr * Q;
                             const int m = h ? 2 : pr / nn, n = tl ? 2 : pr % nn;
-                            if (arow < PP) a = sPe[(m == 1) * 2 + (n == 1)][arow][qe];
-                            if (bcol < P1 * Q) b = sT1[3 * m + n][bz_][qe * Q + qb_];
+                            if (av) a[tl] = sPe[(m == 1) * 2 + (n == 1)][arc][qe];
+                            if (bv) b[tl] = sT1[3 * m + n][bz_][qe * Q + qb_];
                         }
-                        a2 = MF::step(a, b, a2);
+                    }
+#pragma unroll
+                    for (int tl = 0; tl < 2; ++tl) {
+                        const int ke = (h ? 1 : 2) * (tl ? 1 : 2) * Q;
+                        if (ks * 4 < ke) a2[tl] = MF::step(a[tl], b[tl], a2[tl]);
                     }
                 }
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int row = tm * 16 + MF::row(lk, r);
-                    if (row < PP && bcol < P1 * Q) sT2[tl][row][bz_][qb_] = a2[r];
-                }
+                for (int tl = 0; tl < 2; ++tl)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int row = tm * 16 + MF::row(lk, r);
+                        if (row < PP && bv) sT2[tl][row][bz_][qb_] = a2[tl][r];
+                    }
             }
             __syncthreads();
             // beta (MFMA): rows (ab,bb), columns (ae,be,bz), K = (tl, qb)
