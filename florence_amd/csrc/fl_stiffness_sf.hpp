@@ -506,7 +506,8 @@ __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(const double
     const int sp = pi == pk ? pi : (pi + pk == 1 ? 3 : (pi + pk == 2 ? 4 : 5));
     const bool tr = pi > pk;
     const int tid = threadIdx.x;
-    const int wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
+    // wave index read through a scalar register, so tests on it are uniform branches
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63, li = lane & 15, lk = lane >> 4;
     const int32_t *g_of_t = basis + 4;
     const double *f1d = reinterpret_cast<const double *>(basis + L::INTS);
 
@@ -607,14 +608,19 @@ __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(const double
 #pragma unroll
                     for (int tl = 0; tl < 2; ++tl) {
                         const int nn = tl ? 1 : 2, ke = (h ? 1 : 2) * nn * Q;
-                        const int kk = ks * 4 + lk;
                         a[tl] = 0.0;
                         b[tl] = 0.0;
-                        if (ks * 4 < ke && kk < ke) {
-                            const int pr = kk / Q, qe = kk - pr * Q;
+                        if (ks * 4 < ke) {
+                            // clamped loads, zeroed by a select where K or the tile pads
+                            const int kk = ks * 4 + lk;
+                            const bool kv = ks * 4 + 3 < ke || kk < ke;
+                            const int kc = min(kk, ke - 1);
+                            const int pr = kc / Q, qe = kc - pr * Q;
                             const int m = h ? 2 : pr / nn, n = tl ? 2 : pr % nn;
-                            if (av) a[tl] = sPe[(m == 1) * 2 + (n == 1)][arc][qe];
-                            if (bv) b[tl] = sT1[3 * m + n][bz_][qe * Q + qb_];
+                            const double x = sPe[(m == 1) * 2 + (n == 1)][arc][qe];
+                            const double y = sT1[3 * m + n][bz_][qe * Q + qb_];
+                            a[tl] = (kv && av) ? x : 0.0;
+                            b[tl] = (kv && bv) ? y : 0.0;
                         }
                     }
 #pragma unroll
@@ -635,15 +641,20 @@ __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(const double
             // beta (MFMA): rows (ab,bb), columns (ae,be,bz), K = (tl, qb)
 #pragma unroll
             for (int ks = 0; ks < KB; ++ks) {
+                // loads from clamped addresses, zeroed by a select where K or the column
+                // pads (no branches around them)
                 const int kk = ks * 4 + lk;
-                const bool kv = kk < 2 * Q;
-                const int tl = kv ? kk / Q : 0, qb = kv ? kk - (kk / Q) * Q : 0;
+                const bool kv = ks * 4 + 3 < 2 * Q || kk < 2 * Q;
+                const int kc = min(kk, 2 * Q - 1);
+                const int tl = kc / Q, qb = kc - tl * Q;
                 double bf[TNW];
 #pragma unroll
                 for (int v = 0; v < TNW; ++v) {
                     const int bcol = (wave + 4 * v) * 16 + li;
-                    const int ae_ = min(bcol / PP, P1 - 1), be_ = (bcol / P1) % P1, bz_ = bcol % P1;
-                    bf[v] = (kv && bcol < N3) ? sT2[tl][ae_ * P1 + be_][bz_][qb] : 0.0;
+                    const int bc = min(bcol, N3 - 1);
+                    const int ae_ = bc / PP, be_ = (bc / P1) % P1, bz_ = bc % P1;
+                    const double x = sT2[tl][ae_ * P1 + be_][bz_][qb];
+                    bf[v] = (kv && bcol < N3) ? x : 0.0;
                 }
 #pragma unroll
                 for (int tm = 0; tm < MT3; ++tm)
@@ -664,15 +675,23 @@ __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(const double
                 for (int tm = 0; tm < MT3; ++tm)
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
+                        // rows tm 16 + lk + 4 r, lk in [0, 4): whole groups of four are kept
+                        // or dropped at compile time
+                        const int rlo = tm * 16 + 4 * r;
                         const int row = tm * 16 + MF::row(lk, r);
-                        if (row < PP) {
+                        if (rlo < PP && (rlo + 3 < PP || row < PP)) {
                             const int ab = row / P1, bb = row - ab * P1;
                             const int ta = (ab * P1 + ae_) * P1 + az, tb = (bb * P1 + be_) * P1 + bz_;
-                            const int32_t ro = V ? sRow[ta] : -1;
-                            if (ro >= 0)
-                                V[(int64_t)ro + 3 * sRank[tb] + pk] = 0.0 + acc[tm][v][r];
-                            else
-                                Kd[(int64_t)ta * N3 + tb] = acc[tm][v][r];
+                            // one store through a selected address (the 0.0 + value form is
+                            // what the gather would have written for an interior row, and
+                            // leaves a block entry's sums unchanged)
+                            double *dst = Kd + ((int64_t)ta * N3 + tb);
+                            if (V) {
+                                const int32_t ro = sRow[ta];
+                                const int32_t rk = sRank[tb];
+                                dst = ro >= 0 ? V + ((int64_t)ro + 3 * rk + pk) : dst;
+                            }
+                            *dst = 0.0 + acc[tm][v][r];
                         }
                     }
             }
