@@ -189,3 +189,65 @@ def test_sum_factorised_equals_dense(cuda, p):
         print("form {}: |dV|/|V| = {:.3e}".format(form, np.abs(V - V3.cpu().numpy()).max() / np.abs(V2).max()))
         assert np.abs(V - V3.cpu().numpy()).max() <= 1e-13 * np.abs(V2).max(), form
         assert np.array_equal(T, T3.cpu().numpy()), form
+
+
+@pytest.mark.parametrize("et,p,n", [("hex", 4, 2), ("hex", 2, 3), ("tet", 3, 2)])
+def test_gather_forms_agree(cuda, et, p, n):
+    """Every gather form (FL_TUNE_GATHER bits: one element at a time, element-interior
+    nodes through the general kernel) and the factorised stiffness with and without the
+    interior-row fusion (forms 0 and 5) give the same V and T: bitwise for the gather
+    forms over the same element blocks, to 1e-13 across stiffness forms, and all within
+    the oracle bar."""
+    import torch
+    from florence_amd import _lib
+    from florence_amd.assembly import DeviceMeshData, assemble_device
+    if (et, p) == ("hex", 4):
+        m, fs, mat = config2(n=n)
+    else:
+        m = cube_mesh(et, p, n, jitter=0.05, seed=p)
+        fs = function_space(m, p)
+        mat = MooneyRivlin(3, mu1=0.5, mu2=0.5, lamb=20.0)
+    x = deformed(m.points, 0.05, seed=31 + p)
+    dm = DeviceMeshData(m, fs)
+    lib = _lib.load()
+    out = {}
+    try:
+        for sf in (0, 5):
+            for gf in (0, 1, 2, 3):
+                _lib.check(lib.fl_set_tuning(_lib.FL_TUNE_STIFF_SF, sf), "fl_set_tuning")
+                _lib.check(lib.fl_set_tuning(_lib.FL_TUNE_GATHER, gf), "fl_set_tuning")
+                V, T, st = assemble_device(dm, mat, x)
+                torch.cuda.synchronize()
+                assert int(st[0].item()) == 0, (sf, gf)
+                out[(sf, gf)] = (V.cpu().numpy(), T.cpu().numpy())
+    finally:
+        lib.fl_set_tuning(_lib.FL_TUNE_STIFF_SF, 0)
+        lib.fl_set_tuning(_lib.FL_TUNE_GATHER, 0)
+    for sf in (0, 5):
+        V0, T0 = out[(sf, 0)]
+        for gf in (1, 2, 3):
+            V1, T1 = out[(sf, gf)]
+            assert np.array_equal(V0, V1) and np.array_equal(T0, T1), (sf, gf)
+    Va, Ta = out[(0, 0)]
+    Vb, Tb = out[(5, 0)]
+    assert np.abs(Va - Vb).max() <= 1e-13 * np.abs(Vb).max()
+    assert np.abs(Ta - Tb).max() <= RTOL * np.abs(Tb).max()
+    indptr_ref, indices_ref = O.sparsity_pattern(m.elements, m.nnode)
+    Vr, Tr = O.assemble_csr(m.points, m.elements, x, fs.Jm, fs.AllGauss, mat, indptr_ref, indices_ref)
+    assert np.abs(Va - Vr).max() <= RTOL * np.abs(Vr).max()
+    assert np.abs(Ta - Tr).max() <= RTOL * np.abs(Tr).max()
+
+
+def test_orphan_node(cuda):
+    """A point no element uses (its CSR rows are empty, its T entries zero) leaves the
+    assembly of the other nodes unchanged."""
+    import copy
+    m = cube_mesh("hex", 2, 2, jitter=0.05, seed=3)
+    m2 = copy.deepcopy(m)
+    m2.points = np.vstack([np.asarray(m.points), [[2.0, 2.0, 2.0]]])
+    m2.nnode = m2.points.shape[0]
+    fs = function_space(m2, 2)
+    mat = NeoHookean(3, mu=1.0, lamb=10.0)
+    x = deformed(m2.points, 0.05, seed=4)
+    dm, V, T = _check(m2, fs, mat, x, "hex p2 with an orphan point")
+    assert np.all(T[-3:] == 0.0)
