@@ -63,6 +63,25 @@ SF_KERNELS = {
 SF_KERNELS[2] = SF_KERNELS[4] = SF_KERNELS[0]
 
 
+def device_copy_gbs(torch, nbytes, reps=10):
+    """Read+write rate of a plain device-to-device copy of nbytes (GB/s, bytes read plus
+    bytes written), timed with HIP events."""
+    a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(2):
+        b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    return 2 * nbytes / (ms / 1e3) / 1e9
+
+
 @contextlib.contextmanager
 def no_gc():
     """A timed region without Python's cyclic collector: a full collection with torch
@@ -529,12 +548,17 @@ def main():
                    "kernel": "gather_kernel (all degree classes of one assembly)", "bytes_per_launch": gbytes,
                    "interior_nodes_written_by_stiffness": n_interior,
                    "kernel_ms": round(float(phase_ms[3]), 4)}
+    # the practical read+write rate of this GPU: a 4 GiB device-to-device copy, same
+    # clock (HIP events), for comparison with the gather's bytes
+    del V
+    copy_gbs = device_copy_gbs(torch, 4 << 30)
+    gather_roof["copy_gbs_measured"] = round(copy_gbs, 1)
+    gather_roof["frac_of_copy"] = round(gather_gbs / copy_gbs, 4)
     dominant_is_stiffness = phase_ms[1] >= phase_ms[3]
     print("phases ms: qp {:.3f} stiffness {:.3f} traction {:.3f} gather {:.3f}".format(*phase_ms),
           file=sys.stderr, flush=True)
 
     # matrix-free residual-only pass on the same mesh (explicit DF internal force)
-    del V
     dm.release_workspace()
     Tres = torch.empty(3 * dm.nnode, dtype=torch.float64, device="cuda")
     residual_device(dm, mat, x, Tres)
