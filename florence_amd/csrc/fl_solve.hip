@@ -190,13 +190,22 @@ __global__ __launch_bounds__(SV_T) void cg_start_kernel(int64_t nnode, const dou
 template <int NT>
 __device__ inline void block_sum2(const double *__restrict__ part, int64_t n, int stride, double &o0, double &o1,
                                   double *sh0, double *sh1) {
-    double a = 0.0, b = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += NT) {
-        a += part[stride * i];
-        if (stride == 2) b += part[2 * i + 1];
+    // four independent partial sums per thread keep four loads in flight; fixed order
+    double a[4] = {0.0, 0.0, 0.0, 0.0}, b[4] = {0.0, 0.0, 0.0, 0.0};
+    int64_t i = threadIdx.x;
+    for (; i + 3 * NT < n; i += 4 * NT) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            a[u] += part[stride * (i + u * NT)];
+            if (stride == 2) b[u] += part[2 * (i + u * NT) + 1];
+        }
     }
-    sh0[threadIdx.x] = a;
-    sh1[threadIdx.x] = b;
+    for (; i < n; i += NT) {
+        a[0] += part[stride * i];
+        if (stride == 2) b[0] += part[2 * i + 1];
+    }
+    sh0[threadIdx.x] = (a[0] + a[1]) + (a[2] + a[3]);
+    sh1[threadIdx.x] = (b[0] + b[1]) + (b[2] + b[3]);
     __syncthreads();
     for (int off = NT / 2; off > 0; off >>= 1) {
         if (threadIdx.x < off) {
