@@ -91,10 +91,13 @@ __global__ void transpose_jm_kernel(const double *__restrict__ Jm, int npe, int 
 // Jacobians PX[3m+c] = dX_c/dxi_m, Px = dx_c/dxi_m (_KinematicMeasures_.h:64-119,
 // _NeoHookean_.h:27-51, _MooneyRivlin_.h:30-70).  valid = false lanes only join the
 // wave's status ballot.
+// emitC: store, in place of each pair's D^{ik}, the tensor C^{ik}_{mn} = sum_{jl}
+// (dxi_m/dx_j) D^{ik}_{jl} (dxi_n/dx_l) that the sum-factorised stiffness contracts
+// (computed once per point instead of in each of the element's block workgroups).
 template <bool TANGENT>
 __device__ __forceinline__ void qp_point(const MatParams &mp, const double (&PX)[9], const double (&Px)[9], double wg,
                                          int update, bool valid, double *__restrict__ rec,
-                                         int32_t *__restrict__ status) {
+                                         int32_t *__restrict__ status, bool emitC = false) {
     double iX[9], ix[9];
     const double detX = invdet3x3(PX, iX);
     const double detx = invdet3x3(Px, ix);
@@ -158,6 +161,7 @@ __device__ __forceinline__ void qp_point(const MatParams &mp, const double (&PX)
         for (int p = 0; p < 6; ++p) {
             const int i = p < 3 ? p : (p == 5 ? 1 : 0);
             const int k = p < 3 ? p : (p == 3 ? 1 : 2);
+            double D[9];
 #pragma unroll
             for (int j = 0; j < 3; ++j)
 #pragma unroll
@@ -168,8 +172,24 @@ __device__ __forceinline__ void qp_point(const MatParams &mp, const double (&PX)
                     const double cijkl = c1 * (2.0 * b[3 * i + j] * b[3 * k + l] - b[3 * i + k] * b[3 * j + l] -
                                                b[3 * i + l] * b[3 * j + k]) +
                                          a1 * (dik * djl + dil * djk) + a2 * dij * dkl;
-                    rec[Q_D + 9 * p + 3 * j + l] = detJ * (cijkl + dik * sig[3 * j + l]);
+                    D[3 * j + l] = detJ * (cijkl + dik * sig[3 * j + l]);
                 }
+            if (emitC) {
+                double W[9];
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+#pragma unroll
+                    for (int n = 0; n < 3; ++n)
+                        W[3 * j + n] = D[3 * j] * ix[n] + D[3 * j + 1] * ix[3 + n] + D[3 * j + 2] * ix[6 + n];
+#pragma unroll
+                for (int m = 0; m < 3; ++m)
+#pragma unroll
+                    for (int n = 0; n < 3; ++n)
+                        rec[Q_D + 9 * p + 3 * m + n] = ix[m] * W[n] + ix[3 + m] * W[3 + n] + ix[6 + m] * W[6 + n];
+            } else {
+#pragma unroll
+                for (int t = 0; t < 9; ++t) rec[Q_D + 9 * p + t] = D[t];
+            }
         }
         rec[Q_S + 0] = detJ * sig[0];
         rec[Q_S + 1] = detJ * sig[4];
@@ -703,12 +723,14 @@ static int launch_traction_sf(int npe, int G, const double *qp, int qs, int soff
 template <bool TANGENT>
 static int launch_qp_sf(const MatParams &mp, const double *points, const int32_t *elements, const double *Eulerx,
                         const void *basis, const double *w, int npe, int G, int64_t nelem, int update, double *qp,
-                        int32_t *status, const int32_t *elem_mat, const double *mat_table, hipStream_t st) {
+                        int32_t *status, const int32_t *elem_mat, const double *mat_table, hipStream_t st,
+                        int emitC = 0) {
     const int32_t *b = static_cast<const int32_t *>(basis);
 #define FL_QSF_CASE(P, QQ)                                                                                       \
     if (npe == P * P * P && G == QQ * QQ * QQ) {                                                                 \
         qp_sf_kernel<P, QQ, TANGENT><<<(unsigned)nelem, 256, 0, st>>>(mp, points, elements, Eulerx, b, w, nelem, \
-                                                                       update, qp, status, elem_mat, mat_table);  \
+                                                                       update, qp, status, elem_mat, mat_table,   \
+                                                                       emitC);                                    \
         FL_LAUNCH_CHECK();                                                                                       \
         return 0;                                                                                                \
     }
@@ -973,10 +995,13 @@ int fl_df_assemble_mt(int32_t material, const double *params, const int32_t *ele
     // the factorised point, traction and (FP64) stiffness kernels whenever the tensor
     // basis is given, so T is the same for every tangent precision
     const bool sf_ok = tensor_basis != nullptr && sf_shape(npe, G) && tuning_value(TUNE_STIFF_SF, 0) != 1;
+    const bool use_sf = precision == FL_PREC_FP64 && sf_ok;
     const int64_t nq = nelem * G;
     if (sf_ok) {
+        // the factorised FP64 stiffness reads C^{ik} from the records; the dense FP32/BF16
+        // contractions read D^{ik}
         const int rq = launch_qp_sf<true>(mp, points, elements, Eulerx, tensor_basis, AllGauss, npe, G, nelem, update,
-                                          w.qp, status, elem_mat, mat_table, st);
+                                          w.qp, status, elem_mat, mat_table, st, use_sf ? 1 : 0);
         if (rq) return rq;
     } else {
         qp_kernel<true><<<(unsigned)((nq + 255) / 256), 256, 0, st>>>(mp, points, elements, Eulerx, Jm, AllGauss, G,
@@ -985,7 +1010,6 @@ int fl_df_assemble_mt(int32_t material, const double *params, const int32_t *ele
         FL_LAUNCH_CHECK();
     }
     prof_mark(st);
-    const bool use_sf = precision == FL_PREC_FP64 && sf_ok;
     // the factorised blocks are stored in tensor node order: tensor_of_l of the basis table,
     // and each node-element entry's stored row read next to the entry itself
     const int32_t *kperm = use_sf ? static_cast<const int32_t *>(tensor_basis) + 4 + G + npe : nullptr;

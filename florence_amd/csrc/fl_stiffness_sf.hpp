@@ -64,7 +64,7 @@ __global__ __launch_bounds__(256) void qp_sf_kernel(MatParams mp0, const double 
                                                     const int32_t *__restrict__ basis, const double *__restrict__ w,
                                                     int64_t nelem, int update, double *__restrict__ qp,
                                                     int32_t *__restrict__ status, const int32_t *__restrict__ elem_mat,
-                                                    const double *__restrict__ mat_table) {
+                                                    const double *__restrict__ mat_table, int emitC) {
     typedef SfLayout<P1, Q> L;
     constexpr int Q3 = L::Q3, N3 = L::N3, PP = L::PP;
     constexpr int NT = 256;
@@ -140,7 +140,8 @@ __global__ __launch_bounds__(256) void qp_sf_kernel(MatParams mp0, const double 
                 Px[3 * m + c] = sAP[((3 + c) * 3 + m) * Q3 + tq];
             }
         const int g = min(max(g_of_t[tq], 0), Q3 - 1);
-        qp_point<TANGENT>(mp, PX, Px, w[g], update, valid, qp + (e * Q3 + g) * (TANGENT ? QS : QS_RES), status);
+        qp_point<TANGENT>(mp, PX, Px, w[g], update, valid, qp + (e * Q3 + g) * (TANGENT ? QS : QS_RES), status,
+                          emitC != 0);
     }
 }
 
@@ -181,28 +182,15 @@ __global__ __launch_bounds__(256, 2) void stiffness_sf_kernel(const double *__re
     for (int t = tid; t < 2 * P1 * Q; t += NT) sFb[t / (P1 * Q)][(t / Q) % P1][t % Q] = f1d[(4 * P1) * Q + t];
     // table entries are clamped into range: a table that does not match the launch
     // shape gives wrong blocks, never an access outside them
-    // C[mn][t] = sum_jl ix[j][m] D[j][l] ix[l][n] at tensor point t
+    // C[mn][t] = sum_jl ix[j][m] D[j][l] ix[l][n] at tensor point t, as the point kernel stored it
     const double *qe_ = qp + e * (int64_t)(Q3)*QS;
     for (int t = tid; t < Q3; t += NT) {
         const double *rec = qe_ + (int64_t)min(max(g_of_t[t], 0), Q3 - 1) * QS;
-        double ix[9], D[9];
-#pragma unroll
-        for (int i = 0; i < 9; ++i) ix[i] = rec[Q_IX + i];
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-#pragma unroll
-            for (int l = 0; l < 3; ++l) D[3 * j + l] = rec[Q_D + 9 * sp + (tr ? 3 * l + j : 3 * j + l)];
-        double W[9];
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-#pragma unroll
-            for (int n = 0; n < 3; ++n)
-                W[3 * j + n] = D[3 * j] * ix[n] + D[3 * j + 1] * ix[3 + n] + D[3 * j + 2] * ix[6 + n];
+        // the record holds C^{ik} of the pair (qp_point with emitC); C^{ki} = (C^{ik})^T
 #pragma unroll
         for (int m = 0; m < 3; ++m)
 #pragma unroll
-            for (int n = 0; n < 3; ++n)
-                sC[3 * m + n][t] = ix[m] * W[n] + ix[3 + m] * W[3 + n] + ix[6 + m] * W[6 + n];
+            for (int n = 0; n < 3; ++n) sC[3 * m + n][t] = rec[Q_D + 9 * sp + (tr ? 3 * n + m : 3 * m + n)];
     }
     __syncthreads();
 
@@ -313,24 +301,11 @@ __global__ __launch_bounds__(256, 4) void stiffness_sf_mfma_kernel(const double 
     const double *qe_ = qp + e * (int64_t)(Q3)*QS;
     for (int t = tid; t < Q3; t += NT) {
         const double *rec = qe_ + (int64_t)min(max(g_of_t[t], 0), Q3 - 1) * QS;
-        double ix[9], D[9];
-#pragma unroll
-        for (int i = 0; i < 9; ++i) ix[i] = rec[Q_IX + i];
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-#pragma unroll
-            for (int l = 0; l < 3; ++l) D[3 * j + l] = rec[Q_D + 9 * sp + (tr ? 3 * l + j : 3 * j + l)];
-        double W[9];
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-#pragma unroll
-            for (int n = 0; n < 3; ++n)
-                W[3 * j + n] = D[3 * j] * ix[n] + D[3 * j + 1] * ix[3 + n] + D[3 * j + 2] * ix[6 + n];
+        // the record holds C^{ik} of the pair (qp_point with emitC); C^{ki} = (C^{ik})^T
 #pragma unroll
         for (int m = 0; m < 3; ++m)
 #pragma unroll
-            for (int n = 0; n < 3; ++n)
-                sC[3 * m + n][t] = ix[m] * W[n] + ix[3 + m] * W[3 + n] + ix[6 + m] * W[6 + n];
+            for (int n = 0; n < 3; ++n) sC[3 * m + n][t] = rec[Q_D + 9 * sp + (tr ? 3 * n + m : 3 * m + n)];
     }
     __syncthreads();
 
@@ -547,24 +522,11 @@ __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(const double
     const double *qe_ = qp + e * (int64_t)(Q3)*QS;
     for (int t = tid; t < Q3; t += NT) {
         const double *rec = qe_ + (int64_t)min(max(g_of_t[t], 0), Q3 - 1) * QS;
-        double ix[9], D[9];
-#pragma unroll
-        for (int i = 0; i < 9; ++i) ix[i] = rec[Q_IX + i];
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-#pragma unroll
-            for (int l = 0; l < 3; ++l) D[3 * j + l] = rec[Q_D + 9 * sp + (tr ? 3 * l + j : 3 * j + l)];
-        double W[9];
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-#pragma unroll
-            for (int n = 0; n < 3; ++n)
-                W[3 * j + n] = D[3 * j] * ix[n] + D[3 * j + 1] * ix[3 + n] + D[3 * j + 2] * ix[6 + n];
+        // the record holds C^{ik} of the pair (qp_point with emitC); C^{ki} = (C^{ik})^T
 #pragma unroll
         for (int m = 0; m < 3; ++m)
 #pragma unroll
-            for (int n = 0; n < 3; ++n)
-                sC[3 * m + n][t] = ix[m] * W[n] + ix[3 + m] * W[3 + n] + ix[6 + m] * W[6 + n];
+            for (int n = 0; n < 3; ++n) sC[3 * m + n][t] = rec[Q_D + 9 * sp + (tr ? 3 * n + m : 3 * m + n)];
     }
     __syncthreads();
 
