@@ -54,9 +54,9 @@ struct SfLayout {
 //     B[f][m][ab][qe,qz]    = sum_ae T1^{[m=1]}[ae][qe] A[f][[m=0]][ab,ae][qz]
 //     P[f][m][qb,qe,qz]     = sum_ab T2^{[m=2]}[ab][qb] B[f][m][ab][qe,qz]
 //
-// (f = X_0..2, x_0..2), then each point runs qp_point.  Same records as qp_kernel, to
-// rounding (the node sum is taken in another order).  LDS: X/x staging shares its
-// space with B, A with P.
+// (f = X_0..2, x_0..2; X then x, one pass each), then each point runs qp_point.  Same
+// records as qp_kernel, to rounding (the node sum is taken in another order).  LDS:
+// the staging shares its space with B, A with P.
 template <int P1, int Q, bool TANGENT>
 __global__ __launch_bounds__(256) void qp_sf_kernel(MatParams mp0, const double *__restrict__ points,
                                                     const int32_t *__restrict__ elements,
@@ -68,7 +68,10 @@ __global__ __launch_bounds__(256) void qp_sf_kernel(MatParams mp0, const double 
     typedef SfLayout<P1, Q> L;
     constexpr int Q3 = L::Q3, N3 = L::N3, PP = L::PP;
     constexpr int NT = 256;
-    constexpr int SZ_X = 6 * N3, SZ_B = 6 * 3 * P1 * Q * Q, SZ_A = 6 * 2 * PP * Q, SZ_P = 6 * 3 * Q3;
+    static_assert(Q3 <= NT, "one point per thread");
+    // two passes (X, then x) of three fields each keep the LDS small (~30 KB at p = 4)
+    constexpr int NF = 3;
+    constexpr int SZ_X = NF * N3, SZ_B = NF * 3 * P1 * Q * Q, SZ_A = NF * 2 * PP * Q, SZ_P = NF * 3 * Q3;
     __shared__ double sF[3][2][P1][Q];
     __shared__ double sXB[SZ_X > SZ_B ? SZ_X : SZ_B];
     __shared__ double sAP[SZ_A > SZ_P ? SZ_A : SZ_P];
@@ -81,68 +84,64 @@ __global__ __launch_bounds__(256) void qp_sf_kernel(MatParams mp0, const double 
     const MatParams mp = element_material(mp0, e, elem_mat, mat_table);
     for (int t = tid; t < 3 * 2 * P1 * Q; t += NT) (&sF[0][0][0][0])[t] = f1d[t];
     const int32_t *conn = elements + e * (int64_t)N3;
-    for (int t = tid; t < N3; t += NT) {
-        const int64_t nd = conn[min(max(loc[t], 0), N3 - 1)];
+    const bool valid = tid < Q3;
+    const int tq = valid ? tid : 0;
+    double PJ[2][9];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            sXB[c * N3 + t] = points[3 * nd + c];
-            sXB[(3 + c) * N3 + t] = Eulerx[3 * nd + c];
+    for (int pass = 0; pass < 2; ++pass) {
+        // the previous pass's last reads of sXB and sAP precede the barriers below
+        const double *src = pass ? Eulerx : points;
+        for (int t = tid; t < N3; t += NT) {
+            const int64_t nd = conn[min(max(loc[t], 0), N3 - 1)];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) sXB[c * N3 + t] = src[3 * nd + c];
         }
-    }
-    __syncthreads();
-    // A[f][s0][(ab,ae)][qz]
-    for (int t = tid; t < SZ_A; t += NT) {
-        const int qz = t % Q, r = t / Q;
-        const int abe = r % PP, r2 = r / PP;
-        const int s0 = r2 % 2, f = r2 / 2;
-        const double *xf = sXB + f * N3 + abe * P1;
-        double acc = 0.0;
+        __syncthreads();
+        // A[f][s0][(ab,ae)][qz]
+        for (int t = tid; t < SZ_A; t += NT) {
+            const int qz = t % Q, r = t / Q;
+            const int abe = r % PP, r2 = r / PP;
+            const int s0 = r2 % 2, f = r2 / 2;
+            const double *xf = sXB + f * N3 + abe * P1;
+            double acc = 0.0;
 #pragma unroll
-        for (int az = 0; az < P1; ++az) acc += sF[0][s0][az][qz] * xf[az];
-        sAP[t] = acc;
-    }
-    __syncthreads();
-    // B[f][m][ab][(qe,qz)]
-    for (int t = tid; t < SZ_B; t += NT) {
-        const int qez = t % (Q * Q), r = t / (Q * Q);
-        const int ab = r % P1, r2 = r / P1;
-        const int m = r2 % 3, f = r2 / 3;
-        const int qe = qez / Q, qz = qez % Q;
-        const double *a = sAP + ((f * 2 + (m == 0)) * PP + ab * P1) * Q + qz;
-        double acc = 0.0;
+            for (int az = 0; az < P1; ++az) acc += sF[0][s0][az][qz] * xf[az];
+            sAP[t] = acc;
+        }
+        __syncthreads();
+        // B[f][m][ab][(qe,qz)]
+        for (int t = tid; t < SZ_B; t += NT) {
+            const int qez = t % (Q * Q), r = t / (Q * Q);
+            const int ab = r % P1, r2 = r / P1;
+            const int m = r2 % 3, f = r2 / 3;
+            const int qe = qez / Q, qz = qez % Q;
+            const double *a = sAP + ((f * 2 + (m == 0)) * PP + ab * P1) * Q + qz;
+            double acc = 0.0;
 #pragma unroll
-        for (int ae = 0; ae < P1; ++ae) acc += sF[1][m == 1][ae][qe] * a[ae * Q];
-        sXB[t] = acc;
-    }
-    __syncthreads();
-    // P[f][m][(qb,qe,qz)]
-    for (int t = tid; t < SZ_P; t += NT) {
-        const int tq = t % Q3, r = t / Q3;
-        const int m = r % 3, f = r / 3;
-        const int qb = tq / (Q * Q), qez = tq % (Q * Q);
-        const double *bb = sXB + (f * 3 + m) * P1 * Q * Q + qez;
-        double acc = 0.0;
+            for (int ae = 0; ae < P1; ++ae) acc += sF[1][m == 1][ae][qe] * a[ae * Q];
+            sXB[t] = acc;
+        }
+        __syncthreads();
+        // P[f][m][(qb,qe,qz)]
+        for (int t = tid; t < SZ_P; t += NT) {
+            const int tqq = t % Q3, r = t / Q3;
+            const int m = r % 3, f = r / 3;
+            const int qb = tqq / (Q * Q), qez = tqq % (Q * Q);
+            const double *bb = sXB + (f * 3 + m) * P1 * Q * Q + qez;
+            double acc = 0.0;
 #pragma unroll
-        for (int ab = 0; ab < P1; ++ab) acc += sF[2][m == 2][ab][qb] * bb[ab * Q * Q];
-        sAP[t] = acc;
-    }
-    __syncthreads();
-    for (int t0 = 0; t0 < Q3; t0 += NT) {
-        const int t = t0 + tid;
-        const bool valid = t < Q3;
-        const int tq = valid ? t : 0;
-        double PX[9], Px[9];
+            for (int ab = 0; ab < P1; ++ab) acc += sF[2][m == 2][ab][qb] * bb[ab * Q * Q];
+            sAP[t] = acc;
+        }
+        __syncthreads();
 #pragma unroll
         for (int m = 0; m < 3; ++m)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                PX[3 * m + c] = sAP[(c * 3 + m) * Q3 + tq];
-                Px[3 * m + c] = sAP[((3 + c) * 3 + m) * Q3 + tq];
-            }
-        const int g = min(max(g_of_t[tq], 0), Q3 - 1);
-        qp_point<TANGENT>(mp, PX, Px, w[g], update, valid, qp + (e * Q3 + g) * (TANGENT ? QS : QS_RES), status,
-                          emitC != 0);
+            for (int c = 0; c < 3; ++c) PJ[pass][3 * m + c] = sAP[(c * 3 + m) * Q3 + tq];
     }
+    const int g = min(max(g_of_t[tq], 0), Q3 - 1);
+    qp_point<TANGENT>(mp, PJ[0], PJ[1], w[g], update, valid, qp + (e * Q3 + g) * (TANGENT ? QS : QS_RES), status,
+                      emitC != 0);
 }
 
 template <int P1, int Q>
