@@ -35,7 +35,10 @@ namespace fl {
 typedef double double4_t __attribute__((ext_vector_type(4)));
 
 // per quadrature point record
-constexpr int QS = 72;        // stride (doubles)
+// Point records are stored field-major per element: field f of point g of element e at
+// qp[(e * QS + f) * G + g] (QS_RES for residual-only records), so the threads of an
+// element, one per point, write and read each field coalesced.
+constexpr int QS = 72;        // fields per point (doubles)
 constexpr int Q_IX = 0;       // dxi/dx: ix[j*3+m] = d xi_m / d x_j
 constexpr int Q_D = 9;        // 6 pairs x 9: D^{ik}[j*3+l]
 constexpr int Q_S = 63;       // detJ*sigma: s00 s11 s22 s01 s02 s12
@@ -96,7 +99,7 @@ __global__ void transpose_jm_kernel(const double *__restrict__ Jm, int npe, int 
 // (computed once per point instead of in each of the element's block workgroups).
 template <bool TANGENT>
 __device__ __forceinline__ void qp_point(const MatParams &mp, const double (&PX)[9], const double (&Px)[9], double wg,
-                                         int update, bool valid, double *__restrict__ rec,
+                                         int update, bool valid, double *__restrict__ rec, int fs,
                                          int32_t *__restrict__ status, bool emitC = false) {
     double iX[9], ix[9];
     const double detX = invdet3x3(PX, iX);
@@ -156,7 +159,7 @@ __device__ __forceinline__ void qp_point(const MatParams &mp, const double (&PX)
     if (!valid) return;
     if (TANGENT) {
 #pragma unroll
-        for (int i = 0; i < 9; ++i) rec[Q_IX + i] = ix[i];
+        for (int i = 0; i < 9; ++i) rec[(Q_IX + i) * fs] = ix[i];
 #pragma unroll
         for (int p = 0; p < 6; ++p) {
             const int i = p < 3 ? p : (p == 5 ? 1 : 0);
@@ -185,29 +188,29 @@ __device__ __forceinline__ void qp_point(const MatParams &mp, const double (&PX)
                 for (int m = 0; m < 3; ++m)
 #pragma unroll
                     for (int n = 0; n < 3; ++n)
-                        rec[Q_D + 9 * p + 3 * m + n] = ix[m] * W[n] + ix[3 + m] * W[3 + n] + ix[6 + m] * W[6 + n];
+                        rec[(Q_D + 9 * p + 3 * m + n) * fs] = ix[m] * W[n] + ix[3 + m] * W[3 + n] + ix[6 + m] * W[6 + n];
             } else {
 #pragma unroll
-                for (int t = 0; t < 9; ++t) rec[Q_D + 9 * p + t] = D[t];
+                for (int t = 0; t < 9; ++t) rec[(Q_D + 9 * p + t) * fs] = D[t];
             }
         }
-        rec[Q_S + 0] = detJ * sig[0];
-        rec[Q_S + 1] = detJ * sig[4];
-        rec[Q_S + 2] = detJ * sig[8];
-        rec[Q_S + 3] = detJ * sig[1];
-        rec[Q_S + 4] = detJ * sig[2];
-        rec[Q_S + 5] = detJ * sig[5];
-        rec[Q_DETJ] = detJ;
+        rec[(Q_S + 0) * fs] = detJ * sig[0];
+        rec[(Q_S + 1) * fs] = detJ * sig[4];
+        rec[(Q_S + 2) * fs] = detJ * sig[8];
+        rec[(Q_S + 3) * fs] = detJ * sig[1];
+        rec[(Q_S + 4) * fs] = detJ * sig[2];
+        rec[(Q_S + 5) * fs] = detJ * sig[5];
+        rec[(Q_DETJ) * fs] = detJ;
     } else {
 #pragma unroll
-        for (int i = 0; i < 9; ++i) rec[i] = ix[i];
-        rec[QR_S + 0] = detJ * sig[0];
-        rec[QR_S + 1] = detJ * sig[4];
-        rec[QR_S + 2] = detJ * sig[8];
-        rec[QR_S + 3] = detJ * sig[1];
-        rec[QR_S + 4] = detJ * sig[2];
-        rec[QR_S + 5] = detJ * sig[5];
-        rec[QR_DETJ] = detJ;
+        for (int i = 0; i < 9; ++i) rec[(i) * fs] = ix[i];
+        rec[(QR_S + 0) * fs] = detJ * sig[0];
+        rec[(QR_S + 1) * fs] = detJ * sig[4];
+        rec[(QR_S + 2) * fs] = detJ * sig[8];
+        rec[(QR_S + 3) * fs] = detJ * sig[1];
+        rec[(QR_S + 4) * fs] = detJ * sig[2];
+        rec[(QR_S + 5) * fs] = detJ * sig[5];
+        rec[(QR_DETJ) * fs] = detJ;
     }
 }
 
@@ -258,7 +261,8 @@ __global__ __launch_bounds__(256) void qp_kernel(MatParams mp0, const double *__
         }
     }
     if (!valid) PX[0] = PX[4] = PX[8] = Px[0] = Px[4] = Px[8] = 1.0;
-    qp_point<TANGENT>(mp, PX, Px, valid ? w[g] : 1.0, update, valid, qp + t * (TANGENT ? QS : QS_RES), status);
+    qp_point<TANGENT>(mp, PX, Px, valid ? w[g] : 1.0, update, valid, qp + e * (int64_t)G * (TANGENT ? QS : QS_RES) + g,
+                      G, status);
 }
 
 // ------------------------------------------------------------------ phase 2
@@ -279,10 +283,11 @@ __global__ __launch_bounds__(256) void traction_kernel(const double *__restrict_
     const double *qe = qp + e * (int64_t)G * qs;
     for (int idx = threadIdx.x; idx < G * 9; idx += blockDim.x) {
         const int g = idx / 9, m = (idx / 3) % 3, i = idx % 3;
-        const double *rec = qe + (int64_t)g * qs;
-        const double *S = rec + soff;   // s00 s11 s22 s01 s02 s12
-        const double Si[3][3] = {{S[0], S[3], S[4]}, {S[3], S[1], S[5]}, {S[4], S[5], S[2]}};
-        P[idx] = rec[0 * 3 + m] * Si[i][0] + rec[1 * 3 + m] * Si[i][1] + rec[2 * 3 + m] * Si[i][2];
+        const double *rec = qe + g;   // field f at rec[f * G]
+        const double *S = rec + (int64_t)soff * G;   // s00 s11 s22 s01 s02 s12
+        const double Si[3][3] = {{S[0], S[3 * G], S[4 * G]}, {S[3 * G], S[1 * G], S[5 * G]},
+                                 {S[4 * G], S[5 * G], S[2 * G]}};
+        P[idx] = rec[(0 * 3 + m) * G] * Si[i][0] + rec[(1 * 3 + m) * G] * Si[i][1] + rec[(2 * 3 + m) * G] * Si[i][2];
     }
     __syncthreads();
     for (int idx = threadIdx.x; idx < 3 * npe; idx += blockDim.x) {

@@ -140,7 +140,7 @@ __global__ __launch_bounds__(64 * WGM *WGN, MINW) void stiffness_kernel(const do
                 pj[s][2] = jt[TPG * s + 2 * NP];
             }
 #pragma unroll
-            for (int m = 0; m < 9; ++m) pix[m] = qe[(int64_t)g * QS + Q_IX + m];
+            for (int m = 0; m < 9; ++m) pix[m] = qe[(int64_t)(Q_IX + m) * G + g];
         } else {
 #pragma unroll
             for (int s = 0; s < IPT; ++s) pj[s][0] = pj[s][1] = pj[s][2] = 0.0;
@@ -151,7 +151,7 @@ __global__ __launch_bounds__(64 * WGM *WGN, MINW) void stiffness_kernel(const do
         for (int s = 0; s < PDN; ++s) {
             const int t = tid + s * NTH;
             const int gq = t / 9, f = t - 9 * gq, gg = g0 + gq;
-            pd[s] = (t < QC * 9 && gg < G) ? qe[(int64_t)gg * QS + Q_D + 9 * pair + f] : 0.0;
+            pd[s] = (t < QC * 9 && gg < G) ? qe[(int64_t)(Q_D + 9 * pair + f) * G + gg] : 0.0;
         }
     };
     auto commit = [&](int buf) {

@@ -54,7 +54,8 @@ __global__ __launch_bounds__(64 * WGM *WGN) void stiffness_bf16_kernel(const dou
         for (int idx = tid; idx < QC * 18; idx += NTH) {
             const int gq = idx / 18, f = idx - 18 * gq, g = g0 + gq;
             double v = 0.0;
-            if (g < G) v = f < 9 ? qe[(int64_t)g * QS + Q_IX + f] : qe[(int64_t)g * QS + Q_D + 9 * pair + (f - 9)];
+            if (g < G)
+                v = f < 9 ? qe[(int64_t)(Q_IX + f) * G + g] : qe[(int64_t)(Q_D + 9 * pair + (f - 9)) * G + g];
             sQ[idx] = v;
         }
         __syncthreads();

@@ -140,8 +140,8 @@ __global__ __launch_bounds__(256) void qp_sf_kernel(MatParams mp0, const double 
             for (int c = 0; c < 3; ++c) PJ[pass][3 * m + c] = sAP[(c * 3 + m) * Q3 + tq];
     }
     const int g = min(max(g_of_t[tq], 0), Q3 - 1);
-    qp_point<TANGENT>(mp, PJ[0], PJ[1], w[g], update, valid, qp + (e * Q3 + g) * (TANGENT ? QS : QS_RES), status,
-                      emitC != 0);
+    qp_point<TANGENT>(mp, PJ[0], PJ[1], w[g], update, valid, qp + e * (int64_t)Q3 * (TANGENT ? QS : QS_RES) + g,
+                      Q3, status, emitC != 0);
 }
 
 template <int P1, int Q>
@@ -184,12 +184,12 @@ __global__ __launch_bounds__(256, 2) void stiffness_sf_kernel(const double *__re
     // C[mn][t] = sum_jl ix[j][m] D[j][l] ix[l][n] at tensor point t, as the point kernel stored it
     const double *qe_ = qp + e * (int64_t)(Q3)*QS;
     for (int t = tid; t < Q3; t += NT) {
-        const double *rec = qe_ + (int64_t)min(max(g_of_t[t], 0), Q3 - 1) * QS;
+        const double *rec = qe_ + min(max(g_of_t[t], 0), Q3 - 1);   // field f at rec[f * Q3]
         // the record holds C^{ik} of the pair (qp_point with emitC); C^{ki} = (C^{ik})^T
 #pragma unroll
         for (int m = 0; m < 3; ++m)
 #pragma unroll
-            for (int n = 0; n < 3; ++n) sC[3 * m + n][t] = rec[Q_D + 9 * sp + (tr ? 3 * n + m : 3 * m + n)];
+            for (int n = 0; n < 3; ++n) sC[3 * m + n][t] = rec[(Q_D + 9 * sp + (tr ? 3 * n + m : 3 * m + n)) * Q3];
     }
     __syncthreads();
 
@@ -299,12 +299,12 @@ __global__ __launch_bounds__(256, 4) void stiffness_sf_mfma_kernel(const double 
     for (int t = tid; t < 3 * 2 * P1 * Q; t += NT) (&sF[0][0][0][0])[t] = f1d[t];
     const double *qe_ = qp + e * (int64_t)(Q3)*QS;
     for (int t = tid; t < Q3; t += NT) {
-        const double *rec = qe_ + (int64_t)min(max(g_of_t[t], 0), Q3 - 1) * QS;
+        const double *rec = qe_ + min(max(g_of_t[t], 0), Q3 - 1);   // field f at rec[f * Q3]
         // the record holds C^{ik} of the pair (qp_point with emitC); C^{ki} = (C^{ik})^T
 #pragma unroll
         for (int m = 0; m < 3; ++m)
 #pragma unroll
-            for (int n = 0; n < 3; ++n) sC[3 * m + n][t] = rec[Q_D + 9 * sp + (tr ? 3 * n + m : 3 * m + n)];
+            for (int n = 0; n < 3; ++n) sC[3 * m + n][t] = rec[(Q_D + 9 * sp + (tr ? 3 * n + m : 3 * m + n)) * Q3];
     }
     __syncthreads();
 
@@ -520,12 +520,12 @@ __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(const double
         }
     const double *qe_ = qp + e * (int64_t)(Q3)*QS;
     for (int t = tid; t < Q3; t += NT) {
-        const double *rec = qe_ + (int64_t)min(max(g_of_t[t], 0), Q3 - 1) * QS;
+        const double *rec = qe_ + min(max(g_of_t[t], 0), Q3 - 1);   // field f at rec[f * Q3]
         // the record holds C^{ik} of the pair (qp_point with emitC); C^{ki} = (C^{ik})^T
 #pragma unroll
         for (int m = 0; m < 3; ++m)
 #pragma unroll
-            for (int n = 0; n < 3; ++n) sC[3 * m + n][t] = rec[Q_D + 9 * sp + (tr ? 3 * n + m : 3 * m + n)];
+            for (int n = 0; n < 3; ++n) sC[3 * m + n][t] = rec[(Q_D + 9 * sp + (tr ? 3 * n + m : 3 * m + n)) * Q3];
     }
     __syncthreads();
 
@@ -663,8 +663,8 @@ __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(const double
 // Element tractions of tensor-product hexes by the same factorisation:
 // t_(a,i) = sum_q sum_m dN_a/dxi_m (q) P[q][m][i], P = (dxi/dx)^T (detJ sigma), contracted
 // zeta, eta, beta in turn for the nine (m, i).  One workgroup per element; te is written
-// in local node order like traction_kernel's.  qs / soff: record stride and offset of
-// detJ sigma (full or residual-only point records).
+// in local node order like traction_kernel's.  qs / soff: fields per point and the first
+// field of detJ sigma (full or residual-only point records).
 template <int P1, int Q>
 __global__ __launch_bounds__(256) void traction_sf_kernel(const double *__restrict__ qp, int qs, int soff,
                                                           const int32_t *__restrict__ basis, int64_t nelem,
@@ -689,14 +689,15 @@ __global__ __launch_bounds__(256) void traction_sf_kernel(const double *__restri
     for (int t = tid; t < N3; t += NT) sLoc[t] = min(max(loc[t], 0), N3 - 1);
     const double *qe_ = qp + e * (int64_t)Q3 * qs;
     for (int t = tid; t < Q3; t += NT) {
-        const double *rec = qe_ + (int64_t)min(max(g_of_t[t], 0), Q3 - 1) * qs;
-        const double *S = rec + soff;   // s00 s11 s22 s01 s02 s12
-        const double Si[3][3] = {{S[0], S[3], S[4]}, {S[3], S[1], S[5]}, {S[4], S[5], S[2]}};
+        const double *rec = qe_ + min(max(g_of_t[t], 0), Q3 - 1);   // field f at rec[f * Q3]
+        const double *S = rec + soff * Q3;   // s00 s11 s22 s01 s02 s12
+        const double Si[3][3] = {{S[0], S[3 * Q3], S[4 * Q3]}, {S[3 * Q3], S[1 * Q3], S[5 * Q3]},
+                                 {S[4 * Q3], S[5 * Q3], S[2 * Q3]}};
 #pragma unroll
         for (int m = 0; m < 3; ++m)
 #pragma unroll
             for (int i = 0; i < 3; ++i)
-                sP[3 * m + i][t] = rec[m] * Si[i][0] + rec[3 + m] * Si[i][1] + rec[6 + m] * Si[i][2];
+                sP[3 * m + i][t] = rec[m * Q3] * Si[i][0] + rec[(3 + m) * Q3] * Si[i][1] + rec[(6 + m) * Q3] * Si[i][2];
     }
     __syncthreads();
     for (int t = tid; t < 9 * P1 * Q * Q; t += NT) {
