@@ -429,6 +429,9 @@ def main():
     ap.add_argument("--dense", action="store_true", help="dense MFMA stiffness even where sum factorisation applies")
     ap.add_argument("--gather-form", type=int, default=0, choices=[0, 1, 2, 3],
                     help="gather form bits: 1 one element at a time, 2 no single-element kernel")
+    ap.add_argument("--no-direct", action="store_true",
+                    help="store element blocks and gather them per node instead of the colour-ordered direct "
+                         "scatter from the stiffness kernel")
     ap.add_argument("--sf-form", type=int, default=0, choices=[0, 3, 5],
                     help="sum-factorised stiffness form: 3 all-VALU, 5 MFMA one type per reduction (0: the "
                          "library default, MFMA two types per reduction)")
@@ -500,8 +503,9 @@ def main():
         print("copy calibration: {} bytes read, {} bytes written".format(a.numel() * 8, a.numel() * 8),
               file=sys.stderr, flush=True)
         del a, b
+    direct = not args.no_direct
     for _ in range(args.warmup):
-        assemble_device(dm, mat, x, V, T)
+        assemble_device(dm, mat, x, V, T, direct=direct)
     torch.cuda.synchronize()
     status = dm.status.cpu().numpy().tolist()
 
@@ -511,7 +515,7 @@ def main():
     with no_gc():
         start.record()
         for _ in range(args.steps):
-            assemble_device(dm, mat, x, V, T)
+            assemble_device(dm, mat, x, V, T, direct=direct)
         stop.record()
         torch.cuda.synchronize()
     total_ms = start.elapsed_time(stop)

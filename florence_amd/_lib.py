@@ -18,6 +18,10 @@ FL_GATHER_BUCKETS = 6
 FL_TUNE_STIFF_MINW = 1
 FL_TUNE_STIFF_VARIANT_BASE = 2
 STIFF_VARIANTS = {0: 1, 1: 2, 2: 1, 3: 2, 4: 3}   # padded node-count class -> number of tile variants
+FL_E_ARG = -1
+FL_E_HIP = -2
+FL_E_SPACE = -3
+FL_E_LIMIT = -4
 FL_PREC_FP64 = 0
 FL_PREC_FP32 = 1
 FL_PREC_BF16 = 2
@@ -48,6 +52,13 @@ SIGNATURES = {
                                          _i32, _vp, _vp, _vp, _vp, _vp, _P32, _P32, _vp, _vp, _vp, _i32, _vp, _vp,
                                          _vp, _sz, _vp, _vp]),
     "fl_tensor_basis_bytes": (_sz, [_i32, _i32]),
+    "fl_pattern_colour": (ctypes.c_int, [_vp, _i64, _i32, _i64, _i32, _vp, _vp, _vp, _P32]),
+    "fl_pattern_scatter_table_bytes": (_sz, [_i64, _i32]),
+    "fl_pattern_scatter_table": (ctypes.c_int, [_vp, _i64, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "fl_df_workspace_size_cs": (_sz, [_i32, _i64, _i32]),
+    "fl_df_assemble_cs": (ctypes.c_int, [_i32, _PD, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i64, _i32, _i64,
+                                         _i32, _vp, _vp, _vp, _vp, _vp, _P32, _P32, _vp, _vp, _vp, _i32, _vp, _vp,
+                                         _i32, _P32, _vp, _vp, _vp, _sz, _vp, _vp]),
     "fl_pattern_rank_table_bytes": (_sz, [_i64, _i32]),
     "fl_pattern_rank_table": (ctypes.c_int, [_vp, _i64, _i32, _vp, _vp, _vp, _vp, _vp]),
     "fl_df_assemble_ex": (ctypes.c_int, [_i32, _PD, _vp, _vp, _vp, _vp, _vp, _i32, _i64, _i32, _i64, _i32,

@@ -86,7 +86,7 @@ class DeviceMeshData(object):
     int32 connectivity, basis tables, the node graph and the reference CSR pattern."""
 
     def __init__(self, mesh, function_space, device="cuda", build_indices=True, build_ranks=True,
-                 tensor_basis=True):
+                 tensor_basis=True, build_colours=True):
         torch = _torch()
         lib = _lib.load()
         self.device = torch.device(device)
@@ -155,6 +155,41 @@ class DeviceMeshData(object):
                                                         int(round(self.ngauss ** (1 / 3.))))):
                 raise _lib.FlorenceAMDError("tensor basis table size mismatch")
             self.tensor_basis = torch.as_tensor(tb, device=self.device)
+        # the direct scatter of factorised hexes: element colours and the column-block table
+        self.ncolour = 0
+        self.colour_offsets = None
+        self.colour_elems = None
+        self.scatter = None
+        if build_colours and self.tensor_basis is not None and self.max_degree <= 32767:
+            self._build_colour_plan(np.ascontiguousarray(el, dtype=np.int32))
+
+    def _build_colour_plan(self, el32, max_colours=64):
+        torch = _torch()
+        lib = _lib.load()
+        c = _lib.ctypes
+        colour = np.empty(self.nelem, dtype=np.int32)
+        elems = np.empty(self.nelem, dtype=np.int32)
+        offsets = (c.c_int32 * (max_colours + 1))()
+        nc = c.c_int32(0)
+        _lib.check(lib.fl_pattern_colour(el32.ctypes.data_as(c.c_void_p), self.nelem, self.npe, self.nnode,
+                                         max_colours, colour.ctypes.data_as(c.c_void_p),
+                                         elems.ctypes.data_as(c.c_void_p), offsets, c.byref(nc)),
+                   "fl_pattern_colour")
+        self.ncolour = int(nc.value)
+        self.colour_offsets = offsets
+        self.colour_of_elem = torch.as_tensor(colour, device=self.device)
+        self.colour_elems = torch.as_tensor(elems, device=self.device)
+        self.scatter = torch.empty(int(lib.fl_pattern_scatter_table_bytes(self.nelem, self.npe)) // 2,
+                                   dtype=torch.int16, device=self.device)
+        self.status.zero_()
+        _lib.check(lib.fl_pattern_scatter_table(_lib.ptr(self.elements), self.nelem, self.npe,
+                                                _lib.ptr(self.node_elem_ptr), _lib.ptr(self.node_elem),
+                                                _lib.ptr(self.node_nbr_ptr), _lib.ptr(self.node_nbrs),
+                                                _lib.ptr(self.colour_of_elem), _lib.ptr(self.tensor_basis),
+                                                _lib.ptr(self.scatter), _lib.ptr(self.status), _lib.stream_ptr()),
+                   "fl_pattern_scatter_table")
+        if int(self.status[0].item()) & _lib.FL_S_PATTERN:
+            raise _lib.FlorenceAMDError("inconsistent sparsity pattern or colouring (scatter table)")
 
     def workspace(self, residual_only=False):
         torch = _torch()
@@ -198,11 +233,13 @@ def _as_device_coords(Eulerx, device):
 
 
 def assemble_device(dm, material, Eulerx, V=None, T=None, requires_geometry_update=1, stream=None,
-                    precision="fp64", csr="fp64"):
+                    precision="fp64", csr="fp64", direct=True):
     """Raw device assembly on DeviceMeshData: returns (V, T, status) torch tensors.
     ``precision`` selects the element-block contraction: "fp64" (the reference's
     arithmetic), "fp32" (FP32 MFMA) or "bf16" (BF16 MFMA), FP32 accumulation; T and the
-    V sums stay FP64.  ``csr="fp32"`` (with fp32/bf16) stores V as float32."""
+    V sums stay FP64.  ``csr="fp32"`` (with fp32/bf16) stores V as float32.  ``direct``
+    (FP64 hexes with a colour plan): the stiffness kernel writes V itself, colour by
+    colour, instead of element blocks that a gather sums per node."""
     torch = _torch()
     lib = _lib.load()
     x = _as_device_coords(Eulerx, dm.device)
@@ -219,15 +256,17 @@ def assemble_device(dm, material, Eulerx, V=None, T=None, requires_geometry_upda
     if mid is None:
         raise NotImplementedError("no low-level assembler for material {}".format(type(material).__name__))
     prec = _lib.PRECISIONS[precision] | (_lib.FL_PREC_CSR32 if csr == "fp32" else 0)
-    _lib.check(lib.fl_df_assemble_mt(mid, prm, None, None, _lib.ptr(dm.points), _lib.ptr(dm.elements), _lib.ptr(x),
-                                     _lib.ptr(dm.Jm), _lib.ptr(dm.AllGauss), dm.ngauss, dm.nelem, dm.npe, dm.nnode,
-                                     int(requires_geometry_update), _lib.ptr(dm.node_elem_ptr),
-                                     _lib.ptr(dm.node_elem), _lib.ptr(dm.node_nbr_ptr), _lib.ptr(dm.node_nbrs),
-                                     _lib.ptr(dm.node_order), dm.bucket_offsets, dm.bucket_degree,
-                                     _lib.ptr(dm.indptr), _lib.ptr(V), _lib.ptr(T), prec, _lib.ptr(dm.ranks),
-                                     _lib.ptr(dm.tensor_basis), _lib.ptr(ws), ws.numel(), _lib.ptr(dm.status),
-                                     _lib.stream_ptr(stream)),
-               "fl_df_assemble_mt")
+    args = (mid, prm, None, None, _lib.ptr(dm.points), _lib.ptr(dm.elements), _lib.ptr(x), _lib.ptr(dm.Jm),
+            _lib.ptr(dm.AllGauss), dm.ngauss, dm.nelem, dm.npe, dm.nnode, int(requires_geometry_update),
+            _lib.ptr(dm.node_elem_ptr), _lib.ptr(dm.node_elem), _lib.ptr(dm.node_nbr_ptr), _lib.ptr(dm.node_nbrs),
+            _lib.ptr(dm.node_order), dm.bucket_offsets, dm.bucket_degree, _lib.ptr(dm.indptr), _lib.ptr(V),
+            _lib.ptr(T), prec, _lib.ptr(dm.ranks), _lib.ptr(dm.tensor_basis))
+    tail = (_lib.ptr(ws), ws.numel(), _lib.ptr(dm.status), _lib.stream_ptr(stream))
+    if direct and dm.scatter is not None:
+        _lib.check(lib.fl_df_assemble_cs(*(args + (dm.ncolour, dm.colour_offsets, _lib.ptr(dm.colour_elems),
+                                                   _lib.ptr(dm.scatter)) + tail)), "fl_df_assemble_cs")
+    else:
+        _lib.check(lib.fl_df_assemble_mt(*(args + tail)), "fl_df_assemble_mt")
     return V, T, dm.status
 
 

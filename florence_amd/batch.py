@@ -87,9 +87,11 @@ class BatchedAssembly(object):
     its element-block workspace stays small; all groups share one workspace.
     """
 
-    def __init__(self, batch, precision="fp64", tuning=None, device="cuda", max_block_entries=1.5e9):
+    def __init__(self, batch, precision="fp64", tuning=None, device="cuda", max_block_entries=1.5e9,
+                 direct=True):
         import torch
         self.precision = precision
+        self.direct = direct
         self.device = torch.device(device)
         self.batch = batch
         keys = {}
@@ -127,7 +129,7 @@ class BatchedAssembly(object):
         import torch
         batch = self.batch
         cm = _Concat([batch[i][0] for i in idx])
-        dm = DeviceMeshData(cm, fs, device=self.device)
+        dm = DeviceMeshData(cm, fs, device=self.device, build_colours=self.direct)
         elem_mat = np.repeat(np.arange(len(idx), dtype=np.int32), np.diff(cm.elem_offsets))
         table = np.array([batch[i][2].device_parameters() for i in idx], dtype=np.float64)
         mat_ids = {MATERIAL_IDS[type(batch[i][2]).__name__] for i in idx}
@@ -174,16 +176,20 @@ class BatchedAssembly(object):
                    "fl_set_tuning")
         prm, _keep = _lib.double_array(g["params"])
         ws = self.workspace
-        _lib.check(lib.fl_df_assemble_mt(g["material"], prm, _lib.ptr(g["elem_mat"]), _lib.ptr(g["table"]),
-                                         _lib.ptr(dm.points), _lib.ptr(dm.elements), _lib.ptr(g["x"]),
-                                         _lib.ptr(dm.Jm), _lib.ptr(dm.AllGauss), dm.ngauss, dm.nelem, dm.npe,
-                                         dm.nnode, 1, _lib.ptr(dm.node_elem_ptr), _lib.ptr(dm.node_elem),
-                                         _lib.ptr(dm.node_nbr_ptr), _lib.ptr(dm.node_nbrs), _lib.ptr(dm.node_order),
-                                         dm.bucket_offsets, dm.bucket_degree, _lib.ptr(dm.indptr),
-                                         _lib.ptr(g["V"]), _lib.ptr(g["T"]), _lib.PRECISIONS[self.precision],
-                                         _lib.ptr(dm.ranks), _lib.ptr(dm.tensor_basis), _lib.ptr(ws), ws.numel(),
-                                         _lib.ptr(dm.status), _lib.stream_ptr()),
-                   "fl_df_assemble_mt")
+        args = (g["material"], prm, _lib.ptr(g["elem_mat"]), _lib.ptr(g["table"]), _lib.ptr(dm.points),
+                _lib.ptr(dm.elements), _lib.ptr(g["x"]), _lib.ptr(dm.Jm), _lib.ptr(dm.AllGauss), dm.ngauss,
+                dm.nelem, dm.npe, dm.nnode, 1, _lib.ptr(dm.node_elem_ptr), _lib.ptr(dm.node_elem),
+                _lib.ptr(dm.node_nbr_ptr), _lib.ptr(dm.node_nbrs), _lib.ptr(dm.node_order), dm.bucket_offsets,
+                dm.bucket_degree, _lib.ptr(dm.indptr), _lib.ptr(g["V"]), _lib.ptr(g["T"]),
+                _lib.PRECISIONS[self.precision], _lib.ptr(dm.ranks), _lib.ptr(dm.tensor_basis))
+        tail = (_lib.ptr(ws), ws.numel(), _lib.ptr(dm.status), _lib.stream_ptr())
+        if self.direct and dm.scatter is not None:
+            # the direct scatter where it applies (FP64 factorised hexes), else the same
+            # call as fl_df_assemble_mt
+            _lib.check(lib.fl_df_assemble_cs(*(args + (dm.ncolour, dm.colour_offsets, _lib.ptr(dm.colour_elems),
+                                                       _lib.ptr(dm.scatter)) + tail)), "fl_df_assemble_cs")
+        else:
+            _lib.check(lib.fl_df_assemble_mt(*(args + tail)), "fl_df_assemble_mt")
 
     def assemble(self):
         """All groups, eagerly (one library call per group)."""

@@ -703,6 +703,36 @@ static int launch_stiffness_sf(int npe, int G, const double *qp, const void *bas
     return FL_E_ARG;
 }
 
+// The factorised stiffness straight into the CSR, one launch per colour in colour order
+// (stiffness_sf_mfma2_kernel's DIRECT form).  offsets (host): the colour ranges of elems.
+static int launch_stiffness_direct(int npe, int G, const double *qp, const void *basis, const int32_t *elements,
+                                   const int32_t *indptr, int ncolour, const int32_t *offsets,
+                                   const int32_t *elems, const int16_t *scat, double *V, hipStream_t st) {
+    const int32_t *b = static_cast<const int32_t *>(basis);
+    for (int c = 0; c < ncolour; ++c) {
+        const int64_t n = offsets[c + 1] - offsets[c];
+        if (n <= 0) continue;
+        const dim3 grid((unsigned)((n + 7) / 8 * 8 * 9));
+        const int32_t *el = elems + offsets[c];
+#define FL_SF_CASE(P, QQ)                                                                                  \
+    if (npe == P * P * P && G == QQ * QQ * QQ) {                                                           \
+        stiffness_sf_mfma2_kernel<P, QQ, true><<<grid, 256, 0, st>>>(qp, b, n, nullptr, nullptr, V, el, elements, \
+                                                                     indptr, scat);                        \
+        FL_LAUNCH_CHECK();                                                                                 \
+        continue;                                                                                          \
+    }
+        FL_SF_CASE(2, 2)
+        FL_SF_CASE(3, 3)
+        FL_SF_CASE(4, 4)
+        FL_SF_CASE(5, 5)
+        FL_SF_CASE(5, 6)
+#undef FL_SF_CASE
+        set_error("no sum-factorised kernel for %d nodes and %d points", npe, G);
+        return FL_E_ARG;
+    }
+    return 0;
+}
+
 // Tdirect: T entries of element-interior nodes written here (their gather is skipped)
 static int launch_traction_sf(int npe, int G, const double *qp, int qs, int soff, const void *basis, int64_t nelem,
                               double *te, const int32_t *elements, const int32_t *node_elem_ptr, double *Tdirect,
@@ -845,17 +875,19 @@ struct DFWork {
 // bytes per element-block entry for each tangent precision
 static size_t ke_bytes(int precision) { return precision == FL_PREC_FP64 ? sizeof(double) : sizeof(float); }
 
+// blocks = false: no element-block buffer and tables (residual-only pass, direct scatter)
 static DFWork carve_df(void *ws, size_t bytes, int G, int64_t nelem, int npe, bool tangent, size_t *need,
-                       int precision = FL_PREC_FP64) {
+                       int precision = FL_PREC_FP64, bool blocks = true) {
     const int NP = padded_np(npe);
     Carve cv(ws, bytes);
     DFWork w;
     w.JmT = cv.take<double>((size_t)G * 3 * NP);
     w.qp = cv.take<double>((size_t)nelem * G * (tangent ? QS : QS_RES));
     w.te = cv.take<double>((size_t)nelem * 3 * npe);
-    w.Ke = tangent ? (void *)cv.take<char>((size_t)nelem * 9 * npe * npe * ke_bytes(precision)) : nullptr;
-    w.slot = tangent ? cv.take<int32_t>((size_t)nelem * npe) : nullptr;
-    w.irow = tangent ? cv.take<int32_t>((size_t)nelem * npe * 4) : nullptr;
+    blocks = blocks && tangent;
+    w.Ke = blocks ? (void *)cv.take<char>((size_t)nelem * 9 * npe * npe * ke_bytes(precision)) : nullptr;
+    w.slot = blocks ? cv.take<int32_t>((size_t)nelem * npe) : nullptr;
+    w.irow = blocks ? cv.take<int32_t>((size_t)nelem * npe * 4) : nullptr;
     *need = cv.off + 256;
     if (!cv.ok) w.JmT = nullptr;
     return w;
@@ -947,14 +979,26 @@ int fl_df_assemble_ex(int32_t material, const double *params, const double *poin
                              workspace_bytes, status, stream);
 }
 
-int fl_df_assemble_mt(int32_t material, const double *params, const int32_t *elem_mat, const double *mat_table,
-                      const double *points, const int32_t *elements, const double *Eulerx, const double *Jm,
-                      const double *AllGauss, int32_t G, int64_t nelem, int32_t npe, int64_t nnode, int32_t update,
-                      const int32_t *node_elem_ptr, const int32_t *node_elem, const int32_t *node_nbr_ptr,
-                      const int32_t *node_nbrs, const int32_t *node_order, const int32_t *bucket_offsets,
-                      const int32_t *bucket_degree, const int32_t *indptr, double *V, double *T, int32_t precision,
-                      const int16_t *ranks, const void *tensor_basis, void *workspace, size_t workspace_bytes,
-                      int32_t *status, void *stream) {
+}  // extern "C"
+
+namespace fl {
+// the colouring and scatter table of the direct form (offsets on the host)
+struct ColourPlan {
+    int32_t ncolour;
+    const int32_t *offsets;
+    const int32_t *elems;
+    const int16_t *scat;
+};
+}  // namespace fl
+
+static int df_assemble(int32_t material, const double *params, const int32_t *elem_mat, const double *mat_table,
+                       const double *points, const int32_t *elements, const double *Eulerx, const double *Jm,
+                       const double *AllGauss, int32_t G, int64_t nelem, int32_t npe, int64_t nnode, int32_t update,
+                       const int32_t *node_elem_ptr, const int32_t *node_elem, const int32_t *node_nbr_ptr,
+                       const int32_t *node_nbrs, const int32_t *node_order, const int32_t *bucket_offsets,
+                       const int32_t *bucket_degree, const int32_t *indptr, double *V, double *T, int32_t precision,
+                       const int16_t *ranks, const void *tensor_basis, const ColourPlan *cp, void *workspace,
+                       size_t workspace_bytes, int32_t *status, void *stream) {
     hipStream_t st = as_stream(stream);
     if (!check_common(material, params, G, nelem, npe, nnode)) return FL_E_ARG;
     if ((elem_mat == nullptr) != (mat_table == nullptr)) {
@@ -977,13 +1021,20 @@ int fl_df_assemble_mt(int32_t material, const double *params, const int32_t *ele
         set_error("fl_df_assemble: gather plan does not cover the %lld nodes", (long long)nnode);
         return FL_E_ARG;
     }
+    // the factorised point, traction and (FP64) stiffness kernels whenever the tensor
+    // basis is given, so T is the same for every tangent precision
+    const bool sf_ok = tensor_basis != nullptr && sf_shape(npe, G) && tuning_value(TUNE_STIFF_SF, 0) != 1;
+    const bool use_sf = precision == FL_PREC_FP64 && sf_ok;
+    // the direct scatter takes the default factorised form only
+    const bool direct = cp != nullptr && use_sf && tuning_value(TUNE_STIFF_SF, 0) != 3 &&
+                        tuning_value(TUNE_STIFF_SF, 0) != 5;
     size_t need = 0;
-    DFWork w = carve_df(workspace, workspace_bytes, G, nelem, npe, true, &need, precision);
+    DFWork w = carve_df(workspace, workspace_bytes, G, nelem, npe, true, &need, precision, !direct);
     if (!w.JmT) {
         set_error("fl_df_assemble: workspace too small (%zu < %zu)", workspace_bytes, need);
         return FL_E_SPACE;
     }
-    for (int b = 0; b < FL_GATHER_BUCKETS; ++b) {
+    for (int b = 0; b < FL_GATHER_BUCKETS && !direct; ++b) {
         if (bucket_offsets[b + 1] > bucket_offsets[b] &&
             gather_lds_bytes(gather_form(b, bucket_degree[b], npe), bucket_degree[b], npe) > 160 * 1024) {
             set_error("fl_df_assemble: node degree %d needs more LDS than a CU has", bucket_degree[b]);
@@ -997,10 +1048,6 @@ int fl_df_assemble_mt(int32_t material, const double *params, const int32_t *ele
     clear_status_kernel<<<1, 64, 0, st>>>(status);
     transpose_jm_kernel<<<(G * 3 * NP + 255) / 256, 256, 0, st>>>(Jm, npe, G, NP, w.JmT);
     FL_LAUNCH_CHECK();
-    // the factorised point, traction and (FP64) stiffness kernels whenever the tensor
-    // basis is given, so T is the same for every tangent precision
-    const bool sf_ok = tensor_basis != nullptr && sf_shape(npe, G) && tuning_value(TUNE_STIFF_SF, 0) != 1;
-    const bool use_sf = precision == FL_PREC_FP64 && sf_ok;
     const int64_t nq = nelem * G;
     if (sf_ok) {
         // the factorised FP64 stiffness reads C^{ik} from the records; the dense FP32/BF16
@@ -1017,8 +1064,8 @@ int fl_df_assemble_mt(int32_t material, const double *params, const int32_t *ele
     prof_mark(st);
     // the factorised blocks are stored in tensor node order: tensor_of_l of the basis table,
     // and each node-element entry's stored row read next to the entry itself
-    const int32_t *kperm = use_sf ? static_cast<const int32_t *>(tensor_basis) + 4 + G + npe : nullptr;
-    const int32_t *kinv = use_sf ? static_cast<const int32_t *>(tensor_basis) + 4 + G : nullptr;   // local_of_t
+    const int32_t *kperm = use_sf && !direct ? static_cast<const int32_t *>(tensor_basis) + 4 + G + npe : nullptr;
+    const int32_t *kinv = use_sf && !direct ? static_cast<const int32_t *>(tensor_basis) + 4 + G : nullptr;   // local_of_t
     const int32_t *eslot = nullptr;
     if (kperm) {
         const int64_t ne = nelem * npe;
@@ -1029,10 +1076,12 @@ int fl_df_assemble_mt(int32_t material, const double *params, const int32_t *ele
     // element-interior rows straight from the default factorised kernel into V (and their
     // T entries from the traction kernel) when the lowest gather class holds exactly
     // those nodes (its degree equals npe); not with update != 1, whose T is zero
-    const bool fuse_interior = use_sf && update == 1 && tuning_value(TUNE_STIFF_SF, 0) != 3 &&
+    const bool fuse_interior = use_sf && !direct && update == 1 && tuning_value(TUNE_STIFF_SF, 0) != 3 &&
                                tuning_value(TUNE_STIFF_SF, 0) != 5 && bucket_offsets[1] > bucket_offsets[0] &&
                                bucket_degree[0] == npe;
-    int rc = use_sf ? launch_stiffness_sf(npe, G, w.qp, tensor_basis, nelem, (double *)w.Ke, elements, node_elem_ptr,
+    int rc = direct ? launch_stiffness_direct(npe, G, w.qp, tensor_basis, elements, indptr, cp->ncolour, cp->offsets,
+                                              cp->elems, cp->scat, V, st)
+           : use_sf ? launch_stiffness_sf(npe, G, w.qp, tensor_basis, nelem, (double *)w.Ke, elements, node_elem_ptr,
                                           indptr, w.irow, fuse_interior ? V : nullptr, st)
            : precision == FL_PREC_FP64
                  ? launch_stiffness_t<double, double>(NP, w.JmT, w.qp, npe, G, nelem, (double *)w.Ke, st)
@@ -1059,6 +1108,14 @@ int fl_df_assemble_mt(int32_t material, const double *params, const int32_t *ele
         FL_LAUNCH_CHECK();
     }
     prof_mark(st);
+    if (direct) {
+        // K is complete; T by node from the element vectors
+        gather_traction_kernel<<<(unsigned)((3 * nnode + 255) / 256), 256, 0, st>>>(npe, node_elem_ptr, node_elem,
+                                                                                     w.te, nnode, T);
+        FL_LAUNCH_CHECK();
+        prof_mark(st);
+        return 0;
+    }
     for (int b = fuse_interior ? 1 : 0; b < FL_GATHER_BUCKETS; ++b) {
         const int32_t nb = bucket_offsets[b + 1] - bucket_offsets[b];
         if (nb <= 0) continue;
@@ -1081,6 +1138,58 @@ int fl_df_assemble_mt(int32_t material, const double *params, const int32_t *ele
     }
     prof_mark(st);
     return 0;
+}
+
+extern "C" {
+
+int fl_df_assemble_mt(int32_t material, const double *params, const int32_t *elem_mat, const double *mat_table,
+                      const double *points, const int32_t *elements, const double *Eulerx, const double *Jm,
+                      const double *AllGauss, int32_t G, int64_t nelem, int32_t npe, int64_t nnode, int32_t update,
+                      const int32_t *node_elem_ptr, const int32_t *node_elem, const int32_t *node_nbr_ptr,
+                      const int32_t *node_nbrs, const int32_t *node_order, const int32_t *bucket_offsets,
+                      const int32_t *bucket_degree, const int32_t *indptr, double *V, double *T, int32_t precision,
+                      const int16_t *ranks, const void *tensor_basis, void *workspace, size_t workspace_bytes,
+                      int32_t *status, void *stream) {
+    return df_assemble(material, params, elem_mat, mat_table, points, elements, Eulerx, Jm, AllGauss, G, nelem, npe,
+                       nnode, update, node_elem_ptr, node_elem, node_nbr_ptr, node_nbrs, node_order, bucket_offsets,
+                       bucket_degree, indptr, V, T, precision, ranks, tensor_basis, nullptr, workspace,
+                       workspace_bytes, status, stream);
+}
+
+size_t fl_df_workspace_size_cs(int32_t ngauss, int64_t nelem, int32_t npe) {
+    if (padded_np(npe) < 0) return 0;
+    size_t need = 0;
+    carve_df(nullptr, 0, ngauss, nelem, npe, true, &need, FL_PREC_FP64, false);
+    return need;
+}
+
+int fl_df_assemble_cs(int32_t material, const double *params, const int32_t *elem_mat, const double *mat_table,
+                      const double *points, const int32_t *elements, const double *Eulerx, const double *Jm,
+                      const double *AllGauss, int32_t G, int64_t nelem, int32_t npe, int64_t nnode, int32_t update,
+                      const int32_t *node_elem_ptr, const int32_t *node_elem, const int32_t *node_nbr_ptr,
+                      const int32_t *node_nbrs, const int32_t *node_order, const int32_t *bucket_offsets,
+                      const int32_t *bucket_degree, const int32_t *indptr, double *V, double *T, int32_t precision,
+                      const int16_t *ranks, const void *tensor_basis, int32_t ncolour, const int32_t *colour_offsets,
+                      const int32_t *colour_elems, const int16_t *scatter, void *workspace, size_t workspace_bytes,
+                      int32_t *status, void *stream) {
+    if (ncolour < 1 || ncolour > 64 || !colour_offsets || !colour_elems || !scatter) {
+        set_error("fl_df_assemble_cs: invalid colour plan (ncolour %d)", ncolour);
+        return FL_E_ARG;
+    }
+    if (colour_offsets[0] != 0 || colour_offsets[ncolour] != nelem) {
+        set_error("fl_df_assemble_cs: colour offsets do not cover the %lld elements", (long long)nelem);
+        return FL_E_ARG;
+    }
+    for (int c = 0; c < ncolour; ++c)
+        if (colour_offsets[c + 1] < colour_offsets[c]) {
+            set_error("fl_df_assemble_cs: colour offsets decrease at colour %d", c);
+            return FL_E_ARG;
+        }
+    const ColourPlan cp = {ncolour, colour_offsets, colour_elems, scatter};
+    return df_assemble(material, params, elem_mat, mat_table, points, elements, Eulerx, Jm, AllGauss, G, nelem, npe,
+                       nnode, update, node_elem_ptr, node_elem, node_nbr_ptr, node_nbrs, node_order, bucket_offsets,
+                       bucket_degree, indptr, V, T, precision, ranks, tensor_basis, &cp, workspace, workspace_bytes,
+                       status, stream);
 }
 
 static int df_residual(int32_t material, const double *params, const double *points, const int32_t *elements,

@@ -53,6 +53,22 @@ struct Carve {
 enum TuneKey { TUNE_STIFF_MINW = 1, TUNE_STIFF_VARIANT_BASE = 2, TUNE_STIFF_SF = 7, TUNE_GATHER = 8, TUNE_KEYS = 9 };
 int tuning_value(int key, int dflt);
 
+// Order of the direct scatter table of hexes of p1^3 nodes: the order in which the
+// factorised stiffness kernel stores an element's entries.  Per element and slice az,
+// per column tile s of 16 columns, per lane of the wave that owns the tile, the lane's
+// sf_row_tiles x 4 rows.  Row 16 tm + lane / 16 + 4 r is (ab, bb) = divmod(row, p1),
+// column 16 s + lane % 16 is ((ae p1 + be) p1 + bz); the entry couples row node
+// (ab p1 + ae) p1 + az with column node (bb p1 + be) p1 + bz.  Slots outside the p1^2
+// rows or p1^3 columns are unused.
+__host__ __device__ inline int sf_row_tiles(int p1) { return (p1 * p1 + 15) / 16; }
+__host__ __device__ inline int sf_col_tiles(int p1) { return (p1 * p1 * p1 + 15) / 16; }
+__host__ __device__ inline int64_t sf_scatter_per_element(int p1) {
+    return (int64_t)p1 * sf_col_tiles(p1) * 64 * sf_row_tiles(p1) * 4;
+}
+__host__ __device__ inline int64_t sf_scatter_index(int p1, int64_t e, int az, int s, int lane, int tm, int r) {
+    return ((((e * p1 + az) * sf_col_tiles(p1) + s) * 64 + lane) * sf_row_tiles(p1) + tm) * 4 + r;
+}
+
 // Exclusive prefix sum of int64 values in place (n entries, last entry = total), device.
 int exclusive_scan_i64(int64_t *data, int64_t n, int64_t *scratch, size_t scratch_count, hipStream_t st);
 size_t scan_scratch_count(int64_t n);

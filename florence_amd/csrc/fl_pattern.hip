@@ -286,6 +286,63 @@ __global__ void rank_table_kernel(const int32_t *__restrict__ elements, int64_t 
     ranks[t] = ok ? (int16_t)lo : (int16_t)0;
 }
 
+// The direct scatter's table, one entry per slot of sf_scatter_index's order for hexes
+// of p1^3 nodes: the position of the column node b in the sorted neighbour list of the
+// row node a, with the sign bit set when an element of a lower colour holds both a and b
+// (that element has already written the entry); unused slots hold 0.  Tensor nodes map
+// to the element's local nodes through local_of_t of the tensor basis.  Status bit
+// FL_S_PATTERN on a missing node, an oversized list, a basis of another p1, or two
+// elements of one colour that share a node.
+__global__ void scatter_table_kernel(const int32_t *__restrict__ elements, int64_t nelem, int p1,
+                                     const int32_t *__restrict__ node_elem_ptr, const int32_t *__restrict__ node_elem,
+                                     const int32_t *__restrict__ nbr_ptr, const int32_t *__restrict__ nbrs,
+                                     const int32_t *__restrict__ colour, const int32_t *__restrict__ basis,
+                                     int16_t *__restrict__ table, int32_t *__restrict__ status) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t per = sf_scatter_per_element(p1);
+    if (t >= nelem * per) return;
+    const int64_t e = t / per;
+    int64_t u = t - e * per;
+    const int mt = sf_row_tiles(p1), nt = sf_col_tiles(p1);
+    const int r = (int)(u % 4);
+    u /= 4;
+    const int tm = (int)(u % mt);
+    u /= mt;
+    const int lane = (int)(u % 64);
+    u /= 64;
+    const int s = (int)(u % nt);
+    const int az = (int)(u / nt);
+    const int pp = p1 * p1, npe = pp * p1;
+    const int row = 16 * tm + (lane >> 4) + 4 * r, col = 16 * s + (lane & 15);
+    if (row >= pp || col >= npe) {
+        table[t] = 0;
+        return;
+    }
+    const int ab = row / p1, bb = row % p1, ae = col / pp, be = (col / p1) % p1, bz = col % p1;
+    const int ta = (ab * p1 + ae) * p1 + az, tb = (bb * p1 + be) * p1 + bz;
+    const int32_t *loc = basis + 4 + basis[1] * basis[1] * basis[1];
+    const int la = min(max(loc[ta], 0), npe - 1), lb = min(max(loc[tb], 0), npe - 1);
+    const int32_t a = elements[e * npe + la], b = elements[e * npe + lb];
+    const int32_t p0 = nbr_ptr[a], deg = nbr_ptr[a + 1] - p0;
+    int lo = 0, hi = deg;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (b < nbrs[p0 + mid]) hi = mid; else lo = mid;
+    }
+    bool ok = basis[0] == p1 && deg > 0 && deg <= 32767 && nbrs[p0 + lo] == b;
+    const int32_t ce = colour[e];
+    bool add = false;
+    for (int32_t q = node_elem_ptr[a]; q < node_elem_ptr[a + 1]; ++q) {
+        const int32_t e2 = node_elem[q] / npe;
+        if (e2 != e && colour[e2] == ce) ok = false;
+        if (colour[e2] >= ce || add) continue;
+        for (int32_t q2 = node_elem_ptr[b]; q2 < node_elem_ptr[b + 1]; ++q2)
+            if (node_elem[q2] / npe == e2) add = true;
+    }
+    if (!ok) atomicOr(status, FL_S_PATTERN);
+    table[t] = ok ? (int16_t)(add ? lo - 32768 : lo) : (int16_t)0;
+}
+
 }  // namespace fl
 
 using namespace fl;
@@ -410,6 +467,81 @@ int fl_pattern_rank_table(const int32_t *elements, int64_t nelem, int32_t npe, c
     const int64_t n = nelem * npe * (int64_t)npe;
     rank_table_kernel<<<(unsigned)((n + 255) / 256), 256, 0, as_stream(stream)>>>(elements, nelem, npe, node_nbr_ptr,
                                                                                  node_nbrs, ranks, status);
+    FL_LAUNCH_CHECK();
+    return 0;
+}
+
+int fl_pattern_colour(const int32_t *elements, int64_t nelem, int32_t npe, int64_t nnode, int32_t max_colours,
+                      int32_t *colour_of_elem, int32_t *colour_elems, int32_t *colour_offsets, int32_t *ncolour) {
+    if (!elements || !colour_of_elem || !colour_elems || !colour_offsets || !ncolour || nelem <= 0 || npe <= 0 ||
+        nnode <= 0 || max_colours < 1 || max_colours > 64 || nelem >= INT32_MAX) {
+        set_error("fl_pattern_colour: invalid argument");
+        return FL_E_ARG;
+    }
+    // greedy first fit in element order; used[a] = colours already holding node a
+    std::vector<uint64_t> used((size_t)nnode, 0);
+    std::vector<int64_t> count(64, 0);
+    int nc = 0;
+    for (int64_t e = 0; e < nelem; ++e) {
+        const int32_t *conn = elements + e * npe;
+        uint64_t taken = 0;
+        for (int a = 0; a < npe; ++a) {
+            if (conn[a] < 0 || conn[a] >= nnode) {
+                set_error("fl_pattern_colour: element %lld has node %d outside [0, %lld)", (long long)e, conn[a],
+                          (long long)nnode);
+                return FL_E_ARG;
+            }
+            taken |= used[conn[a]];
+        }
+        int c = 0;
+        while (c < 64 && ((taken >> c) & 1)) ++c;
+        if (c >= max_colours) {
+            set_error("fl_pattern_colour: element %lld needs more than %d colours", (long long)e, max_colours);
+            return FL_E_LIMIT;
+        }
+        colour_of_elem[e] = c;
+        ++count[c];
+        nc = std::max(nc, c + 1);
+        for (int a = 0; a < npe; ++a) used[conn[a]] |= 1ull << c;
+    }
+    colour_offsets[0] = 0;
+    for (int c = 0; c < nc; ++c) colour_offsets[c + 1] = colour_offsets[c] + (int32_t)count[c];
+    std::vector<int32_t> cur(colour_offsets, colour_offsets + nc);
+    for (int64_t e = 0; e < nelem; ++e) colour_elems[cur[colour_of_elem[e]]++] = (int32_t)e;
+    *ncolour = nc;
+    return 0;
+}
+
+// p1 of a hex of npe = p1^3 nodes (0 otherwise)
+static int cube_root(int32_t npe) {
+    for (int p1 = 1; p1 * p1 * p1 <= npe; ++p1)
+        if (p1 * p1 * p1 == npe) return p1;
+    return 0;
+}
+
+size_t fl_pattern_scatter_table_bytes(int64_t nelem, int32_t npe) {
+    const int p1 = npe > 0 ? cube_root(npe) : 0;
+    return nelem > 0 && p1 > 0 ? sizeof(int16_t) * (size_t)(nelem * sf_scatter_per_element(p1)) : 0;
+}
+
+int fl_pattern_scatter_table(const int32_t *elements, int64_t nelem, int32_t npe, const int32_t *node_elem_ptr,
+                             const int32_t *node_elem, const int32_t *node_nbr_ptr, const int32_t *node_nbrs,
+                             const int32_t *colour_of_elem, const void *tensor_basis, int16_t *table, int32_t *status,
+                             void *stream) {
+    const int p1 = npe > 0 ? cube_root(npe) : 0;
+    if (!elements || !node_elem_ptr || !node_elem || !node_nbr_ptr || !node_nbrs || !colour_of_elem ||
+        !tensor_basis || !table || !status || nelem <= 0 || p1 <= 0) {
+        set_error("fl_pattern_scatter_table: invalid argument (hexes of p1^3 nodes and their tensor basis)");
+        return FL_E_ARG;
+    }
+    const int64_t n = nelem * sf_scatter_per_element(p1);
+    if ((n + 255) / 256 > (int64_t)UINT32_MAX) {
+        set_error("fl_pattern_scatter_table: too many entries for one launch");
+        return FL_E_LIMIT;
+    }
+    scatter_table_kernel<<<(unsigned)((n + 255) / 256), 256, 0, as_stream(stream)>>>(
+        elements, nelem, p1, node_elem_ptr, node_elem, node_nbr_ptr, node_nbrs, colour_of_elem,
+        static_cast<const int32_t *>(tensor_basis), table, status);
     FL_LAUNCH_CHECK();
     return 0;
 }

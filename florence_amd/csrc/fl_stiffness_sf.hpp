@@ -443,12 +443,23 @@ __global__ __launch_bounds__(256) void interior_rows_kernel(const int32_t *__res
 // kernel.  With V given, rows of element-interior nodes go straight into the CSR
 // instead of the block buffer (the gather then skips those nodes).  LDS: T2 of the
 // phase's two types and the tables (~51 KB at p = 4, three workgroups per CU).
-template <int P1, int Q>
-__global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(const double *__restrict__ qp,
-                                                                    const int32_t *__restrict__ basis,
-                                                                    int64_t nelem, double *__restrict__ Ke,
-                                                                    const int32_t *__restrict__ irow,
-                                                                    double *__restrict__ V) {
+//
+// DIRECT: every entry goes straight into the CSR and no block is stored.  The launch
+// covers the elements elist[0, nelem) of one colour (no two of them share a node, so
+// no two workgroups of the launch write one entry); scat (sf_scatter_index's order:
+// the order this kernel stores in) holds each entry's column block in its row, with
+// the sign bit set where an element of an earlier colour holds both nodes: that entry
+// is added to, the others are stored.  Launched colour by colour on one stream, each
+// CSR entry is the sum of its elements' contributions in colour order.  A slice's
+// entries are fetched with one vector load per column tile when the slice starts, and
+// the loads of the entries added to precede the slice's stores: stores count in the
+// same memory counter as loads, so a load issued after a store waits for it.
+template <int P1, int Q, bool DIRECT = false>
+__global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(
+    const double *__restrict__ qp, const int32_t *__restrict__ basis, int64_t nelem, double *__restrict__ Ke,
+    const int32_t *__restrict__ irow, double *__restrict__ V, const int32_t *__restrict__ elist = nullptr,
+    const int32_t *__restrict__ elements = nullptr, const int32_t *__restrict__ indptr = nullptr,
+    const int16_t *__restrict__ scat = nullptr) {
     typedef SfLayout<P1, Q> L;
     typedef Mfma16<double> MF;
     constexpr int Q3 = L::Q3, N3 = L::N3, PP = L::PP;
@@ -474,8 +485,9 @@ __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(const double
     const int xcd = (int)(wid & 7);
     const int64_t slot = wid >> 3;
     const int blk = (int)(slot % 9);
-    const int64_t e = (slot / 9) * 8 + xcd;
-    if (e >= nelem) return;
+    const int64_t idx = (slot / 9) * 8 + xcd;
+    if (idx >= nelem) return;
+    const int64_t e = DIRECT ? (int64_t)elist[idx] : idx;
     const int pi = blk / 3, pk = blk % 3;
     const int sp = pi == pk ? pi : (pi + pk == 1 ? 3 : (pi + pk == 2 ? 4 : 5));
     const bool tr = pi > pk;
@@ -485,7 +497,12 @@ __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(const double
     const int32_t *g_of_t = basis + 4;
     const double *f1d = reinterpret_cast<const double *>(basis + L::INTS);
 
-    if (V) {
+    if (DIRECT) {
+        // CSR offset of row 3a+pi of every tensor node t
+        const int32_t *loc = basis + 4 + Q3;
+        const int32_t *conn = elements + e * N3;
+        for (int t = tid; t < N3; t += NT) sRow[t] = indptr[3 * (int64_t)conn[min(max(loc[t], 0), N3 - 1)] + pi];
+    } else if (V) {
         // interior_rows_kernel's table: per tensor node three row offsets and the rank
         const int32_t *ir = irow + e * (int64_t)N3 * 4;
         for (int t = tid; t < N3; t += NT) {
@@ -529,8 +546,18 @@ __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(const double
     }
     __syncthreads();
 
-    double *Kd = Ke + (e * 9 + pi * 3 + pk) * (int64_t)N3 * N3;
+    double *Kd = DIRECT ? nullptr : Ke + (e * 9 + pi * 3 + pk) * (int64_t)N3 * N3;
+    typedef short TV __attribute__((ext_vector_type(MT3 * 4)));
     for (int az = 0; az < P1; ++az) {
+        // this slice's scatter entries: lane's rows (tm, r) of column tile wave + 4 v
+        TV tv[TNW];
+        if (DIRECT) {
+#pragma unroll
+            for (int v = 0; v < TNW; ++v)
+                tv[v] = wave + 4 * v < NT3
+                            ? *reinterpret_cast<const TV *>(scat + sf_scatter_index(P1, e, az, wave + 4 * v, lane, 0, 0))
+                            : TV{};
+        }
         // zeta (VALU): task (mn, bz, qe) forms its factor once and sweeps qb
         for (int t = tid; t < 9 * P1 * Q; t += NT) {
             const int mn = t / (P1 * Q), bz = (t / Q) % P1, qe = t % Q;
@@ -626,6 +653,50 @@ __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(const double
             // the next phase's eta rewrites sT2; the next slice's zeta only rewrites sT1,
             // whose last reads precede the barrier after this phase's eta
             if (h == 0) __syncthreads();
+        }
+        if (DIRECT) {
+            // CSR positions and, where an earlier colour has written, the values to add
+            // to: every load before the first store
+            int32_t off[TNW][MT3][4];
+            double prev[TNW][MT3][4];
+#pragma unroll
+            for (int v = 0; v < TNW; ++v) {
+                const int bcol = (wave + 4 * v) * 16 + li;
+                const int ae_ = min(bcol, N3 - 1) / PP;
+#pragma unroll
+                for (int tm = 0; tm < MT3; ++tm)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int row = min(tm * 16 + MF::row(lk, r), PP - 1);
+                        const int ta = ((row / P1) * P1 + ae_) * P1 + az;
+                        const int sv = tv[v][tm * 4 + r];
+                        off[v][tm][r] = sRow[ta] + 3 * (sv & 0x7fff) + pk;
+                        const bool live = wave + 4 * v < NT3 && bcol < N3 && tm * 16 + MF::row(lk, r) < PP;
+                        prev[v][tm][r] = live && sv < 0 ? V[off[v][tm][r]] : 0.0;
+                    }
+            }
+            // vmcnt(0) on every path (expcnt and lgkmcnt left open): no load of this phase
+            // is still counted when the next slice starts, so the waits the compiler places
+            // there for its registers do not also wait for these stores
+            __builtin_amdgcn_s_waitcnt(0x0F70);
+            // the first contribution is stored as 0.0 + value, the sum the gather starts
+            // from; later colours add to it
+#pragma unroll
+            for (int v = 0; v < TNW; ++v) {
+                const int bcol = (wave + 4 * v) * 16 + li;
+                if (wave + 4 * v < NT3 && bcol < N3) {
+#pragma unroll
+                    for (int tm = 0; tm < MT3; ++tm)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int rlo = tm * 16 + 4 * r;
+                            const int row = tm * 16 + MF::row(lk, r);
+                            if (rlo < PP && (rlo + 3 < PP || row < PP))
+                                V[off[v][tm][r]] = prev[v][tm][r] + acc[tm][v][r];
+                        }
+                }
+            }
+            continue;
         }
 #pragma unroll
         for (int v = 0; v < TNW; ++v) {

@@ -90,6 +90,36 @@ int fl_pattern_rank_table(const int32_t *elements_dev, int64_t nelem, int32_t np
                           const int32_t *node_nbr_ptr_dev, const int32_t *node_nbrs_dev, int16_t *ranks_dev,
                           int32_t *status_dev, void *stream);
 
+/* Element colouring of the direct scatter (no reference counterpart: the reference sums
+ * duplicate COO entries on the host).  Greedy first fit in element order: element e takes
+ * the smallest colour no earlier element sharing a node with it holds, so no two elements
+ * of one colour share a node.  Host arrays: elements_host int32[nelem][npe];
+ * colour_of_elem_host int32[nelem]; colour_elems_host int32[nelem], the elements grouped
+ * by colour (ascending within a colour); colour_offsets_host int32[max_colours + 1], of
+ * which [0, ncolour] are set.  max_colours <= 64; FL_E_LIMIT if more are needed. */
+int fl_pattern_colour(const int32_t *elements_host, int64_t nelem, int32_t npe, int64_t nnode,
+                      int32_t max_colours, int32_t *colour_of_elem_host, int32_t *colour_elems_host,
+                      int32_t *colour_offsets_host, int32_t *ncolour);
+
+/* Scatter table of the direct form (no reference counterpart), for hexes of
+ * npe = p1^3 nodes with their tensor basis: for every entry the factorised stiffness
+ * kernel writes, in the order it writes them, the position of the column node b in the
+ * sorted neighbour list of the row node a, with the sign bit set when an element of a
+ * lower colour holds both a and b.  Order: entry
+ * ((((e*p1 + az)*nt + s)*64 + lane)*mt + tm)*4 + r with mt = ceil(p1^2/16),
+ * nt = ceil(p1^3/16), row = 16*tm + lane/16 + 4*r = ab*p1 + bb and
+ * col = 16*s + lane%16 = (ae*p1 + be)*p1 + bz: a and b are the element's tensor nodes
+ * (ab*p1 + ae)*p1 + az and (bb*p1 + be)*p1 + bz (local node local_of_t of the basis);
+ * slots with row >= p1^2 or col >= p1^3 hold 0.  colour_of_elem_dev: fl_pattern_colour's
+ * colours on the device.  status_dev bit FL_S_PATTERN on an inconsistent pattern,
+ * basis or colouring (the caller zeroes it first).  0 bytes for other element sizes. */
+size_t fl_pattern_scatter_table_bytes(int64_t nelem, int32_t npe);
+int fl_pattern_scatter_table(const int32_t *elements_dev, int64_t nelem, int32_t npe,
+                             const int32_t *node_elem_ptr_dev, const int32_t *node_elem_dev,
+                             const int32_t *node_nbr_ptr_dev, const int32_t *node_nbrs_dev,
+                             const int32_t *colour_of_elem_dev, const void *tensor_basis_dev,
+                             int16_t *table_dev, int32_t *status_dev, void *stream);
+
 /* ------------------------------------------------- implicit DF assembly
  * Replaces _GlobalAssemblyDF_<Material> (FiniteElements/Assembly/_Assembly_/
  * _LowLevelAssemblyDF_.h:8-176) in its CSR mode (recompute_sparsity_pattern=0):
@@ -172,6 +202,30 @@ int fl_df_assemble_mt(int32_t material, const double *params, const int32_t *ele
  *   reduction). */
 #define FL_TUNE_STIFF_SF 7
 size_t fl_tensor_basis_bytes(int32_t p1, int32_t q);
+
+/* fl_df_assemble_mt with the direct scatter (no reference counterpart; replaces the
+ * same _GlobalAssemblyDF_ call and its duplicate-summing COO -> CSR conversion,
+ * FiniteElements/Assembly/Assembly.py:207-208): with the
+ * tensor basis and FL_PREC_FP64, the factorised stiffness kernel writes every CSR entry
+ * itself, one launch per colour, and no element block is stored or gathered.  Each entry
+ * is the sum of its elements' contributions in colour order (bitwise reproducible; equal
+ * to fl_df_assemble_mt's V to rounding).  ncolour, colour_offsets (host),
+ * colour_elems_dev: fl_pattern_colour's plan; scatter_dev: fl_pattern_scatter_table
+ * built with the same tensor basis.  workspace: fl_df_workspace_size_cs() bytes on the
+ * direct path; where it does not apply (other precisions, no basis, the other
+ * FL_TUNE_STIFF_SF forms) the call is fl_df_assemble_mt with its workspace size. */
+size_t fl_df_workspace_size_cs(int32_t ngauss, int64_t nelem, int32_t npe);
+int fl_df_assemble_cs(int32_t material, const double *params, const int32_t *elem_material_dev,
+                      const double *material_table_dev, const double *points_dev, const int32_t *elements_dev,
+                      const double *Eulerx_dev, const double *Jm_dev, const double *AllGauss_dev, int32_t ngauss,
+                      int64_t nelem, int32_t npe, int64_t nnode, int32_t requires_geometry_update,
+                      const int32_t *node_elem_ptr_dev, const int32_t *node_elem_dev,
+                      const int32_t *node_nbr_ptr_dev, const int32_t *node_nbrs_dev, const int32_t *node_order_dev,
+                      const int32_t *bucket_offsets, const int32_t *bucket_degree, const int32_t *indptr_dev,
+                      double *V_dev, double *T_dev, int32_t precision, const int16_t *ranks_dev,
+                      const void *tensor_basis_dev, int32_t ncolour, const int32_t *colour_offsets,
+                      const int32_t *colour_elems_dev, const int16_t *scatter_dev, void *workspace_dev,
+                      size_t workspace_bytes, int32_t *status_dev, void *stream);
 
 /* --------------------------------------------- residual-only (explicit) pass
  * Replaces the mechanics branch of _GlobalAssemblyExplicit_DF_DPF_

@@ -216,7 +216,7 @@ def test_gather_forms_agree(cuda, et, p, n):
             for gf in (0, 1, 2, 3):
                 _lib.check(lib.fl_set_tuning(_lib.FL_TUNE_STIFF_SF, sf), "fl_set_tuning")
                 _lib.check(lib.fl_set_tuning(_lib.FL_TUNE_GATHER, gf), "fl_set_tuning")
-                V, T, st = assemble_device(dm, mat, x)
+                V, T, st = assemble_device(dm, mat, x, direct=False)
                 torch.cuda.synchronize()
                 assert int(st[0].item()) == 0, (sf, gf)
                 out[(sf, gf)] = (V.cpu().numpy(), T.cpu().numpy())
@@ -236,6 +236,86 @@ def test_gather_forms_agree(cuda, et, p, n):
     Vr, Tr = O.assemble_csr(m.points, m.elements, x, fs.Jm, fs.AllGauss, mat, indptr_ref, indices_ref)
     assert np.abs(Va - Vr).max() <= RTOL * np.abs(Vr).max()
     assert np.abs(Ta - Tr).max() <= RTOL * np.abs(Tr).max()
+
+
+def test_direct_scatter_table(cuda):
+    """fl_pattern_scatter_table against a restatement from the oracle's CSR pattern, slot
+    by slot in the stiffness kernel's store order: the column block of every (row node,
+    column node) pair of an element, negative (offset by -32768) exactly when an element
+    of a lower colour holds both nodes, 0 in unused slots."""
+    from florence_amd.assembly import DeviceMeshData
+    for p, n in ((2, 3), (4, 2)):
+        m = cube_mesh("hex", p, n, jitter=0.05, seed=2)
+        fs = function_space(m, p)
+        dm = DeviceMeshData(m, fs)
+        assert dm.scatter is not None and dm.ncolour == 8
+        p1, npe = p + 1, dm.npe
+        pp, mt, nt = p1 * p1, (p1 * p1 + 15) // 16, (npe + 15) // 16
+        tab = dm.scatter.cpu().numpy().reshape(dm.nelem, p1, nt, 64, mt, 4)
+        ints = np.frombuffer(dm.tensor_basis.cpu().numpy().tobytes(), dtype=np.int32)
+        q3 = int(ints[1]) ** 3
+        loc = ints[4 + q3:4 + q3 + npe]
+        el = np.asarray(m.elements)
+        colour = dm.colour_of_elem.cpu().numpy()
+        indptr_ref, indices_ref = O.sparsity_pattern(el, m.nnode)
+        node_sets = [set(map(int, r)) for r in el]
+        nadd = nused = 0
+        for e in range(dm.nelem):
+            for az in range(p1):
+                for st in range(nt):
+                    for lane in range(64):
+                        for tm in range(mt):
+                            for r in range(4):
+                                row, col = 16 * tm + lane // 16 + 4 * r, 16 * st + lane % 16
+                                got = tab[e, az, st, lane, tm, r]
+                                if row >= pp or col >= npe:
+                                    assert got == 0
+                                    continue
+                                ab, bb = divmod(row, p1)
+                                ae, be, bz = col // pp, (col // p1) % p1, col % p1
+                                a = int(el[e, loc[(ab * p1 + ae) * p1 + az]])
+                                b = int(el[e, loc[(bb * p1 + be) * p1 + bz]])
+                                nbrs = indices_ref[indptr_ref[3 * a]:indptr_ref[3 * a + 1]:3] // 3
+                                rank = int(np.searchsorted(nbrs, b))
+                                assert nbrs[rank] == b
+                                add = any(colour[f] < colour[e] and a in node_sets[f] and b in node_sets[f]
+                                          for f in range(dm.nelem))
+                                nadd += add
+                                nused += 1
+                                assert got == (rank - 32768 if add else rank), (p, e, az, st, lane, tm, r)
+        # every (row node, column node) pair of every element once; each CSR entry is
+        # stored once and added to by each further element holding it
+        assert nused == dm.nelem * npe * npe
+        assert nadd == dm.nelem * npe * npe - dm.nnz // 9
+
+
+@pytest.mark.parametrize("p,n", [(4, 2), (1, 4), (3, 3)])
+def test_direct_scatter_equals_gather(cuda, p, n):
+    """The direct scatter (colour-ordered stores and adds from the stiffness kernel)
+    meets the oracle bar, repeats bitwise, and equals the gather over stored element
+    blocks to 1e-13 in V (sums of three or more contributions taken in another order)
+    and bitwise in T."""
+    import torch
+    from florence_amd.assembly import DeviceMeshData, assemble_device
+    if p == 4:
+        m, fs, mat = config2(n=n)
+    else:
+        m = cube_mesh("hex", p, n, jitter=0.05, seed=p)
+        fs = function_space(m, p)
+        mat = NeoHookean(3, mu=1.0, lamb=10.0)
+    x = deformed(m.points, 0.05, seed=41 + p)
+    dm, V, T = _check(m, fs, mat, x, "direct scatter hex p{} n={}".format(p, n))
+    assert dm.scatter is not None
+    V2, T2, _ = assemble_device(dm, mat, x)
+    torch.cuda.synchronize()
+    assert np.array_equal(V, V2.cpu().numpy()) and np.array_equal(T, T2.cpu().numpy())
+    Vg, Tg, st = assemble_device(dm, mat, x, direct=False)
+    torch.cuda.synchronize()
+    assert int(st[0].item()) == 0
+    Vg, Tg = Vg.cpu().numpy(), Tg.cpu().numpy()
+    print("direct vs gather: |dV|/|V| = {:.3e}".format(np.abs(V - Vg).max() / np.abs(Vg).max()))
+    assert np.abs(V - Vg).max() <= 1e-13 * np.abs(Vg).max()
+    assert np.array_equal(T, Tg)
 
 
 def test_orphan_node(cuda):

@@ -5,6 +5,10 @@ import ctypes
 import os
 import re
 
+import numpy as np
+import pytest
+
+from cases import cube_mesh
 from florence_amd import _lib
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -41,3 +45,65 @@ def test_host_only_queries():
     assert lib.fl_df_residual_workspace_size(11, 10, 10) > 0
     assert lib.fl_set_tuning(99, 1) != 0
     assert b"unknown key" in lib.fl_last_error()
+
+
+def _colour(el, nnode, max_colours=64):
+    lib = _lib.load()
+    el = np.ascontiguousarray(el, dtype=np.int32)
+    nelem, npe = el.shape
+    colour = np.full(nelem, -1, dtype=np.int32)
+    elems = np.full(nelem, -1, dtype=np.int32)
+    offsets = (ctypes.c_int32 * (max_colours + 1))()
+    nc = ctypes.c_int32(0)
+    rc = lib.fl_pattern_colour(el.ctypes.data_as(ctypes.c_void_p), nelem, npe, nnode, max_colours,
+                               colour.ctypes.data_as(ctypes.c_void_p), elems.ctypes.data_as(ctypes.c_void_p),
+                               offsets, ctypes.byref(nc))
+    return rc, int(nc.value), colour, elems, np.array(offsets[:nc.value + 1])
+
+
+def _first_fit(el, nnode):
+    """Greedy first-fit colouring in element order, by brute force over earlier
+    elements (no two elements sharing a node get one colour)."""
+    nodes = [set(map(int, r)) for r in el]
+    colour = []
+    for e, s in enumerate(nodes):
+        taken = {colour[f] for f in range(e) if nodes[f] & s}
+        c = 0
+        while c in taken:
+            c += 1
+        colour.append(c)
+    return np.array(colour, dtype=np.int32)
+
+
+@pytest.mark.parametrize("et,p,n", [("hex", 1, 4), ("hex", 4, 3), ("tet", 2, 2)])
+def test_colouring_is_greedy_first_fit(et, p, n):
+    """fl_pattern_colour (host only): the first-fit colours of a brute-force greedy pass,
+    no node shared inside a colour, elements grouped by colour in ascending order; a
+    structured hex grid takes 8 colours (its 2 x 2 x 2 parity classes)."""
+    m = cube_mesh(et, p, n, jitter=0.0, seed=0)
+    el = np.asarray(m.elements)
+    rc, nc, colour, elems, offsets = _colour(el, m.nnode)
+    assert rc == 0
+    assert np.array_equal(colour, _first_fit(el, m.nnode))
+    assert nc == colour.max() + 1
+    if et == "hex":
+        assert nc == 8
+    for c in range(nc):
+        grp = elems[offsets[c]:offsets[c + 1]]
+        assert np.all(colour[grp] == c) and np.all(np.diff(grp) > 0)
+        nodes = el[grp].ravel()
+        assert np.unique(nodes).size == nodes.size, c
+    assert offsets[0] == 0 and offsets[-1] == el.shape[0]
+    assert np.array_equal(np.sort(elems), np.arange(el.shape[0]))
+
+
+def test_colouring_limits():
+    m = cube_mesh("hex", 1, 3, jitter=0.0, seed=0)
+    el = np.asarray(m.elements)
+    rc, _, _, _, _ = _colour(el, m.nnode, max_colours=4)
+    assert rc == _lib.FL_E_LIMIT
+    assert b"more than 4 colours" in _lib.load().fl_last_error()
+    bad = el.copy()
+    bad[0, 0] = m.nnode
+    rc, _, _, _, _ = _colour(bad, m.nnode)
+    assert rc == _lib.FL_E_ARG
