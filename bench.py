@@ -7,8 +7,10 @@ nodes jittered +-0.03h (seed 0), Mooney-Rivlin (mu1=0.5, mu2=0.5, lamb=20), eval
 at a twisted current configuration.  Inputs are resident in HBM before timing.
 
 Printed: one JSON line with value = elements/s (whole job), the stiffness kernel's
-roofline (FP64 MFMA), and the CPU baseline (the C restatement of the reference
-assembly timed on a bounded sample of the same mesh, one core).
+roofline (FP64 MFMA, with its HBM view), and the CPU baseline (the C restatement of the
+reference assembly timed on a bounded sample of the same mesh, one core).  The default
+data path is the colour-ordered direct scatter (the stiffness kernel writes the CSR);
+--no-direct times the element-block store + per-node gather instead.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 1]
 """
@@ -61,6 +63,16 @@ SF_KERNELS = {
        "one type per reduction)",
 }
 SF_KERNELS[2] = SF_KERNELS[4] = SF_KERNELS[0]
+DIRECT_KERNEL = ("stiffness_sf_mfma2_kernel<5,6,true> (sum-factorised: zeta on the FP64 VALU, eta and beta on "
+                 "FP64 MFMA, two types per reduction; writes the CSR itself, one launch per element colour)")
+
+
+def direct_scatter_bytes(nelem, npe, G, nnz, table_bytes):
+    """Compulsory HBM bytes of the direct-scatter stiffness phase: the 54 C-tensor fields
+    of every point read once, the scatter table read once, every CSR value written once,
+    and each further contribution to an entry (9 nelem npe^2 - nnz of them) read back
+    once before it is added."""
+    return 8 * nelem * G * 54 + table_bytes + 8 * nnz + 8 * (9 * nelem * npe * npe - nnz)
 
 
 def device_copy_gbs(torch, nbytes, reps=10):
@@ -206,7 +218,7 @@ def bench_batch(args):
     # eager per-mesh reference: each mesh alone through the ordinary path, same variant
     t_check = time.perf_counter()
     worst, nbit, eager_ms = 0.0, 0, 0.0
-    variants = {g["key"]: (g["variant"], g.get("sum_factorised", True)) for g in ba.groups}
+    variants = {g["key"]: (g["variant"], g.get("sum_factorised", True), g.get("direct", True)) for g in ba.groups}
     for i, (m, key, mat) in enumerate(batch):
         g = next(gg for gg in ba.groups if i in gg["mesh_ids"])
         dm = DeviceMeshData(m, g["fs"])
@@ -215,9 +227,9 @@ def bench_batch(args):
         X = np.asarray(m.points)
         x = X + disp(X)     # the same expression as set_state, so the same coordinates bitwise
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        assemble_device(dm, mat, x, precision=args.precision)
+        assemble_device(dm, mat, x, precision=args.precision, direct=variants[key][2])
         e0.record()
-        V, T, _ = assemble_device(dm, mat, x, precision=args.precision)
+        V, T, _ = assemble_device(dm, mat, x, precision=args.precision, direct=variants[key][2])
         e1.record()
         torch.cuda.synchronize()
         eager_ms += e0.elapsed_time(e1)
@@ -240,7 +252,9 @@ def bench_batch(args):
                       "nelem": ba.nelem, "groups": len(ba.groups), "seed": args.seed},
            "per_mesh_eager_ms_sum": round(eager_ms, 3),
            "speedup_vs_per_mesh_calls": round(eager_ms / ms, 2),
-           "tuned_variants": {"{}-p{}".format(*g["key"]): ("sum-factorised" if g.get("sum_factorised") and
+           "tuned_variants": {"{}-p{}".format(*g["key"]): (("sum-factorised, direct scatter" if g.get("direct")
+                                                             and g["dm"].scatter is not None else "sum-factorised")
+                                                            if g.get("sum_factorised") and
                                                             g["dm"].tensor_basis is not None
                                                             else "dense v{}".format(g["variant"]))
                               for g in ba.groups},
@@ -527,37 +541,69 @@ def main():
     value = dm.nelem / (ms_per_step / 1e3)
 
     sf = dm.tensor_basis is not None and not args.dense
+    direct = direct and sf and dm.scatter is not None and args.sf_form not in (3, 5)
     p1, q1 = int(round(dm.npe ** (1 / 3.))), int(round(dm.ngauss ** (1 / 3.)))
     flops_exec = (sf_flops_per_element(p1, q1) if sf else stiffness_flops_per_element(dm.npe, dm.ngauss)) * dm.nelem
     achieved = flops_exec / (phase_ms[1] / 1e3) / 1e12
     dense_b = dense_b_flops_per_element(dm.npe, dm.ngauss) * dm.nelem
-    # the default factorised kernel writes element-interior rows itself (nodes of one
-    # element, the lowest gather class when its degree equals npe)
-    fused = sf and args.sf_form not in (3, 5) and dm.bucket_degree[0] == dm.npe
-    n_interior = int(((dm.node_elem_ptr[1:] - dm.node_elem_ptr[:-1]) == 1).sum().item()) if fused else 0
-    gbytes = gather_bytes(dm.nelem, dm.npe, dm.nnode, dm.nnz, n_interior)
-    gather_gbs = gbytes / (phase_ms[3] / 1e3) / 1e9
     stiff_roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                   "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
                   "traffic": measured_traffic("assembly_hex_p4_24cube_pmc.md", "stiffness"),
-                  "kernel": (SF_KERNELS[args.sf_form] if sf else
+                  "kernel": (DIRECT_KERNEL if direct else SF_KERNELS[args.sf_form] if sf else
                              "stiffness_kernel<128,4,4,2,2> (off-diagonal + triangular diagonal launches)"),
                   "flops_per_launch": flops_exec, "kernel_ms": round(float(phase_ms[1]), 4),
                   "dense_b_equivalent_tflops": round(dense_b / (phase_ms[1] / 1e3) / 1e12, 3),
                   "dense_block_equivalent_tflops": round(stiffness_flops_per_element(dm.npe, dm.ngauss) * dm.nelem
                                                          / (phase_ms[1] / 1e3) / 1e12, 3)}
-    gather_roof = {"bound": "hbm", "achieved": round(gather_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                   "frac": round(gather_gbs / HBM_PEAK_GBS, 4),
-                   "traffic": measured_traffic("assembly_hex_p4_24cube_pmc.md", "gather"),
-                   "kernel": "gather_kernel (all degree classes of one assembly)", "bytes_per_launch": gbytes,
-                   "interior_nodes_written_by_stiffness": n_interior,
-                   "kernel_ms": round(float(phase_ms[3]), 4)}
+    if direct:
+        # one launch per colour: "per launch" figures are per assembly (all colours)
+        stiff_roof["launches_per_assembly"] = dm.ncolour
+        sbytes = direct_scatter_bytes(dm.nelem, dm.npe, dm.ngauss, dm.nnz, dm.scatter.numel() * 2)
+        sgbs = sbytes / (phase_ms[1] / 1e3) / 1e9
+        stiff_roof["hbm_view"] = {"bound": "hbm", "achieved": round(sgbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": round(sgbs / HBM_PEAK_GBS, 4), "bytes_per_assembly": sbytes}
+    # the default factorised kernel writes element-interior rows itself (nodes of one
+    # element, the lowest gather class when its degree equals npe)
+    fused = sf and not direct and args.sf_form not in (3, 5) and dm.bucket_degree[0] == dm.npe
+    n_interior = int(((dm.node_elem_ptr[1:] - dm.node_elem_ptr[:-1]) == 1).sum().item()) if fused else 0
+    if direct:
+        tbytes = 8 * (dm.nelem * 3 * dm.npe + 3 * dm.nnode)
+        tgbs = tbytes / (phase_ms[3] / 1e3) / 1e9
+        gather_roof = {"bound": "hbm", "achieved": round(tgbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": round(tgbs / HBM_PEAK_GBS, 4), "bytes_per_launch": tbytes,
+                       "kernel": "gather_traction_kernel (T only: K is complete after the stiffness launches)",
+                       "kernel_ms": round(float(phase_ms[3]), 4)}
+    else:
+        gbytes = gather_bytes(dm.nelem, dm.npe, dm.nnode, dm.nnz, n_interior)
+        gather_gbs = gbytes / (phase_ms[3] / 1e3) / 1e9
+        gather_roof = {"bound": "hbm", "achieved": round(gather_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": round(gather_gbs / HBM_PEAK_GBS, 4),
+                       "traffic": measured_traffic("assembly_hex_p4_24cube_pmc.md", "gather"),
+                       "kernel": "gather_kernel (all degree classes of one assembly)", "bytes_per_launch": gbytes,
+                       "interior_nodes_written_by_stiffness": n_interior,
+                       "kernel_ms": round(float(phase_ms[3]), 4)}
+    # the other data path on the same mesh for comparison: element blocks stored, then
+    # gathered per node (fl_df_assemble_mt)
+    other_path = None
+    if direct:
+        assemble_device(dm, mat, x, V, T, direct=False)
+        torch.cuda.synchronize()
+        o0, o1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        with no_gc():
+            o0.record()
+            for _ in range(3):
+                assemble_device(dm, mat, x, V, T, direct=False)
+            o1.record()
+            torch.cuda.synchronize()
+        other_path = {"form": "element blocks stored, then gathered per node", "ms_per_step": round(
+            o0.elapsed_time(o1) / 3, 4)}
     # the practical read+write rate of this GPU: a 4 GiB device-to-device copy, same
-    # clock (HIP events), for comparison with the gather's bytes
+    # clock (HIP events), for comparison with the HBM-bound phases
     del V
     copy_gbs = device_copy_gbs(torch, 4 << 30)
     gather_roof["copy_gbs_measured"] = round(copy_gbs, 1)
-    gather_roof["frac_of_copy"] = round(gather_gbs / copy_gbs, 4)
+    if not direct:
+        gather_roof["frac_of_copy"] = round(gather_gbs / copy_gbs, 4)
     dominant_is_stiffness = phase_ms[1] >= phase_ms[3]
     print("phases ms: qp {:.3f} stiffness {:.3f} traction {:.3f} gather {:.3f}".format(*phase_ms),
           file=sys.stderr, flush=True)
@@ -613,6 +659,8 @@ def main():
         "phase_ms": {"qp": round(float(phase_ms[0]), 4), "stiffness": round(float(phase_ms[1]), 4),
                      "traction": round(float(phase_ms[2]), 4), "gather": round(float(phase_ms[3]), 4)},
         "cpu_baseline": cpu,
+        "scatter": "direct (colour-ordered, {} colours)".format(dm.ncolour) if direct else "element blocks + gather",
+        "other_data_path": other_path,
         "residual_only": residual_only,
         "status": status,
     }

@@ -143,6 +143,8 @@ class BatchedAssembly(object):
                 "x": torch.as_tensor(np.ascontiguousarray(cm.points), device=self.device).reshape(-1).clone(),
                 "V": torch.zeros(dm.nnz, dtype=torch.float64, device=self.device),
                 "T": torch.zeros(3 * dm.nnode, dtype=torch.float64, device=self.device),
+                "direct": self.direct and bool(tuning[tkey].get("direct", True)) if tuning and tkey in tuning
+                else self.direct,
                 "variant": int(tuning[tkey]["variant"]) if tuning and tkey in tuning else 0,
                 "sum_factorised": bool(tuning[tkey].get("sum_factorised", True)) if tuning and tkey in tuning
                 else True}
@@ -183,7 +185,7 @@ class BatchedAssembly(object):
                 dm.bucket_degree, _lib.ptr(dm.indptr), _lib.ptr(g["V"]), _lib.ptr(g["T"]),
                 _lib.PRECISIONS[self.precision], _lib.ptr(dm.ranks), _lib.ptr(dm.tensor_basis))
         tail = (_lib.ptr(ws), ws.numel(), _lib.ptr(dm.status), _lib.stream_ptr())
-        if self.direct and dm.scatter is not None:
+        if g.get("direct", self.direct) and dm.scatter is not None:
             # the direct scatter where it applies (FP64 factorised hexes), else the same
             # call as fl_df_assemble_mt
             _lib.check(lib.fl_df_assemble_cs(*(args + (dm.ncolour, dm.colour_offsets, _lib.ptr(dm.colour_elems),
