@@ -318,6 +318,63 @@ def test_direct_scatter_equals_gather(cuda, p, n):
     assert np.array_equal(T, Tg)
 
 
+def test_direct_scatter_contract(cuda):
+    """fl_df_assemble_cs through the C-ABI with the smaller workspace of
+    fl_df_workspace_size_cs equals the Python path bitwise; a colour plan that does not
+    cover the elements is refused before any launch; a colouring with two elements of
+    one colour sharing a node is reported by fl_pattern_scatter_table."""
+    import ctypes
+    import torch
+    from florence_amd import _lib
+    from florence_amd.assembly import DeviceMeshData, assemble_device
+    m = cube_mesh("hex", 2, 3, jitter=0.05, seed=7)
+    fs = function_space(m, 2)
+    mat = MooneyRivlin(3, mu1=0.5, mu2=0.5, lamb=20.0)
+    x = deformed(m.points, 0.05, seed=8)
+    dm = DeviceMeshData(m, fs)
+    V1, T1, _ = assemble_device(dm, mat, x)
+    V1, T1 = V1.clone(), T1.clone()
+    lib = _lib.load()
+    ws = torch.empty(int(lib.fl_df_workspace_size_cs(dm.ngauss, dm.nelem, dm.npe)), dtype=torch.uint8,
+                     device="cuda")
+    assert ws.numel() < int(lib.fl_df_workspace_size(dm.ngauss, dm.nelem, dm.npe))
+    V2, T2 = torch.empty_like(V1), torch.empty_like(T1)
+    xd = torch.as_tensor(np.ascontiguousarray(x), device="cuda")
+    prm, _keep = _lib.double_array(mat.device_parameters())
+
+    def call(offsets, ncolour):
+        return lib.fl_df_assemble_cs(mat.material_id, prm, None, None, _lib.ptr(dm.points), _lib.ptr(dm.elements),
+                                     _lib.ptr(xd), _lib.ptr(dm.Jm), _lib.ptr(dm.AllGauss), dm.ngauss, dm.nelem,
+                                     dm.npe, dm.nnode, 1, _lib.ptr(dm.node_elem_ptr), _lib.ptr(dm.node_elem),
+                                     _lib.ptr(dm.node_nbr_ptr), _lib.ptr(dm.node_nbrs), _lib.ptr(dm.node_order),
+                                     dm.bucket_offsets, dm.bucket_degree, _lib.ptr(dm.indptr), _lib.ptr(V2),
+                                     _lib.ptr(T2), _lib.FL_PREC_FP64, _lib.ptr(dm.ranks), _lib.ptr(dm.tensor_basis),
+                                     ncolour, offsets, _lib.ptr(dm.colour_elems), _lib.ptr(dm.scatter),
+                                     _lib.ptr(ws), ws.numel(), _lib.ptr(dm.status), _lib.stream_ptr())
+
+    _lib.check(call(dm.colour_offsets, dm.ncolour), "fl_df_assemble_cs")
+    torch.cuda.synchronize()
+    assert torch.equal(V1, V2) and torch.equal(T1, T2)
+    assert int(dm.status[0].item()) == 0
+    short = (ctypes.c_int32 * 65)(*list(dm.colour_offsets))
+    short[dm.ncolour] = dm.nelem - 1
+    assert call(short, dm.ncolour) == _lib.FL_E_ARG
+    assert b"colour offsets" in lib.fl_last_error()
+    assert call(dm.colour_offsets, 0) == _lib.FL_E_ARG
+    # one colour for every element: neighbours share it, so the table build reports it
+    bad = torch.zeros(dm.nelem, dtype=torch.int32, device="cuda")
+    table = torch.empty_like(dm.scatter)
+    dm.status.zero_()
+    _lib.check(lib.fl_pattern_scatter_table(_lib.ptr(dm.elements), dm.nelem, dm.npe, _lib.ptr(dm.node_elem_ptr),
+                                            _lib.ptr(dm.node_elem), _lib.ptr(dm.node_nbr_ptr),
+                                            _lib.ptr(dm.node_nbrs), _lib.ptr(bad), _lib.ptr(dm.tensor_basis),
+                                            _lib.ptr(table), _lib.ptr(dm.status), _lib.stream_ptr()),
+               "fl_pattern_scatter_table")
+    torch.cuda.synchronize()
+    assert int(dm.status[0].item()) & _lib.FL_S_PATTERN
+    dm.status.zero_()
+
+
 def test_orphan_node(cuda):
     """A point no element uses (its CSR rows are empty, its T entries zero) leaves the
     assembly of the other nodes unchanged."""

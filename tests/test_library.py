@@ -45,6 +45,12 @@ def test_host_only_queries():
     assert lib.fl_df_residual_workspace_size(11, 10, 10) > 0
     assert lib.fl_set_tuning(99, 1) != 0
     assert b"unknown key" in lib.fl_last_error()
+    # the direct scatter: no element blocks in its workspace; its table has one int16 per
+    # slot of the kernel's store order (p=4: 5 slices x 8 column tiles x 64 lanes x 8 rows)
+    assert 0 < lib.fl_df_workspace_size_cs(216, 8, 125) < lib.fl_df_workspace_size(216, 8, 125)
+    assert lib.fl_pattern_scatter_table_bytes(8, 125) == 8 * 5 * 8 * 64 * 8 * 2
+    assert lib.fl_pattern_scatter_table_bytes(8, 27) == 8 * 3 * 2 * 64 * 4 * 2
+    assert lib.fl_pattern_scatter_table_bytes(8, 10) == 0     # not a hex of p1^3 nodes
 
 
 def _colour(el, nnode, max_colours=64):
