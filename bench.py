@@ -355,7 +355,7 @@ def bench_learned(args):
 
 
 def bench_newton(args):
-    """The config-2 solve: hex p=4 unit cube n^3 (jitter 0.03h, seed 0), MR(0.5, 0.5,
+    """The twisted-cube solve: hex p=4 unit cube n^3 (jitter 0.03h, seed 0), MR(0.5, 0.5,
     20), 6^3 Gauss-Lobatto points, x=0 clamped, x=1 twisted 30 degrees, 10 increments.
 
     * For n <= 8 (``--femsolver``): FEMSolver with the reference's Newton semantics
@@ -421,7 +421,7 @@ def bench_newton(args):
         del ln
         torch.cuda.empty_cache()
     plain, fast = out_modes["fp64+previous+fixed"]["wall_s"], out_modes["fp32+extrap+adaptive+csr32"]["wall_s"]
-    out = {"metric": "Newton wall time to FP64 residual 1e-8 (config-2 workload)", "value": fast, "unit": "s",
+    out = {"metric": "Newton wall time to FP64 residual 1e-8 (twisted hex p=4 cube)", "value": fast, "unit": "s",
            "n_gpus": 1, "steps": 1, "warmup": 0, "higher_is_better": False, "vs_baseline": None, "dtype": "fp32",
            "data": "synthetic", "speedup_vs_plain_fp64": round(plain / fast, 3),
            "config": {"workload": "unit cube {0}^3 hex p=4 (jitter 0.03h), 6^3 GLL, MR(0.5,0.5,20), x=0 clamped, "

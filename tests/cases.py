@@ -14,12 +14,12 @@ def function_space(mesh, p, qtype="gauss", norder=None):
     return FunctionSpace(mesh, q, p=p)
 
 
-def config1(n=8, seed=0):
+def tet_p2_cube(n=8, seed=0):
     """Unit cube, n^3 cubes x 6 tets, p=2, degree-4 (11 point) rule, jitter +-0.05h, NH(1, 10)."""
     return cube_tet_p2(n, seed)
 
 
-def config2(n=24, seed=0):
+def hex_p4_cube(n=24, seed=0):
     """Unit cube, n^3 hexes, p=4, 6x6x6 Gauss-Lobatto rule, jitter +-0.03h, MR(0.5, 0.5, 20)."""
     return cube_hex_p4(n, seed)
 

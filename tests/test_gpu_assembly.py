@@ -6,7 +6,7 @@ Bar: CSR indptr/indices bit-exact; K data and residual T within 1e-12 relative
 import numpy as np
 import pytest
 
-from cases import config1, config2, cube_mesh, deformed, function_space
+from cases import tet_p2_cube, hex_p4_cube, cube_mesh, deformed, function_space
 from florence_amd.materials import MooneyRivlin, NeoHookean
 from oracle import df_numpy as O
 
@@ -49,26 +49,26 @@ def test_small_meshes(cuda, et, p, n, mat):
     _check(m, fs, material, deformed(m.points, 0.05, seed=p), "{}{} n={} {}".format(et, p, n, mat))
 
 
-def test_config1_full(cuda):
+def test_tet_p2_cube_full(cuda):
     """3072 p=2 tets, 11-point rule, Neo-Hookean, deformed state."""
-    m, fs, mat = config1()
+    m, fs, mat = tet_p2_cube()
     x = deformed(m.points, 0.04, seed=11)
-    dm, V, T = _check(m, fs, mat, x, "config1")
+    dm, V, T = _check(m, fs, mat, x, "tet_p2_cube")
     # determinism: second call bitwise identical
     _, V2, T2, _ = _gpu_assemble(m, fs, mat, x)
     assert np.array_equal(V, V2) and np.array_equal(T, T2)
 
 
-def test_config2_reduced(cuda):
+def test_hex_p4_cube_n2(cuda):
     """Hex p=4, 6^3 Gauss-Lobatto points, Mooney-Rivlin on a 2^3 cube (8 elements)."""
-    m, fs, mat = config2(n=2)
-    _check(m, fs, mat, deformed(m.points, 0.05, seed=5), "config2 n=2")
+    m, fs, mat = hex_p4_cube(n=2)
+    _check(m, fs, mat, deformed(m.points, 0.05, seed=5), "hex p4 cube n=2")
 
 
 def test_residual_only_matches(cuda):
     import torch
     from florence_amd.assembly import DeviceMeshData, residual_device, assemble_device
-    m, fs, mat = config2(n=2)
+    m, fs, mat = hex_p4_cube(n=2)
     x = deformed(m.points, 0.05, seed=6)
     dm = DeviceMeshData(m, fs)
     T, st = residual_device(dm, mat, x)
@@ -133,7 +133,7 @@ def test_reference_entry_point_equals_rank_table_path(cuda, et, p, n):
 @pytest.mark.parametrize("variant", [0, 1, 2])
 def test_hex_p4_tile_variants(cuda, variant):
     """Every tile-shape variant of the dense (MFMA) contraction the autotuner may pick
-    for 128-node elements meets the oracle bar (config 2 family, 2^3 cube); the
+    for 128-node elements meets the oracle bar (p=4 hex cube, 2^3 elements); the
     sum-factorised path is switched off for it."""
     from florence_amd import _lib
     lib = _lib.load()
@@ -141,7 +141,7 @@ def test_hex_p4_tile_variants(cuda, variant):
     _lib.check(lib.fl_set_tuning(key, variant + 1), "fl_set_tuning")
     _lib.check(lib.fl_set_tuning(_lib.FL_TUNE_STIFF_SF, 1), "fl_set_tuning")
     try:
-        m, fs, mat = config2(n=2)
+        m, fs, mat = hex_p4_cube(n=2)
         _check(m, fs, mat, deformed(m.points, 0.05, seed=9), "hex p4 variant {}".format(variant))
     finally:
         lib.fl_set_tuning(key, 0)
@@ -157,7 +157,7 @@ def test_sum_factorised_equals_dense(cuda, p):
     from florence_amd import _lib
     from florence_amd.assembly import DeviceMeshData, assemble_device
     if p == 4:
-        m, fs, mat = config2(n=2)
+        m, fs, mat = hex_p4_cube(n=2)
     else:
         m = cube_mesh("hex", p, 2, jitter=0.05, seed=p)
         fs = function_space(m, p)
@@ -202,7 +202,7 @@ def test_gather_forms_agree(cuda, et, p, n):
     from florence_amd import _lib
     from florence_amd.assembly import DeviceMeshData, assemble_device
     if (et, p) == ("hex", 4):
-        m, fs, mat = config2(n=n)
+        m, fs, mat = hex_p4_cube(n=n)
     else:
         m = cube_mesh(et, p, n, jitter=0.05, seed=p)
         fs = function_space(m, p)
@@ -298,7 +298,7 @@ def test_direct_scatter_equals_gather(cuda, p, n):
     import torch
     from florence_amd.assembly import DeviceMeshData, assemble_device
     if p == 4:
-        m, fs, mat = config2(n=n)
+        m, fs, mat = hex_p4_cube(n=n)
     else:
         m = cube_mesh("hex", p, n, jitter=0.05, seed=p)
         fs = function_space(m, p)

@@ -23,7 +23,7 @@ class _Formulation(object):
         self.function_spaces = (fs, None, None)
 
 
-def test_config1_newton_matches_oracle(cuda):
+def test_tet_p2_cube_newton_matches_oracle(cuda):
     ref = np.load(GOLD)
     m, fs, mat = cube_tet_p2(8)
     assert np.array_equal(np.asarray(m.points), ref["points"]), "mesh generator changed; regenerate the fixture"

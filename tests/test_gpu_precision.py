@@ -50,7 +50,7 @@ def test_fp32_tangent_error(cuda, case):
 
 
 @pytest.mark.parametrize("precision,bar", [("fp32", 1e-7), ("bf16", 1e-6)])
-def test_mixed_newton_config1(cuda, precision, bar):
+def test_mixed_newton_tet_p2_cube(cuda, precision, bar):
     from florence_amd.boundary_condition import BoundaryCondition
     from florence_amd.solver import FEMSolver, LinearSolver
     from florence_amd.workloads import clamp_and_pull

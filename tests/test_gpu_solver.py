@@ -1,4 +1,4 @@
-"""GPU SpMV and block-Jacobi PCG against scipy on the assembled config-1 tangent."""
+"""GPU SpMV and block-Jacobi PCG against scipy on the assembled tangent of the p=2 tet cube."""
 import numpy as np
 import pytest
 import scipy.sparse as sp

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 import scipy.sparse as sp
 
-from cases import config1, cube_mesh, deformed, function_space
+from cases import tet_p2_cube, cube_mesh, deformed, function_space
 from florence_amd.bases import (FeketePointsTet, GaussLobattoPointsHex, HexBases, NodeArrangementHex, TetBases)
 from florence_amd.materials import MooneyRivlin, NeoHookean
 from florence_amd.quadrature import GaussLobattoQuadrature, QuadratureRule
@@ -163,7 +163,7 @@ def test_assembly_equals_energy_derivatives(et, p, qt, q, mat):
 
 
 def test_rigid_translation_and_equilibrium():
-    m, fs, mat = config1(n=2)
+    m, fs, mat = tet_p2_cube(n=2)
     x = deformed(m.points, 0.05, seed=4)
     indptr, indices = O.sparsity_pattern(m.elements, m.nnode)
     V, T = O.assemble_csr(m.points, m.elements, x, fs.Jm, fs.AllGauss, mat, indptr, indices)
