@@ -100,7 +100,8 @@ __global__ void transpose_jm_kernel(const double *__restrict__ Jm, int npe, int 
 template <bool TANGENT>
 __device__ __forceinline__ void qp_point(const MatParams &mp, const double (&PX)[9], const double (&Px)[9], double wg,
                                          int update, bool valid, double *__restrict__ rec, int fs,
-                                         int32_t *__restrict__ status, bool emitC = false) {
+                                         int32_t *__restrict__ status, bool emitC = false,
+                                         double *__restrict__ out = nullptr) {
     double iX[9], ix[9];
     const double detX = invdet3x3(PX, iX);
     const double detx = invdet3x3(Px, ix);
@@ -157,6 +158,18 @@ __device__ __forceinline__ void qp_point(const MatParams &mp, const double (&PX)
         atomicAdd(status + 1, (int)__popcll(m_np));
     }
     if (!valid) return;
+    if (out) {
+        // dxi/dx and detJ sigma (s00 s11 s22 s01 s02 s12) for a fused traction
+#pragma unroll
+        for (int i = 0; i < 9; ++i) out[i] = ix[i];
+        out[9] = detJ * sig[0];
+        out[10] = detJ * sig[4];
+        out[11] = detJ * sig[8];
+        out[12] = detJ * sig[1];
+        out[13] = detJ * sig[2];
+        out[14] = detJ * sig[5];
+    }
+    if (!rec) return;
     if (TANGENT) {
 #pragma unroll
         for (int i = 0; i < 9; ++i) rec[(Q_IX + i) * fs] = ix[i];
@@ -194,13 +207,16 @@ __device__ __forceinline__ void qp_point(const MatParams &mp, const double (&PX)
                 for (int t = 0; t < 9; ++t) rec[(Q_D + 9 * p + t) * fs] = D[t];
             }
         }
-        rec[(Q_S + 0) * fs] = detJ * sig[0];
-        rec[(Q_S + 1) * fs] = detJ * sig[4];
-        rec[(Q_S + 2) * fs] = detJ * sig[8];
-        rec[(Q_S + 3) * fs] = detJ * sig[1];
-        rec[(Q_S + 4) * fs] = detJ * sig[2];
-        rec[(Q_S + 5) * fs] = detJ * sig[5];
-        rec[(Q_DETJ) * fs] = detJ;
+        if (!out) {
+            // read by the traction kernels (a fused traction has them already)
+            rec[(Q_S + 0) * fs] = detJ * sig[0];
+            rec[(Q_S + 1) * fs] = detJ * sig[4];
+            rec[(Q_S + 2) * fs] = detJ * sig[8];
+            rec[(Q_S + 3) * fs] = detJ * sig[1];
+            rec[(Q_S + 4) * fs] = detJ * sig[2];
+            rec[(Q_S + 5) * fs] = detJ * sig[5];
+            rec[(Q_DETJ) * fs] = detJ;
+        }
     } else {
 #pragma unroll
         for (int i = 0; i < 9; ++i) rec[(i) * fs] = ix[i];
@@ -734,38 +750,18 @@ static int launch_stiffness_direct(int npe, int G, const double *qp, const void 
 }
 
 // Tdirect: T entries of element-interior nodes written here (their gather is skipped)
-static int launch_traction_sf(int npe, int G, const double *qp, int qs, int soff, const void *basis, int64_t nelem,
-                              double *te, const int32_t *elements, const int32_t *node_elem_ptr, double *Tdirect,
-                              hipStream_t st) {
-    const int32_t *b = static_cast<const int32_t *>(basis);
-#define FL_TSF_CASE(P, QQ)                                                                                \
-    if (npe == P * P * P && G == QQ * QQ * QQ) {                                                          \
-        traction_sf_kernel<P, QQ><<<(unsigned)nelem, 256, 0, st>>>(qp, qs, soff, b, nelem, te, elements,  \
-                                                                    node_elem_ptr, Tdirect);              \
-        FL_LAUNCH_CHECK();                                                                                \
-        return 0;                                                                                         \
-    }
-    FL_TSF_CASE(2, 2)
-    FL_TSF_CASE(3, 3)
-    FL_TSF_CASE(4, 4)
-    FL_TSF_CASE(5, 5)
-    FL_TSF_CASE(5, 6)
-#undef FL_TSF_CASE
-    set_error("no sum-factorised traction kernel for %d nodes and %d points", npe, G);
-    return FL_E_ARG;
-}
-
 template <bool TANGENT>
 static int launch_qp_sf(const MatParams &mp, const double *points, const int32_t *elements, const double *Eulerx,
                         const void *basis, const double *w, int npe, int G, int64_t nelem, int update, double *qp,
                         int32_t *status, const int32_t *elem_mat, const double *mat_table, hipStream_t st,
-                        int emitC = 0) {
+                        int emitC = 0, double *te = nullptr, const int32_t *node_elem_ptr = nullptr,
+                        double *Tdirect = nullptr) {
     const int32_t *b = static_cast<const int32_t *>(basis);
 #define FL_QSF_CASE(P, QQ)                                                                                       \
     if (npe == P * P * P && G == QQ * QQ * QQ) {                                                                 \
         qp_sf_kernel<P, QQ, TANGENT><<<(unsigned)nelem, 256, 0, st>>>(mp, points, elements, Eulerx, b, w, nelem, \
                                                                        update, qp, status, elem_mat, mat_table,   \
-                                                                       emitC);                                    \
+                                                                       emitC, te, node_elem_ptr, Tdirect);        \
         FL_LAUNCH_CHECK();                                                                                       \
         return 0;                                                                                                \
     }
@@ -1049,11 +1045,18 @@ static int df_assemble(int32_t material, const double *params, const int32_t *el
     transpose_jm_kernel<<<(G * 3 * NP + 255) / 256, 256, 0, st>>>(Jm, npe, G, NP, w.JmT);
     FL_LAUNCH_CHECK();
     const int64_t nq = nelem * G;
+    // element-interior rows straight from the default factorised kernel into V (and their
+    // T entries from the fused traction) when the lowest gather class holds exactly
+    // those nodes (its degree equals npe); not with update != 1, whose T is zero
+    const bool fuse_interior = use_sf && !direct && update == 1 && tuning_value(TUNE_STIFF_SF, 0) != 3 &&
+                               tuning_value(TUNE_STIFF_SF, 0) != 5 && bucket_offsets[1] > bucket_offsets[0] &&
+                               bucket_degree[0] == npe;
     if (sf_ok) {
         // the factorised FP64 stiffness reads C^{ik} from the records; the dense FP32/BF16
-        // contractions read D^{ik}
+        // contractions read D^{ik}.  The element tractions are contracted in the same kernel.
         const int rq = launch_qp_sf<true>(mp, points, elements, Eulerx, tensor_basis, AllGauss, npe, G, nelem, update,
-                                          w.qp, status, elem_mat, mat_table, st, use_sf ? 1 : 0);
+                                          w.qp, status, elem_mat, mat_table, st, use_sf ? 1 : 0, w.te, node_elem_ptr,
+                                          fuse_interior ? T : nullptr);
         if (rq) return rq;
     } else {
         qp_kernel<true><<<(unsigned)((nq + 255) / 256), 256, 0, st>>>(mp, points, elements, Eulerx, Jm, AllGauss, G,
@@ -1073,12 +1076,6 @@ static int df_assemble(int32_t material, const double *params, const int32_t *el
         FL_LAUNCH_CHECK();
         eslot = w.slot;
     }
-    // element-interior rows straight from the default factorised kernel into V (and their
-    // T entries from the traction kernel) when the lowest gather class holds exactly
-    // those nodes (its degree equals npe); not with update != 1, whose T is zero
-    const bool fuse_interior = use_sf && !direct && update == 1 && tuning_value(TUNE_STIFF_SF, 0) != 3 &&
-                               tuning_value(TUNE_STIFF_SF, 0) != 5 && bucket_offsets[1] > bucket_offsets[0] &&
-                               bucket_degree[0] == npe;
     int rc = direct ? launch_stiffness_direct(npe, G, w.qp, tensor_basis, elements, indptr, cp->ncolour, cp->offsets,
                                               cp->elems, cp->scat, V, st)
            : use_sf ? launch_stiffness_sf(npe, G, w.qp, tensor_basis, nelem, (double *)w.Ke, elements, node_elem_ptr,
@@ -1090,11 +1087,7 @@ static int df_assemble(int32_t material, const double *params, const int32_t *el
                        : launch_stiffness_bf16(NP, w.JmT, w.qp, npe, G, nelem, (float *)w.Ke, st);
     if (rc) return rc;
     prof_mark(st);
-    if (sf_ok) {
-        rc = launch_traction_sf(npe, G, w.qp, QS, Q_S, tensor_basis, nelem, w.te, elements, node_elem_ptr,
-                                fuse_interior ? T : nullptr, st);
-        if (rc) return rc;
-    } else {
+    if (!sf_ok) {
         const size_t lds_tr = (size_t)G * 9 * 8;
         if (lds_tr > 64 * 1024)
             FL_HIP_CHECK(hipFuncSetAttribute((const void *)traction_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1215,8 +1208,9 @@ static int df_residual(int32_t material, const double *params, const double *poi
     prof_begin(st, PROF_DF_RESIDUAL);
     clear_status_kernel<<<1, 64, 0, st>>>(status);
     if (sf_ok) {
+        // no records: the tractions are contracted in the point kernel
         const int rq = launch_qp_sf<false>(mp, points, elements, Eulerx, tensor_basis, AllGauss, npe, G, nelem, 1,
-                                           w.qp, status, nullptr, nullptr, st);
+                                           nullptr, status, nullptr, nullptr, st, 0, w.te);
         if (rq) return rq;
     } else {
         transpose_jm_kernel<<<(G * 3 * NP + 255) / 256, 256, 0, st>>>(Jm, npe, G, NP, w.JmT);
@@ -1227,11 +1221,7 @@ static int df_residual(int32_t material, const double *params, const double *poi
         FL_LAUNCH_CHECK();
     }
     prof_mark(st);
-    if (sf_ok) {
-        const int rt = launch_traction_sf(npe, G, w.qp, QS_RES, QR_S, tensor_basis, nelem, w.te, nullptr, nullptr,
-                                          nullptr, st);
-        if (rt) return rt;
-    } else {
+    if (!sf_ok) {
         const size_t lds_tr = (size_t)G * 9 * 8;
         if (lds_tr > 64 * 1024)
             FL_HIP_CHECK(hipFuncSetAttribute((const void *)traction_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,

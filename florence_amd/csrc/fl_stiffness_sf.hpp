@@ -46,6 +46,74 @@ struct SfLayout {
     static constexpr int INTS = 4 + Q3 + 2 * N3 + (Q3 % 2);   // 8-byte aligned
 };
 
+// Element tractions t_(a,i) = sum_q sum_m dN_a/dxi_m (q) P[q][m][i] from the point
+// tensors P[3 m + i][t] (tensor point t) in LDS: contracted zeta, eta, beta in turn for the
+// nine (m, i), one workgroup per element.  U1 [9][P1][Q^2] and U2 [9][PP][Q] are LDS
+// work areas; U2 may share P's space (P's last reads precede the barrier before U2 is
+// written).  te in local node order; with T, a node of this element only gets its T
+// entry directly (as 0.0 + value, the accumulated form).
+template <int P1, int Q>
+__device__ __forceinline__ void traction_sf_contract(const double (&sF)[3][2][P1][Q], const double *sP,
+                                                     double *sU1, double *sU2, const int32_t *__restrict__ basis,
+                                                     int64_t e, double *__restrict__ te,
+                                                     const int32_t *__restrict__ elements,
+                                                     const int32_t *__restrict__ node_elem_ptr,
+                                                     double *__restrict__ T) {
+    typedef SfLayout<P1, Q> L;
+    constexpr int Q3 = L::Q3, N3 = L::N3, PP = L::PP;
+    constexpr int NT = 256;
+    const int tid = threadIdx.x;
+    const int32_t *loc = basis + 4 + Q3;
+    for (int t = tid; t < 9 * P1 * Q * Q; t += NT) {
+        const int mi = t / (P1 * Q * Q), r = t - mi * P1 * Q * Q;
+        const int az = r / (Q * Q), qe = (r / Q) % Q, qb = r % Q;
+        const double *f = sF[0][mi / 3 == 0][az];
+        const double *p = sP + mi * Q3 + (qb * Q + qe) * Q;
+        double s = 0.0;
+#pragma unroll
+        for (int qz = 0; qz < Q; ++qz) s += f[qz] * p[qz];
+        sU1[(mi * P1 + az) * Q * Q + qe * Q + qb] = s;
+    }
+    __syncthreads();
+    for (int t = tid; t < 9 * PP * Q; t += NT) {
+        const int mi = t / (PP * Q), r = t - mi * PP * Q;
+        const int ae = r / (P1 * Q), az = (r / Q) % P1, qb = r % Q;
+        const double *f = sF[1][mi / 3 == 1][ae];
+        double s = 0.0;
+#pragma unroll
+        for (int qe = 0; qe < Q; ++qe) s += f[qe] * sU1[(mi * P1 + az) * Q * Q + qe * Q + qb];
+        sU2[(mi * PP + ae * P1 + az) * Q + qb] = s;
+    }
+    __syncthreads();
+    for (int t = tid; t < 3 * N3; t += NT) {
+        const int tn = t / 3, i = t - 3 * tn;
+        const int ab = tn / PP, aeaz = tn - ab * PP;
+        double s = 0.0;
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            const double *f = sF[2][m == 2][ab];
+#pragma unroll
+            for (int qb = 0; qb < Q; ++qb) s += f[qb] * sU2[((3 * m + i) * PP + aeaz) * Q + qb];
+        }
+        const int lt = min(max(loc[tn], 0), N3 - 1);
+        te[e * 3 * N3 + 3 * lt + i] = s;
+        if (T) {
+            const int32_t a = elements[e * N3 + lt];
+            if (node_elem_ptr[a + 1] - node_elem_ptr[a] == 1) T[3 * (int64_t)a + i] = 0.0 + s;
+        }
+    }
+}
+
+// P[3 m + i] = sum_j (dxi_m/dx_j) (detJ sigma)_ij of one point from dxi/dx (9) and detJ
+// sigma (s00 s11 s22 s01 s02 s12)
+__device__ __forceinline__ void traction_point_tensor(const double *ix, const double *S, double (&P)[9]) {
+    const double Si[3][3] = {{S[0], S[3], S[4]}, {S[3], S[1], S[5]}, {S[4], S[5], S[2]}};
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) P[3 * m + i] = ix[m] * Si[i][0] + ix[3 + m] * Si[i][1] + ix[6 + m] * Si[i][2];
+}
+
 // Phase 1 for tensor-product hexes: one workgroup per element.  The element's X and x
 // are staged in LDS in tensor node order and the parent Jacobians at all points,
 // dX_c/dxi_m = sum_a dN_a/dxi_m X_a,c, are contracted one direction at a time:
@@ -56,7 +124,10 @@ struct SfLayout {
 //
 // (f = X_0..2, x_0..2; X then x, one pass each), then each point runs qp_point.  Same
 // records as qp_kernel, to rounding (the node sum is taken in another order).  LDS:
-// the staging shares its space with B, A with P.
+// the staging shares its space with B, A with P.  With te given, the element tractions
+// follow in the same workgroup (traction_sf_contract: the point tensors in the P area,
+// the zeta results in the B area, the eta results in the P area again), and the
+// records -- if qp is given at all -- omit detJ sigma and detJ.
 template <int P1, int Q, bool TANGENT>
 __global__ __launch_bounds__(256) void qp_sf_kernel(MatParams mp0, const double *__restrict__ points,
                                                     const int32_t *__restrict__ elements,
@@ -64,7 +135,10 @@ __global__ __launch_bounds__(256) void qp_sf_kernel(MatParams mp0, const double 
                                                     const int32_t *__restrict__ basis, const double *__restrict__ w,
                                                     int64_t nelem, int update, double *__restrict__ qp,
                                                     int32_t *__restrict__ status, const int32_t *__restrict__ elem_mat,
-                                                    const double *__restrict__ mat_table, int emitC) {
+                                                    const double *__restrict__ mat_table, int emitC,
+                                                    double *__restrict__ te = nullptr,
+                                                    const int32_t *__restrict__ node_elem_ptr = nullptr,
+                                                    double *__restrict__ T = nullptr) {
     typedef SfLayout<P1, Q> L;
     constexpr int Q3 = L::Q3, N3 = L::N3, PP = L::PP;
     constexpr int NT = 256;
@@ -140,8 +214,26 @@ __global__ __launch_bounds__(256) void qp_sf_kernel(MatParams mp0, const double 
             for (int c = 0; c < 3; ++c) PJ[pass][3 * m + c] = sAP[(c * 3 + m) * Q3 + tq];
     }
     const int g = min(max(g_of_t[tq], 0), Q3 - 1);
-    qp_point<TANGENT>(mp, PJ[0], PJ[1], w[g], update, valid, qp + e * (int64_t)Q3 * (TANGENT ? QS : QS_RES) + g,
-                      Q3, status, emitC != 0);
+    // te given: the element tractions are contracted here from the point values, and
+    // the records (qp, when given) are only what the stiffness kernels read
+    double pv[15];
+    qp_point<TANGENT>(mp, PJ[0], PJ[1], w[g], update, valid,
+                      qp ? qp + e * (int64_t)Q3 * (TANGENT ? QS : QS_RES) + g : nullptr, Q3, status, emitC != 0,
+                      te ? pv : nullptr);
+    if (te) {
+        static_assert(9 * P1 * Q * Q <= (SZ_X > SZ_B ? SZ_X : SZ_B), "U1 in the B area");
+        static_assert(9 * PP * Q <= (SZ_A > SZ_P ? SZ_A : SZ_P) && 9 * Q3 <= (SZ_A > SZ_P ? SZ_A : SZ_P),
+                      "P and U2 in the P area");
+        __syncthreads();   // the last reads of sAP (PJ) precede
+        if (valid) {
+            double P[9];
+            traction_point_tensor(pv, pv + 9, P);
+#pragma unroll
+            for (int mi = 0; mi < 9; ++mi) sAP[mi * Q3 + tq] = P[mi];
+        }
+        __syncthreads();
+        traction_sf_contract<P1, Q>(sF, sAP, sXB, sAP, basis, e, te, elements, node_elem_ptr, T);
+    }
 }
 
 template <int P1, int Q>
@@ -727,88 +819,6 @@ __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(
                         }
                     }
             }
-        }
-    }
-}
-
-// Element tractions of tensor-product hexes by the same factorisation:
-// t_(a,i) = sum_q sum_m dN_a/dxi_m (q) P[q][m][i], P = (dxi/dx)^T (detJ sigma), contracted
-// zeta, eta, beta in turn for the nine (m, i).  One workgroup per element; te is written
-// in local node order like traction_kernel's.  qs / soff: fields per point and the first
-// field of detJ sigma (full or residual-only point records).
-template <int P1, int Q>
-__global__ __launch_bounds__(256) void traction_sf_kernel(const double *__restrict__ qp, int qs, int soff,
-                                                          const int32_t *__restrict__ basis, int64_t nelem,
-                                                          double *__restrict__ te, const int32_t *__restrict__ elements,
-                                                          const int32_t *__restrict__ node_elem_ptr,
-                                                          double *__restrict__ T) {
-    typedef SfLayout<P1, Q> L;
-    constexpr int Q3 = L::Q3, N3 = L::N3, PP = L::PP;
-    constexpr int NT = 256;
-    __shared__ double sF[3][2][P1][Q];
-    __shared__ double sP[9][Q3];
-    __shared__ double sU1[9][P1][Q * Q];
-    __shared__ double sU2[9][PP][Q];
-    __shared__ int32_t sLoc[N3];
-    const int64_t e = blockIdx.x;
-    if (e >= nelem) return;
-    const int tid = threadIdx.x;
-    const int32_t *g_of_t = basis + 4;
-    const int32_t *loc = basis + 4 + Q3;
-    const double *f1d = reinterpret_cast<const double *>(basis + L::INTS);
-    for (int t = tid; t < 3 * 2 * P1 * Q; t += NT) (&sF[0][0][0][0])[t] = f1d[t];
-    for (int t = tid; t < N3; t += NT) sLoc[t] = min(max(loc[t], 0), N3 - 1);
-    const double *qe_ = qp + e * (int64_t)Q3 * qs;
-    for (int t = tid; t < Q3; t += NT) {
-        const double *rec = qe_ + min(max(g_of_t[t], 0), Q3 - 1);   // field f at rec[f * Q3]
-        const double *S = rec + soff * Q3;   // s00 s11 s22 s01 s02 s12
-        const double Si[3][3] = {{S[0], S[3 * Q3], S[4 * Q3]}, {S[3 * Q3], S[1 * Q3], S[5 * Q3]},
-                                 {S[4 * Q3], S[5 * Q3], S[2 * Q3]}};
-#pragma unroll
-        for (int m = 0; m < 3; ++m)
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-                sP[3 * m + i][t] = rec[m * Q3] * Si[i][0] + rec[(3 + m) * Q3] * Si[i][1] + rec[(6 + m) * Q3] * Si[i][2];
-    }
-    __syncthreads();
-    for (int t = tid; t < 9 * P1 * Q * Q; t += NT) {
-        const int mi = t / (P1 * Q * Q), r = t - mi * P1 * Q * Q;
-        const int az = r / (Q * Q), qe = (r / Q) % Q, qb = r % Q;
-        const double *f = sF[0][mi / 3 == 0][az];
-        const double *p = sP[mi] + (qb * Q + qe) * Q;
-        double s = 0.0;
-#pragma unroll
-        for (int qz = 0; qz < Q; ++qz) s += f[qz] * p[qz];
-        sU1[mi][az][qe * Q + qb] = s;
-    }
-    __syncthreads();
-    for (int t = tid; t < 9 * PP * Q; t += NT) {
-        const int mi = t / (PP * Q), r = t - mi * PP * Q;
-        const int ae = r / (P1 * Q), az = (r / Q) % P1, qb = r % Q;
-        const double *f = sF[1][mi / 3 == 1][ae];
-        double s = 0.0;
-#pragma unroll
-        for (int qe = 0; qe < Q; ++qe) s += f[qe] * sU1[mi][az][qe * Q + qb];
-        sU2[mi][ae * P1 + az][qb] = s;
-    }
-    __syncthreads();
-    for (int t = tid; t < 3 * N3; t += NT) {
-        const int tn = t / 3, i = t - 3 * tn;
-        const int ab = tn / PP, aeaz = tn - ab * PP;
-        double s = 0.0;
-#pragma unroll
-        for (int m = 0; m < 3; ++m) {
-            const double *f = sF[2][m == 2][ab];
-#pragma unroll
-            for (int qb = 0; qb < Q; ++qb) s += f[qb] * sU2[3 * m + i][aeaz][qb];
-        }
-        const int lt = sLoc[tn];
-        te[e * 3 * N3 + 3 * lt + i] = s;
-        if (T) {
-            // a node of this element only: its T entry is this element's (as 0.0 + value,
-            // the accumulated form)
-            const int32_t a = elements[e * N3 + lt];
-            if (node_elem_ptr[a + 1] - node_elem_ptr[a] == 1) T[3 * (int64_t)a + i] = 0.0 + s;
         }
     }
 }
