@@ -171,8 +171,11 @@ __device__ __forceinline__ void qp_point(const MatParams &mp, const double (&PX)
     }
     if (!rec) return;
     if (TANGENT) {
+        // dxi/dx: read by the dense contractions (the C tensors already hold it)
+        if (!emitC) {
 #pragma unroll
-        for (int i = 0; i < 9; ++i) rec[(Q_IX + i) * fs] = ix[i];
+            for (int i = 0; i < 9; ++i) rec[(Q_IX + i) * fs] = ix[i];
+        }
 #pragma unroll
         for (int p = 0; p < 6; ++p) {
             const int i = p < 3 ? p : (p == 5 ? 1 : 0);
