@@ -44,3 +44,44 @@ def test_batch_graph_matches_per_mesh(cuda):
         assert np.array_equal(Tb, T.cpu().numpy()), (i, key)
     assert all(s[0] == 0 for s in ba.statuses())
     _lib.load().fl_set_tuning(_lib.FL_TUNE_STIFF_SF, 0)
+
+
+def test_tuned_batch_matches_per_mesh(cuda, tmp_path):
+    """The autotuner's choice per group (dense tile variant, factorised element blocks +
+    gather, or the direct scatter) is recorded in the cache, and each mesh of the tuned
+    batch equals that mesh assembled alone on the chosen path, bitwise."""
+    import torch
+    from florence_amd import _lib, autotune
+    from florence_amd.assembly import DeviceMeshData, assemble_device
+    from florence_amd.batch import BatchedAssembly, random_batch, _np_class
+
+    batch = random_batch(10, seed=5, nelem_range=(20, 300))
+    ba = BatchedAssembly(batch)
+    disp = lambda X: 0.02 * np.sin(np.pi * X[:, [1, 2, 0]]) + 0.01 * X
+    ba.set_state(disp)
+    cache = autotune.tune_batch(ba, repeats=1, cache={})
+    for g in ba.groups:
+        entry = cache[ba.tuning_key(g["key"], g["fs"])]
+        assert entry["direct"] == g["direct"] and entry["sum_factorised"] == g["sum_factorised"]
+        assert len(entry["assembly_ms"]) == len(entry["candidates"])
+        if g["dm"].scatter is not None:
+            assert "sum-factorised, direct scatter" in entry["candidates"]
+    ba.assemble()
+    torch.cuda.synchronize()
+    try:
+        for i, (m, key, mat) in enumerate(batch):
+            g = next(gg for gg in ba.groups if i in gg["mesh_ids"])
+            dm = DeviceMeshData(m, g["fs"])
+            _lib.load().fl_set_tuning(_lib.FL_TUNE_STIFF_VARIANT_BASE + _np_class(dm.npe), g["variant"] + 1)
+            _lib.load().fl_set_tuning(_lib.FL_TUNE_STIFF_SF, 2 if g["sum_factorised"] else 1)
+            X = np.asarray(m.points)
+            V, T, st = assemble_device(dm, mat, X + disp(X), direct=g["direct"])
+            torch.cuda.synchronize()
+            _, _, data, Tb = ba.mesh_result(i)
+            assert np.array_equal(data, V.cpu().numpy()), (i, key, g["direct"])
+            assert np.array_equal(Tb, T.cpu().numpy()), (i, key)
+    finally:
+        _lib.load().fl_set_tuning(_lib.FL_TUNE_STIFF_SF, 0)
+        for npc in range(5):
+            _lib.load().fl_set_tuning(_lib.FL_TUNE_STIFF_VARIANT_BASE + npc, 0)
+    assert all(s[0] == 0 for s in ba.statuses())
