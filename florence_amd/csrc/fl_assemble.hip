@@ -731,6 +731,10 @@ static int launch_stiffness_direct(int npe, int G, const double *qp, const void 
     for (int c = 0; c < ncolour; ++c) {
         const int64_t n = offsets[c + 1] - offsets[c];
         if (n <= 0) continue;
+        if ((n + 7) / 8 * 8 * 9 > (int64_t)INT32_MAX) {
+            set_error("too many elements of one colour for one launch");
+            return FL_E_LIMIT;
+        }
         const dim3 grid((unsigned)((n + 7) / 8 * 8 * 9));
         const int32_t *el = elems + offsets[c];
 #define FL_SF_CASE(P, QQ)                                                                                  \
