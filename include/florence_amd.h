@@ -211,7 +211,9 @@ size_t fl_tensor_basis_bytes(int32_t p1, int32_t q);
  * is the sum of its elements' contributions in colour order (bitwise reproducible; equal
  * to fl_df_assemble_mt's V to rounding).  ncolour, colour_offsets (host),
  * colour_elems_dev: fl_pattern_colour's plan; scatter_dev: fl_pattern_scatter_table
- * built with the same tensor basis.  workspace: fl_df_workspace_size_cs() bytes on the
+ * built from the same colouring and the same tensor basis (a table of another
+ * colouring gives wrong sums; the table build itself reports a colouring in which
+ * elements of one colour share a node).  workspace: fl_df_workspace_size_cs() bytes on the
  * direct path; where it does not apply (other precisions, no basis, the other
  * FL_TUNE_STIFF_SF forms) the call is fl_df_assemble_mt with its workspace size. */
 size_t fl_df_workspace_size_cs(int32_t ngauss, int64_t nelem, int32_t npe);
