@@ -318,6 +318,36 @@ def test_direct_scatter_equals_gather(cuda, p, n):
     assert np.array_equal(T, Tg)
 
 
+@pytest.mark.parametrize("shape", ["cube, shuffled elements", "cylinder"])
+def test_direct_scatter_irregular(cuda, shape):
+    """The direct scatter on colourings that are not the 2 x 2 x 2 parity classes: a
+    p=2 hex cube with its elements in a random order (greedy colours then follow no
+    lattice) and a p=2 hex cylinder (curved elements).  Oracle bar, and the block path
+    to 1e-13."""
+    import torch
+    from florence_amd.assembly import DeviceMeshData, assemble_device
+    from florence_amd.mesh import Mesh
+    if shape == "cylinder":
+        m = Mesh()
+        m.Cylinder(radius=0.5, length=2.0, nlong=4, nrad=1, element_type="hex")
+        m.GetHighOrderMesh(p=2)
+    else:
+        m = cube_mesh("hex", 2, 3, jitter=0.05, seed=11)
+        perm = np.random.default_rng(12).permutation(m.nelem)
+        m.elements = np.ascontiguousarray(np.asarray(m.elements)[perm])
+    fs = function_space(m, 2)
+    mat = NeoHookean(3, mu=1.0, lamb=10.0)
+    x = deformed(m.points, 0.04, seed=13)
+    dm, V, T = _check(m, fs, mat, x, "direct scatter, {}".format(shape))
+    assert dm.scatter is not None and dm.ncolour >= 8
+    print("{}: {} colours".format(shape, dm.ncolour))
+    Vg, Tg, _ = assemble_device(dm, mat, x, direct=False)
+    torch.cuda.synchronize()
+    Vg = Vg.cpu().numpy()
+    assert np.abs(V - Vg).max() <= 1e-13 * np.abs(Vg).max()
+    assert np.array_equal(T, Tg.cpu().numpy())
+
+
 def test_direct_scatter_contract(cuda):
     """fl_df_assemble_cs through the C-ABI with the smaller workspace of
     fl_df_workspace_size_cs equals the Python path bitwise; a colour plan that does not
