@@ -348,6 +348,31 @@ def test_direct_scatter_irregular(cuda, shape):
     assert np.array_equal(T, Tg.cpu().numpy())
 
 
+@pytest.mark.parametrize("et,p,direct", [("hex", 2, True), ("hex", 2, False), ("tet", 2, False)])
+def test_reference_configuration(cuda, et, p, direct):
+    """requires_geometry_update = 0 (integration on the reference configuration, the
+    reference's `_KinematicMeasures_.h:94-99` branch): K meets the oracle bar on the direct
+    scatter, the block path and the dense tet path, and T is zero."""
+    import torch
+    from florence_amd.assembly import DeviceMeshData, assemble_device
+    m = cube_mesh(et, p, 2, jitter=0.05, seed=17)
+    fs = function_space(m, p)
+    mat = MooneyRivlin(3, mu1=0.5, mu2=0.5, lamb=20.0)
+    x = deformed(m.points, 0.05, seed=18)
+    dm = DeviceMeshData(m, fs)
+    V, T, st = assemble_device(dm, mat, x, requires_geometry_update=0, direct=direct)
+    torch.cuda.synchronize()
+    assert int(st[0].item()) == 0
+    indptr_ref, indices_ref = O.sparsity_pattern(m.elements, m.nnode)
+    Vr, Tr = O.assemble_csr(m.points, m.elements, x, fs.Jm, fs.AllGauss, mat, indptr_ref, indices_ref,
+                            requires_geometry_update=0)
+    V = V.cpu().numpy()
+    err = np.abs(V - Vr).max() / np.abs(Vr).max()
+    print("{}{} update=0 direct={}: |dK|/|K| = {:.3e}".format(et, p, direct, err))
+    assert err <= RTOL
+    assert np.all(Tr == 0.0) and np.all(T.cpu().numpy() == 0.0)
+
+
 def test_direct_scatter_contract(cuda):
     """fl_df_assemble_cs through the C-ABI with the smaller workspace of
     fl_df_workspace_size_cs equals the Python path bitwise; a colour plan that does not
