@@ -39,7 +39,9 @@ __global__ __launch_bounds__(64 * SP_W) void spmv_kernel(const int32_t *__restri
     const int64_t a = (int64_t)blockIdx.x * SP_W + wave;
     double d = 0.0;
     if (a < nnode) {
-        const int32_t p0 = nbr_ptr[a], deg = nbr_ptr[a + 1] - p0;
+        // a node whose three dofs are all fixed keeps x: its rows are not read
+        const bool all_fixed = free_mask && !free_mask[3 * a] && !free_mask[3 * a + 1] && !free_mask[3 * a + 2];
+        const int32_t p0 = nbr_ptr[a], deg = all_fixed ? 0 : nbr_ptr[a + 1] - p0;
         const VT *r0 = V + indptr[3 * a];
         const VT *r1 = V + indptr[3 * a + 1];
         const VT *r2 = V + indptr[3 * a + 2];
@@ -238,7 +240,9 @@ __global__ __launch_bounds__(64 * SP_W) void pcg_spmv_kernel(const int32_t *__re
     const double beta = s[4];
     double d = 0.0;
     if (a < nnode) {
-        const int32_t p0 = nbr_ptr[a], deg = nbr_ptr[a + 1] - p0;
+        // a node whose three dofs are all fixed has p = 0 and q = 0: its rows are not read
+        const bool all_fixed = free_mask && !free_mask[3 * a] && !free_mask[3 * a + 1] && !free_mask[3 * a + 2];
+        const int32_t p0 = nbr_ptr[a], deg = all_fixed ? 0 : nbr_ptr[a + 1] - p0;
         const VT *r0 = V + indptr[3 * a];
         const VT *r1 = V + indptr[3 * a + 1];
         const VT *r2 = V + indptr[3 * a + 2];
