@@ -136,9 +136,17 @@ class LockstepNewton(object):
         out.mul_(self.free_f)
         return out
 
-    def _solve(self, rhs, du):
+    def _solve(self, rhs, du, done=None):
+        """du = K^-1 rhs on the free dofs of the members not yet converged (``done``: a
+        bool per member); a converged member is held, so its rows drop out of every
+        SpMV, as if each member were solved alone."""
+        import torch
         du.zero_()
-        self.lin.Solve(self.dm, self.V, self.free, rhs, du)
+        mask = self.free
+        if done is not None and done.any():
+            live = torch.as_tensor(~done, device=self.free.device)
+            mask = self.free * self._per_dof(live).to(self.free.dtype)
+        self.lin.Solve(self.dm, self.V, mask, rhs, du)
 
     def _eta(self, r, C):
         """Linear tolerance of the next step at (worst member's) relative residual r,
@@ -255,7 +263,7 @@ class LockstepNewton(object):
             while True:
                 eta = self._eta(r_cur, C)
                 self.lin.rtol = eta
-                t_lin += timed(self._solve, rhs, du)
+                t_lin += timed(self._solve, rhs, du, done)
                 u.add_(du)
                 it += 1
                 if first_linear:
