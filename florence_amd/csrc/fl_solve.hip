@@ -248,15 +248,12 @@ __global__ __launch_bounds__(64 * SP_W) void pcg_spmv_kernel(const int32_t *__re
         const VT *r2 = V + indptr[3 * a + 2];
         double s0 = 0.0, s1 = 0.0, s2 = 0.0;
         for (int r = lane; r < deg; r += 64) {
+            // z and p_old are +0 on fixed dofs (r starts masked, the block inverse has
+            // identity rows there, q and p_new are set to 0 there): no mask is needed
             const int64_t b = nbrs[p0 + r];
-            double x0 = z[3 * b] + beta * p_old[3 * b];
-            double x1 = z[3 * b + 1] + beta * p_old[3 * b + 1];
-            double x2 = z[3 * b + 2] + beta * p_old[3 * b + 2];
-            if (free_mask) {
-                x0 = free_mask[3 * b] ? x0 : 0.0;
-                x1 = free_mask[3 * b + 1] ? x1 : 0.0;
-                x2 = free_mask[3 * b + 2] ? x2 : 0.0;
-            }
+            const double x0 = z[3 * b] + beta * p_old[3 * b];
+            const double x1 = z[3 * b + 1] + beta * p_old[3 * b + 1];
+            const double x2 = z[3 * b + 2] + beta * p_old[3 * b + 2];
             s0 += (double)r0[3 * r] * x0 + (double)r0[3 * r + 1] * x1 + (double)r0[3 * r + 2] * x2;
             s1 += (double)r1[3 * r] * x0 + (double)r1[3 * r + 1] * x1 + (double)r1[3 * r + 2] * x2;
             s2 += (double)r2[3 * r] * x0 + (double)r2[3 * r + 1] * x1 + (double)r2[3 * r + 2] * x2;
