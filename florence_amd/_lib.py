@@ -11,6 +11,9 @@ LIB_PATH = os.path.join(_HERE, "libflorence_amd.so")
 
 FL_MAT_NEOHOOKEAN = 1
 FL_MAT_MOONEYRIVLIN = 2
+FL_MAT_LINEARELASTIC = 3
+FL_MAT_NEARLYINCOMPRESSIBLEMOONEYRIVLIN = 4
+FL_MAT_ANISOTROPICMOONEYRIVLIN_1 = 5
 FL_S_NONPOSITIVE_J = 1
 FL_S_NONFINITE = 2
 FL_S_PATTERN = 4
@@ -69,6 +72,9 @@ SIGNATURES = {
                                       _vp, _vp, _vp, _vp, _sz, _vp, _vp]),
     "fl_df_residual_tb": (ctypes.c_int, [_i32, _PD, _vp, _vp, _vp, _vp, _vp, _i32, _i64, _i32, _i64,
                                          _vp, _vp, _vp, _vp, _vp, _sz, _vp, _vp]),
+    "fl_df_residual_mt": (ctypes.c_int, [_i32, _PD, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i64, _i32, _i64,
+                                         _vp, _vp, _vp, _vp, _vp, _sz, _vp, _vp]),
+    "fl_material_table_stride": (_i32, [_i32]),
     "fl_csr_spmv": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp]),
     "fl_pcg_workspace_size": (_sz, [_i64]),
     "fl_pcg": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, ctypes.c_double, _i32, _i32, _vp, _sz,

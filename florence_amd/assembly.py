@@ -224,6 +224,25 @@ def device_mesh_data(fem_solver, mesh, function_space):
     return cache[1]
 
 
+def material_table(dm, material):
+    """Device (elem_material, table) of a law with per-element data (the fibre
+    directions of AnisotropicMooneyRivlin_1), cached on the mesh data per material
+    object and orientation array; (None, None) for the other laws."""
+    torch = _torch()
+    ori = getattr(material, "anisotropic_orientations", None)
+    key = (id(material), id(ori), tuple(material.device_parameters()))
+    cache = getattr(dm, "_mat_table", None)
+    if cache is not None and cache[0] == key:
+        return cache[1], cache[2]
+    tab = material.device_table(dm.nelem)
+    if tab is None:
+        return None, None
+    em = torch.as_tensor(tab[0], device=dm.device)
+    mt = torch.as_tensor(np.ascontiguousarray(tab[1]), device=dm.device)
+    dm._mat_table = (key, em, mt)
+    return em, mt
+
+
 def _as_device_coords(Eulerx, device):
     torch = _torch()
     if isinstance(Eulerx, torch.Tensor):
@@ -256,7 +275,8 @@ def assemble_device(dm, material, Eulerx, V=None, T=None, requires_geometry_upda
     if mid is None:
         raise NotImplementedError("no low-level assembler for material {}".format(type(material).__name__))
     prec = _lib.PRECISIONS[precision] | (_lib.FL_PREC_CSR32 if csr == "fp32" else 0)
-    args = (mid, prm, None, None, _lib.ptr(dm.points), _lib.ptr(dm.elements), _lib.ptr(x), _lib.ptr(dm.Jm),
+    em, mt = material_table(dm, material)
+    args = (mid, prm, _lib.ptr(em), _lib.ptr(mt), _lib.ptr(dm.points), _lib.ptr(dm.elements), _lib.ptr(x), _lib.ptr(dm.Jm),
             _lib.ptr(dm.AllGauss), dm.ngauss, dm.nelem, dm.npe, dm.nnode, int(requires_geometry_update),
             _lib.ptr(dm.node_elem_ptr), _lib.ptr(dm.node_elem), _lib.ptr(dm.node_nbr_ptr), _lib.ptr(dm.node_nbrs),
             _lib.ptr(dm.node_order), dm.bucket_offsets, dm.bucket_degree, _lib.ptr(dm.indptr), _lib.ptr(V),
@@ -282,12 +302,13 @@ def residual_device(dm, material, Eulerx, T=None, stream=None):
     mid = MATERIAL_IDS.get(type(material).__name__)
     if mid is None:
         raise NotImplementedError("no low-level assembler for material {}".format(type(material).__name__))
-    _lib.check(lib.fl_df_residual_tb(mid, prm, _lib.ptr(dm.points), _lib.ptr(dm.elements), _lib.ptr(x),
-                                     _lib.ptr(dm.Jm), _lib.ptr(dm.AllGauss), dm.ngauss, dm.nelem, dm.npe, dm.nnode,
-                                     _lib.ptr(dm.node_elem_ptr), _lib.ptr(dm.node_elem), _lib.ptr(T),
-                                     _lib.ptr(dm.tensor_basis), _lib.ptr(ws), ws.numel(), _lib.ptr(dm.status),
-                                     _lib.stream_ptr(stream)),
-               "fl_df_residual_tb")
+    em, mt = material_table(dm, material)
+    _lib.check(lib.fl_df_residual_mt(mid, prm, _lib.ptr(em), _lib.ptr(mt), _lib.ptr(dm.points),
+                                     _lib.ptr(dm.elements), _lib.ptr(x), _lib.ptr(dm.Jm), _lib.ptr(dm.AllGauss),
+                                     dm.ngauss, dm.nelem, dm.npe, dm.nnode, _lib.ptr(dm.node_elem_ptr),
+                                     _lib.ptr(dm.node_elem), _lib.ptr(T), _lib.ptr(dm.tensor_basis), _lib.ptr(ws),
+                                     ws.numel(), _lib.ptr(dm.status), _lib.stream_ptr(stream)),
+               "fl_df_residual_mt")
     return T, dm.status
 
 

@@ -6,7 +6,9 @@
 //  1. qp_kernel       one thread per (element, quadrature point): gather X, x; the
 //                     kinematics of _KinematicMeasures_.h:64-119 (parent Jacobians,
 //                     adjugate inverses, detJ = w |det dx/dxi|, F); the Neo-Hookean /
-//                     Mooney-Rivlin laws of _NeoHookean_.h:27-51 / _MooneyRivlin_.h:30-70.
+//                     Mooney-Rivlin laws of _NeoHookean_.h:27-51 / _MooneyRivlin_.h:30-70
+//                     (and the linear elastic, nearly incompressible and anisotropic
+//                     Mooney-Rivlin laws, material_law_gen below).
 //                     Emits per point: dxi/dx (9), the six coupling blocks
 //                     D^{ik}_{jl} = detJ (c_ijkl + delta_ik sigma_jl) (54) and detJ*sigma (6).
 //  2. stiffness_kernel one workgroup per (element, block pair i<=k): the constitutive
@@ -52,8 +54,125 @@ __constant__ int c_pair_k[6] = {0, 1, 2, 1, 2, 2};
 
 struct MatParams {
     int material;
-    double mu, mu1, mu2, lamb;
+    double mu, mu1, mu2, lamb, mu3;
+    double n0, n1, n2;   // the element's fibre direction (FL_MAT_ANISOTROPICMOONEYRIVLIN_1)
 };
+
+// the laws whose tangent needs the general form below (qp kernels instantiated with GEN)
+__host__ __device__ inline bool general_law(int material) { return material > FL_MAT_MOONEYRIVLIN; }
+
+// rows of the per-element material table: {mu, mu1, mu2, lamb} or, for the anisotropic
+// law, {mu, mu1, mu2, lamb, mu3, N0, N1, N2}
+__host__ __device__ inline int material_table_stride(int material) {
+    return material == FL_MAT_ANISOTROPICMOONEYRIVLIN_1 ? 8 : 4;
+}
+
+// cofactor of F (row-major): H_ij = dJ/dF_ij = J F^{-T}
+__device__ __forceinline__ void cofactor3(const double (&F)[9], double (&H)[9]) {
+    H[0] = F[4] * F[8] - F[5] * F[7];
+    H[1] = F[5] * F[6] - F[3] * F[8];
+    H[2] = F[3] * F[7] - F[4] * F[6];
+    H[3] = F[2] * F[7] - F[1] * F[8];
+    H[4] = F[0] * F[8] - F[2] * F[6];
+    H[5] = F[1] * F[6] - F[0] * F[7];
+    H[6] = F[1] * F[5] - F[2] * F[4];
+    H[7] = F[2] * F[3] - F[0] * F[5];
+    H[8] = F[0] * F[4] - F[1] * F[3];
+}
+
+// The other low-level laws: Cauchy stress sig and the coefficients of a spatial tangent
+//   c_ijkl = c1 (2 b_ij b_kl - b_ik b_jl - b_il b_jk) + a1 (d_ik d_jl + d_il d_jk) + a2 d_ij d_kl
+//          + A_ij d_kl + d_ij A_kl + B_ik d_jl + B_il d_jk + d_ik B_jl + d_il B_jk + e4 g_ij g_kl
+// (A, B, g symmetric), and geo = 0 where the low-level assembler has no geometric term.
+//  * linear elastic (_LinearElastic_.h:35-49): sigma = 2 mu e + lamb tr(e) I, e = sym(F) - I;
+//    c = mu (d_ik d_jl + d_il d_jk) + lamb d_ij d_kl; no geometric stiffness
+//    (AOT_Assembler.py:84-93).
+//  * nearly incompressible Mooney-Rivlin (_NearlyIncompressibleMooneyRivlin_.h:34-81) with
+//    (alpha, beta, kappa) = (mu1, mu2, lamb) as the generated assembler constructs it
+//    (AOT_Assembler.py:69-70): with H = cof F, g = H H^T, c0 = J^(-5/3), c1' = sqrt(tr g),
+//    c4 = J^-3, sigma = 2 alpha c0 b - 2/3 alpha c0 tr(b) I + beta c4 tr(g) c1' I
+//    - 3 beta c4 c1' g + kappa (J-1) I.
+//  * anisotropic Mooney-Rivlin 1 (_AnisotropicMooneyRivlin_1_.h:40-89) with n = H N:
+//    the Mooney-Rivlin terms with mu1+2mu2+mu3 in the isotropic part, plus
+//    2mu3/J (FN FN^T + (n.n) I - n n^T) in sigma.
+struct GenTangent {
+    double c1, a1, a2, e4, geo;
+    double A[9], B[9], g[9];
+};
+
+__device__ __forceinline__ void material_law_gen(const MatParams &mp, const double (&F)[9], double J,
+                                                 const double (&b)[9], double (&sig)[9], GenTangent &t) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) t.A[i] = t.B[i] = t.g[i] = 0.0;
+    t.c1 = t.e4 = 0.0;
+    t.geo = 1.0;
+    if (mp.material == FL_MAT_LINEARELASTIC) {
+        const double tre = F[0] + F[4] + F[8] - 3.0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                sig[3 * i + j] = mp.mu * (F[3 * i + j] + F[3 * j + i] - (i == j ? 2.0 : 0.0)) +
+                                 (i == j ? mp.lamb * tre : 0.0);
+        t.a1 = mp.mu;
+        t.a2 = mp.lamb;
+        t.geo = 0.0;
+    } else if (mp.material == FL_MAT_NEARLYINCOMPRESSIBLEMOONEYRIVLIN) {
+        const double alpha = mp.mu1, beta = mp.mu2, kappa = mp.lamb;
+        double Hc[9];
+        cofactor3(F, Hc);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                t.g[3 * i + j] = Hc[3 * i] * Hc[3 * j] + Hc[3 * i + 1] * Hc[3 * j + 1] + Hc[3 * i + 2] * Hc[3 * j + 2];
+        const double trb = b[0] + b[4] + b[8], trg = t.g[0] + t.g[4] + t.g[8];
+        const double c0 = pow(J, -5.0 / 3.0), q1 = sqrt(trg), q2 = 1.0 / q1, q3 = trg * q1, c4 = 1.0 / (J * J * J);
+        const double diag = -2.0 / 3.0 * alpha * c0 * trb + beta * c4 * q3 + kappa * (J - 1.0);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) sig[i] = 2.0 * alpha * c0 * b[i] - 3.0 * beta * c4 * q1 * t.g[i];
+        sig[0] += diag;
+        sig[4] += diag;
+        sig[8] += diag;
+        t.a1 = 2.0 / 3.0 * alpha * c0 * trb - beta * c4 * q3 - kappa * (J - 1.0);
+        t.a2 = 4.0 * alpha / 9.0 * c0 * trb + beta * c4 * q3 + kappa * (2.0 * J - 1.0);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            t.A[i] = -4.0 / 3.0 * alpha * c0 * b[i] - 3.0 * beta * c4 * q1 * t.g[i];
+            t.B[i] = 3.0 * beta * c4 * q1 * t.g[i];
+        }
+        t.e4 = 3.0 * beta * c4 * q2;
+    } else {   // FL_MAT_ANISOTROPICMOONEYRIVLIN_1
+        double Hc[9];
+        cofactor3(F, Hc);
+        const double N[3] = {mp.n0, mp.n1, mp.n2};
+        double FN[3], HN[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            FN[i] = F[3 * i] * N[0] + F[3 * i + 1] * N[1] + F[3 * i + 2] * N[2];
+            HN[i] = Hc[3 * i] * N[0] + Hc[3 * i + 1] * N[1] + Hc[3 * i + 2] * N[2];
+        }
+        const double iHN = HN[0] * HN[0] + HN[1] * HN[1] + HN[2] * HN[2], trb = b[0] + b[4] + b[8];
+        const double s3 = 2.0 * mp.mu3 / J;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const double bb = b[3 * i] * b[j] + b[3 * i + 1] * b[3 + j] + b[3 * i + 2] * b[6 + j];
+                sig[3 * i + j] = 2.0 * mp.mu1 / J * b[3 * i + j] + 2.0 * mp.mu2 / J * (trb * b[3 * i + j] - bb) +
+                                 s3 * (FN[i] * FN[j] - HN[i] * HN[j]);
+                t.A[3 * i + j] = -2.0 * s3 * HN[i] * HN[j];
+                t.B[3 * i + j] = s3 * HN[i] * HN[j];
+            }
+        const double diag = -2.0 * (mp.mu1 + 2.0 * mp.mu2 + mp.mu3) / J + mp.lamb * (J - 1.0) + s3 * iHN;
+        sig[0] += diag;
+        sig[4] += diag;
+        sig[8] += diag;
+        t.c1 = 2.0 * mp.mu2 / J;
+        t.a1 = 2.0 * (mp.mu1 + 2.0 * mp.mu2 + mp.mu3) / J - mp.lamb * (J - 1.0) - s3 * iHN;
+        t.a2 = mp.lamb * (2.0 * J - 1.0) + 2.0 * s3 * iHN;
+    }
+}
 
 __device__ inline double invdet3x3(const double *s, double *d) {
     d[0] = s[4] * s[8] - s[5] * s[7];
@@ -97,7 +216,7 @@ __global__ void transpose_jm_kernel(const double *__restrict__ Jm, int npe, int 
 // emitC: store, in place of each pair's D^{ik}, the tensor C^{ik}_{mn} = sum_{jl}
 // (dxi_m/dx_j) D^{ik}_{jl} (dxi_n/dx_l) that the sum-factorised stiffness contracts
 // (computed once per point instead of in each of the element's block workgroups).
-template <bool TANGENT>
+template <bool TANGENT, bool GEN = false>
 __device__ __forceinline__ void qp_point(const MatParams &mp, const double (&PX)[9], const double (&Px)[9], double wg,
                                          int update, bool valid, double *__restrict__ rec, int fs,
                                          int32_t *__restrict__ status, bool emitC = false,
@@ -120,7 +239,13 @@ __device__ __forceinline__ void qp_point(const MatParams &mp, const double (&PX)
 #pragma unroll
         for (int j = 0; j < 3; ++j) b[3 * i + j] = F[3 * i] * F[3 * j] + F[3 * i + 1] * F[3 * j + 1] + F[3 * i + 2] * F[3 * j + 2];
     double sig[9], c1, a1, a2;
-    if (mp.material == FL_MAT_NEOHOOKEAN) {
+    GenTangent gt;
+    if (GEN) {
+        material_law_gen(mp, F, J, b, sig, gt);
+        c1 = gt.c1;
+        a1 = gt.a1;
+        a2 = gt.a2;
+    } else if (mp.material == FL_MAT_NEOHOOKEAN) {
         // sigma = mu/J (b - I) + lamb (J-1) I; c = a1 (d_ik d_jl + d_il d_jk) + a2 d_ij d_kl
 #pragma unroll
         for (int i = 0; i < 9; ++i) sig[i] = mp.mu / J * (b[i] - ((i % 4) == 0 ? 1.0 : 0.0));
@@ -188,10 +313,17 @@ __device__ __forceinline__ void qp_point(const MatParams &mp, const double (&PX)
                     const double dik = i == k ? 1.0 : 0.0, djl = j == l ? 1.0 : 0.0;
                     const double dil = i == l ? 1.0 : 0.0, djk = j == k ? 1.0 : 0.0;
                     const double dij = i == j ? 1.0 : 0.0, dkl = k == l ? 1.0 : 0.0;
-                    const double cijkl = c1 * (2.0 * b[3 * i + j] * b[3 * k + l] - b[3 * i + k] * b[3 * j + l] -
-                                               b[3 * i + l] * b[3 * j + k]) +
-                                         a1 * (dik * djl + dil * djk) + a2 * dij * dkl;
-                    D[3 * j + l] = detJ * (cijkl + dik * sig[3 * j + l]);
+                    double cijkl = c1 * (2.0 * b[3 * i + j] * b[3 * k + l] - b[3 * i + k] * b[3 * j + l] -
+                                         b[3 * i + l] * b[3 * j + k]) +
+                                   a1 * (dik * djl + dil * djk) + a2 * dij * dkl;
+                    if (GEN) {
+                        cijkl += gt.A[3 * i + j] * dkl + dij * gt.A[3 * k + l] + gt.B[3 * i + k] * djl +
+                                 gt.B[3 * i + l] * djk + dik * gt.B[3 * j + l] + dil * gt.B[3 * j + k] +
+                                 gt.e4 * gt.g[3 * i + j] * gt.g[3 * k + l];
+                        D[3 * j + l] = detJ * (cijkl + gt.geo * dik * sig[3 * j + l]);
+                    } else {
+                        D[3 * j + l] = detJ * (cijkl + dik * sig[3 * j + l]);
+                    }
                 }
             if (emitC) {
                 double W[9];
@@ -238,18 +370,24 @@ __device__ __forceinline__ MatParams element_material(const MatParams &mp0, int6
                                                       const double *__restrict__ mat_table) {
     MatParams mp = mp0;
     if (elem_mat) {
-        const double *row = mat_table + 4 * (int64_t)elem_mat[e];
+        const double *row = mat_table + material_table_stride(mp0.material) * (int64_t)elem_mat[e];
         mp.mu = row[0];
         mp.mu1 = row[1];
         mp.mu2 = row[2];
         mp.lamb = row[3];
+        if (mp0.material == FL_MAT_ANISOTROPICMOONEYRIVLIN_1) {
+            mp.mu3 = row[4];
+            mp.n0 = row[5];
+            mp.n1 = row[6];
+            mp.n2 = row[7];
+        }
     }
     return mp;
 }
 
 // One thread per point: the parent Jacobians summed over the element's nodes.
-// elem_mat/mat_table (optional): per-element material parameters {mu, mu1, mu2, lamb}
-template <bool TANGENT>
+// elem_mat/mat_table (optional): per-element material parameters (material_table_stride)
+template <bool TANGENT, bool GEN>
 __global__ __launch_bounds__(256) void qp_kernel(MatParams mp0, const double *__restrict__ points,
                                                  const int32_t *__restrict__ elements,
                                                  const double *__restrict__ Eulerx, const double *__restrict__ Jm,
@@ -280,8 +418,8 @@ __global__ __launch_bounds__(256) void qp_kernel(MatParams mp0, const double *__
         }
     }
     if (!valid) PX[0] = PX[4] = PX[8] = Px[0] = Px[4] = Px[8] = 1.0;
-    qp_point<TANGENT>(mp, PX, Px, valid ? w[g] : 1.0, update, valid, qp + e * (int64_t)G * (TANGENT ? QS : QS_RES) + g,
-                      G, status);
+    qp_point<TANGENT, GEN>(mp, PX, Px, valid ? w[g] : 1.0, update, valid,
+                           qp + e * (int64_t)G * (TANGENT ? QS : QS_RES) + g, G, status);
 }
 
 // ------------------------------------------------------------------ phase 2
@@ -764,13 +902,18 @@ static int launch_qp_sf(const MatParams &mp, const double *points, const int32_t
                         int emitC = 0, double *te = nullptr, const int32_t *node_elem_ptr = nullptr,
                         double *Tdirect = nullptr) {
     const int32_t *b = static_cast<const int32_t *>(basis);
-#define FL_QSF_CASE(P, QQ)                                                                                       \
-    if (npe == P * P * P && G == QQ * QQ * QQ) {                                                                 \
-        qp_sf_kernel<P, QQ, TANGENT><<<(unsigned)nelem, 256, 0, st>>>(mp, points, elements, Eulerx, b, w, nelem, \
-                                                                       update, qp, status, elem_mat, mat_table,   \
-                                                                       emitC, te, node_elem_ptr, Tdirect);        \
-        FL_LAUNCH_CHECK();                                                                                       \
-        return 0;                                                                                                \
+#define FL_QSF_CASE(P, QQ)                                                                                     \
+    if (npe == P * P * P && G == QQ * QQ * QQ) {                                                               \
+        if (general_law(mp.material))                                                                          \
+            qp_sf_kernel<P, QQ, TANGENT, true><<<(unsigned)nelem, 256, 0, st>>>(                               \
+                mp, points, elements, Eulerx, b, w, nelem, update, qp, status, elem_mat, mat_table, emitC, te,  \
+                node_elem_ptr, Tdirect);                                                                       \
+        else                                                                                                   \
+            qp_sf_kernel<P, QQ, TANGENT, false><<<(unsigned)nelem, 256, 0, st>>>(                              \
+                mp, points, elements, Eulerx, b, w, nelem, update, qp, status, elem_mat, mat_table, emitC, te,  \
+                node_elem_ptr, Tdirect);                                                                       \
+        FL_LAUNCH_CHECK();                                                                                     \
+        return 0;                                                                                              \
     }
     FL_QSF_CASE(2, 2)
     FL_QSF_CASE(3, 3)
@@ -897,7 +1040,7 @@ static DFWork carve_df(void *ws, size_t bytes, int G, int64_t nelem, int npe, bo
 }
 
 static bool check_common(int material, const double *params, int G, int64_t nelem, int npe, int64_t nnode) {
-    if (material != FL_MAT_NEOHOOKEAN && material != FL_MAT_MOONEYRIVLIN) {
+    if (material < FL_MAT_NEOHOOKEAN || material > FL_MAT_ANISOTROPICMOONEYRIVLIN_1) {
         set_error("unknown material id %d", material);
         return false;
     }
@@ -913,6 +1056,8 @@ static bool check_common(int material, const double *params, int G, int64_t nele
     return true;
 }
 
+// params: {mu, mu1, mu2, lamb} and, for the anisotropic law, mu3 in params[4] (its
+// fibre directions only come per element, through the material table)
 static MatParams mat_params(int material, const double *p) {
     MatParams m;
     m.material = material;
@@ -920,7 +1065,21 @@ static MatParams mat_params(int material, const double *p) {
     m.mu1 = p[1];
     m.mu2 = p[2];
     m.lamb = p[3];
+    m.mu3 = material == FL_MAT_ANISOTROPICMOONEYRIVLIN_1 ? p[4] : 0.0;
+    m.n0 = m.n1 = m.n2 = 0.0;
     return m;
+}
+
+static bool check_table(int material, const int32_t *elem_mat, const double *mat_table, const char *who) {
+    if ((elem_mat == nullptr) != (mat_table == nullptr)) {
+        set_error("%s: elem_material and material_table go together", who);
+        return false;
+    }
+    if (material == FL_MAT_ANISOTROPICMOONEYRIVLIN_1 && !elem_mat) {
+        set_error("%s: the anisotropic Mooney-Rivlin law takes its fibre directions from a material table", who);
+        return false;
+    }
+    return true;
 }
 
 }  // namespace fl
@@ -1004,10 +1163,7 @@ static int df_assemble(int32_t material, const double *params, const int32_t *el
                        size_t workspace_bytes, int32_t *status, void *stream) {
     hipStream_t st = as_stream(stream);
     if (!check_common(material, params, G, nelem, npe, nnode)) return FL_E_ARG;
-    if ((elem_mat == nullptr) != (mat_table == nullptr)) {
-        set_error("fl_df_assemble_mt: elem_material and material_table go together");
-        return FL_E_ARG;
-    }
+    if (!check_table(material, elem_mat, mat_table, "fl_df_assemble_mt")) return FL_E_ARG;
     // FL_PREC_CSR32 (with an FP32/BF16 contraction): V is written as float[nnz]
     const bool v32 = (precision & FL_PREC_CSR32) != 0;
     precision &= ~FL_PREC_CSR32;
@@ -1066,9 +1222,12 @@ static int df_assemble(int32_t material, const double *params, const int32_t *el
                                           fuse_interior ? T : nullptr);
         if (rq) return rq;
     } else {
-        qp_kernel<true><<<(unsigned)((nq + 255) / 256), 256, 0, st>>>(mp, points, elements, Eulerx, Jm, AllGauss, G,
-                                                                       nelem, npe, update, w.qp, status, elem_mat,
-                                                                       mat_table);
+        if (general_law(material))
+            qp_kernel<true, true><<<(unsigned)((nq + 255) / 256), 256, 0, st>>>(
+                mp, points, elements, Eulerx, Jm, AllGauss, G, nelem, npe, update, w.qp, status, elem_mat, mat_table);
+        else
+            qp_kernel<true, false><<<(unsigned)((nq + 255) / 256), 256, 0, st>>>(
+                mp, points, elements, Eulerx, Jm, AllGauss, G, nelem, npe, update, w.qp, status, elem_mat, mat_table);
         FL_LAUNCH_CHECK();
     }
     prof_mark(st);
@@ -1192,13 +1351,14 @@ int fl_df_assemble_cs(int32_t material, const double *params, const int32_t *ele
                        status, stream);
 }
 
-static int df_residual(int32_t material, const double *params, const double *points, const int32_t *elements,
-                       const double *Eulerx, const double *Jm, const double *AllGauss, int32_t G, int64_t nelem,
-                       int32_t npe, int64_t nnode, const int32_t *node_elem_ptr, const int32_t *node_elem, double *T,
-                       const void *tensor_basis, void *workspace, size_t workspace_bytes, int32_t *status,
-                       void *stream) {
+static int df_residual(int32_t material, const double *params, const int32_t *elem_mat, const double *mat_table,
+                       const double *points, const int32_t *elements, const double *Eulerx, const double *Jm,
+                       const double *AllGauss, int32_t G, int64_t nelem, int32_t npe, int64_t nnode,
+                       const int32_t *node_elem_ptr, const int32_t *node_elem, double *T, const void *tensor_basis,
+                       void *workspace, size_t workspace_bytes, int32_t *status, void *stream) {
     hipStream_t st = as_stream(stream);
     if (!check_common(material, params, G, nelem, npe, nnode)) return FL_E_ARG;
+    if (!check_table(material, elem_mat, mat_table, "fl_df_residual")) return FL_E_ARG;
     if (!points || !elements || !Eulerx || !Jm || !AllGauss || !node_elem_ptr || !node_elem || !T || !status) {
         set_error("fl_df_residual: null pointer");
         return FL_E_ARG;
@@ -1217,14 +1377,18 @@ static int df_residual(int32_t material, const double *params, const double *poi
     if (sf_ok) {
         // no records: the tractions are contracted in the point kernel
         const int rq = launch_qp_sf<false>(mp, points, elements, Eulerx, tensor_basis, AllGauss, npe, G, nelem, 1,
-                                           nullptr, status, nullptr, nullptr, st, 0, w.te);
+                                           nullptr, status, elem_mat, mat_table, st, 0, w.te);
         if (rq) return rq;
     } else {
         transpose_jm_kernel<<<(G * 3 * NP + 255) / 256, 256, 0, st>>>(Jm, npe, G, NP, w.JmT);
         FL_LAUNCH_CHECK();
         const int64_t nq = nelem * G;
-        qp_kernel<false><<<(unsigned)((nq + 255) / 256), 256, 0, st>>>(mp, points, elements, Eulerx, Jm, AllGauss, G,
-                                                                        nelem, npe, 1, w.qp, status, nullptr, nullptr);
+        if (general_law(material))
+            qp_kernel<false, true><<<(unsigned)((nq + 255) / 256), 256, 0, st>>>(
+                mp, points, elements, Eulerx, Jm, AllGauss, G, nelem, npe, 1, w.qp, status, elem_mat, mat_table);
+        else
+            qp_kernel<false, false><<<(unsigned)((nq + 255) / 256), 256, 0, st>>>(
+                mp, points, elements, Eulerx, Jm, AllGauss, G, nelem, npe, 1, w.qp, status, elem_mat, mat_table);
         FL_LAUNCH_CHECK();
     }
     prof_mark(st);
@@ -1248,8 +1412,8 @@ int fl_df_residual(int32_t material, const double *params, const double *points,
                    const double *Eulerx, const double *Jm, const double *AllGauss, int32_t G, int64_t nelem,
                    int32_t npe, int64_t nnode, const int32_t *node_elem_ptr, const int32_t *node_elem, double *T,
                    void *workspace, size_t workspace_bytes, int32_t *status, void *stream) {
-    return df_residual(material, params, points, elements, Eulerx, Jm, AllGauss, G, nelem, npe, nnode, node_elem_ptr,
-                       node_elem, T, nullptr, workspace, workspace_bytes, status, stream);
+    return df_residual(material, params, nullptr, nullptr, points, elements, Eulerx, Jm, AllGauss, G, nelem, npe,
+                       nnode, node_elem_ptr, node_elem, T, nullptr, workspace, workspace_bytes, status, stream);
 }
 
 int fl_df_residual_tb(int32_t material, const double *params, const double *points, const int32_t *elements,
@@ -1257,8 +1421,22 @@ int fl_df_residual_tb(int32_t material, const double *params, const double *poin
                       int32_t npe, int64_t nnode, const int32_t *node_elem_ptr, const int32_t *node_elem, double *T,
                       const void *tensor_basis, void *workspace, size_t workspace_bytes, int32_t *status,
                       void *stream) {
-    return df_residual(material, params, points, elements, Eulerx, Jm, AllGauss, G, nelem, npe, nnode, node_elem_ptr,
-                       node_elem, T, tensor_basis, workspace, workspace_bytes, status, stream);
+    return df_residual(material, params, nullptr, nullptr, points, elements, Eulerx, Jm, AllGauss, G, nelem, npe,
+                       nnode, node_elem_ptr, node_elem, T, tensor_basis, workspace, workspace_bytes, status, stream);
+}
+
+int fl_df_residual_mt(int32_t material, const double *params, const int32_t *elem_mat, const double *mat_table,
+                      const double *points, const int32_t *elements, const double *Eulerx, const double *Jm,
+                      const double *AllGauss, int32_t G, int64_t nelem, int32_t npe, int64_t nnode,
+                      const int32_t *node_elem_ptr, const int32_t *node_elem, double *T, const void *tensor_basis,
+                      void *workspace, size_t workspace_bytes, int32_t *status, void *stream) {
+    return df_residual(material, params, elem_mat, mat_table, points, elements, Eulerx, Jm, AllGauss, G, nelem, npe,
+                       nnode, node_elem_ptr, node_elem, T, tensor_basis, workspace, workspace_bytes, status, stream);
+}
+
+int32_t fl_material_table_stride(int32_t material) {
+    if (material < FL_MAT_NEOHOOKEAN || material > FL_MAT_ANISOTROPICMOONEYRIVLIN_1) return 0;
+    return material_table_stride(material);
 }
 
 }  // extern "C"

@@ -128,7 +128,7 @@ __device__ __forceinline__ void traction_point_tensor(const double *ix, const do
 // follow in the same workgroup (traction_sf_contract: the point tensors in the P area,
 // the zeta results in the B area, the eta results in the P area again), and the
 // records -- if qp is given at all -- omit detJ sigma and detJ.
-template <int P1, int Q, bool TANGENT>
+template <int P1, int Q, bool TANGENT, bool GEN>
 __global__ __launch_bounds__(256) void qp_sf_kernel(MatParams mp0, const double *__restrict__ points,
                                                     const int32_t *__restrict__ elements,
                                                     const double *__restrict__ Eulerx,
@@ -217,7 +217,7 @@ __global__ __launch_bounds__(256) void qp_sf_kernel(MatParams mp0, const double 
     // te given: the element tractions are contracted here from the point values, and
     // the records (qp, when given) are only what the stiffness kernels read
     double pv[15];
-    qp_point<TANGENT>(mp, PJ[0], PJ[1], w[g], update, valid,
+    qp_point<TANGENT, GEN>(mp, PJ[0], PJ[1], w[g], update, valid,
                       qp ? qp + e * (int64_t)Q3 * (TANGENT ? QS : QS_RES) + g : nullptr, Q3, status, emitC != 0,
                       te ? pv : nullptr);
     if (te) {

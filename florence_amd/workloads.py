@@ -2,18 +2,20 @@
 boundary conditions for the benchmark configurations.
 
 * ``cube_tet_p2``  -- unit cube, n^3 cubes split into 6 tets, p=2 (10 nodes), the
-  degree-4 (11 point) tetrahedral rule, interior nodes jittered by U(-0.05h, 0.05h),
-  Neo-Hookean(mu=1, lamb=10); x=0 clamped, x=1 given u_x=0.2 (5 load increments).
+  degree-4 (11 point) tetrahedral rule, interior nodes jittered by U(-0.05h, 0.05h)
+  with h = 1/n, the linear cell edge; Neo-Hookean(mu=1, lamb=10); x=0 clamped, x=1
+  given u_x=0.2 (5 load increments).
 * ``cube_hex_p4``  -- unit cube, n^3 hexes, p=4 (125 nodes), 6x6x6 Gauss-Lobatto rule,
-  interior nodes jittered by U(-0.03h, 0.03h), Mooney-Rivlin(mu1=0.5, mu2=0.5,
-  lamb=20); x=0 clamped, x=1 twisted 30 degrees about the x axis through
-  (y, z) = (0.5, 0.5) (10 load increments).
+  interior nodes jittered by U(-0.03h, 0.03h) with h = 1/(n p), the node spacing along
+  an element edge; Mooney-Rivlin(mu1=0.5, mu2=0.5, lamb=20); x=0 clamped, x=1 twisted
+  30 degrees about the x axis through (y, z) = (0.5, 0.5) (10 load increments).
 
-h is the node spacing along an element edge, 1/(n p); "interior" means strictly inside
-the cube so the boundary faces stay planar.  (Taking h as the linear element edge
-1/n folds the p=4 elements: +-0.03/n moves the Gauss-Lobatto nodes next to a corner
-enough to make det(dX/dxi) negative at 1,479 corner points of the 24^3 mesh.)  All
-randomness is numpy PCG64(seed).
+"Interior" means strictly inside the cube so the boundary faces stay planar.  For the
+p=4 hexes h is the node spacing: taking h as the cell edge 1/n folds the elements
+(+-0.03/n moves the Gauss-Lobatto nodes next to a corner enough to make det(dX/dxi)
+negative at 1,479 corner points of the 24^3 mesh).  The p=2 tets keep h = 1/n: their
+mid-edge nodes sit 1/(2n) apart, ten times the jitter.  All randomness is numpy
+PCG64(seed).
 """
 import numpy as np
 
@@ -25,20 +27,21 @@ from .quadrature import QuadratureRule
 __all__ = ["cube_mesh", "cube_tet_p2", "cube_hex_p4", "clamp_and_pull", "clamp_and_twist", "twist_field"]
 
 
-def cube_mesh(element_type, p, n, jitter=0.0, seed=0):
+def cube_mesh(element_type, p, n, jitter=0.0, seed=0, spacing="node"):
     """Unit cube with n^3 linear cells (hexes, or 6 tets each), raised to degree p;
-    ``jitter`` is the half-width of the uniform noise in units of the node spacing
-    h = 1/(n p)."""
+    ``jitter`` is the half-width of the uniform noise in units of h: the node spacing
+    1/(n p) (``spacing="node"``) or the cell edge 1/n (``spacing="cell"``)."""
     m = Mesh()
     m.Cube(n=n, element_type=element_type)
     m.GetHighOrderMesh(p=p)
     if jitter:
-        m.JitterInteriorNodes(jitter / (n * p), seed=seed, lower=(0, 0, 0), upper=(1, 1, 1))
+        h = 1.0 / n if spacing == "cell" else 1.0 / (n * p)
+        m.JitterInteriorNodes(jitter * h, seed=seed, lower=(0, 0, 0), upper=(1, 1, 1))
     return m
 
 
 def cube_tet_p2(n=8, seed=0):
-    m = cube_mesh("tet", 2, n, jitter=0.05, seed=seed)
+    m = cube_mesh("tet", 2, n, jitter=0.05, seed=seed, spacing="cell")
     fs = FunctionSpace(m, QuadratureRule(norder=3, mesh_type="tet"), p=2)
     return m, fs, NeoHookean(3, mu=1.0, lamb=10.0)
 

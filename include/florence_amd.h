@@ -25,6 +25,22 @@ extern "C" {
  * FiniteElements/Assembly/_LowLevelAssembly_.py:47-60) */
 #define FL_MAT_NEOHOOKEAN 1   /* params: mu, lamb         (_NeoHookean_.h:27-51)  */
 #define FL_MAT_MOONEYRIVLIN 2 /* params: mu1, mu2, lamb   (_MooneyRivlin_.h:30-70) */
+/* params: mu, lamb (_LinearElastic_.h:26-52); no geometric stiffness, as the generated
+ * _GlobalAssemblyDF__LinearElastic_ (AOT_Assembler.py:84-93) */
+#define FL_MAT_LINEARELASTIC 3
+/* params: mu1, mu2, lamb, taken as alpha, beta, kappa of
+ * _NearlyIncompressibleMooneyRivlin_.h:28-87, the constants the generated assembler
+ * constructs the law from (AOT_Assembler.py:69-70) */
+#define FL_MAT_NEARLYINCOMPRESSIBLEMOONEYRIVLIN 4
+/* params: mu1, mu2, lamb and mu3 = params[4] (_AnisotropicMooneyRivlin_1_.h:30-95); the
+ * element's fibre direction N comes from the per-element material table only (the
+ * reference passes anisotropic_orientations[elem], AOT_Assembler.py:78-82), so the
+ * law needs the _mt / _cs entry points with a table of stride 8 */
+#define FL_MAT_ANISOTROPICMOONEYRIVLIN_1 5
+/* doubles per row of a per-element material table: {mu, mu1, mu2, lamb} (stride 4) or,
+ * for FL_MAT_ANISOTROPICMOONEYRIVLIN_1, {mu, mu1, mu2, lamb, mu3, N0, N1, N2} (stride 8);
+ * 0 for an unknown id.  No reference counterpart. */
+int32_t fl_material_table_stride(int32_t material);
 
 /* error codes */
 #define FL_E_ARG -1     /* invalid argument / unsupported shape     */
@@ -127,7 +143,8 @@ int fl_pattern_scatter_table(const int32_t *elements_dev, int64_t nelem, int32_t
  * pattern built above and into T[3*nnode].  Values of V are summed per CSR entry
  * in ascending element order, so the result is bitwise reproducible.
  *
- * params (host): {mu, mu1, mu2, lamb}; Jm: (3, npe, ngauss) reference layout;
+ * params (host): {mu, mu1, mu2, lamb} (mu3 in params[4] for the anisotropic law);
+ * Jm: (3, npe, ngauss) reference layout;
  * AllGauss: (ngauss).  requires_geometry_update: 1 integrates on the current
  * configuration (the reference's default), 0 on the reference configuration
  * with no traction.  workspace: fl_df_workspace_size() bytes.  status_dev: int32[2],
@@ -170,7 +187,8 @@ int fl_df_assemble_ex(int32_t material, const double *params, const double *poin
 
 /* Per-element materials (no reference counterpart: the reference assembles one
  * material per call).  elem_material_dev: int32[nelem] rows into material_table_dev:
- * f64[nmat][4] = {mu, mu1, mu2, lamb} for the law `material`; both NULL = params.
+ * f64[nmat][fl_material_table_stride(material)] for the law `material`; both NULL =
+ * params (not allowed for FL_MAT_ANISOTROPICMOONEYRIVLIN_1, whose rows carry N).
  * Used to assemble many meshes as one (batched form).  ranks_dev: the table of
  * fl_pattern_rank_table, or NULL (the gather then searches the neighbour lists on
  * every call; same result).  Issues no memset, so the call can be captured into a HIP
@@ -250,6 +268,17 @@ int fl_df_residual_tb(int32_t material, const double *params, const double *poin
                       const int32_t *node_elem_ptr_dev, const int32_t *node_elem_dev, double *T_dev,
                       const void *tensor_basis_dev, void *workspace_dev, size_t workspace_bytes,
                       int32_t *status_dev, void *stream);
+
+/* The same pass with a per-element material table (as fl_df_assemble_mt; required for
+ * FL_MAT_ANISOTROPICMOONEYRIVLIN_1, whose rows carry the fibre directions, as the
+ * reference's explicit assembler takes anisotropic_orientations,
+ * _LowLevelAssemblyExplicit_DF_DPF_.h:484) and the optional tensor basis. */
+int fl_df_residual_mt(int32_t material, const double *params, const int32_t *elem_material_dev,
+                      const double *material_table_dev, const double *points_dev, const int32_t *elements_dev,
+                      const double *Eulerx_dev, const double *Jm_dev, const double *AllGauss_dev, int32_t ngauss,
+                      int64_t nelem, int32_t npe, int64_t nnode, const int32_t *node_elem_ptr_dev,
+                      const int32_t *node_elem_dev, double *T_dev, const void *tensor_basis_dev, void *workspace_dev,
+                      size_t workspace_bytes, int32_t *status_dev, void *stream);
 
 /* ------------------------------------------------------ Newton linear algebra
  * The reference reduces the tangent to the free dofs with scipy fancy indexing

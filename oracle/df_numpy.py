@@ -108,6 +108,130 @@ def mooney_rivlin(F, mu1, mu2, lamb):
     return sig, voigt(c)
 
 
+def linear_elastic(F, mu, lamb):
+    """Cauchy stress and Voigt tangent -- LLDispatch/CythonSource/_LinearElastic_.h:26-52.
+
+    strain = (GradU + GradU^T)/2 with GradU = F - I
+    sigma  = 2 mu strain + lamb tr(strain) I
+    c_ijkl = mu (d_ik d_jl + d_il d_jk) + lamb d_ij d_kl   (independent of F)
+    The low-level assembler drops the geometric stiffness for this law
+    (AOT_Assembler.py:84-93).
+    """
+    I = _delta()
+    GradU = F - I
+    strain = 0.5 * (GradU + np.swapaxes(GradU, -1, -2))
+    tre = np.trace(strain, axis1=-2, axis2=-1)
+    sig = 2.0 * mu * strain + (lamb * tre)[..., None, None] * I
+    II_ijkl = np.einsum("ij,kl->ijkl", I, I)
+    II_ikjl = np.einsum("ik,jl->ijkl", I, I)
+    II_iljk = np.einsum("il,jk->ijkl", I, I)
+    c = mu * (II_ikjl + II_iljk) + lamb * II_ijkl
+    H = np.broadcast_to(voigt(c), F.shape[:-2] + (6, 6)).copy()
+    return sig, H
+
+
+def nearly_incompressible_mooney_rivlin(F, alpha, beta, kappa):
+    """Cauchy stress and Voigt tangent --
+    LLDispatch/CythonSource/_NearlyIncompressibleMooneyRivlin_.h:28-87.
+
+    H = cof(F), b = F F^T, g = H H^T; c0 = J^(-5/3), c1 = sqrt(tr g), c2 = 1/c1,
+    c3 = tr(g) c1, c4 = J^-3:
+    sigma = 2 alpha c0 b - 2/3 alpha c0 tr(b) I + beta c4 c3 I - 3 beta c4 c1 g
+            + kappa (J-1) I
+    c_ijkl = -4/3 alpha c0 (b_ij d_kl + d_ij b_kl) + 4/9 alpha c0 tr(b) d_ij d_kl
+             + 2/3 alpha c0 tr(b) (d_ik d_jl + d_il d_jk)
+             + beta c4 c3 (d_ij d_kl - d_ik d_jl - d_il d_jk)
+             - 3 beta c4 c1 (g_ij d_kl + d_ij g_kl)
+             + 3 beta c4 c1 (g_ik d_jl + g_il d_jk + d_ik g_jl + d_il g_jk)
+             + 3 beta c4 c2 g_ij g_kl
+             + kappa (2J-1) d_ij d_kl - kappa (J-1) (d_ik d_jl + d_il d_jk)
+    The low-level assembler constructs it from (mu1, mu2, lamb) as (alpha, beta, kappa)
+    (AOT_Assembler.py:69-70, 114-115), which is how the dispatcher below calls it.
+    """
+    I = _delta()
+    J = np.linalg.det(F)
+    Hc = J[..., None, None] * np.swapaxes(np.linalg.inv(F), -1, -2)   # cofactor(F)
+    b = F @ np.swapaxes(F, -1, -2)
+    g = Hc @ np.swapaxes(Hc, -1, -2)
+    trb = np.trace(b, axis1=-2, axis2=-1)
+    trg = np.trace(g, axis1=-2, axis2=-1)
+    c0 = J ** (-5.0 / 3.0)
+    c1 = np.sqrt(trg)
+    c2 = 1.0 / c1
+    c3 = trg * c1
+    c4 = 1.0 / (J * J * J)
+    E = lambda v: v[..., None, None]
+    sig = E(2.0 * alpha * c0) * b - E(2.0 / 3.0 * alpha * c0 * trb) * I + E(beta * c4 * c3) * I \
+        - E(3.0 * beta * c4 * c1) * g + E(kappa * (J - 1.0)) * I
+    E4 = lambda v: v[..., None, None, None, None]
+    Ib = np.broadcast_to(I, b.shape)
+    II_ijkl = np.einsum("ij,kl->ijkl", I, I)
+    II_ikjl = np.einsum("ik,jl->ijkl", I, I)
+    II_iljk = np.einsum("il,jk->ijkl", I, I)
+    bI = np.einsum("...ij,...kl->...ijkl", b, Ib) + np.einsum("...ij,...kl->...ijkl", Ib, b)
+    gI = np.einsum("...ij,...kl->...ijkl", g, Ib) + np.einsum("...ij,...kl->...ijkl", Ib, g)
+    # gI_ikjl + gI_iljk + Ig_ikjl + Ig_iljk
+    gIp = np.einsum("...ik,...jl->...ijkl", g, Ib) + np.einsum("...il,...jk->...ijkl", g, Ib) \
+        + np.einsum("...ik,...jl->...ijkl", Ib, g) + np.einsum("...il,...jk->...ijkl", Ib, g)
+    gg = np.einsum("...ij,...kl->...ijkl", g, g)
+    c = E4(-4.0 / 3.0 * alpha * c0) * bI + E4(4.0 * alpha / 9.0 * c0 * trb) * II_ijkl \
+        + E4(2.0 / 3.0 * alpha * c0 * trb) * (II_ikjl + II_iljk) \
+        + E4(beta * c4 * c3) * (II_ijkl - II_ikjl - II_iljk) \
+        - E4(3.0 * beta * c4 * c1) * gI + E4(3.0 * beta * c4 * c1) * gIp + E4(3.0 * beta * c4 * c2) * gg \
+        + E4(kappa * (2.0 * J - 1.0)) * II_ijkl - E4(kappa * (J - 1.0)) * (II_ikjl + II_iljk)
+    return sig, voigt(c)
+
+
+def anisotropic_mooney_rivlin_1(F, N, mu1, mu2, mu3, lamb):
+    """Cauchy stress and Voigt tangent --
+    LLDispatch/CythonSource/_AnisotropicMooneyRivlin_1_.h:30-95; N is the element's
+    fibre direction (..., 3) (the assembler passes anisotropic_orientations[elem],
+    AOT_Assembler.py:78-82).
+
+    H = cof(F), FN = F N, HN = H N:
+    sigma = 2mu1/J b + 2mu2/J (tr(b) b - b b) - 2(mu1+2mu2+mu3)/J I + lamb(J-1) I
+            + 2mu3/J FN FN^T + 2mu3/J (HN.HN) I - 2mu3/J HN HN^T
+    c_ijkl = 2mu2/J (2 b_ij b_kl - b_ik b_jl - b_il b_jk)
+             + (2(mu1+2mu2+mu3)/J - lamb(J-1)) (d_ik d_jl + d_il d_jk) + lamb(2J-1) d_ij d_kl
+             - 4mu3/J (d_ij n_k n_l + n_i n_j d_kl)
+             + 2mu3/J (HN.HN) (2 d_ij d_kl - d_ik d_jl - d_il d_jk)
+             + 2mu3/J (d_ik n_j n_l + d_il n_j n_k + d_jl n_i n_k + d_jk n_i n_l),  n = HN
+    """
+    I = _delta()
+    J = np.linalg.det(F)
+    Hc = J[..., None, None] * np.swapaxes(np.linalg.inv(F), -1, -2)
+    b = F @ np.swapaxes(F, -1, -2)
+    N = np.broadcast_to(N, F.shape[:-1])
+    FN = np.einsum("...ij,...j->...i", F, N)
+    HN = np.einsum("...ij,...j->...i", Hc, N)
+    oFN = np.einsum("...i,...j->...ij", FN, FN)
+    oHN = np.einsum("...i,...j->...ij", HN, HN)
+    iHN = np.einsum("...i,...i->...", HN, HN)
+    trb = np.trace(b, axis1=-2, axis2=-1)
+    E = lambda v: v[..., None, None]
+    sig = E(2.0 * mu1 / J) * b + E(2.0 * mu2 / J) * (E(trb) * b - b @ b) \
+        - E(2.0 * (mu1 + 2.0 * mu2 + mu3) / J) * I + E(lamb * (J - 1.0)) * I \
+        + E(2.0 * mu3 / J) * oFN + E(2.0 * mu3 / J * iHN) * I - E(2.0 * mu3 / J) * oHN
+    E4 = lambda v: v[..., None, None, None, None]
+    Ib = np.broadcast_to(I, b.shape)
+    II_ijkl = np.einsum("ij,kl->ijkl", I, I)
+    II_ikjl = np.einsum("ik,jl->ijkl", I, I)
+    II_iljk = np.einsum("il,jk->ijkl", I, I)
+    bb = np.einsum("...ij,...kl->...ijkl", b, b)
+    bb_ikjl = np.einsum("...ik,...jl->...ijkl", b, b)
+    bb_iljk = np.einsum("...il,...jk->...ijkl", b, b)
+    IHN = np.einsum("...ij,...kl->...ijkl", Ib, oHN)
+    HNI = np.einsum("...ij,...kl->...ijkl", oHN, Ib)
+    # IHN_ikjl + IHN_iljk + IHN_jlik + IHN_jkil
+    IHNp = np.einsum("...ik,...jl->...ijkl", Ib, oHN) + np.einsum("...il,...jk->...ijkl", Ib, oHN) \
+        + np.einsum("...jl,...ik->...ijkl", Ib, oHN) + np.einsum("...jk,...il->...ijkl", Ib, oHN)
+    c = E4(2.0 * mu2 / J) * (2.0 * bb - bb_ikjl - bb_iljk) \
+        + E4(2.0 * (mu1 + 2.0 * mu2 + mu3) / J - lamb * (J - 1.0)) * (II_ikjl + II_iljk) \
+        + E4(lamb * (2.0 * J - 1.0)) * II_ijkl - E4(4.0 * mu3 / J) * (IHN + HNI) \
+        + E4(2.0 * mu3 / J * iHN) * (2.0 * II_ijkl - II_ikjl - II_iljk) + E4(2.0 * mu3 / J) * IHNp
+    return sig, voigt(c)
+
+
 def voigt(c):
     """4th-order (..., 3,3,3,3) -> (..., 6, 6): H[A,B] = (c_ijkl + c_ijlk)/2 with
     A=(i,j), B=(k,l) in Voigt order (00,11,22,01,02,12) -- Tensor/Numeric.pyx:181-218."""
@@ -118,14 +242,36 @@ def voigt(c):
     return H
 
 
-def kinetic_measures(material, F):
-    """Dispatch on the material name (as _LowLevelAssembly_.py:47-60)."""
-    name = type(material).__name__ if not isinstance(material, dict) else material["name"]
+def material_name(material):
+    return type(material).__name__ if not isinstance(material, dict) else material["name"]
+
+
+def has_geometric_stiffness(material):
+    """The low-level LinearElastic assembler is generated without the geometric term
+    (AOT_Assembler.py:84-93); every other law keeps it."""
+    return material_name(material) != "LinearElastic"
+
+
+def kinetic_measures(material, F, orientation=None):
+    """Dispatch on the material name (as _LowLevelAssembly_.py:47-60), with the
+    constants the generated assemblers construct each law from
+    (AOT_Assembler.py:64-72).  orientation: (E, 3) fibre directions of the elements of
+    F (E, G, 3, 3), for AnisotropicMooneyRivlin_1."""
+    name = material_name(material)
     p = material if isinstance(material, dict) else material.__dict__
     if name == "NeoHookean":
         return neo_hookean(F, p["mu"], p["lamb"])
     if name == "MooneyRivlin":
         return mooney_rivlin(F, p["mu1"], p["mu2"], p["lamb"])
+    if name == "LinearElastic":
+        return linear_elastic(F, p["mu"], p["lamb"])
+    if name == "NearlyIncompressibleMooneyRivlin":
+        return nearly_incompressible_mooney_rivlin(F, p["mu1"], p["mu2"], p["lamb"])
+    if name == "AnisotropicMooneyRivlin_1":
+        N = np.asarray(orientation, dtype=np.float64)
+        if F.ndim == 4:
+            N = N[:, None, :]
+        return anisotropic_mooney_rivlin_1(F, N, p["mu1"], p["mu2"], p["mu3"], p["lamb"])
     raise NotImplementedError(name)
 
 
@@ -155,13 +301,15 @@ def total_traction(sig):
     return np.stack([s[..., 0], s[..., 4], s[..., 8], s[..., 1], s[..., 2], s[..., 5]], axis=-1)
 
 
-def local_stiffness_traction(SG, sig, H, detJ, requires_geometry_update=1):
+def local_stiffness_traction(SG, sig, H, detJ, requires_geometry_update=1, geometric=True):
     """Element stiffness (E, n, n) and traction (E, n).
 
     Constitutive: sum_g detJ_g * B (H B^T)  (_ConstitutiveStiffnessDF_.h:109-134)
     Traction:     sum_g detJ_g * B s        (:136-148)
     Geometric:    K[3a+d, 3b+d] += detJ_g * gA . sigma . gB  (_GeometricStiffness_.h:61-100)
     Sum:          stiffness += geometric (_LowLevelAssemblyDF_.h:128-130)
+    geometric=False: the term is left out (the LinearElastic assembler,
+    AOT_Assembler.py:84-93).
     """
     B = constitutive_B(SG)                                  # (E,G,n,6)
     E, G, n = B.shape[0], B.shape[1], B.shape[2]
@@ -174,6 +322,8 @@ def local_stiffness_traction(SG, sig, H, detJ, requires_geometry_update=1):
     if requires_geometry_update == 1:
         s = total_traction(sig)
         t = np.einsum("egnA,egA,eg->en", B, s, detJ)
+    if not geometric:
+        return K, t
     # geometric stiffness with symmetric sigma (uses s00,s01,s02,s11,s12,s22 as the C code)
     sg = sig.copy()
     sg[..., 1, 0] = sg[..., 0, 1]
@@ -193,13 +343,20 @@ def element_matrices(points, elements, Eulerx, Jm, AllGauss, material, requires_
     """(K_e, t_e) for a slice of elements: gather -> kinematics -> kinetics ->
     integrands, i.e. the body of the element loop of _LowLevelAssemblyDF_.h:69-130."""
     el = np.asarray(elements, dtype=np.int64)
-    if elem_slice is not None:
-        el = el[elem_slice]
+    sl = elem_slice if elem_slice is not None else slice(None)
+    el = el[sl]
     X = np.asarray(points)[el]
     x = np.asarray(Eulerx)[el]
     SG, F, detJ = kinematic_measures(Jm, AllGauss, X, x, requires_geometry_update)
-    sig, H = kinetic_measures(material, F)
-    return local_stiffness_traction(SG, sig, H, detJ, requires_geometry_update)
+    sig, H = kinetic_measures(material, F, element_orientations(material, sl))
+    return local_stiffness_traction(SG, sig, H, detJ, requires_geometry_update, has_geometric_stiffness(material))
+
+
+def element_orientations(material, sl):
+    """Rows of anisotropic_orientations (nelem, 3) for an element slice, or None."""
+    ori = material.get("anisotropic_orientations") if isinstance(material, dict) else \
+        getattr(material, "anisotropic_orientations", None)
+    return None if ori is None else np.asarray(ori, dtype=np.float64)[sl]
 
 
 # ------------------------------------------------------------------ sparse pattern
@@ -288,7 +445,7 @@ def residual_only(points, elements, Eulerx, Jm, AllGauss, material, chunk=4096):
         X = np.asarray(points)[el[sl]]
         x = np.asarray(Eulerx)[el[sl]]
         SG, F, detJ = kinematic_measures(Jm, AllGauss, X, x, 1)
-        sig, _ = kinetic_measures(material, F)
+        sig, _ = kinetic_measures(material, F, element_orientations(material, sl))
         t = np.einsum("egaj,egij,eg->eai", SG, sig, detJ).reshape(-1, 3 * npe)
         for k in range(t.shape[0]):
             dof = (3 * el[s + k][:, None] + np.arange(3)[None, :]).ravel()

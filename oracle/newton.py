@@ -25,21 +25,28 @@ def _c_lib():
     if not os.path.exists(so):
         return None
     lib = ctypes.CDLL(so)
-    lib.oracle_df_assemble.restype = ctypes.c_int
+    lib.oracle_df_assemble_ex.restype = ctypes.c_int
     return lib
 
 
+_IDS = {"NeoHookean": 1, "MooneyRivlin": 2, "LinearElastic": 3, "NearlyIncompressibleMooneyRivlin": 4,
+        "AnisotropicMooneyRivlin_1": 5}
+
+
 def make_assembler(mesh, fs, material, indptr, indices):
+    """x -> (V, T): the C restatement (liboracle_df.so) when built, else the numpy one."""
     lib = _c_lib()
     el = np.ascontiguousarray(mesh.elements, dtype=np.int64)
     pts = np.ascontiguousarray(mesh.points, dtype=np.float64)
     Jm, w = np.ascontiguousarray(fs.Jm), np.ascontiguousarray(fs.AllGauss.ravel())
     ip, ix = np.ascontiguousarray(indptr, dtype=np.int32), np.ascontiguousarray(indices, dtype=np.int32)
     name = type(material).__name__
-    prm = np.array([getattr(material, "mu", 0.0) or 0.0, getattr(material, "mu1", 0.0) or 0.0,
-                    getattr(material, "mu2", 0.0) or 0.0, material.lamb], dtype=np.float64)
-    mid = {"NeoHookean": 1, "MooneyRivlin": 2}[name]
-    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    f = lambda a: 0.0 if getattr(material, a, None) is None else float(getattr(material, a))
+    prm = np.array([f("mu"), f("mu1"), f("mu2"), f("lamb"), f("mu3")], dtype=np.float64)
+    ori = getattr(material, "anisotropic_orientations", None)
+    ori = None if ori is None else np.ascontiguousarray(ori, dtype=np.float64)
+    mid = _IDS[name]
+    P = lambda a: None if a is None else a.ctypes.data_as(ctypes.c_void_p)
 
     def assemble(x):
         x = np.ascontiguousarray(x, dtype=np.float64)
@@ -47,9 +54,10 @@ def make_assembler(mesh, fs, material, indptr, indices):
             return O.assemble_csr(pts, el, x, Jm, w, material, ip, ix)
         V = np.zeros(ix.size)
         T = np.zeros(3 * pts.shape[0])
-        rc = lib.oracle_df_assemble(P(pts), P(el), P(x), P(Jm), P(w), ctypes.c_int64(w.size),
-                                    ctypes.c_int64(el.shape[1]), ctypes.c_int(mid), P(prm), ctypes.c_int(1),
-                                    P(ip), P(ix), P(V), P(T), ctypes.c_int64(0), ctypes.c_int64(el.shape[0]))
+        rc = lib.oracle_df_assemble_ex(P(pts), P(el), P(x), P(Jm), P(w), ctypes.c_int64(w.size),
+                                       ctypes.c_int64(el.shape[1]), ctypes.c_int(mid), P(prm), P(ori),
+                                       ctypes.c_int(1), P(ip), P(ix), P(V), P(T), ctypes.c_int64(0),
+                                       ctypes.c_int64(el.shape[0]))
         if rc != 0:
             raise RuntimeError("oracle assembly failed")
         return V, T
@@ -57,8 +65,14 @@ def make_assembler(mesh, fs, material, indptr, indices):
     return assemble
 
 
-def solve_static(mesh, fs, material, dirichlet_flags, nincr, tol=1e-8, maxit=50, break_at_stagnation=True):
-    """Returns (TotalDisp (nnode, 3, nincr), NRConvergence dict, iterations per increment)."""
+def solve_static(mesh, fs, material, dirichlet_flags, nincr, tol=1e-8, maxit=50, break_at_stagnation=True,
+                 record=None):
+    """Returns (TotalDisp (nnode, 3, nincr), NRConvergence dict, iterations per increment).
+
+    record (optional dict) receives per increment: "first_iterate" (the displacement after
+    the first Newton update, i.e. the predictor step), "abs_residual" (the absolute
+    ||R_free|| after every iteration) and "predictor_residual" (||R_free|| of the
+    predictor, the reference's NormForces at increment 0)."""
     nnode = mesh.points.shape[0]
     ndof = 3 * nnode
     indptr, indices = O.sparsity_pattern(mesh.elements, nnode)
@@ -87,6 +101,9 @@ def solve_static(mesh, fs, material, dirichlet_flags, nincr, tol=1e-8, maxit=50,
             if np.isclose(NormForces, 0.0):
                 NormForces = 1e-14
         norm_residual = np.linalg.norm(Residual) / NormForces
+        if record is not None:
+            record.setdefault("predictor_residual", []).append(np.linalg.norm(Residual))
+            record.setdefault("abs_residual", []).append([])
         Eulerx[cols_out] += LF * applied
         Iter = 0
         hist = []
@@ -100,6 +117,10 @@ def solve_static(mesh, fs, material, dirichlet_flags, nincr, tol=1e-8, maxit=50,
             K = sp.csr_matrix((V, indices, indptr), shape=(ndof, ndof))
             Residual[cols_in] = T[cols_in] - NodalForces[cols_in]
             absn = np.linalg.norm(Residual[cols_in])
+            if record is not None:
+                record["abs_residual"][-1].append(absn)
+                if Iter == 0:
+                    record.setdefault("first_iterate", []).append(Eulerx.reshape(-1, 3) - mesh.points)
             if Iter == 0:
                 NormForces = absn if not np.isclose(absn, 0.0) else 1e-14
             norm_residual = abs(absn / NormForces)

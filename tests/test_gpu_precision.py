@@ -44,7 +44,7 @@ def test_fp32_tangent_error(cuda, case):
           flush=True)
     assert e64 <= 1e-12
     assert e32 <= 1e-5
-    assert e16 <= 2e-2      # BF16 operands (8-bit mantissa), FP32 accumulation
+    assert e16 <= 5e-3      # BF16 operands (unit roundoff 2^-8 = 3.9e-3), FP32 accumulation
     assert torch.equal(T32, T64) and torch.equal(T16, T64)
     assert int(st[0].item()) == 0
 
@@ -69,7 +69,7 @@ def test_mixed_newton_tet_p2_cube(cuda, precision, bar):
     res = solver.Solve(formulation=Form(), mesh=m, material=mat, boundary_condition=bc)
     assert not res.newton_raphson_failed_to_converge
     disp = res.TotalDisp[:, :, -1].cpu().numpy()
-    err = np.abs(disp - ref["disp_final"]).max() / np.abs(ref["disp_final"]).max()
+    err = np.abs(disp - ref["disp"][-1]).max() / np.abs(ref["disp"][-1]).max()
     print("{} tangent newton iterations {} (fp64: {}) rel disp err {:.2e} wall {:.3f}s final residuals {}".format(
         precision, res.newton_iterations, ref["iterations"].tolist(), err, res.wall_time,
         ["{:.1e}".format(v[-1]) for v in res.NRConvergence.values()]), flush=True)

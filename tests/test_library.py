@@ -51,6 +51,8 @@ def test_host_only_queries():
     assert lib.fl_pattern_scatter_table_bytes(8, 125) == 8 * 5 * 8 * 64 * 8 * 2
     assert lib.fl_pattern_scatter_table_bytes(8, 27) == 8 * 3 * 2 * 64 * 4 * 2
     assert lib.fl_pattern_scatter_table_bytes(8, 10) == 0     # not a hex of p1^3 nodes
+    # per-element material rows: 4 doubles, 8 for the anisotropic law (its fibre direction)
+    assert [lib.fl_material_table_stride(m) for m in range(7)] == [0, 4, 4, 4, 4, 8, 0]
 
 
 def _colour(el, nnode, max_colours=64):

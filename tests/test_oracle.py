@@ -14,7 +14,8 @@ import scipy.sparse as sp
 
 from cases import tet_p2_cube, cube_mesh, deformed, function_space
 from florence_amd.bases import (FeketePointsTet, GaussLobattoPointsHex, HexBases, NodeArrangementHex, TetBases)
-from florence_amd.materials import MooneyRivlin, NeoHookean
+from florence_amd.materials import (AnisotropicMooneyRivlin_1, LinearElastic, MooneyRivlin,
+                                    NearlyIncompressibleMooneyRivlin, NeoHookean)
 from florence_amd.quadrature import GaussLobattoQuadrature, QuadratureRule
 from oracle import df_numpy as O
 from oracle import energy as EN
@@ -137,10 +138,42 @@ def test_materials_at_identity():
     assert np.abs(H[0] - lin(2 * 0.5 + 2 * 0.7, 20.0 + 4 * 0.7)).max() < 1e-13
 
 
+def test_new_materials_at_identity():
+    """sigma(I) = 0 for the three other low-level laws (any fibre direction), and the
+    linear elastic tangent is LinearElastic.H_Voigt (MaterialBase.py:92-99)."""
+    I = np.eye(3)[None]
+    lin = lambda mu, lam: lam * np.block([[np.ones((3, 3)), np.zeros((3, 3))], [np.zeros((3, 6))]]) + \
+        mu * np.diag([2, 2, 2, 1, 1, 1.0])
+    sig, H = O.linear_elastic(I, 1.3, 7.0)
+    assert np.abs(sig).max() == 0.0
+    assert np.abs(H[0] - lin(1.3, 7.0)).max() < 1e-15
+    sig, H = O.nearly_incompressible_mooney_rivlin(I, 0.6, 0.3, 15.0)
+    assert np.abs(sig).max() < 1e-13
+    N = np.array([0.48, -0.6, 0.64])
+    sig, H = O.anisotropic_mooney_rivlin_1(I, N, 0.5, 0.4, 0.7, 20.0)
+    assert np.abs(sig).max() < 1e-13
+
+
+def _materials(nelem=None, seed=5):
+    rng = np.random.default_rng(seed)
+    ori = None
+    if nelem is not None:
+        ori = rng.standard_normal((nelem, 3))
+        ori /= np.linalg.norm(ori, axis=1)[:, None]
+    return {"NH": NeoHookean(3, mu=1.0, lamb=10.0), "MR": MooneyRivlin(3, mu1=0.5, mu2=0.5, lamb=20.0),
+            "LE": LinearElastic(3, mu=1.0, lamb=10.0),
+            "NIMR": NearlyIncompressibleMooneyRivlin(3, mu1=0.6, mu2=0.3, lamb=15.0),
+            "AMR1": AnisotropicMooneyRivlin_1(3, mu1=0.5, mu2=0.4, mu3=0.7, lamb=20.0,
+                                             anisotropic_orientations=ori)}
+
+
 def test_material_tangent_symmetries():
     rng = np.random.default_rng(0)
     F = np.eye(3) + 0.2 * rng.standard_normal((5, 3, 3))
-    for sig, H in [O.neo_hookean(F, 1.0, 10.0), O.mooney_rivlin(F, 0.5, 0.5, 20.0)]:
+    N = rng.standard_normal((5, 3))
+    for sig, H in [O.neo_hookean(F, 1.0, 10.0), O.mooney_rivlin(F, 0.5, 0.5, 20.0), O.linear_elastic(F, 1.0, 10.0),
+                   O.nearly_incompressible_mooney_rivlin(F, 0.6, 0.3, 15.0),
+                   O.anisotropic_mooney_rivlin_1(F, N, 0.5, 0.4, 0.7, 20.0)]:
         assert np.abs(H - np.swapaxes(H, -1, -2)).max() < 1e-12
         assert np.abs(sig - np.swapaxes(sig, -1, -2)).max() < 1e-13
 
@@ -148,11 +181,11 @@ def test_material_tangent_symmetries():
 # ------------------------------------------------------ energy oracle (autograd)
 @pytest.mark.parametrize("et,p,qt,q", [("tet", 2, "gauss", 3), ("hex", 2, "gauss", 3), ("hex", 3, "lobatto", 5),
                                        ("tet", 3, "gauss", 4)])
-@pytest.mark.parametrize("mat", ["NH", "MR"])
+@pytest.mark.parametrize("mat", ["NH", "MR", "NIMR", "AMR1"])
 def test_assembly_equals_energy_derivatives(et, p, qt, q, mat):
     m = cube_mesh(et, p, 1, jitter=0.05, seed=1)
     fs = function_space(m, p, qtype=qt, norder=q)
-    material = NeoHookean(3, mu=1.0, lamb=10.0) if mat == "NH" else MooneyRivlin(3, mu1=0.5, mu2=0.5, lamb=20.0)
+    material = _materials(m.nelem)[mat]
     x = deformed(m.points, 0.03, seed=3)
     indptr, indices = O.sparsity_pattern(m.elements, m.nnode)
     V, T = O.assemble_csr(m.points, m.elements, x, fs.Jm, fs.AllGauss, material, indptr, indices)
@@ -160,6 +193,52 @@ def test_assembly_equals_energy_derivatives(et, p, qt, q, mat):
     g, Hs = EN.gradient_and_hessian(m.points, m.elements, x, fs.Jm, fs.AllGauss, material)
     assert np.abs(T - g).max() <= 1e-12 * np.abs(g).max()
     assert np.abs(K - Hs).max() <= 1e-12 * np.abs(Hs).max()
+
+
+def _dense(m, fs, material, x, update=1):
+    indptr, indices = O.sparsity_pattern(m.elements, m.nnode)
+    V, T = O.assemble_csr(m.points, m.elements, x, fs.Jm, fs.AllGauss, material, indptr, indices,
+                          requires_geometry_update=update)
+    return sp.csr_matrix((V, indices, indptr), shape=(3 * m.nnode,) * 2).toarray(), T
+
+
+@pytest.mark.parametrize("et,p", [("tet", 2), ("hex", 2)])
+def test_linear_elastic_pins(et, p):
+    """The linear elastic low-level assembler (no geometric term) is pinned by:
+    * at x = X, with and without the geometry update: K is the Hessian of the
+      small-strain energy sum w |det dX/dxi| (mu e:e + lamb/2 tr(e)^2), and T = 0;
+    * without the update T = 0 at any x (the traction is only formed with the update,
+      _ConstitutiveStiffnessDF_.h:136-148; the gradients stay those of the current
+      configuration, _KinematicMeasures_.h:94-108, so K is not x-independent);
+    * at a rigid rotation x = R X: K = Q K0 Q^T with Q = diag(R) per node -- an
+      isotropic constitutive term rotates with the gradients, while a geometric term
+      with sigma(R) != 0 would not;
+    * near X: T(X + eps d) = eps K0 d + O(eps^2), which fixes the stress law (linear
+      in F with sigma(I) = 0) and the traction integral."""
+    m = cube_mesh(et, p, 1, jitter=0.05, seed=1)
+    fs = function_space(m, p)
+    mat = LinearElastic(3, mu=1.0, lamb=10.0)
+    X = np.asarray(m.points)
+    _, Hs = EN.gradient_and_hessian(X, m.elements, X, fs.Jm, fs.AllGauss, mat)
+    scale = np.abs(Hs).max()
+    K0, T0 = _dense(m, fs, mat, X, update=0)
+    assert np.abs(K0 - Hs).max() <= 1e-12 * scale and np.abs(T0).max() == 0.0
+    _, T0 = _dense(m, fs, mat, deformed(X, 0.05, seed=3), update=0)
+    assert np.abs(T0).max() == 0.0
+    K1, T1 = _dense(m, fs, mat, X, update=1)
+    assert np.abs(K1 - Hs).max() <= 1e-12 * scale and np.abs(T1).max() <= 1e-14
+    t = 0.4
+    R = np.array([[1, 0, 0], [0, np.cos(t), -np.sin(t)], [0, np.sin(t), np.cos(t)]]) @ \
+        np.array([[np.cos(0.3), 0, np.sin(0.3)], [0, 1, 0], [-np.sin(0.3), 0, np.cos(0.3)]])
+    KR, _ = _dense(m, fs, mat, X @ R.T + 0.1, update=1)
+    Q = np.kron(np.eye(m.nnode), R)
+    assert np.abs(KR - Q @ Hs @ Q.T).max() <= 1e-12 * scale
+    d = np.random.default_rng(4).standard_normal(X.shape)
+    errs = []
+    for eps in (1e-4, 5e-5):
+        _, Te = _dense(m, fs, mat, X + eps * d, update=1)
+        errs.append(np.abs(Te - eps * (Hs @ d.ravel())).max() / (eps * np.abs(Hs @ d.ravel()).max()))
+    assert errs[0] < 1e-3 and 1.7 < errs[0] / errs[1] < 2.3   # first order in eps
 
 
 def test_rigid_translation_and_equilibrium():
@@ -201,28 +280,33 @@ def _c_oracle():
     return lib
 
 
-def c_assemble(m, fs, mat, x, indptr, indices):
+def c_assemble(m, fs, mat, x, indptr, indices, update=1):
     lib = _c_oracle()
     el = np.ascontiguousarray(m.elements, dtype=np.int64)
     V = np.zeros(indices.size)
     T = np.zeros(3 * m.nnode)
     prm = np.ascontiguousarray(mat.device_parameters())
-    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    ori = None if mat.anisotropic_orientations is None else \
+        np.ascontiguousarray(mat.anisotropic_orientations, dtype=np.float64)
+    P = lambda a: None if a is None else a.ctypes.data_as(ctypes.c_void_p)
     pts, xx = np.ascontiguousarray(m.points), np.ascontiguousarray(x)
     Jm, w = np.ascontiguousarray(fs.Jm), np.ascontiguousarray(fs.AllGauss.ravel())
     ip, ix = np.ascontiguousarray(indptr, dtype=np.int32), np.ascontiguousarray(indices, dtype=np.int32)
-    rc = lib.oracle_df_assemble(P(pts), P(el), P(xx), P(Jm), P(w), ctypes.c_int64(fs.AllGauss.shape[0]),
-                                ctypes.c_int64(el.shape[1]), ctypes.c_int(mat.material_id), P(prm), ctypes.c_int(1),
-                                P(ip), P(ix), P(V), P(T), ctypes.c_int64(0), ctypes.c_int64(el.shape[0]))
+    lib.oracle_df_assemble_ex.restype = ctypes.c_int
+    rc = lib.oracle_df_assemble_ex(P(pts), P(el), P(xx), P(Jm), P(w), ctypes.c_int64(fs.AllGauss.shape[0]),
+                                   ctypes.c_int64(el.shape[1]), ctypes.c_int(mat.material_id), P(prm), P(ori),
+                                   ctypes.c_int(update), P(ip), P(ix), P(V), P(T), ctypes.c_int64(0),
+                                   ctypes.c_int64(el.shape[0]))
     assert rc == 0
     return V, T
 
 
-@pytest.mark.parametrize("et,p,mat", [("tet", 2, "NH"), ("hex", 2, "MR"), ("tet", 3, "MR")])
+@pytest.mark.parametrize("et,p,mat", [("tet", 2, "NH"), ("hex", 2, "MR"), ("tet", 3, "MR"), ("tet", 2, "LE"),
+                                      ("hex", 2, "NIMR"), ("tet", 3, "AMR1")])
 def test_c_restatement_matches_numpy(et, p, mat):
     m = cube_mesh(et, p, 2, jitter=0.05, seed=7)
     fs = function_space(m, p)
-    material = NeoHookean(3, mu=1.0, lamb=10.0) if mat == "NH" else MooneyRivlin(3, mu1=0.5, mu2=0.5, lamb=20.0)
+    material = _materials(m.nelem)[mat]
     x = deformed(m.points, 0.04, seed=8)
     indptr, indices = O.sparsity_pattern(m.elements, m.nnode)
     V, T = O.assemble_csr(m.points, m.elements, x, fs.Jm, fs.AllGauss, material, indptr, indices)

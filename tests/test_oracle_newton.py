@@ -1,0 +1,89 @@
+"""Pins of the Newton oracle (oracle/newton.py) through its golden fixtures, by what
+the mathematics fixes rather than by re-running the same code:
+
+* prescribed dofs: after increment k the fixed dofs hold exactly (k+1)/n of the applied
+  Dirichlet values (the reference's LoadFactor scaling, FEMSolver.py:846, 943-946,
+  BoundaryCondition.py:908-919), to rounding;
+* predictor: the first Newton update of increment k solves the linearised equilibrium
+  K_ff(u_{k-1}) du_f = -K_fd(u_{k-1}) du_D (FEMSolver.py:857-862 with
+  BoundaryCondition.py:860-892's sign), checked by a product with the pinned assembly;
+* equilibrium: the stored converged displacement of every increment satisfies the
+  reference's exit test under the pinned assembly (FEMSolver.py:981-1004): the free
+  residual is below the tolerance in absolute terms or relative to the residual after
+  the increment's first iteration;
+* quadratic convergence: every relative residual of an increment's history is at most
+  the square of the one before (r_{i+1} <= r_i^2), which a wrong sign, a dropped
+  geometric term or a stale tangent would break;
+* the recorded histories and the increment-0 predictor norm are those of the pinned
+  assembly.
+
+The assembly used here is the C restatement, itself pinned to the numpy oracle
+(tests/test_oracle.py::test_c_restatement_matches_numpy), which is pinned to the
+autograd energy Hessians."""
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from florence_amd.workloads import clamp_and_pull, clamp_and_twist, cube_hex_p4, cube_tet_p2
+from oracle import df_numpy as O
+from oracle import newton
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+TOL = 1e-8
+
+CASES = {
+    "c1_newton.npz": (lambda: cube_tet_p2(8), clamp_and_pull, 5),
+    "c2_newton_n2.npz": (lambda: cube_hex_p4(2), lambda m: clamp_and_twist(m, 30.0), 10),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_newton_fixture_is_pinned(name):
+    build, bc, nincr = CASES[name]
+    f = np.load(os.path.join(GOLD, name))
+    m, fs, mat = build()
+    X = np.asarray(m.points)
+    assert np.array_equal(X, f["points"]), "mesh generator changed; regenerate the fixture"
+    flags = np.asarray(bc(m), dtype=np.float64).ravel()
+    assert np.array_equal(np.isnan(flags), np.isnan(f["flags"].ravel()))
+    out = ~np.isnan(flags)
+    free = ~out
+    ip, ix = O.sparsity_pattern(m.elements, m.nnode)
+    asm = newton.make_assembler(m, fs, mat, ip, ix)
+    csr = lambda V: sp.csr_matrix((V, ix, ip), shape=(3 * m.nnode,) * 2)
+    dUD = np.zeros(3 * m.nnode)
+    dUD[out] = flags[out] / nincr
+    V0, _ = asm(X)
+    assert np.isclose(np.linalg.norm((csr(V0) @ dUD)[free]), f["predictor_residual"][0], rtol=1e-12)
+    assert f["disp"].shape == (nincr,) + X.shape and len(f["iterations"]) == nincr
+    prev = np.zeros_like(X)
+    amax = np.abs(flags[out]).max()
+    for k in range(nincr):
+        u, first = f["disp"][k], f["first"][k]
+        # prescribed values
+        assert np.abs(u.ravel()[out] - (k + 1) * flags[out] / nincr).max() <= 1e-14 * amax, k
+        assert np.abs(first.ravel()[out] - (k + 1) * flags[out] / nincr).max() <= 1e-14 * amax, k
+        # predictor: linearised equilibrium at the previous converged state
+        Vp, _ = asm(X + prev)
+        Kp = csr(Vp)
+        du = (first - prev).ravel().copy()
+        du[out] = 0.0
+        lin = (Kp @ du + Kp @ dUD)[free]
+        assert np.linalg.norm(lin) <= 1e-10 * np.linalg.norm((Kp @ dUD)[free]), k
+        # equilibrium under the reference's exit test
+        _, T1 = asm(X + first)
+        _, Tk = asm(X + u)
+        R1, Rk = np.linalg.norm(T1[free]), np.linalg.norm(Tk[free])
+        ah = f["abs_history"][k]
+        ah = ah[~np.isnan(ah)]
+        assert np.isclose(R1, ah[0], rtol=1e-10) and np.isclose(Rk, ah[-1], rtol=1e-6, atol=1e-15), k
+        assert Rk < TOL or Rk / R1 <= TOL, (k, Rk, Rk / R1)
+        # quadratic contraction of the relative history
+        h = f["nr_history"][k]
+        h = h[~np.isnan(h)]
+        assert len(h) == f["iterations"][k] and h[0] == 1.0
+        assert np.allclose(h, ah / ah[0], rtol=1e-12)
+        assert np.all(h[1:] <= h[:-1] ** 2), (k, h)
+        prev = u
