@@ -76,6 +76,19 @@ def direct_scatter_bytes(nelem, npe, G, nnz, table_bytes):
     return 8 * nelem * G * 54 + table_bytes + 8 * nnz + 8 * (9 * nelem * npe * npe - nnz)
 
 
+def compulsory_bytes(nelem, npe, nnode, nnz):
+    """Algorithmic HBM bytes of one K+T assembly (the per-unit figure of the path's
+    measurement plan): every CSR value written once (8 nnz), T written once (8 x 3
+    nnode), X and x read once (2 x 8 x 3 nnode) and the int32 connectivity read once
+    (4 nelem npe)."""
+    return 8 * nnz + 8 * 3 * nnode + 16 * 3 * nnode + 4 * nelem * npe
+
+
+PMC_SUMMARY = "assembly_hex_p4_24cube_pmc.md"
+PMC_SOURCE = ("profiles/" + PMC_SUMMARY + ": rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, of the committed bench.py "
+              "on this workload (copy-calibrated, KiB per dispatch summed over one assembly's launches)")
+
+
 def device_copy_gbs(torch, nbytes, reps=10):
     """Read+write rate of a plain device-to-device copy of nbytes (GB/s, bytes read plus
     bytes written), timed with HIP events."""
@@ -135,12 +148,31 @@ def measured_traffic(summary, phase):
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(mesh, fs, mat, x, nsample, max_threads=16):
-    """Time oracle/liboracle_df.so (C restatement of _GlobalAssemblyDF_) on the first
-    ``nsample`` elements of the mesh, split into equal chunks over the host cores (at most
-    ``max_threads``, the host share of one GPU): each thread assembles its chunk into the
-    chunk's own CSR pattern (the library call releases the GIL and keeps no global
-    state).  value = elements / wall time of the parallel section."""
+def cpu_baseline(mesh, fs, mat, x, nsample, max_threads=16, one_core_sample=192):
+    """Time oracle/liboracle_df.so (C restatement of _GlobalAssemblyDF_) on bounded
+    samples of the mesh: on one core (the reference kernel is serial) over the first
+    ``one_core_sample`` elements, and on every core of this process's affinity set (at
+    most ``max_threads``: a GPU box gives one GPU's process 16 cores, whatever the
+    machine has) over the first ``nsample`` elements split into equal chunks, each thread
+    assembling its chunk into the chunk's own CSR pattern (the library call releases the
+    GIL and keeps no global state) -- the reference's element-partitioned pool model.
+    value = elements / wall time of the parallel section."""
+    one = _cpu_run(mesh, fs, mat, x, one_core_sample, 1)
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncpu = os.cpu_count() or 1
+    threads = max(1, min(max_threads, ncpu))
+    allc = _cpu_run(mesh, fs, mat, x, max(nsample, 16 * threads), threads)
+    if one is None or allc is None:
+        return None
+    allc["affinity_cores"] = ncpu
+    allc["one_core"] = {"value": one["value"], "cores": 1, "sample": one["sample"]}
+    allc["speedup_all_cores_vs_one"] = round(allc["value"] / one["value"], 2)
+    return allc
+
+
+def _cpu_run(mesh, fs, mat, x, nsample, threads):
     import concurrent.futures as cf
     from oracle import df_numpy as O
     so = os.path.join(ROOT, "oracle", "liboracle_df.so")
@@ -148,11 +180,7 @@ def cpu_baseline(mesh, fs, mat, x, nsample, max_threads=16):
         return None
     lib = ctypes.CDLL(so)
     lib.oracle_df_assemble.restype = ctypes.c_int
-    try:
-        ncpu = len(os.sched_getaffinity(0))
-    except AttributeError:
-        ncpu = os.cpu_count() or 1
-    threads = max(1, min(max_threads, ncpu, nsample))
+    threads = max(1, min(threads, nsample))
     chunk = nsample // threads
     prm = np.ascontiguousarray(mat.device_parameters())
     Jm, w = np.ascontiguousarray(fs.Jm), np.ascontiguousarray(fs.AllGauss.ravel())
@@ -185,10 +213,10 @@ def cpu_baseline(mesh, fs, mat, x, nsample, max_threads=16):
     per_core = chunk / np.mean([d for _, d in res])
     return {"value": threads * chunk / dt, "unit": "elements/s", "cores": threads, "kind": "port",
             "per_core": round(float(per_core), 2),
-            "sample": "first {} elements of the same p=4 hex mesh in {} chunks of {} on {} host threads, each "
-                      "chunk K+T into its own CSR pattern, {:.1f} s wall; C restatement of _GlobalAssemblyDF_ "
-                      "(dense B(HB^T) per point), gcc -O3 -march=native".format(threads * chunk, threads, chunk,
-                                                                               threads, dt)}
+            "sample": "first {} elements of the same p=4 hex mesh in {} chunk(s) of {} on {} host thread(s), each "
+                      "chunk K+T into its own CSR pattern, {:.1f} s wall, {:.1f} s of CPU; C restatement of "
+                      "_GlobalAssemblyDF_ (dense B(HB^T) per point), gcc -O3 -march=native".format(
+                          threads * chunk, threads, chunk, threads, dt, sum(d for _, d in res))}
 
 
 def bench_batch(args):
@@ -378,6 +406,55 @@ def bench_learned(args):
     print(json.dumps(out), flush=True)
 
 
+WEIGHTS = os.path.join(ROOT, "florence_amd", "data", "warmstart_twist_hex_p4.npz")
+
+
+def newton_comparison(args, log):
+    """The Newton half of the metric, on a workload that fits the default run: held-out
+    members of the learned family (hex p=4 unit cubes of 2^3 cells, jitter 0.03h,
+    MR(0.5,0.5,20), 6^3 GLL points, x=0 clamped, x=1 twist U[0,60] deg + stretch
+    U[0.8,1.5], 30 increments) solved in lockstep to |R_free,i| <= 1e-8 R0_i per member.
+    Plain FP64 path: FP64 tangent and CSR, the reference's predictor start.  Fast path:
+    FP32 tangent, FP32-stored CSR, the committed learned warm start.  Both arms use the
+    SAME linear solver and the same tolerance policy, for every policy listed."""
+    import torch
+    from florence_amd.learned import LockstepNewton, TwistStretchFamily, WarmStartMLP
+    fam = TwistStretchFamily(n=2, nincr=30)
+    net = WarmStartMLP.load(WEIGHTS)
+    seeds = range(10 ** 6, 10 ** 6 + args.newton_members)
+    arms = (("plain_fp64", "fp64", "previous", "fp64"), ("mixed_learned", "fp32", "learned", "fp32"))
+    out = {"workload": "{} held-out members (seeds 1e6..) of the learned family: unit cubes 2^3 hex p=4, jitter "
+                       "0.03h, MR(0.5,0.5,20), 6^3 GLL, x=0 clamped, x=1 twist U[0,60] deg + stretch U[0.8,1.5], "
+                       "30 increments, lockstep, |R_free,i| <= 1e-8 R0_i".format(args.newton_members),
+           "plain_fp64": "FP64 tangent, FP64 CSR, reference predictor start",
+           "mixed_learned": "FP32 tangent, FP32-stored CSR, learned warm start (FP64 residual arbitrates)",
+           "policies": {}}
+    for linear, forcing in (("mg", "fixed"), ("mg", "adaptive"), ("bj", "adaptive")):
+        pol = {}
+        for name, prec, start, csr in arms:
+            ln = LockstepNewton(fam, seeds, precision=prec, forcing=forcing, csr=csr, linear=linear)
+            ln.solve(start, predictor=net)           # warm-up (first-touch, kernel attributes)
+            with no_gc():
+                r = ln.solve(start, predictor=net, record=True)
+            pol[name] = {"wall_s": round(r["wall_s"], 3), "linear_s": round(r["linear_s"], 3),
+                         "assembly_s": round(r["assembly_s"], 3), "newton_iterations_per_increment":
+                         r["lockstep_iterations"], "newton_iterations": int(sum(r["lockstep_iterations"])),
+                         "linear_iterations": int(sum(r["pcg_iterations"])), "failed": r["failed"],
+                         "final": r["trajectory"][-1]}
+            del ln
+            torch.cuda.empty_cache()
+        a, b = pol["plain_fp64"], pol["mixed_learned"]
+        fa, fb = a.pop("final"), b.pop("final")
+        diff = float(np.abs(fb - fa).max() / np.abs(fa).max())
+        key = "{}+{}".format("multigrid" if linear == "mg" else "block_jacobi", forcing)
+        pol["speedup"] = round(a["wall_s"] / b["wall_s"], 3)
+        pol["max_rel_diff_displacement"] = diff
+        out["policies"][key] = pol
+        log("newton {}: plain {:.3f}s ({} its), mixed+learned {:.3f}s ({} its), {:.2f}x".format(
+            key, a["wall_s"], a["newton_iterations"], b["wall_s"], b["newton_iterations"], pol["speedup"]))
+    return out
+
+
 def bench_newton(args):
     """The twisted-cube solve: hex p=4 unit cube n^3 (jitter 0.03h, seed 0), MR(0.5, 0.5,
     20), 6^3 Gauss-Lobatto points, x=0 clamped, x=1 twisted 30 degrees, 10 increments.
@@ -409,10 +486,14 @@ def bench_newton(args):
         for prec in ("fp64", "fp32"):
             bc = BoundaryCondition()
             bc.SetDirichletCriteria(clamp_and_twist, mesh, 30.0)
+            if args.linear == "mg":
+                from florence_amd.multigrid import MultigridSolver
+                lin = MultigridSolver(rtol=args.lin_rtol, maxit=5000)
+            else:
+                lin = LinearSolver(rtol=args.lin_rtol, maxit=500000, check_every=25)
             solver = FEMSolver(optimise=True, number_of_load_increments=10, newton_raphson_tolerance=1e-8,
-                               linear_solver=LinearSolver(rtol=args.lin_rtol, maxit=500000, check_every=25),
-                               tangent_precision=prec, csr_precision="fp32" if prec != "fp64" else "fp64",
-                               report_log_level=1)
+                               linear_solver=lin, tangent_precision=prec,
+                               csr_precision="fp32" if prec != "fp64" else "fp64", report_log_level=1)
             with no_gc():
                 res = solver.Solve(formulation=Form(), mesh=mesh, material=mat, boundary_condition=bc)
             final = res.TotalDisp[:, :, -1].cpu().numpy()
@@ -420,6 +501,7 @@ def bench_newton(args):
             key = "femsolver+" + prec
             out_modes[key] = {"wall_s": round(res.wall_time, 3), "assembly_s": round(res.assembly_time, 3),
                               "newton_iterations": list(res.newton_iterations),
+                              "linear_iterations": list(res.cg_iterations),
                               "cg_iterations_total": int(sum(res.cg_iterations)),
                               "failed": bool(res.newton_raphson_failed_to_converge),
                               "max_rel_diff_vs_fp64": float(np.abs(final - ref).max() / np.abs(ref).max())}
@@ -428,8 +510,10 @@ def bench_newton(args):
             torch.cuda.empty_cache()
     fam = TwistStretchFamily(n=args.n, nincr=10, theta_range=(30.0, 30.0), stretch_range=(1.0, 1.0))
     ref = None
-    for prec, start, forcing, csr in (("fp64", "previous", "fixed", "fp64"), ("fp32", "extrap", "adaptive", "fp32")):
-        ln = LockstepNewton(fam, [0], precision=prec, forcing=forcing, csr=csr, lin_rtol=args.lin_rtol)
+    for prec, start, forcing, csr in (("fp64", "previous", "fixed", "fp64"), ("fp64", "previous", "adaptive", "fp64"),
+                                      ("fp32", "extrap", "adaptive", "fp32")):
+        ln = LockstepNewton(fam, [0], precision=prec, forcing=forcing, csr=csr, lin_rtol=args.lin_rtol,
+                            linear=args.linear)
         log("lockstep {} {} {} csr {}: {} dofs, nnz {}".format(prec, start, forcing, csr, 3 * ln.dm.nnode, ln.dm.nnz))
         with no_gc():
             r = ln.solve(start, record=True, log=log)
@@ -439,18 +523,21 @@ def bench_newton(args):
         out_modes[key] = {"wall_s": round(r["wall_s"], 3), "linear_s": round(r["linear_s"], 3),
                           "assembly_s": round(r["assembly_s"], 3), "residual_s": round(r["residual_s"], 3),
                           "newton_iterations": r["lockstep_iterations"], "pcg_iterations_total":
-                          int(sum(r["pcg_iterations"])), "failed": r["failed"],
+                          int(sum(r["pcg_iterations"])), "linear_iterations": list(r["pcg_iterations"]),
+                          "failed": r["failed"],
                           "max_rel_diff_vs_plain": float(np.abs(final - ref).max() / np.abs(ref).max())}
         log("{}: {}".format(key, out_modes[key]))
         del ln
         torch.cuda.empty_cache()
-    plain, fast = out_modes["fp64+previous+fixed"]["wall_s"], out_modes["fp32+extrap+adaptive+csr32"]["wall_s"]
+    plain, fast = out_modes["fp64+previous+adaptive"]["wall_s"], out_modes["fp32+extrap+adaptive+csr32"]["wall_s"]
     out = {"metric": "Newton wall time to FP64 residual 1e-8 (twisted hex p=4 cube)", "value": fast, "unit": "s",
            "n_gpus": 1, "steps": 1, "warmup": 0, "higher_is_better": False, "vs_baseline": None, "dtype": "fp32",
            "data": "synthetic", "speedup_vs_plain_fp64": round(plain / fast, 3),
            "config": {"workload": "unit cube {0}^3 hex p=4 (jitter 0.03h), 6^3 GLL, MR(0.5,0.5,20), x=0 clamped, "
-                                  "x=1 twisted 30 deg, 10 increments, block-Jacobi PCG".format(args.n),
-                      "lin_rtol": args.lin_rtol},
+                                  "x=1 twisted 30 deg, 10 increments, {1}".format(
+                                      args.n, "p-multigrid flexible CG" if args.linear == "mg" else
+                                      "block-Jacobi PCG"),
+                      "lin_rtol": args.lin_rtol, "linear": args.linear},
            "modes": out_modes}
     print(json.dumps(out), flush=True)
 
@@ -487,8 +574,12 @@ def main():
     ap.add_argument("--save-weights", default="")
     ap.add_argument("--lin-rtol", type=float, default=1e-10, help="newton: PCG relative tolerance")
     ap.add_argument("--femsolver", action="store_true", help="newton: also run FEMSolver (reference semantics)")
+    ap.add_argument("--linear", default="mg", choices=["mg", "bj"],
+                    help="newton: linear solver of the Newton steps (p-multigrid or block-Jacobi CG)")
     ap.add_argument("--copy-mib", type=int, default=0, help="assembly: run one D2D copy of this size first "
                                                              "(calibrates PMC byte counters)")
+    ap.add_argument("--newton-members", type=int, default=16,
+                    help="assembly: held-out members of the Newton comparison in the default line (0 = skip)")
     ap.add_argument("--nmesh", type=int, default=256)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16"])
@@ -571,12 +662,22 @@ def main():
     flops_exec = (sf_flops_per_element(p1, q1) if sf else stiffness_flops_per_element(dm.npe, dm.ngauss)) * dm.nelem
     achieved = flops_exec / (phase_ms[1] / 1e3) / 1e12
     dense_b = dense_b_flops_per_element(dm.npe, dm.ngauss) * dm.nelem
-    stiff_roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                  "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-                  "traffic": measured_traffic("assembly_hex_p4_24cube_pmc.md", "stiffness"),
+    # the dominant kernel against the assembly's compulsory bytes: the dense-B flop
+    # count cannot bound a sum-factorised kernel (it executes 37x fewer flops), so the
+    # fraction is taken on bytes and the executed-flop rate is kept beside it
+    cbytes = compulsory_bytes(dm.nelem, dm.npe, dm.nnode, dm.nnz)
+    cgbs = cbytes / (phase_ms[1] / 1e3) / 1e9
+    stiff_roof = {"bound": "hbm", "achieved": round(cgbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                  "frac": round(cgbs / HBM_PEAK_GBS, 4),
+                  "traffic": measured_traffic(PMC_SUMMARY, "stiffness"), "traffic_source": PMC_SOURCE,
+                  "algorithmic_bytes_per_assembly": cbytes,
                   "kernel": (DIRECT_KERNEL if direct else SF_KERNELS[args.sf_form] if sf else
                              "stiffness_kernel<128,4,4,2,2> (off-diagonal + triangular diagonal launches)"),
-                  "flops_per_launch": flops_exec, "kernel_ms": round(float(phase_ms[1]), 4),
+                  "kernel_ms": round(float(phase_ms[1]), 4),
+                  "mfma_view": {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
+                                "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+                                "flops_per_assembly": flops_exec,
+                                "flops": "executed (sum-factorised count)" if sf else "executed (dense blocks)"},
                   "dense_b_equivalent_tflops": round(dense_b / (phase_ms[1] / 1e3) / 1e12, 3),
                   "dense_block_equivalent_tflops": round(stiffness_flops_per_element(dm.npe, dm.ngauss) * dm.nelem
                                                          / (phase_ms[1] / 1e3) / 1e12, 3)}
@@ -585,8 +686,10 @@ def main():
         stiff_roof["launches_per_assembly"] = dm.ncolour
         sbytes = direct_scatter_bytes(dm.nelem, dm.npe, dm.ngauss, dm.nnz, dm.scatter.numel() * 2)
         sgbs = sbytes / (phase_ms[1] / 1e3) / 1e9
-        stiff_roof["hbm_view"] = {"bound": "hbm", "achieved": round(sgbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                  "frac": round(sgbs / HBM_PEAK_GBS, 4), "bytes_per_assembly": sbytes}
+        stiff_roof["scatter_view"] = {"bound": "hbm", "achieved": round(sgbs, 1), "peak": HBM_PEAK_GBS,
+                                      "unit": "GB/s", "frac": round(sgbs / HBM_PEAK_GBS, 4),
+                                      "bytes_per_assembly": sbytes,
+                                      "bytes": "C fields, scatter table, V written, add-loads of later colours"}
     # the default factorised kernel writes element-interior rows itself (nodes of one
     # element, the lowest gather class when its degree equals npe)
     fused = sf and not direct and args.sf_form not in (3, 5) and dm.bucket_degree[0] == dm.npe
@@ -661,6 +764,9 @@ def main():
     cpu = None
     if args.cpu_sample > 0:
         cpu = cpu_baseline(mesh, fs, mat, x_host, args.cpu_sample)
+    del dm, x, T, Tres
+    torch.cuda.empty_cache()
+    newton = newton_comparison(args, lambda m: print(m, file=sys.stderr, flush=True)) if args.newton_members else None
 
     out = {
         "metric": "K+residual assembly elements/s (DF, hex p=4, FP64)",
@@ -687,6 +793,7 @@ def main():
         "scatter": "direct (colour-ordered, {} colours)".format(dm.ncolour) if direct else "element blocks + gather",
         "other_data_path": other_path,
         "residual_only": residual_only,
+        "newton": newton,
         "status": status,
     }
     print(json.dumps(out), flush=True)

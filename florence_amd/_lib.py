@@ -32,6 +32,8 @@ PRECISIONS = {"fp64": FL_PREC_FP64, "fp32": FL_PREC_FP32, "bf16": FL_PREC_BF16}
 FL_PREC_CSR32 = 16
 FL_TUNE_STIFF_SF = 7
 FL_TUNE_GATHER = 8
+FL_RECORDS_D = 0
+FL_RECORDS_C = 1
 
 # every symbol declared in include/florence_amd.h -> (restype, argtypes)
 _vp, _i32, _i64, _sz, _dp = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_size_t, ctypes.c_void_p
@@ -75,6 +77,15 @@ SIGNATURES = {
     "fl_df_residual_mt": (ctypes.c_int, [_i32, _PD, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i64, _i32, _i64,
                                          _vp, _vp, _vp, _vp, _vp, _sz, _vp, _vp]),
     "fl_material_table_stride": (_i32, [_i32]),
+    "fl_df_records_form": (_i32, [_i32, _vp, _i32, _i32]),
+    "fl_mg_galerkin_workspace_size": (_sz, [_i64, _i32, _i64]),
+    "fl_mg_galerkin": (ctypes.c_int, [_vp, _sz, _i32, _i64, _i32, _i32, _vp, _i32, _vp, _i64, _vp, _vp, _vp, _vp,
+                                      _vp, _P32, _P32, _vp, _vp, _vp, _vp, _sz, _vp, _vp]),
+    "fl_mg_workspace_size": (_sz, [_vp, _i32]),
+    "fl_mg_pcg": (ctypes.c_int, [_vp, _i32, _i32, _vp, _vp, ctypes.c_double, _i32, _i32, _i32, ctypes.c_double,
+                                 _i32, _vp, _sz, _P32, ctypes.POINTER(ctypes.c_double), _vp]),
+    "fl_mesh_refine_workspace_size": (_sz, [_i64, _i64, _i32, _i32]),
+    "fl_mesh_refine": (ctypes.c_int, [_vp, _i64, _vp, _i64, _i32, _vp, _i32, _i32, _vp, _vp, _P64, _vp, _sz, _vp]),
     "fl_csr_spmv": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp]),
     "fl_pcg_workspace_size": (_sz, [_i64]),
     "fl_pcg": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, ctypes.c_double, _i32, _i32, _vp, _sz,

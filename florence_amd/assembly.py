@@ -96,6 +96,8 @@ class DeviceMeshData(object):
         self.nelem, self.npe = int(el.shape[0]), int(el.shape[1])
         self.nnode = int(np.asarray(mesh.points).shape[0])
         self.ngauss = int(function_space.AllGauss.shape[0])
+        self.element_type = getattr(mesh, "element_type", None)
+        self.function_space = function_space
         self.points = torch.as_tensor(np.ascontiguousarray(mesh.points, dtype=np.float64), device=self.device)
         self.elements = torch.as_tensor(np.ascontiguousarray(el, dtype=np.int32), device=self.device)
         self.Jm = torch.as_tensor(np.ascontiguousarray(function_space.Jm, dtype=np.float64), device=self.device)

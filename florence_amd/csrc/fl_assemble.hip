@@ -428,6 +428,7 @@ __global__ __launch_bounds__(256) void qp_kernel(MatParams mp0, const double *__
 #include "fl_stiffness.hpp"
 #include "fl_stiffness_bf16.hpp"
 #include "fl_stiffness_sf.hpp"
+#include "fl_mg_galerkin.hpp"
 namespace fl {
 
 // ------------------------------------------------------------------ phase 3
@@ -1021,14 +1022,16 @@ struct DFWork {
 // bytes per element-block entry for each tangent precision
 static size_t ke_bytes(int precision) { return precision == FL_PREC_FP64 ? sizeof(double) : sizeof(float); }
 
-// blocks = false: no element-block buffer and tables (residual-only pass, direct scatter)
+// blocks = false: no element-block buffer and tables (residual-only pass, direct scatter);
+// records = false: no point records (the factorised residual pass contracts the
+// tractions inside its point kernel)
 static DFWork carve_df(void *ws, size_t bytes, int G, int64_t nelem, int npe, bool tangent, size_t *need,
-                       int precision = FL_PREC_FP64, bool blocks = true) {
+                       int precision = FL_PREC_FP64, bool blocks = true, bool records = true) {
     const int NP = padded_np(npe);
     Carve cv(ws, bytes);
     DFWork w;
     w.JmT = cv.take<double>((size_t)G * 3 * NP);
-    w.qp = cv.take<double>((size_t)nelem * G * (tangent ? QS : QS_RES));
+    w.qp = records ? cv.take<double>((size_t)nelem * G * (tangent ? QS : QS_RES)) : nullptr;
     w.te = cv.take<double>((size_t)nelem * 3 * npe);
     blocks = blocks && tangent;
     w.Ke = blocks ? (void *)cv.take<char>((size_t)nelem * 9 * npe * npe * ke_bytes(precision)) : nullptr;
@@ -1351,6 +1354,12 @@ int fl_df_assemble_cs(int32_t material, const double *params, const int32_t *ele
                        status, stream);
 }
 
+// the residual pass takes the factorised point kernel (tractions contracted in it, no
+// point records) for tensor-product hexes given their basis table
+static bool residual_sf(const void *tensor_basis, int npe, int G) {
+    return tensor_basis != nullptr && sf_shape(npe, G) && tuning_value(TUNE_STIFF_SF, 0) != 1;
+}
+
 static int df_residual(int32_t material, const double *params, const int32_t *elem_mat, const double *mat_table,
                        const double *points, const int32_t *elements, const double *Eulerx, const double *Jm,
                        const double *AllGauss, int32_t G, int64_t nelem, int32_t npe, int64_t nnode,
@@ -1363,15 +1372,15 @@ static int df_residual(int32_t material, const double *params, const int32_t *el
         set_error("fl_df_residual: null pointer");
         return FL_E_ARG;
     }
+    const bool sf_ok = residual_sf(tensor_basis, npe, G);
     size_t need = 0;
-    DFWork w = carve_df(workspace, workspace_bytes, G, nelem, npe, false, &need);
+    DFWork w = carve_df(workspace, workspace_bytes, G, nelem, npe, false, &need, FL_PREC_FP64, false, !sf_ok);
     if (!w.JmT) {
         set_error("fl_df_residual: workspace too small (%zu < %zu)", workspace_bytes, need);
         return FL_E_SPACE;
     }
     const int NP = padded_np(npe);
     const MatParams mp = mat_params(material, params);
-    const bool sf_ok = tensor_basis != nullptr && sf_shape(npe, G) && tuning_value(TUNE_STIFF_SF, 0) != 1;
     prof_begin(st, PROF_DF_RESIDUAL);
     clear_status_kernel<<<1, 64, 0, st>>>(status);
     if (sf_ok) {
@@ -1432,6 +1441,99 @@ int fl_df_residual_mt(int32_t material, const double *params, const int32_t *ele
                       void *workspace, size_t workspace_bytes, int32_t *status, void *stream) {
     return df_residual(material, params, elem_mat, mat_table, points, elements, Eulerx, Jm, AllGauss, G, nelem, npe,
                        nnode, node_elem_ptr, node_elem, T, tensor_basis, workspace, workspace_bytes, status, stream);
+}
+
+// the form of the point records fl_df_assemble_* leaves in its workspace: the C tensors
+// of the factorised FP64 path or dxi/dx and D of the dense path
+static int records_form(int32_t precision, const void *tensor_basis, int npe, int G) {
+    precision &= ~FL_PREC_CSR32;
+    return precision == FL_PREC_FP64 && tensor_basis != nullptr && sf_shape(npe, G) &&
+                   tuning_value(TUNE_STIFF_SF, 0) != 1
+               ? FL_RECORDS_C
+               : FL_RECORDS_D;
+}
+
+int32_t fl_df_records_form(int32_t precision, const void *tensor_basis, int32_t npe, int32_t ngauss) {
+    return records_form(precision, tensor_basis, npe, ngauss);
+}
+
+size_t fl_mg_galerkin_workspace_size(int64_t nelem, int32_t npe_c, int64_t nnode_c) {
+    if (nelem <= 0 || npe_c <= 0 || nnode_c <= 0) return 0;
+    Carve cv(nullptr, 0);
+    cv.take<double>((size_t)nelem * 9 * npe_c * npe_c);
+    cv.take<double>((size_t)nelem * 3 * npe_c);
+    cv.take<double>((size_t)3 * nnode_c);
+    return cv.off + 256;
+}
+
+int fl_mg_galerkin(const void *df_workspace, size_t df_workspace_bytes, int32_t G, int64_t nelem, int32_t npe,
+                   int32_t records, const double *JmCt, int32_t npe_c, const int32_t *elements_c, int64_t nnode_c,
+                   const int32_t *node_elem_ptr_c, const int32_t *node_elem_c, const int32_t *node_nbr_ptr_c,
+                   const int32_t *node_nbrs_c, const int32_t *node_order_c, const int32_t *bucket_offsets_c,
+                   const int32_t *bucket_degree_c, const int32_t *indptr_c, const int16_t *ranks_c, double *V_c,
+                   void *workspace, size_t workspace_bytes, int32_t *status, void *stream) {
+    hipStream_t st = as_stream(stream);
+    if (!df_workspace || !JmCt || !elements_c || !node_elem_ptr_c || !node_elem_c || !node_nbr_ptr_c ||
+        !node_nbrs_c || !node_order_c || !bucket_offsets_c || !bucket_degree_c || !indptr_c || !V_c || !status ||
+        G <= 0 || nelem <= 0 || nnode_c <= 0 || padded_np(npe) < 0 ||
+        (records != FL_RECORDS_C && records != FL_RECORDS_D)) {
+        set_error("fl_mg_galerkin: invalid argument");
+        return FL_E_ARG;
+    }
+    if (npe_c != 4 && npe_c != 8 && npe_c != 27) {
+        set_error("fl_mg_galerkin: coarse elements of %d nodes are not supported (4, 8 or 27)", npe_c);
+        return FL_E_ARG;
+    }
+    if (bucket_offsets_c[0] != 0 || bucket_offsets_c[FL_GATHER_BUCKETS] != nnode_c) {
+        set_error("fl_mg_galerkin: gather plan does not cover the %lld coarse nodes", (long long)nnode_c);
+        return FL_E_ARG;
+    }
+    size_t need = 0;
+    DFWork fw = carve_df(const_cast<void *>(df_workspace), df_workspace_bytes, G, nelem, npe, true, &need,
+                         FL_PREC_FP64, false);
+    if (!fw.JmT) {
+        set_error("fl_mg_galerkin: the fine workspace (%zu bytes) holds no point records for these sizes",
+                  df_workspace_bytes);
+        return FL_E_SPACE;
+    }
+    Carve cv(workspace, workspace_bytes);
+    double *Kc = cv.take<double>((size_t)nelem * 9 * npe_c * npe_c);
+    double *te = cv.take<double>((size_t)nelem * 3 * npe_c);
+    double *Tdum = cv.take<double>((size_t)3 * nnode_c);
+    if (!cv.ok) {
+        set_error("fl_mg_galerkin: workspace too small (%zu < %zu)", workspace_bytes, cv.off + 256);
+        return FL_E_SPACE;
+    }
+    for (int b = 0; b < FL_GATHER_BUCKETS; ++b) {
+        if (bucket_offsets_c[b + 1] > bucket_offsets_c[b] &&
+            gather_lds_bytes(gather_form(b, bucket_degree_c[b], npe_c), bucket_degree_c[b], npe_c) > 160 * 1024) {
+            set_error("fl_mg_galerkin: coarse node degree %d needs more LDS than a CU has", bucket_degree_c[b]);
+            return FL_E_LIMIT;
+        }
+    }
+    clear_status_kernel<<<1, 64, 0, st>>>(status);
+    const int64_t nte = nelem * 3 * npe_c;
+    zero_kernel<<<(unsigned)((nte + 255) / 256), 256, 0, st>>>(te, nte);
+    FL_LAUNCH_CHECK();
+    const int form_c = records == FL_RECORDS_C ? 1 : 0;
+    if (npe_c == 4)
+        mg_galerkin_kernel<4><<<(unsigned)nelem, 256, 0, st>>>(fw.qp, G, form_c, JmCt, nelem, Kc);
+    else if (npe_c == 8)
+        mg_galerkin_kernel<8><<<(unsigned)nelem, 256, 0, st>>>(fw.qp, G, form_c, JmCt, nelem, Kc);
+    else
+        mg_galerkin_kernel<27><<<(unsigned)nelem, 256, 0, st>>>(fw.qp, G, form_c, JmCt, nelem, Kc);
+    FL_LAUNCH_CHECK();
+    // the coarse CSR: owner-computes rows summed in ascending element order, as the fine gather
+    for (int b = 0; b < FL_GATHER_BUCKETS; ++b) {
+        const int32_t nb = bucket_offsets_c[b + 1] - bucket_offsets_c[b];
+        if (nb <= 0) continue;
+        const int rc = launch_gather<double, double>(b, nb, elements_c, npe_c, node_order_c + bucket_offsets_c[b],
+                                                     node_elem_ptr_c, node_elem_c, node_nbr_ptr_c, node_nbrs_c,
+                                                     indptr_c, Kc, te, bucket_degree_c[b], ranks_c, nullptr, nullptr,
+                                                     nullptr, V_c, Tdum, status, st);
+        if (rc) return rc;
+    }
+    return 0;
 }
 
 int32_t fl_material_table_stride(int32_t material) {

@@ -32,7 +32,8 @@ Each Newton iteration of the warm-started modes assembles K and T at the current
 state, solves, and evaluates the residual-only FP64 kernel at the new state; K is
 assembled again only if another step is needed.
 
-Linear solves: ``forcing="fixed"`` solves every step to ``lin_rtol`` (the stand-in for
+Linear solves (``linear="bj"``: block-Jacobi PCG; ``"mg"``: the p-multigrid flexible CG
+of multigrid.py): ``forcing="fixed"`` solves every step to ``lin_rtol`` (the stand-in for
 the reference's direct solve); ``forcing="adaptive"`` is an inexact Newton method
 whose PCG tolerance per step follows the model |R_next| ~ max(C |R|^2, eta |R|), C
 measured on the first (tightly solved) step and refined by every later one (see
@@ -93,7 +94,7 @@ class LockstepNewton(object):
     """Incremental Newton of many family members as one device problem."""
 
     def __init__(self, family, seeds, precision="fp64", tol=1e-8, lin_rtol=1e-10, maxit=25, forcing="fixed",
-                 eta_max=1e-2, csr="fp64", device="cuda"):
+                 eta_max=1e-2, csr="fp64", device="cuda", linear="bj"):
         import torch
         if forcing not in ("fixed", "adaptive"):
             raise ValueError(forcing)
@@ -118,7 +119,14 @@ class LockstepNewton(object):
         self.X = self.dm.points.reshape(-1)
         self.V = torch.empty(self.dm.nnz, dtype=torch.float32 if csr == "fp32" else torch.float64, device=dev)
         self.T = torch.empty(3 * self.dm.nnode, dtype=torch.float64, device=dev)
-        self.lin = LinearSolver(rtol=lin_rtol, maxit=50000, check_every=10)
+        # "bj": block-Jacobi PCG; "mg": p-multigrid flexible CG (its coarse operators come
+        # from the point records of the assembly that produced V)
+        if linear == "mg":
+            from .multigrid import MultigridSolver
+            self.lin = MultigridSolver(rtol=lin_rtol, maxit=2000)
+        else:
+            self.lin = LinearSolver(rtol=lin_rtol, maxit=50000, check_every=10)
+        self.linear = linear
 
     def _norms(self, R):
         import torch
@@ -146,7 +154,7 @@ class LockstepNewton(object):
         if done is not None and done.any():
             live = torch.as_tensor(~done, device=self.free.device)
             mask = self.free * self._per_dof(live).to(self.free.dtype)
-        self.lin.Solve(self.dm, self.V, mask, rhs, du)
+        self.lin.Solve(self.dm, self.V, mask, rhs, du, precision=self.precision)
 
     def _eta(self, r, C):
         """Linear tolerance of the next step at (worst member's) relative residual r,

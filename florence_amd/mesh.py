@@ -13,12 +13,15 @@ Generators restated for the workloads of this path:
 * ``ConvertHexesToTets`` -- the 6-tet split of each linear hex sharing the 0-7
   diagonal, all first tets first (Mesh.py:8757-8802, NodeArrangement.py:696-726);
 * ``GetHighOrderMesh`` -- p-refinement placing GLL nodes in hexes and Fekete nodes in
-  tets through the linear map of each element (HigherOrderMeshingHex.py:19-120,
-  HigherOrderMeshingTet.py:19-140).  Shared nodes are merged on coordinates rounded
-  to 10 decimals as in the reference; vertices keep their linear-mesh numbers and
-  new nodes are numbered in order of first appearance (element by element, local
-  node order).  The reference numbers its merged nodes differently; the assembled
-  operator is a function of the connectivity handed to it, not of the numbering.
+  tets through the linear map of each element (HigherOrderMeshingHex.py:19-133,
+  HigherOrderMeshingTet.py:19-133).  ``numbering="reference"`` runs the device
+  generator (libflorence_amd fl_mesh_refine), which numbers the nodes as the
+  reference does (rank of the rounded coordinates in numpy's raw-byte order).  The
+  default ``numbering="first"`` (host) merges nodes on the same rounded coordinates
+  but keeps the vertices' linear numbers and numbers new nodes in order of first
+  appearance (element by element, local node order): the same mesh up to a node
+  permutation, with element-coherent node ids (the reference's order scatters an
+  element's nodes over the whole id range, which costs the CSR rows their locality).
 * ``Cylinder`` -- a structured hex/tet discretisation of a cylinder along x used by
   the batched workloads.
 """
@@ -136,9 +139,11 @@ class Mesh(object):
         self.degree = 1
         self.nelem = self.elements.shape[0]
 
-    def GetHighOrderMesh(self, p=1, silent=True, Decimals=10, **kwargs):
+    def GetHighOrderMesh(self, p=1, silent=True, Decimals=10, numbering="first", **kwargs):
         if not isinstance(p, int) or p < 1:
             raise ValueError("Value of p={} is not acceptable. Provide p>=1.".format(p))
+        if numbering not in ("first", "reference"):
+            raise ValueError("numbering is 'first' or 'reference'")
         if self.degree is None:
             self.InferPolynomialDegree()
         if self.degree == p:
@@ -148,18 +153,14 @@ class Mesh(object):
         C = p - 1
         el = np.asarray(self.elements, dtype=np.int64)
         X = np.asarray(self.points, dtype=np.float64)
-        if self.element_type == "hex":
-            eps = GaussLobattoPointsHex(C)
-            Neval = LinearHexBases(eps)
-            nv = 8
-        elif self.element_type == "tet":
-            eps = FeketePointsTet(C)
-            Neval = LinearTetBases(eps)
-            nv = 4
-        else:
-            raise NotImplementedError(self.element_type)
-        Neval[np.abs(Neval) < 1e-12] = 0.0
-        npe = eps.shape[0]
+        Neval, nv = self._linear_bases_at_nodes(C)
+        npe = Neval.shape[1]
+        if numbering == "reference":
+            from .meshgen import refine_on_device
+            self.points, self.elements = refine_on_device(X, el, Neval, Decimals)
+            self.degree = p
+            self.ChangeType()
+            return
         nelem = el.shape[0]
         # physical coordinates of the non-vertex nodes of every element
         vert = X[el]                                               # (nelem, nv, 3)
@@ -184,6 +185,19 @@ class Mesh(object):
         self.points, self.elements = points, elements
         self.degree = p
         self.ChangeType()
+
+    def _linear_bases_at_nodes(self, C):
+        """Neval (nv, npe): the linear basis at the degree C+1 nodes, values below 1e-12
+        set to 0 (HigherOrderMeshingHex.py:30-46, HigherOrderMeshingTet.py:32-49)."""
+        if self.element_type == "hex":
+            Neval, nv = LinearHexBases(GaussLobattoPointsHex(C)), 8
+        elif self.element_type == "tet":
+            Neval, nv = LinearTetBases(FeketePointsTet(C)), 4
+        else:
+            raise NotImplementedError(self.element_type)
+        Neval = np.array(Neval, dtype=np.float64)
+        Neval[np.abs(Neval) < 1e-12] = 0.0
+        return Neval, nv
 
     # ------------------------------------------------------------------ helpers
     def JitterInteriorNodes(self, amplitude, seed=0, lower=None, upper=None, tol=1e-12):

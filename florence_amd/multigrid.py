@@ -1,0 +1,315 @@
+"""p-multigrid linear solver of the Newton steps (no reference counterpart: the
+reference solves every step with a sparse direct solver, Solver/LinearSolver.py:208-300).
+
+``MultigridSolver`` has the ``Solve(dm, V, free_mask, b, x)`` interface of
+``solver.LinearSolver`` and runs ``fl_mg_pcg``: flexible CG on the masked assembled
+tangent, preconditioned by one V-cycle over nested nodal spaces on the mesh's own
+elements:
+
+* hex p=4 -> p=2 lattice -> vertices; hex p=2, p=3 -> vertices; tet p>=2 -> vertices;
+* coarse operators are Galerkin products P^T K P, formed on the device from the point
+  records of the assembly that produced V (``fl_mg_galerkin``), so each Newton step's
+  coarse operators follow its tangent;
+* transfers are the nodal interpolation of the coarse spaces (a coarse basis function
+  is a fine-space function), built here once per mesh as per-node CSR lists.
+
+Host set-up (per mesh): the coarse nodes are the fine nodes at the coarse lattice
+positions of each element (found from the local nodes' reference coordinates), their
+basis gradients at the fine quadrature points and the transfer weights come from a
+Vandermonde solve in the monomial basis of the coarse space, and the coarse
+patterns are built by the library's fl_pattern_* calls.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .bases import FeketePointsTet, GaussLobattoPointsHex, HexBases, TetBases
+
+__all__ = ["MultigridSolver", "PMultigridHierarchy", "hierarchy_tables", "local_node_coordinates", "coarse_degrees"]
+
+
+class FlMgLevel(ctypes.Structure):
+    _fields_ = [("nnode", ctypes.c_int64), ("indptr", ctypes.c_void_p), ("node_nbr_ptr", ctypes.c_void_p),
+                ("node_nbrs", ctypes.c_void_p), ("V", ctypes.c_void_p), ("free_mask", ctypes.c_void_p),
+                ("P_ptr", ctypes.c_void_p), ("P_idx", ctypes.c_void_p), ("P_w", ctypes.c_void_p),
+                ("R_ptr", ctypes.c_void_p), ("R_idx", ctypes.c_void_p), ("R_w", ctypes.c_void_p)]
+
+
+def local_node_coordinates(element_type, npe):
+    """Reference coordinates (npe, 3) of the local nodes, checked against the bases
+    (N_a(xi_b) = delta_ab)."""
+    if element_type == "hex":
+        p1 = int(round(npe ** (1.0 / 3.0)))
+        C = p1 - 2
+        xi = GaussLobattoPointsHex(C)
+        N, _ = HexBases(C, xi)
+    elif element_type == "tet":
+        p = {4: 1, 10: 2, 20: 3, 35: 4}[npe]
+        C = p - 1
+        xi = FeketePointsTet(C)
+        N, _ = TetBases(C, xi)
+    else:
+        raise NotImplementedError(element_type)
+    if np.abs(N - np.eye(npe)).max() > 1e-10:
+        raise _lib.FlorenceAMDError("local node coordinates do not match the bases")
+    return np.asarray(xi, dtype=np.float64)
+
+
+def coarse_degrees(element_type, p):
+    """Degrees of the coarse levels below a fine degree p."""
+    if element_type == "hex":
+        return {4: [2, 1], 3: [1], 2: [1]}.get(p, [])
+    return [1] if p >= 2 else []
+
+
+def _exponents(element_type, p):
+    if element_type == "hex":
+        return [(i, j, k) for k in range(p + 1) for j in range(p + 1) for i in range(p + 1)]
+    return [(i, j, k) for s in range(p + 1) for i in range(s + 1) for j in range(s + 1 - i) for k in [s - i - j]]
+
+
+def _monomials(ex, z):
+    """Values and gradients (3, npts, nm) of the monomials x^i y^j z^k at points z."""
+    z = np.asarray(z, dtype=np.float64)
+    V = np.stack([z[:, 0] ** i * z[:, 1] ** j * z[:, 2] ** k for i, j, k in ex], axis=1)
+    pw = lambda x, n: n * x ** (n - 1) if n > 0 else np.zeros_like(x)
+    G = np.stack([np.stack([pw(z[:, 0], i) * z[:, 1] ** j * z[:, 2] ** k for i, j, k in ex], axis=1),
+                  np.stack([z[:, 0] ** i * pw(z[:, 1], j) * z[:, 2] ** k for i, j, k in ex], axis=1),
+                  np.stack([z[:, 0] ** i * z[:, 1] ** j * pw(z[:, 2], k) for i, j, k in ex], axis=1)])
+    return V, G
+
+
+def coarse_basis(element_type, pc, nodes, points):
+    """Lagrange basis of degree pc through ``nodes`` (nc, 3): values (npts, nc) and
+    gradients (3, npts, nc) at ``points``."""
+    ex = _exponents(element_type, pc)
+    if len(ex) != nodes.shape[0]:
+        raise _lib.FlorenceAMDError("coarse space of degree {} has {} functions for {} nodes".format(
+            pc, len(ex), nodes.shape[0]))
+    Vn, _ = _monomials(ex, nodes)
+    coef = np.linalg.inv(Vn)                     # N_a = sum_m coef[m, a] phi_m
+    V, G = _monomials(ex, points)
+    return V @ coef, np.einsum("dpm,ma->dpa", G, coef)
+
+
+def _coarse_locals(element_type, pc, xi):
+    """Local fine nodes lying on the degree-pc lattice of the reference element."""
+    if element_type == "hex":
+        if pc not in (1, 2):
+            raise NotImplementedError("hexahedral coarse levels are the vertices or the p=2 lattice")
+        lat = np.array([-1.0, 1.0]) if pc == 1 else np.array([-1.0, 0.0, 1.0])
+        on = np.all(np.min(np.abs(xi[:, :, None] - lat[None, None, :]), axis=2) < 1e-12, axis=1)
+    else:
+        verts = np.array([[-1, -1, -1], [1, -1, -1], [-1, 1, -1], [-1, -1, 1.0]])
+        if pc != 1:
+            raise NotImplementedError("tetrahedral coarse levels are the vertices")
+        on = np.any(np.all(np.abs(xi[:, None, :] - verts[None, :, :]) < 1e-12, axis=2), axis=1)
+    return np.nonzero(on)[0]
+
+
+def _csr_from_first(elements, weights, coarse_elements, n_fine):
+    """Prolongation CSR over fine nodes: fine node f takes the weights of the first
+    element containing it, W[local, :] over that element's coarse nodes."""
+    flat = elements.ravel()
+    _, first = np.unique(flat, return_index=True)
+    if first.size != n_fine:
+        raise _lib.FlorenceAMDError("a node belongs to no element")
+    e, l = np.divmod(first, elements.shape[1])
+    W = weights[l]                               # (n_fine, nc)
+    cols = coarse_elements[e]                    # (n_fine, nc)
+    keep = np.abs(W) > 1e-14
+    counts = keep.sum(axis=1)
+    ptr = np.zeros(n_fine + 1, dtype=np.int64)
+    np.cumsum(counts, out=ptr[1:])
+    return ptr, cols[keep].astype(np.int64), W[keep]
+
+
+def _transpose(ptr, idx, w, n_coarse):
+    rows = np.repeat(np.arange(ptr.size - 1), np.diff(ptr))
+    order = np.lexsort((rows, idx))              # by coarse node, then fine node
+    cidx = idx[order]
+    rptr = np.zeros(n_coarse + 1, dtype=np.int64)
+    np.cumsum(np.bincount(cidx, minlength=n_coarse), out=rptr[1:])
+    return rptr, rows[order], w[order]
+
+
+class _Level(object):
+    """A coarse level: connectivity, node ids in the fine mesh, device pattern and
+    transfer tables to the next finer level."""
+
+    def __init__(self, device, degree, elements, fine_ids, JmCt, P, R):
+        import torch
+        lib = _lib.load()
+        i32 = torch.int32
+        self.degree = degree
+        self.nelem, self.npe = elements.shape
+        self.nnode = int(fine_ids.size)
+        self.fine_ids = torch.as_tensor(fine_ids.astype(np.int64), device=device)
+        self.elements = torch.as_tensor(np.ascontiguousarray(elements, dtype=np.int32), device=device)
+        self.JmCt = torch.as_tensor(np.ascontiguousarray(JmCt), device=device)
+        as32 = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.int32), device=device)
+        asf = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device=device)
+        self.P_ptr, self.P_idx, self.P_w = as32(P[0]), as32(P[1]), asf(P[2])
+        self.R_ptr, self.R_idx, self.R_w = as32(R[0]), as32(R[1]), asf(R[2])
+        n, ne, npe = self.nnode, self.nelem, self.npe
+        self.node_elem_ptr = torch.empty(n + 1, dtype=i32, device=device)
+        self.node_elem = torch.empty(ne * npe, dtype=i32, device=device)
+        self.node_nbr_ptr = torch.empty(n + 1, dtype=i32, device=device)
+        ws = torch.empty(int(lib.fl_pattern_workspace_size(n)), dtype=torch.uint8, device=device)
+        nnz = ctypes.c_int64(0)
+        maxdeg = ctypes.c_int32(0)
+        P_ = _lib.ptr
+        _lib.check(lib.fl_pattern_graph(P_(self.elements), ne, npe, n, P_(self.node_elem_ptr), P_(self.node_elem),
+                                        P_(self.node_nbr_ptr), ctypes.byref(nnz), ctypes.byref(maxdeg), P_(ws),
+                                        ws.numel(), _lib.stream_ptr()), "fl_pattern_graph (coarse level)")
+        del ws
+        self.node_nnz = int(nnz.value)
+        self.node_nbrs = torch.empty(self.node_nnz, dtype=i32, device=device)
+        self.indptr = torch.empty(3 * n + 1, dtype=i32, device=device)
+        _lib.check(lib.fl_pattern_fill(P_(self.elements), ne, npe, n, P_(self.node_elem_ptr), P_(self.node_elem),
+                                       P_(self.node_nbr_ptr), P_(self.node_nbrs), P_(self.indptr), None,
+                                       _lib.stream_ptr()), "fl_pattern_fill (coarse level)")
+        nb = _lib.FL_GATHER_BUCKETS
+        self.node_order = torch.empty(n, dtype=i32, device=device)
+        self.bucket_offsets = (ctypes.c_int32 * (nb + 1))()
+        self.bucket_degree = (ctypes.c_int32 * nb)()
+        _lib.check(lib.fl_pattern_gather_plan(P_(self.node_nbr_ptr), n, P_(self.node_order), self.bucket_offsets,
+                                              self.bucket_degree, _lib.stream_ptr()), "fl_pattern_gather_plan")
+        self.status = torch.zeros(2, dtype=i32, device=device)
+        self.ranks = torch.empty(int(lib.fl_pattern_rank_table_bytes(ne, npe)) // 2, dtype=torch.int16,
+                                 device=device)
+        _lib.check(lib.fl_pattern_rank_table(P_(self.elements), ne, npe, P_(self.node_nbr_ptr), P_(self.node_nbrs),
+                                             P_(self.ranks), P_(self.status), _lib.stream_ptr()),
+                   "fl_pattern_rank_table (coarse level)")
+        self.V = torch.empty(9 * self.node_nnz, dtype=torch.float64, device=device)
+        self.ws = torch.empty(int(lib.fl_mg_galerkin_workspace_size(ne, npe, n)), dtype=torch.uint8, device=device)
+        import torch
+        self.dofs = (3 * self.fine_ids[:, None] + torch.arange(3, device=device)[None, :]).reshape(-1)
+        self.free = None
+
+    def set_free(self, free_fine):
+        """Coarse free dofs: those of the same node in the fine mesh (formed on every
+        solve: the caller's mask may change between solves)."""
+        self.free = free_fine[self.dofs].contiguous()
+        return self.free
+
+
+def hierarchy_tables(element_type, elements, nnode, quadrature_points):
+    """Host tables of the coarse levels below a mesh of ``elements`` (nelem, npe):
+    one dict per level, coarsest last, with its degree, connectivity ``elements``
+    (coarse node ids), ``fine_ids`` (the fine-mesh node of each coarse node), ``JmCt``
+    (ngauss, npe_c, 3) -- the coarse basis gradients at the fine quadrature points --
+    and the prolongation ``P`` = (ptr, idx, w) over the next finer level's nodes with
+    its transpose ``R``."""
+    el = np.asarray(elements, dtype=np.int64)
+    npe = el.shape[1]
+    xi = local_node_coordinates(element_type, npe)
+    p = int(round(npe ** (1.0 / 3.0))) - 1 if element_type == "hex" else {4: 1, 10: 2, 20: 3, 35: 4}.get(npe, 0)
+    degrees = coarse_degrees(element_type, p)
+    if not degrees:
+        raise _lib.FlorenceAMDError("no coarse space below {} p={}".format(element_type, p))
+    z = np.asarray(quadrature_points, dtype=np.float64)
+    out = []
+    fine_el, fine_xi, fine_n, fine_ids = el, xi, int(nnode), np.arange(int(nnode))
+    for pc in degrees:
+        loc = _coarse_locals(element_type, pc, fine_xi)
+        cxi = fine_xi[loc]
+        ids_in_fine, cel = np.unique(fine_el[:, loc], return_inverse=True)
+        cel = cel.reshape(fine_el.shape[0], loc.size)
+        # the coarse basis at the fine quadrature points (gradients) and at the finer
+        # level's nodes (prolongation weights)
+        _, G = coarse_basis(element_type, pc, cxi, z)
+        W, _ = coarse_basis(element_type, pc, cxi, fine_xi)              # (npe_finer, nc)
+        P = _csr_from_first(fine_el, W, cel, fine_n)
+        out.append({"degree": pc, "elements": cel, "fine_ids": fine_ids[ids_in_fine],
+                    "JmCt": np.ascontiguousarray(np.transpose(G, (1, 2, 0))), "P": P,
+                    "R": _transpose(P[0], P[1], P[2], ids_in_fine.size)})
+        fine_el, fine_xi, fine_n, fine_ids = cel, cxi, ids_in_fine.size, fine_ids[ids_in_fine]
+    return out
+
+
+class PMultigridHierarchy(object):
+    """The coarse levels of a DeviceMeshData (built once per mesh and function space)."""
+
+    def __init__(self, dm, element_type, function_space):
+        z = np.asarray(function_space.quadrature.points, dtype=np.float64)
+        if z.shape[0] != dm.ngauss:
+            raise _lib.FlorenceAMDError("quadrature points do not match the mesh data")
+        tabs = hierarchy_tables(element_type, dm.elements.cpu().numpy(), dm.nnode, z)
+        self.levels = [_Level(dm.device, t["degree"], t["elements"], t["fine_ids"], t["JmCt"], t["P"], t["R"])
+                       for t in tabs]
+
+
+class MultigridSolver(object):
+    """Newton-step linear solver: p-multigrid preconditioned flexible CG (fl_mg_pcg).
+
+    cheb_degree / cheb_range: Chebyshev smoother on [lmax/range, lmax]; coarse_iterations:
+    block-Jacobi CG steps on the coarsest level.  ``rtol``, ``maxit``, ``check_every`` and
+    ``iterations`` as LinearSolver."""
+
+    def __init__(self, rtol=1e-12, maxit=2000, check_every=1, cheb_degree=3, cheb_range=20.0,
+                 coarse_iterations=30):
+        self.linear_solver = "pmg"
+        self.rtol = rtol
+        self.maxit = maxit
+        self.check_every = check_every
+        self.cheb_degree = cheb_degree
+        self.cheb_range = cheb_range
+        self.coarse_iterations = coarse_iterations
+        self.iterations = []
+        self._ws = None
+        self._h = None
+
+    def hierarchy(self, dm):
+        if self._h is None or self._h[0] is not dm:
+            et = dm.element_type
+            self._h = (dm, PMultigridHierarchy(dm, et, dm.function_space))
+        return self._h[1]
+
+    def galerkin(self, dm, precision="fp64"):
+        """Coarse operators of every level from the last assembly's point records."""
+        lib = _lib.load()
+        h = self.hierarchy(dm)
+        ws = dm.workspace()
+        rec = int(lib.fl_df_records_form(_lib.PRECISIONS[precision], _lib.ptr(dm.tensor_basis), dm.npe, dm.ngauss))
+        P_ = _lib.ptr
+        for lv in h.levels:
+            _lib.check(lib.fl_mg_galerkin(P_(ws), ws.numel(), dm.ngauss, dm.nelem, dm.npe, rec, P_(lv.JmCt), lv.npe,
+                                          P_(lv.elements), lv.nnode, P_(lv.node_elem_ptr), P_(lv.node_elem),
+                                          P_(lv.node_nbr_ptr), P_(lv.node_nbrs), P_(lv.node_order),
+                                          lv.bucket_offsets, lv.bucket_degree, P_(lv.indptr), P_(lv.ranks),
+                                          P_(lv.V), P_(lv.ws), lv.ws.numel(), P_(lv.status), _lib.stream_ptr()),
+                       "fl_mg_galerkin")
+        return h
+
+    def Solve(self, dm, V, free_mask, b, x, precision="fp64"):
+        import torch
+        lib = _lib.load()
+        h = self.galerkin(dm, precision)
+        nl = 1 + len(h.levels)
+        levels = (FlMgLevel * nl)()
+        P_ = lambda t: None if t is None else t.data_ptr()
+        levels[0] = FlMgLevel(dm.nnode, P_(dm.indptr), P_(dm.node_nbr_ptr), P_(dm.node_nbrs), P_(V), P_(free_mask),
+                              None, None, None, None, None, None)
+        for i, lv in enumerate(h.levels):
+            fr = lv.set_free(free_mask)
+            levels[i + 1] = FlMgLevel(lv.nnode, P_(lv.indptr), P_(lv.node_nbr_ptr), P_(lv.node_nbrs), P_(lv.V),
+                                      P_(fr), P_(lv.P_ptr), P_(lv.P_idx), P_(lv.P_w), P_(lv.R_ptr), P_(lv.R_idx),
+                                      P_(lv.R_w))
+        nn = (ctypes.c_int64 * nl)(dm.nnode, *[lv.nnode for lv in h.levels])
+        need = int(lib.fl_mg_workspace_size(nn, nl))
+        if self._ws is None or self._ws.numel() < need:
+            self._ws = torch.empty(need, dtype=torch.uint8, device=dm.device)
+        it = ctypes.c_int32(0)
+        rel = ctypes.c_double(0.0)
+        _lib.check(lib.fl_mg_pcg(levels, nl, 1 if V.dtype == torch.float32 else 0, _lib.ptr(b), _lib.ptr(x),
+                                 float(self.rtol), int(self.maxit), int(self.check_every), int(self.cheb_degree),
+                                 float(self.cheb_range), int(self.coarse_iterations), _lib.ptr(self._ws),
+                                 self._ws.numel(), ctypes.byref(it), ctypes.byref(rel), _lib.stream_ptr()),
+                   "fl_mg_pcg")
+        self.iterations.append(int(it.value))
+        if not (rel.value <= self.rtol) and it.value >= self.maxit:
+            raise _lib.FlorenceAMDError("multigrid CG did not reach rtol={} in {} iterations (rel={})".format(
+                self.rtol, it.value, rel.value))
+        return x

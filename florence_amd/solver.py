@@ -15,8 +15,9 @@ Semantics restated from Florence/Solver/FEMSolver.py:
 
 Everything between the iterations stays in HBM: assembly (libflorence_amd
 ``fl_df_assemble``), the masked reduction and the linear solve (``fl_pcg``, block-Jacobi
-CG) -- the reference's sparse direct solve has no GPU counterpart here.  Host work
-per iteration is the scalar bookkeeping (a few norms).
+CG, or ``multigrid.MultigridSolver``'s p-multigrid flexible CG) -- the reference's
+sparse direct solve has no GPU counterpart here.  Host work per iteration is the scalar
+bookkeeping (a few norms).
 """
 import time
 
@@ -41,7 +42,7 @@ class LinearSolver(object):
         self.iterations = []
         self._ws = None
 
-    def Solve(self, dm, V, free_mask, b, x):
+    def Solve(self, dm, V, free_mask, b, x, precision=None):
         import torch
         lib = _lib.load()
         if self._ws is None or self._ws.numel() < lib.fl_pcg_workspace_size(dm.nnode):
@@ -209,7 +210,7 @@ class FEMSolver(object):
         self._Eulerx = Eulerx
         while self.norm_residual > tol or Iter == 0:
             torch.neg(Residual, out=rhs)
-            self.linear_solver.Solve(dm, V, free, rhs, dU)
+            self.linear_solver.Solve(dm, V, free, rhs, dU, precision=self.tangent_precision)
             Eulerx = Eulerx + dU
             self._Eulerx = Eulerx
             assemble(Eulerx)

@@ -136,6 +136,25 @@ int fl_pattern_scatter_table(const int32_t *elements_dev, int64_t nelem, int32_t
                              const int32_t *colour_of_elem_dev, const void *tensor_basis_dev,
                              int16_t *table_dev, int32_t *status_dev, void *stream);
 
+/* ------------------------------------------------- high-order meshes
+ * Replaces GetHighOrderMesh (MeshGeneration/Mesh.py:1339-1409) -> HighOrderMeshHex /
+ * HighOrderMeshTet_SEMISTABLE (HigherOrderMeshing/HigherOrderMeshingHex.py:19-133,
+ * HigherOrderMeshingTet.py:19-133) with Decimals = decimals and check_duplicates: the
+ * p-refined mesh of a linear one with the reference's node numbering.  Every element's
+ * non-vertex nodes are placed at Neval^T x_vertices (Neval: f64 [nv][npe], the linear
+ * basis at the element's p-nodes in local order, values below 1e-12 set to 0, device);
+ * the new nodes are merged on their coordinates rounded to `decimals`, the whole list
+ * again at decimals - 2, and every node numbered by the byte order of its rounded
+ * coordinates (numpy's unique of raw-byte rows, which the reference calls).  Inputs:
+ * points_lin f64 [nnode_lin][3], elements_lin int32 [nelem][nv] (ids < nnode_lin).
+ * Outputs: points f64 [*nnode][3] (capacity nnode_lin + nelem (npe - nv) rows),
+ * elements int32 [nelem][npe]; *nnode (host).  Synchronous.
+ * workspace: fl_mesh_refine_workspace_size() bytes. */
+size_t fl_mesh_refine_workspace_size(int64_t nnode_lin, int64_t nelem, int32_t nv, int32_t npe);
+int fl_mesh_refine(const double *points_lin_dev, int64_t nnode_lin, const int32_t *elements_lin_dev, int64_t nelem,
+                   int32_t nv, const double *Neval_dev, int32_t npe, int32_t decimals, double *points_dev,
+                   int32_t *elements_dev, int64_t *nnode, void *workspace_dev, size_t workspace_bytes, void *stream);
+
 /* ------------------------------------------------- implicit DF assembly
  * Replaces _GlobalAssemblyDF_<Material> (FiniteElements/Assembly/_Assembly_/
  * _LowLevelAssemblyDF_.h:8-176) in its CSR mode (recompute_sparsity_pattern=0):
@@ -280,38 +299,62 @@ int fl_df_residual_mt(int32_t material, const double *params, const int32_t *ele
                       const int32_t *node_elem_dev, double *T_dev, const void *tensor_basis_dev, void *workspace_dev,
                       size_t workspace_bytes, int32_t *status_dev, void *stream);
 
-/* ------------------------------------------------------ Newton linear algebra
- * The reference reduces the tangent to the free dofs with scipy fancy indexing
- * (BoundaryCondition.py:842-857, GetReducedMatrices) and solves with a sparse direct
- * solver (Solver/LinearSolver.py:208-300, LinearSolver.Solve).  Here the reduction is
- * a mask free_mask[3*nnode] (1 = free dof; NULL = all free): the operator is
- * A = M K M + (I - M), M = diag(free_mask), on the unmodified CSR of fl_df_assemble.
+/* ------------------------------------------------------ p-multigrid solve
+ * No reference counterpart: the reference solves every Newton step with a sparse
+ * direct solver (Solver/LinearSolver.py:208-300).  fl_mg_pcg solves A x = b for the
+ * masked operator of fl_pcg by flexible CG preconditioned with one V-cycle over nested
+ * nodal spaces on the mesh's elements (level 0 = the assembled CSR, coarser levels =
+ * the p=2 lattice or the vertices of each element), Chebyshev-smoothed (degree
+ * cheb_degree on [lmax/cheb_range, lmax] of the block-Jacobi-scaled operator, lmax by
+ * power iteration per call), with coarse_iterations block-Jacobi CG steps on the
+ * coarsest level.  Stopping, b, x, iterations and rel_residual as fl_pcg.
  *
- * fl_csr_spmv: y = A x (y = K x when free_mask is NULL).
- * fl_pcg: block-Jacobi (3x3 node blocks) preconditioned conjugate gradient for
- * A x = b, x starting from 0, stopping when ||r|| <= rtol ||b|| (tested every
- * check_every iterations, which costs one host synchronisation) or after maxit
- * iterations.  b must vanish on fixed dofs; x does.  Synchronous: returns the
- * iteration count and final relative residual in host *iterations / *rel_residual.
- * workspace: fl_pcg_workspace_size(nnode) bytes.
+ * A coarse level's operator is the Galerkin product P^T K P of the fine tangent;
+ * fl_mg_galerkin forms it from the point records the last fl_df_assemble_* call on a
+ * workspace left there (so it must follow the assembly whose V is level 0, on the same
+ * workspace): records = fl_df_records_form(precision, tensor_basis, npe, ngauss) of
+ * that assembly; JmCt: f64 [ngauss][npe_c][3], the coarse basis gradients dNc_a/dxi_m
+ * at the fine quadrature points; the coarse mesh (elements_c [nelem][npe_c], npe_c =
+ * 4, 8 or 27) and its pattern, gather plan and optional rank table are built by the
+ * fl_pattern_* calls above.  workspace: fl_mg_galerkin_workspace_size() bytes.
  */
-int fl_csr_spmv(const int32_t *indptr_dev, const int32_t *node_nbr_ptr_dev, const int32_t *node_nbrs_dev,
-                const double *V_dev, int64_t nnode, const uint8_t *free_mask_dev, const double *x_dev,
-                double *y_dev, void *stream);
-size_t fl_pcg_workspace_size(int64_t nnode);
-int fl_pcg(const int32_t *indptr_dev, const int32_t *node_nbr_ptr_dev, const int32_t *node_nbrs_dev,
-           const double *V_dev, int64_t nnode, const uint8_t *free_mask_dev, const double *b_dev, double *x_dev,
-           double rtol, int32_t maxit, int32_t check_every, void *workspace_dev, size_t workspace_bytes,
-           int32_t *iterations, double *rel_residual, void *stream);
-/* The same two operations on FP32-stored CSR values (FL_PREC_CSR32); products and
- * every vector stay FP64.  No reference counterpart (the mixed-precision form). */
-int fl_csr_spmv_v32(const int32_t *indptr_dev, const int32_t *node_nbr_ptr_dev, const int32_t *node_nbrs_dev,
-                    const float *V_dev, int64_t nnode, const uint8_t *free_mask_dev, const double *x_dev,
-                    double *y_dev, void *stream);
-int fl_pcg_v32(const int32_t *indptr_dev, const int32_t *node_nbr_ptr_dev, const int32_t *node_nbrs_dev,
-               const float *V_dev, int64_t nnode, const uint8_t *free_mask_dev, const double *b_dev, double *x_dev,
-               double rtol, int32_t maxit, int32_t check_every, void *workspace_dev, size_t workspace_bytes,
-               int32_t *iterations, double *rel_residual, void *stream);
+#define FL_RECORDS_D 0   /* dxi/dx and the D blocks (dense point kernel) */
+#define FL_RECORDS_C 1   /* the C tensors of the factorised FP64 path */
+int32_t fl_df_records_form(int32_t precision, const void *tensor_basis_dev, int32_t npe, int32_t ngauss);
+size_t fl_mg_galerkin_workspace_size(int64_t nelem, int32_t npe_c, int64_t nnode_c);
+int fl_mg_galerkin(const void *df_workspace_dev, size_t df_workspace_bytes, int32_t ngauss, int64_t nelem,
+                   int32_t npe, int32_t records, const double *JmCt_dev, int32_t npe_c, const int32_t *elements_c_dev,
+                   int64_t nnode_c, const int32_t *node_elem_ptr_c_dev, const int32_t *node_elem_c_dev,
+                   const int32_t *node_nbr_ptr_c_dev, const int32_t *node_nbrs_c_dev, const int32_t *node_order_c_dev,
+                   const int32_t *bucket_offsets_c, const int32_t *bucket_degree_c, const int32_t *indptr_c_dev,
+                   const int16_t *ranks_c_dev, double *V_c_dev, void *workspace_dev, size_t workspace_bytes,
+                   int32_t *status_dev, void *stream);
+
+/* One level of the hierarchy (device pointers).  V: double[nnz] (float on level 0 when
+ * fl_mg_pcg's v32 is set).  P_ptr/P_idx/P_w: the prolongation from this level to the
+ * next finer one, as CSR over the FINER level's nodes (node f of the finer level takes
+ * sum_j P_w[j] u[P_idx[j]] of this level's nodal values); R_*: its transpose, CSR over
+ * this level's nodes.  Unused on level 0. */
+#define FL_MG_MAX_LEVELS 4
+typedef struct {
+    int64_t nnode;
+    const int32_t *indptr;
+    const int32_t *node_nbr_ptr;
+    const int32_t *node_nbrs;
+    const void *V;
+    const uint8_t *free_mask;
+    const int32_t *P_ptr;
+    const int32_t *P_idx;
+    const double *P_w;
+    const int32_t *R_ptr;
+    const int32_t *R_idx;
+    const double *R_w;
+} FlMgLevel;
+size_t fl_mg_workspace_size(const int64_t *nnodes, int32_t nlevels);
+int fl_mg_pcg(const FlMgLevel *levels, int32_t nlevels, int32_t v32, const double *b_dev, double *x_dev,
+              double rtol, int32_t maxit, int32_t check_every, int32_t cheb_degree, double cheb_range,
+              int32_t coarse_iterations, void *workspace_dev, size_t workspace_bytes, int32_t *iterations,
+              double *rel_residual, void *stream);
 
 /* ------------------------------------------------------------- profiling
  * No reference counterpart (the reference only records wall time,
