@@ -764,6 +764,8 @@ def main():
     cpu = None
     if args.cpu_sample > 0:
         cpu = cpu_baseline(mesh, fs, mat, x_host, args.cpu_sample)
+    sizes = {"nelem": dm.nelem, "nnode": dm.nnode, "nnz": dm.nnz, "ngauss": dm.ngauss}
+    ncolour = dm.ncolour
     del dm, x, T, Tres
     torch.cuda.empty_cache()
     newton = newton_comparison(args, lambda m: print(m, file=sys.stderr, flush=True)) if args.newton_members else None
@@ -783,14 +785,13 @@ def main():
         "data": "synthetic",
         "config": {"workload": "unit cube 24^3 hex p=4 (13824 elements, 125 nodes/element), 6^3 Gauss-Lobatto "
                                "points, Mooney-Rivlin(0.5,0.5,20), twisted state, K+T into reference CSR "
-                               "(nnz {})".format(dm.nnz),
-                   "nelem": dm.nelem, "nnode": dm.nnode, "nnz": dm.nnz, "ngauss": dm.ngauss},
+                               "(nnz {})".format(sizes["nnz"]), **sizes},
         "roofline": stiff_roof if dominant_is_stiffness else gather_roof,
         "roofline_other_phase": gather_roof if dominant_is_stiffness else stiff_roof,
         "phase_ms": {"qp": round(float(phase_ms[0]), 4), "stiffness": round(float(phase_ms[1]), 4),
                      "traction": round(float(phase_ms[2]), 4), "gather": round(float(phase_ms[3]), 4)},
         "cpu_baseline": cpu,
-        "scatter": "direct (colour-ordered, {} colours)".format(dm.ncolour) if direct else "element blocks + gather",
+        "scatter": "direct (colour-ordered, {} colours)".format(ncolour) if direct else "element blocks + gather",
         "other_data_path": other_path,
         "residual_only": residual_only,
         "newton": newton,

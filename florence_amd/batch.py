@@ -131,10 +131,13 @@ class BatchedAssembly(object):
         cm = _Concat([batch[i][0] for i in idx])
         dm = DeviceMeshData(cm, fs, device=self.device, build_colours=self.direct)
         elem_mat = np.repeat(np.arange(len(idx), dtype=np.int32), np.diff(cm.elem_offsets))
-        table = np.array([batch[i][2].device_parameters() for i in idx], dtype=np.float64)
         mat_ids = {MATERIAL_IDS[type(batch[i][2]).__name__] for i in idx}
         if len(mat_ids) != 1:
             raise ValueError("one material law per (type, p) group")
+        stride = int(_lib.load().fl_material_table_stride(next(iter(mat_ids))))
+        table = np.array([batch[i][2].device_table_row() for i in idx], dtype=np.float64)
+        if table.shape[1] != stride:
+            raise ValueError("material table rows of {} values, the library reads {}".format(table.shape[1], stride))
         tkey = self.tuning_key(key, fs)
         return {"key": key, "mesh_ids": idx, "concat": cm, "fs": fs, "dm": dm,
                 "elem_mat": torch.as_tensor(elem_mat, device=self.device),

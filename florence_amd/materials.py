@@ -71,6 +71,12 @@ class Material(object):
         per-element data, or None: element e uses row e, {mu, mu1, mu2, lamb, mu3, N}."""
         return None
 
+    def device_table_row(self):
+        """This law's row of a per-element material table of the C-ABI (several
+        materials of one law in one launch): {mu, mu1, mu2, lamb}, the first four
+        entries of device_parameters (fl_material_table_stride = 4)."""
+        return self.device_parameters()[:4]
+
 
 class NeoHookean(Material):
     """W(C) = mu/2 (C:I - 3) - mu ln J + lamb/2 (J-1)^2 (NeoHookean.py:5-10)."""

@@ -9,7 +9,12 @@ elements:
 * hex p=4 -> p=2 lattice -> vertices; hex p=2, p=3 -> vertices; tet p>=2 -> vertices;
 * coarse operators are Galerkin products P^T K P, formed on the device from the point
   records of the assembly that produced V (``fl_mg_galerkin``), so each Newton step's
-  coarse operators follow its tangent;
+  coarse operators follow its tangent.  They are formed element by element (each
+  element's own interpolation of the coarse space), which is P^T K P wherever the
+  coarse space is nested in the fine one; on the reference's 6-tet split, whose
+  neighbouring tets cut a shared cube face along different diagonals, the vertex space
+  is not nested, and the coarse operator is that element-by-element sum (still SPD; the
+  flexible CG tolerates the mismatch with P);
 * transfers are the nodal interpolation of the coarse spaces (a coarse basis function
   is a fine-space function), built here once per mesh as per-node CSR lists.
 

@@ -1,8 +1,15 @@
 """Batched form: several meshes of mixed (type, p) assembled through one captured HIP
 graph; each mesh's CSR block must equal assembling that mesh alone (same kernel
-variant: bitwise), and every graph replay must equal the eager batch."""
+variant: bitwise), and every graph replay must equal the eager batch.  Each mesh of
+the first test is also checked against the FP64 oracle: pattern bit-exact, K and T
+within 1e-12 relative, or within twice the oracle's own FP64 spread where that is
+larger (T of a small-strain state at p=3: the reference's parent gradients sum absolute
+coordinates, and the same oracle on a translated copy of the mesh moves T by up to
+1.8e-12 relative)."""
 import numpy as np
 import pytest
+
+from cases import oracle_with_spread, within_oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -44,6 +51,20 @@ def test_batch_graph_matches_per_mesh(cuda):
         assert np.array_equal(Tb, T.cpu().numpy()), (i, key)
     assert all(s[0] == 0 for s in ba.statuses())
     _lib.load().fl_set_tuning(_lib.FL_TUNE_STIFF_SF, 0)
+    # and every mesh of the batch against the FP64 oracle
+    from oracle import df_numpy as O
+    items, got = [], []
+    for i, (m, key, mat) in enumerate(batch):
+        g = next(gg for gg in ba.groups if i in gg["mesh_ids"])
+        ip, ix, data, Tb = ba.mesh_result(i)
+        ipr, ixr = O.sparsity_pattern(m.elements, m.nnode)
+        assert np.array_equal(ip, ipr) and np.array_equal(ix, ixr), (i, key)
+        X = np.asarray(m.points)
+        items.append((m, g["fs"], mat, X + disp(X), ipr, ixr))
+        got.append((data, Tb))
+    for i, (Vr, Tr, dV, dT), (data, Tb) in zip(range(len(batch)), oracle_with_spread(items), got):
+        ok, ek, et, bk, bt = within_oracle(data, Tb, Vr, Tr, dV, dT)
+        assert ok, (i, batch[i][1], ek, et, bk, bt)
 
 
 def test_tuned_batch_matches_per_mesh(cuda, tmp_path):

@@ -53,6 +53,14 @@ def test_host_only_queries():
     assert lib.fl_pattern_scatter_table_bytes(8, 10) == 0     # not a hex of p1^3 nodes
     # per-element material rows: 4 doubles, 8 for the anisotropic law (its fibre direction)
     assert [lib.fl_material_table_stride(m) for m in range(7)] == [0, 4, 4, 4, 4, 8, 0]
+    # the rows the host puts in such a table (the batch groups) have the library's stride
+    from florence_amd.materials import MATERIAL_IDS, LinearElastic, MooneyRivlin, NeoHookean, \
+        NearlyIncompressibleMooneyRivlin
+    for mat in (NeoHookean(3, mu=1.5, lamb=7.0), MooneyRivlin(3, mu1=0.5, mu2=0.25, lamb=9.0),
+                LinearElastic(3, mu=1.0, lamb=2.0), NearlyIncompressibleMooneyRivlin(3, mu1=0.4, mu2=0.3, lamb=5.0)):
+        row = mat.device_table_row()
+        assert row.shape == (lib.fl_material_table_stride(MATERIAL_IDS[type(mat).__name__]),)
+        assert np.array_equal(row, mat.device_parameters()[:4])
 
 
 def _colour(el, nnode, max_colours=64):

@@ -313,3 +313,36 @@ def test_c_restatement_matches_numpy(et, p, mat):
     Vc, Tc = c_assemble(m, fs, material, x, indptr, indices)
     assert np.abs(V - Vc).max() <= 1e-13 * np.abs(V).max()
     assert np.abs(T - Tc).max() <= 1e-13 * np.abs(T).max()
+
+
+def test_oracle_translation_spread():
+    """The reference algorithm's own FP64 spread, which the batch tests allow for: its
+    parent gradients sum absolute coordinates (_KinematicMeasures_.h:88-89), so on a
+    small-strain p=3 hex state the same oracle on the mesh translated by a quarter unit
+    moves T by more than 1e-12 relative (K by far less); a long-double evaluation of
+    the deformation gradient shows the FP64 value itself is that far from exact."""
+    from cases import ORACLE_SHIFT, oracle_with_spread
+    from florence_amd.batch import default_rule, random_batch
+    from florence_amd.function_space import FunctionSpace
+    batch = random_batch(14, seed=3, nelem_range=(20, 400))
+    m, key, mat = batch[10]
+    assert key == ("hex", 3) and m.nelem == 343
+    fs = FunctionSpace(m, default_rule(*key), p=key[1])
+    X = np.asarray(m.points)
+    x = X + 0.02 * np.sin(np.pi * X[:, [1, 2, 0]]) + 0.01 * X
+    ip, ix = O.sparsity_pattern(m.elements, m.nnode)
+    (V, T, dV, dT), = oracle_with_spread([(m, fs, mat, x, ip, ix)])
+    assert dT / np.abs(T).max() > 1e-12 and dV / np.abs(V).max() < 1e-13
+    # translation invariance itself holds to that spread (a dropped term would not)
+    assert np.abs(ORACLE_SHIFT).max() == 0.75
+    # F in FP64 (reference formula) against a long-double evaluation of the same formula
+    el = np.asarray(m.elements, dtype=np.int64)
+    SG, F, detJ = O.kinematic_measures(fs.Jm, fs.AllGauss, X[el], x[el], 1)
+    L = np.longdouble
+    PGX = np.einsum("iag,eal->egil", fs.Jm.astype(L), X[el].astype(L))
+    adj = np.stack([np.cross(PGX[..., 1, :], PGX[..., 2, :]), np.cross(PGX[..., 2, :], PGX[..., 0, :]),
+                    np.cross(PGX[..., 0, :], PGX[..., 1, :])], axis=-1)
+    iX = adj / np.einsum("...i,...i->...", PGX[..., 0, :], adj[..., :, 0])[..., None, None]
+    FL = np.swapaxes(np.einsum("egja,eal->egjl", np.einsum("egjk,kag->egja", iX, fs.Jm.astype(L)),
+                               x[el].astype(L)), -1, -2)
+    assert np.abs(F - FL).max() > 1e-14
