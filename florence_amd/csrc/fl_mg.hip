@@ -455,7 +455,8 @@ static int vcycle(Mg &m, int l, const double *rhs) {
     const FlMgLevel &L = m.lv[l];
     if (l == m.nl - 1) return coarse_solve(m, l);
     double *dl = nullptr;
-    int rc = chebyshev(m, l, rhs, &dl);
+    // pre-smoothing from x = 0 on this level's right-hand side
+    int rc = chebyshev(m, l, rhs ? rhs : w.b, &dl);
     if (rc) return rc;
     rc = level_cheb_step(m, l, dl, nullptr, nullptr, 0.0, 0.0);   // r = b - A x
     if (rc) return rc;
@@ -528,8 +529,8 @@ size_t fl_mg_workspace_size(const int64_t *nnodes, int32_t nlevels) {
     for (int l = 0; l < nlevels; ++l) lv[l].nnode = nnodes[l];
     Mg m = {};
     Carve cv(nullptr, 0);
-    // outer FCG vectors of level 0: r, r_old, p, q
-    cv.take<double>(4 * 3 * (size_t)nnodes[0]);
+    // outer FCG vectors of level 0: r, r_old, p, q (carved one by one, as fl_mg_pcg does)
+    for (int k = 0; k < 4; ++k) cv.take<double>(3 * (size_t)nnodes[0]);
     return carve_mg(cv, m, lv, nlevels);
 }
 
