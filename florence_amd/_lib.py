@@ -76,6 +76,8 @@ SIGNATURES = {
                                          _vp, _vp, _vp, _vp, _vp, _sz, _vp, _vp]),
     "fl_df_residual_mt": (ctypes.c_int, [_i32, _PD, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i64, _i32, _i64,
                                          _vp, _vp, _vp, _vp, _vp, _sz, _vp, _vp]),
+    "fl_df_residual_noalloc": (ctypes.c_int, [_i32, _PD, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i64, _i32,
+                                              _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "fl_material_table_stride": (_i32, [_i32]),
     "fl_df_records_form": (_i32, [_i32, _vp, _i32, _i32]),
     "fl_mg_galerkin_workspace_size": (_sz, [_i64, _i32, _i64]),

@@ -28,6 +28,7 @@
 //                     No floating-point atomics: the result is bitwise reproducible.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include "fl_common.hpp"
 #include "fl_profile.hpp"
@@ -1073,6 +1074,44 @@ static MatParams mat_params(int material, const double *p) {
     return m;
 }
 
+// Workspace of fl_df_residual_noalloc: a file-scope device array of a fixed size.  HIP
+// gives every GPU the library runs on its own copy, at its own address, held until the
+// process ends.  In host code the name `scratch` is a host-side stand-in, not a device
+// address: host code only ever uses the per-device address hipGetSymbolAddress returns
+// (scratch_address).  Nothing in it has to start at zero (the residual pass writes every
+// element traction before the gather reads it); calls that overlap on one GPU share it
+// and race.
+static __device__ __align__(256) unsigned char scratch[FL_NOALLOC_SCRATCH_BYTES];
+
+constexpr int FL_NOALLOC_MAX_DEVICES = 64;
+static void *g_scratch_addr[FL_NOALLOC_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+// device address of `scratch` on the current device, looked up on the first call per
+// device (that device current) and kept
+static int scratch_address(void **out) {
+    int dev = -1;
+    FL_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= FL_NOALLOC_MAX_DEVICES) {
+        set_error("fl_df_residual_noalloc: device %d is beyond the %d devices the library tracks", dev,
+                  FL_NOALLOC_MAX_DEVICES);
+        return FL_E_LIMIT;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr) {
+            set_error("fl_df_residual_noalloc: hipGetSymbolAddress(scratch) failed on device %d: %s", dev,
+                      hipGetErrorString(e));
+            return FL_E_HIP;
+        }
+        g_scratch_addr[dev] = p;
+    }
+    *out = g_scratch_addr[dev];
+    return 0;
+}
+
 static bool check_table(int material, const int32_t *elem_mat, const double *mat_table, const char *who) {
     if ((elem_mat == nullptr) != (mat_table == nullptr)) {
         set_error("%s: elem_material and material_table go together", who);
@@ -1364,7 +1403,8 @@ static int df_residual(int32_t material, const double *params, const int32_t *el
                        const double *points, const int32_t *elements, const double *Eulerx, const double *Jm,
                        const double *AllGauss, int32_t G, int64_t nelem, int32_t npe, int64_t nnode,
                        const int32_t *node_elem_ptr, const int32_t *node_elem, double *T, const void *tensor_basis,
-                       void *workspace, size_t workspace_bytes, int32_t *status, void *stream) {
+                       void *workspace, size_t workspace_bytes, int32_t *status, void *stream,
+                       bool profile = true) {
     hipStream_t st = as_stream(stream);
     if (!check_common(material, params, G, nelem, npe, nnode)) return FL_E_ARG;
     if (!check_table(material, elem_mat, mat_table, "fl_df_residual")) return FL_E_ARG;
@@ -1381,7 +1421,7 @@ static int df_residual(int32_t material, const double *params, const int32_t *el
     }
     const int NP = padded_np(npe);
     const MatParams mp = mat_params(material, params);
-    prof_begin(st, PROF_DF_RESIDUAL);
+    if (profile) prof_begin(st, PROF_DF_RESIDUAL);
     clear_status_kernel<<<1, 64, 0, st>>>(status);
     if (sf_ok) {
         // no records: the tractions are contracted in the point kernel
@@ -1400,7 +1440,7 @@ static int df_residual(int32_t material, const double *params, const int32_t *el
                 mp, points, elements, Eulerx, Jm, AllGauss, G, nelem, npe, 1, w.qp, status, elem_mat, mat_table);
         FL_LAUNCH_CHECK();
     }
-    prof_mark(st);
+    if (profile) prof_mark(st);
     if (!sf_ok) {
         const size_t lds_tr = (size_t)G * 9 * 8;
         if (lds_tr > 64 * 1024)
@@ -1409,11 +1449,11 @@ static int df_residual(int32_t material, const double *params, const int32_t *el
         traction_kernel<<<(unsigned)nelem, 256, lds_tr, st>>>(w.JmT, NP, w.qp, QS_RES, QR_S, npe, G, w.te);
         FL_LAUNCH_CHECK();
     }
-    prof_mark(st);
+    if (profile) prof_mark(st);
     gather_traction_kernel<<<(unsigned)((3 * nnode + 255) / 256), 256, 0, st>>>(npe, node_elem_ptr, node_elem, w.te,
                                                                                  nnode, T);
     FL_LAUNCH_CHECK();
-    prof_mark(st);
+    if (profile) prof_mark(st);
     return 0;
 }
 
@@ -1441,6 +1481,30 @@ int fl_df_residual_mt(int32_t material, const double *params, const int32_t *ele
                       void *workspace, size_t workspace_bytes, int32_t *status, void *stream) {
     return df_residual(material, params, elem_mat, mat_table, points, elements, Eulerx, Jm, AllGauss, G, nelem, npe,
                        nnode, node_elem_ptr, node_elem, T, tensor_basis, workspace, workspace_bytes, status, stream);
+}
+
+// fl_df_residual_noalloc: fl_df_residual_mt with its workspace in `scratch` below, for
+// callers that cannot allocate (see scratch_address).
+int fl_df_residual_noalloc(int32_t material, const double *params, const int32_t *elem_mat,
+                           const double *mat_table, const double *points, const int32_t *elements,
+                           const double *Eulerx, const double *Jm, const double *AllGauss, int32_t G, int64_t nelem,
+                           int32_t npe, int64_t nnode, const int32_t *node_elem_ptr, const int32_t *node_elem,
+                           double *T, const void *tensor_basis, int32_t *status, void *stream) {
+    if (!check_common(material, params, G, nelem, npe, nnode)) return FL_E_ARG;
+    size_t need = 0;
+    carve_df(nullptr, 0, G, nelem, npe, false, &need, FL_PREC_FP64, false, !residual_sf(tensor_basis, npe, G));
+    if (need > FL_NOALLOC_SCRATCH_BYTES) {
+        set_error("fl_df_residual_noalloc: %lld elements of %d nodes at %d points need %zu bytes of scratch, above "
+                  "the fixed maximum of %zu (use fl_df_residual_mt with a workspace)",
+                  (long long)nelem, npe, G, need, (size_t)FL_NOALLOC_SCRATCH_BYTES);
+        return FL_E_SPACE;
+    }
+    void *ws = nullptr;
+    const int rc = scratch_address(&ws);
+    if (rc) return rc;
+    return df_residual(material, params, elem_mat, mat_table, points, elements, Eulerx, Jm, AllGauss, G, nelem, npe,
+                       nnode, node_elem_ptr, node_elem, T, tensor_basis, ws, FL_NOALLOC_SCRATCH_BYTES, status, stream,
+                       false);
 }
 
 // the form of the point records fl_df_assemble_* leaves in its workspace: the C tensors

@@ -299,6 +299,23 @@ int fl_df_residual_mt(int32_t material, const double *params, const int32_t *ele
                       const int32_t *node_elem_dev, double *T_dev, const void *tensor_basis_dev, void *workspace_dev,
                       size_t workspace_bytes, int32_t *status_dev, void *stream);
 
+/* fl_df_residual_mt without a workspace argument: the pass's workspace (the transposed
+ * Jm, the element tractions and, off the sum-factorised path, the point records) is a
+ * fixed device array of FL_NOALLOC_SCRATCH_BYTES inside the library, one per GPU, held
+ * until the process ends.  No reference counterpart: it is for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc); the call allocates nothing.  Sizes whose workspace exceeds the maximum
+ * return FL_E_SPACE (fl_df_residual_workspace_size gives the need of the dense form;
+ * a 24^3 p=4 hex cube with its tensor basis needs 42 MB).  Calls that overlap on one GPU
+ * share the array and race; the device is the current one at the call. */
+#define FL_NOALLOC_SCRATCH_BYTES (64u << 20)
+int fl_df_residual_noalloc(int32_t material, const double *params, const int32_t *elem_material_dev,
+                           const double *material_table_dev, const double *points_dev, const int32_t *elements_dev,
+                           const double *Eulerx_dev, const double *Jm_dev, const double *AllGauss_dev,
+                           int32_t ngauss, int64_t nelem, int32_t npe, int64_t nnode,
+                           const int32_t *node_elem_ptr_dev, const int32_t *node_elem_dev, double *T_dev,
+                           const void *tensor_basis_dev, int32_t *status_dev, void *stream);
+
 /* ------------------------------------------------------ p-multigrid solve
  * No reference counterpart: the reference solves every Newton step with a sparse
  * direct solver (Solver/LinearSolver.py:208-300).  fl_mg_pcg solves A x = b for the

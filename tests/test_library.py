@@ -61,6 +61,20 @@ def test_host_only_queries():
         row = mat.device_table_row()
         assert row.shape == (lib.fl_material_table_stride(MATERIAL_IDS[type(mat).__name__]),)
         assert np.array_equal(row, mat.device_parameters()[:4])
+    # the residual pass without allocation refuses (on the host, before any device call)
+    # meshes whose workspace exceeds its fixed array: 200k p=2 tets on the dense path
+    prm, _keep = _lib.double_array(NeoHookean(3, mu=1.0, lamb=10.0).device_parameters())
+    assert lib.fl_df_residual_workspace_size(11, 200000, 10) > lib_noalloc_max()
+    rc = lib.fl_df_residual_noalloc(1, prm, None, None, None, None, None, None, None, 11, 200000, 10, 40000,
+                                    None, None, None, None, None, None)
+    assert rc == -3 and b"fixed maximum" in lib.fl_last_error()
+
+
+def lib_noalloc_max():
+    """FL_NOALLOC_SCRATCH_BYTES of include/florence_amd.h."""
+    with open(os.path.join(ROOT, "include", "florence_amd.h")) as f:
+        m = re.search(r"#define FL_NOALLOC_SCRATCH_BYTES \((\d+)u << (\d+)\)", f.read())
+    return int(m.group(1)) << int(m.group(2))
 
 
 def _colour(el, nnode, max_colours=64):
