@@ -725,6 +725,30 @@ def main():
             torch.cuda.synchronize()
         other_path = {"form": "element blocks stored, then gathered per node", "ms_per_step": round(
             o0.elapsed_time(o1) / 3, 4)}
+    # the precision form's tangents on the same mesh: FP32 (sum-factorised, FP32 MFMA and
+    # accumulation) and BF16 (dense BF16 MFMA) element contractions, blocks gathered into
+    # the FP64 CSR; kinematics, stress and T in FP64
+    precision_forms = {}
+    for prec in ("fp32", "bf16"):
+        assemble_device(dm, mat, x, V, T, precision=prec)
+        torch.cuda.synchronize()
+        _lib.profile_enable(True)
+        q0, q1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        with no_gc():
+            q0.record()
+            for _ in range(3):
+                assemble_device(dm, mat, x, V, T, precision=prec)
+            q1.record()
+            torch.cuda.synchronize()
+        pcalls = _lib.profile_read()
+        _lib.profile_enable(False)
+        pph = np.array([c[1] for c in pcalls if c[0] == 1]).mean(axis=0)
+        precision_forms[prec] = {"ms_per_step": round(q0.elapsed_time(q1) / 3, 4),
+                                 "elements_per_s": round(dm.nelem / (q0.elapsed_time(q1) / 3e3), 1),
+                                 "phase_ms": {k: round(float(v), 4) for k, v in
+                                              zip(("qp", "stiffness", "traction", "gather"), pph)}}
+        print("{} tangent: {:.3f} ms per assembly".format(prec, precision_forms[prec]["ms_per_step"]),
+              file=sys.stderr, flush=True)
     # the practical read+write rate of this GPU: a 4 GiB device-to-device copy, same
     # clock (HIP events), for comparison with the HBM-bound phases
     del V
@@ -793,6 +817,7 @@ def main():
         "cpu_baseline": cpu,
         "scatter": "direct (colour-ordered, {} colours)".format(ncolour) if direct else "element blocks + gather",
         "other_data_path": other_path,
+        "precision_forms": precision_forms,
         "residual_only": residual_only,
         "newton": newton,
         "status": status,

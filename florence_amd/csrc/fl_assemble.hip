@@ -862,11 +862,41 @@ static int launch_stiffness_sf(int npe, int G, const double *qp, const void *bas
     return FL_E_ARG;
 }
 
+// Sum-factorised FP32 stiffness blocks (FP32 MFMA, FP32 accumulation) of tensor-product
+// hexes, stored in tensor node order for the gather (no interior-row fusion).
+static int launch_stiffness_sf32(int npe, int G, const double *qp, const void *basis, int64_t nelem, float *Ke,
+                                 hipStream_t st) {
+    const int64_t groups = (nelem + 7) / 8 * 8 * 9;
+    if (groups > (int64_t)INT32_MAX) {
+        set_error("too many elements for one launch");
+        return FL_E_LIMIT;
+    }
+    const dim3 grid((unsigned)groups);
+    const int32_t *b = static_cast<const int32_t *>(basis);
+#define FL_SF_CASE(P, QQ)                                                                                      \
+    if (npe == P * P * P && G == QQ * QQ * QQ) {                                                               \
+        stiffness_sf_mfma2_kernel<P, QQ, false, float, double><<<grid, 256, 0, st>>>(qp, b, nelem, Ke, nullptr, nullptr); \
+        FL_LAUNCH_CHECK();                                                                                     \
+        return 0;                                                                                              \
+    }
+    FL_SF_CASE(2, 2)
+    FL_SF_CASE(3, 3)
+    FL_SF_CASE(4, 4)
+    FL_SF_CASE(5, 5)
+    FL_SF_CASE(5, 6)
+#undef FL_SF_CASE
+    set_error("no sum-factorised kernel for %d nodes and %d points", npe, G);
+    return FL_E_ARG;
+}
+
 // The factorised stiffness straight into the CSR, one launch per colour in colour order
 // (stiffness_sf_mfma2_kernel's DIRECT form).  offsets (host): the colour ranges of elems.
+// CT: the contraction type (double; float for the FP32 tangent), VT: the CSR values'
+// (double; float for FL_PREC_CSR32 with the FP32 tangent).
+template <typename CT, typename VT>
 static int launch_stiffness_direct(int npe, int G, const double *qp, const void *basis, const int32_t *elements,
                                    const int32_t *indptr, int ncolour, const int32_t *offsets,
-                                   const int32_t *elems, const int16_t *scat, double *V, hipStream_t st) {
+                                   const int32_t *elems, const int16_t *scat, VT *V, hipStream_t st) {
     const int32_t *b = static_cast<const int32_t *>(basis);
     for (int c = 0; c < ncolour; ++c) {
         const int64_t n = offsets[c + 1] - offsets[c];
@@ -877,12 +907,12 @@ static int launch_stiffness_direct(int npe, int G, const double *qp, const void 
         }
         const dim3 grid((unsigned)((n + 7) / 8 * 8 * 9));
         const int32_t *el = elems + offsets[c];
-#define FL_SF_CASE(P, QQ)                                                                                  \
-    if (npe == P * P * P && G == QQ * QQ * QQ) {                                                           \
-        stiffness_sf_mfma2_kernel<P, QQ, true><<<grid, 256, 0, st>>>(qp, b, n, nullptr, nullptr, V, el, elements, \
-                                                                     indptr, scat);                        \
-        FL_LAUNCH_CHECK();                                                                                 \
-        continue;                                                                                          \
+#define FL_SF_CASE(P, QQ)                                                                              \
+    if (npe == P * P * P && G == QQ * QQ * QQ) {                                                       \
+        stiffness_sf_mfma2_kernel<P, QQ, true, CT, VT><<<grid, 256, 0, st>>>(qp, b, n, nullptr, nullptr, V, el, \
+                                                                             elements, indptr, scat);  \
+        FL_LAUNCH_CHECK();                                                                             \
+        continue;                                                                                      \
     }
         FL_SF_CASE(2, 2)
         FL_SF_CASE(3, 3)
@@ -925,6 +955,13 @@ static int launch_qp_sf(const MatParams &mp, const double *points, const int32_t
 #undef FL_QSF_CASE
     set_error("no sum-factorised point kernel for %d nodes and %d points", npe, G);
     return FL_E_ARG;
+}
+
+// the FP32 tangent takes the factorised kernel unless a form other than the default
+// factorised one is selected (FL_TUNE_STIFF_SF 3 and 5 are FP64-only forms: dense there)
+static bool sf32_form() {
+    const int f = tuning_value(TUNE_STIFF_SF, 0);
+    return f != 1 && f != 3 && f != 5;
 }
 
 static bool sf_shape(int npe, int G) {
@@ -1226,8 +1263,11 @@ static int df_assemble(int32_t material, const double *params, const int32_t *el
     // basis is given, so T is the same for every tangent precision
     const bool sf_ok = tensor_basis != nullptr && sf_shape(npe, G) && tuning_value(TUNE_STIFF_SF, 0) != 1;
     const bool use_sf = precision == FL_PREC_FP64 && sf_ok;
-    // the direct scatter takes the default factorised form only
-    const bool direct = cp != nullptr && use_sf && tuning_value(TUNE_STIFF_SF, 0) != 3 &&
+    // the FP32 tangent of tensor-product hexes: the default factorised form in FP32
+    const bool use_sf32 = precision == FL_PREC_FP32 && sf_ok && sf32_form();
+    // the direct scatter takes the default factorised form only (FP64, or FP32 into the
+    // FP64 or the FP32-stored CSR)
+    const bool direct = cp != nullptr && (use_sf || use_sf32) && tuning_value(TUNE_STIFF_SF, 0) != 3 &&
                         tuning_value(TUNE_STIFF_SF, 0) != 5;
     size_t need = 0;
     DFWork w = carve_df(workspace, workspace_bytes, G, nelem, npe, true, &need, precision, !direct);
@@ -1260,8 +1300,8 @@ static int df_assemble(int32_t material, const double *params, const int32_t *el
         // the factorised FP64 stiffness reads C^{ik} from the records; the dense FP32/BF16
         // contractions read D^{ik}.  The element tractions are contracted in the same kernel.
         const int rq = launch_qp_sf<true>(mp, points, elements, Eulerx, tensor_basis, AllGauss, npe, G, nelem, update,
-                                          w.qp, status, elem_mat, mat_table, st, use_sf ? 1 : 0, w.te, node_elem_ptr,
-                                          fuse_interior ? T : nullptr);
+                                          w.qp, status, elem_mat, mat_table, st, use_sf || use_sf32 ? 1 : 0, w.te,
+                                          node_elem_ptr, fuse_interior ? T : nullptr);
         if (rq) return rq;
     } else {
         if (general_law(material))
@@ -1275,8 +1315,9 @@ static int df_assemble(int32_t material, const double *params, const int32_t *el
     prof_mark(st);
     // the factorised blocks are stored in tensor node order: tensor_of_l of the basis table,
     // and each node-element entry's stored row read next to the entry itself
-    const int32_t *kperm = use_sf && !direct ? static_cast<const int32_t *>(tensor_basis) + 4 + G + npe : nullptr;
-    const int32_t *kinv = use_sf && !direct ? static_cast<const int32_t *>(tensor_basis) + 4 + G : nullptr;   // local_of_t
+    const bool tensor_blocks = (use_sf || use_sf32) && !direct;
+    const int32_t *kperm = tensor_blocks ? static_cast<const int32_t *>(tensor_basis) + 4 + G + npe : nullptr;
+    const int32_t *kinv = tensor_blocks ? static_cast<const int32_t *>(tensor_basis) + 4 + G : nullptr;   // local_of_t
     const int32_t *eslot = nullptr;
     if (kperm) {
         const int64_t ne = nelem * npe;
@@ -1284,10 +1325,18 @@ static int df_assemble(int32_t material, const double *params, const int32_t *el
         FL_LAUNCH_CHECK();
         eslot = w.slot;
     }
-    int rc = direct ? launch_stiffness_direct(npe, G, w.qp, tensor_basis, elements, indptr, cp->ncolour, cp->offsets,
-                                              cp->elems, cp->scat, V, st)
+    int rc = direct ? (use_sf ? launch_stiffness_direct<double, double>(npe, G, w.qp, tensor_basis, elements, indptr,
+                                                                        cp->ncolour, cp->offsets, cp->elems, cp->scat,
+                                                                        V, st)
+                       : v32  ? launch_stiffness_direct<float, float>(npe, G, w.qp, tensor_basis, elements, indptr,
+                                                                      cp->ncolour, cp->offsets, cp->elems, cp->scat,
+                                                                      reinterpret_cast<float *>(V), st)
+                              : launch_stiffness_direct<float, double>(npe, G, w.qp, tensor_basis, elements, indptr,
+                                                                       cp->ncolour, cp->offsets, cp->elems, cp->scat,
+                                                                       V, st))
            : use_sf ? launch_stiffness_sf(npe, G, w.qp, tensor_basis, nelem, (double *)w.Ke, elements, node_elem_ptr,
                                           indptr, w.irow, fuse_interior ? V : nullptr, st)
+           : use_sf32 ? launch_stiffness_sf32(npe, G, w.qp, tensor_basis, nelem, (float *)w.Ke, st)
            : precision == FL_PREC_FP64
                  ? launch_stiffness_t<double, double>(NP, w.JmT, w.qp, npe, G, nelem, (double *)w.Ke, st)
                  : precision == FL_PREC_FP32
@@ -1511,10 +1560,9 @@ int fl_df_residual_noalloc(int32_t material, const double *params, const int32_t
 // of the factorised FP64 path or dxi/dx and D of the dense path
 static int records_form(int32_t precision, const void *tensor_basis, int npe, int G) {
     precision &= ~FL_PREC_CSR32;
-    return precision == FL_PREC_FP64 && tensor_basis != nullptr && sf_shape(npe, G) &&
-                   tuning_value(TUNE_STIFF_SF, 0) != 1
-               ? FL_RECORDS_C
-               : FL_RECORDS_D;
+    const bool sf = tensor_basis != nullptr && sf_shape(npe, G) && tuning_value(TUNE_STIFF_SF, 0) != 1;
+    return sf && (precision == FL_PREC_FP64 || (precision == FL_PREC_FP32 && sf32_form())) ? FL_RECORDS_C
+                                                                                          : FL_RECORDS_D;
 }
 
 int32_t fl_df_records_form(int32_t precision, const void *tensor_basis, int32_t npe, int32_t ngauss) {

@@ -546,14 +546,21 @@ __global__ __launch_bounds__(256) void interior_rows_kernel(const int32_t *__res
 // entries are fetched with one vector load per column tile when the slice starts, and
 // the loads of the entries added to precede the slice's stores: stores count in the
 // same memory counter as loads, so a load issued after a store waits for it.
-template <int P1, int Q, bool DIRECT = false>
-__global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(
-    const double *__restrict__ qp, const int32_t *__restrict__ basis, int64_t nelem, double *__restrict__ Ke,
-    const int32_t *__restrict__ irow, double *__restrict__ V, const int32_t *__restrict__ elist = nullptr,
+template <int P1, int Q, bool DIRECT = false, typename CT = double, typename VT = double>
+__global__ __launch_bounds__(256, sizeof(CT) == 8 ? 3 : 4) void stiffness_sf_mfma2_kernel(
+    const double *__restrict__ qp, const int32_t *__restrict__ basis, int64_t nelem, CT *__restrict__ Ke,
+    const int32_t *__restrict__ irow, VT *__restrict__ V, const int32_t *__restrict__ elist = nullptr,
     const int32_t *__restrict__ elements = nullptr, const int32_t *__restrict__ indptr = nullptr,
     const int16_t *__restrict__ scat = nullptr) {
     typedef SfLayout<P1, Q> L;
-    typedef Mfma16<double> MF;
+    typedef Mfma16<CT> MF;
+    // output rows of the beta product (and so the store and scatter-table order): the FP64
+    // MFMA's, rows lk + 4 r.  CT = float (the FP32 tangent: FP32 MFMA and accumulation)
+    // feeds the A rows of each tile permuted, i -> (i >> 2) + 4 (i & 3), so that the FP32
+    // MFMA's rows 4 lk + r come out as the same rows lk + 4 r.
+    typedef Mfma16<double> MO;
+    constexpr bool F64 = sizeof(CT) == 8;
+    static_assert(!F64 || sizeof(VT) == 8, "FP64 blocks go to an FP64 CSR");
     constexpr int Q3 = L::Q3, N3 = L::N3, PP = L::PP;
     constexpr int NT = 256;
     constexpr int MT2 = (PP + 15) / 16, NT2 = (P1 * Q + 15) / 16;
@@ -561,11 +568,11 @@ __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(
     constexpr int KE_MAX = (4 * Q + 3) / 4, KB = (2 * Q + 3) / 4;
     // pair products of the 1D bases: zeta [s*2+t][az][bz][qz] = T0^s[az][qz] T0^t[bz][qz],
     // eta [s*2+t][ae*P1+be][qe] likewise; the beta ones live in registers
-    __shared__ double sPz[4][P1][P1][Q];
-    __shared__ double sPe[4][PP][Q];
-    __shared__ double sC[9][Q3];
-    __shared__ double sT1[9][P1][Q * Q];
-    __shared__ double sT2[2][PP][P1][Q];
+    __shared__ CT sPz[4][P1][P1][Q];
+    __shared__ CT sPe[4][PP][Q];
+    __shared__ CT sC[9][Q3];
+    __shared__ CT sT1[9][P1][Q * Q];
+    __shared__ CT sT2[2][PP][P1][Q];
     // interior rows written straight into the CSR (V != NULL): CSR offset of row 3a+pi of
     // tensor node t when node a lies in this element only (else -1), and the position
     // of tensor node t among the element's nodes sorted by id -- the column block of
@@ -606,11 +613,11 @@ __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(
         const int st = t / (PP * Q), r = t - st * (PP * Q);
         const int a = r / (P1 * Q), b = (r / Q) % P1, q = r % Q;
         const int s0 = st >> 1, t0 = st & 1;
-        (&sPz[0][0][0][0])[t] = f1d[(s0 * P1 + a) * Q + q] * f1d[(t0 * P1 + b) * Q + q];
-        (&sPe[0][0][0])[t] = f1d[((2 + s0) * P1 + a) * Q + q] * f1d[((2 + t0) * P1 + b) * Q + q];
+        (&sPz[0][0][0][0])[t] = (CT)(f1d[(s0 * P1 + a) * Q + q] * f1d[(t0 * P1 + b) * Q + q]);
+        (&sPe[0][0][0])[t] = (CT)(f1d[((2 + s0) * P1 + a) * Q + q] * f1d[((2 + t0) * P1 + b) * Q + q]);
     }
     // this lane's beta A fragments: phase h, step ks, row tile tm -> T2^h[ab][qb] T2^tl[bb][qb]
-    double afr[2][KB][MT3];
+    CT afr[2][KB][MT3];
 #pragma unroll
     for (int h = 0; h < 2; ++h)
 #pragma unroll
@@ -620,11 +627,11 @@ __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(
             const int tl = kv ? kk / Q : 0, qb = kv ? kk - (kk / Q) * Q : 0;
 #pragma unroll
             for (int tm = 0; tm < MT3; ++tm) {
-                const int arow = tm * 16 + li;
+                const int arow = tm * 16 + (F64 ? li : (li >> 2) + 4 * (li & 3));
                 const int ab_ = min(arow / P1, P1 - 1), bb_ = arow % P1;
                 afr[h][ks][tm] = (kv && arow < PP)
-                                     ? f1d[((4 + h) * P1 + ab_) * Q + qb] * f1d[((4 + tl) * P1 + bb_) * Q + qb]
-                                     : 0.0;
+                                     ? (CT)(f1d[((4 + h) * P1 + ab_) * Q + qb] * f1d[((4 + tl) * P1 + bb_) * Q + qb])
+                                     : CT(0);
             }
         }
     const double *qe_ = qp + e * (int64_t)(Q3)*QS;
@@ -634,11 +641,11 @@ __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(
 #pragma unroll
         for (int m = 0; m < 3; ++m)
 #pragma unroll
-            for (int n = 0; n < 3; ++n) sC[3 * m + n][t] = rec[(Q_D + 9 * sp + (tr ? 3 * n + m : 3 * m + n)) * Q3];
+            for (int n = 0; n < 3; ++n) sC[3 * m + n][t] = (CT)rec[(Q_D + 9 * sp + (tr ? 3 * n + m : 3 * m + n)) * Q3];
     }
     __syncthreads();
 
-    double *Kd = DIRECT ? nullptr : Ke + (e * 9 + pi * 3 + pk) * (int64_t)N3 * N3;
+    CT *Kd = DIRECT ? nullptr : Ke + (e * 9 + pi * 3 + pk) * (int64_t)N3 * N3;
     typedef short TV __attribute__((ext_vector_type(MT3 * 4)));
     for (int az = 0; az < P1; ++az) {
         // this slice's scatter entries: lane's rows (tm, r) of column tile wave + 4 v
@@ -654,11 +661,11 @@ __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(
         for (int t = tid; t < 9 * P1 * Q; t += NT) {
             const int mn = t / (P1 * Q), bz = (t / Q) % P1, qe = t % Q;
             const int m = mn / 3, n = mn % 3;
-            const double *f = sPz[(m == 0) * 2 + (n == 0)][az][bz];
+            const CT *f = sPz[(m == 0) * 2 + (n == 0)][az][bz];
 #pragma unroll
             for (int qb = 0; qb < Q; ++qb) {
-                const double *c = sC[mn] + (qb * Q + qe) * Q;
-                double s = 0.0;
+                const CT *c = sC[mn] + (qb * Q + qe) * Q;
+                CT s = CT(0);
 #pragma unroll
                 for (int qz = 0; qz < Q; ++qz) s += f[qz] * c[qz];
                 sT1[mn][bz][qe * Q + qb] = s;
@@ -669,7 +676,7 @@ __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(
 #pragma unroll
         for (int tm = 0; tm < MT3; ++tm)
 #pragma unroll
-            for (int v = 0; v < TNW; ++v) acc[tm][v] = typename MF::acc_t{0.0, 0.0, 0.0, 0.0};
+            for (int v = 0; v < TNW; ++v) acc[tm][v] = typename MF::acc_t{CT(0), CT(0), CT(0), CT(0)};
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             // eta (MFMA) for the phase's types (m's beta factor h, n's tl): rows (ae,be),
@@ -681,15 +688,15 @@ __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(
                 const bool av = arow < PP, bv = bcol < P1 * Q;
                 const int arc = av ? arow : 0;
                 const int bz_ = min(bcol / Q, P1 - 1), qb_ = bcol % Q;
-                typename MF::acc_t a2[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+                typename MF::acc_t a2[2] = {{CT(0), CT(0), CT(0), CT(0)}, {CT(0), CT(0), CT(0), CT(0)}};
 #pragma unroll
                 for (int ks = 0; ks < KE_MAX; ++ks) {
-                    double a[2], b[2];
+                    CT a[2], b[2];
 #pragma unroll
                     for (int tl = 0; tl < 2; ++tl) {
                         const int nn = tl ? 1 : 2, ke = (h ? 1 : 2) * nn * Q;
-                        a[tl] = 0.0;
-                        b[tl] = 0.0;
+                        a[tl] = CT(0);
+                        b[tl] = CT(0);
                         if (ks * 4 < ke) {
                             // clamped loads, zeroed by a select where K or the tile pads
                             const int kk = ks * 4 + lk;
@@ -697,10 +704,10 @@ __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(
                             const int kc = min(kk, ke - 1);
                             const int pr = kc / Q, qe = kc - pr * Q;
                             const int m = h ? 2 : pr / nn, n = tl ? 2 : pr % nn;
-                            const double x = sPe[(m == 1) * 2 + (n == 1)][arc][qe];
-                            const double y = sT1[3 * m + n][bz_][qe * Q + qb_];
-                            a[tl] = (kv && av) ? x : 0.0;
-                            b[tl] = (kv && bv) ? y : 0.0;
+                            const CT x = sPe[(m == 1) * 2 + (n == 1)][arc][qe];
+                            const CT y = sT1[3 * m + n][bz_][qe * Q + qb_];
+                            a[tl] = (kv && av) ? x : CT(0);
+                            b[tl] = (kv && bv) ? y : CT(0);
                         }
                     }
 #pragma unroll
@@ -727,14 +734,14 @@ __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(
                 const bool kv = ks * 4 + 3 < 2 * Q || kk < 2 * Q;
                 const int kc = min(kk, 2 * Q - 1);
                 const int tl = kc / Q, qb = kc - tl * Q;
-                double bf[TNW];
+                CT bf[TNW];
 #pragma unroll
                 for (int v = 0; v < TNW; ++v) {
                     const int bcol = (wave + 4 * v) * 16 + li;
                     const int bc = min(bcol, N3 - 1);
                     const int ae_ = bc / PP, be_ = (bc / P1) % P1, bz_ = bc % P1;
-                    const double x = sT2[tl][ae_ * P1 + be_][bz_][qb];
-                    bf[v] = (kv && bcol < N3) ? x : 0.0;
+                    const CT x = sT2[tl][ae_ * P1 + be_][bz_][qb];
+                    bf[v] = (kv && bcol < N3) ? x : CT(0);
                 }
 #pragma unroll
                 for (int tm = 0; tm < MT3; ++tm)
@@ -746,11 +753,11 @@ __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(
             // whose last reads precede the barrier after this phase's eta
             if (h == 0) __syncthreads();
         }
-        if (DIRECT) {
+        if constexpr (DIRECT) {
             // CSR positions and, where an earlier colour has written, the values to add
             // to: every load before the first store
             int32_t off[TNW][MT3][4];
-            double prev[TNW][MT3][4];
+            VT prev[TNW][MT3][4];
 #pragma unroll
             for (int v = 0; v < TNW; ++v) {
                 const int bcol = (wave + 4 * v) * 16 + li;
@@ -759,12 +766,12 @@ __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(
                 for (int tm = 0; tm < MT3; ++tm)
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
-                        const int row = min(tm * 16 + MF::row(lk, r), PP - 1);
+                        const int row = min(tm * 16 + MO::row(lk, r), PP - 1);
                         const int ta = ((row / P1) * P1 + ae_) * P1 + az;
                         const int sv = tv[v][tm * 4 + r];
                         off[v][tm][r] = sRow[ta] + 3 * (sv & 0x7fff) + pk;
-                        const bool live = wave + 4 * v < NT3 && bcol < N3 && tm * 16 + MF::row(lk, r) < PP;
-                        prev[v][tm][r] = live && sv < 0 ? V[off[v][tm][r]] : 0.0;
+                        const bool live = wave + 4 * v < NT3 && bcol < N3 && tm * 16 + MO::row(lk, r) < PP;
+                        prev[v][tm][r] = live && sv < 0 ? V[off[v][tm][r]] : VT(0);
                     }
             }
             // vmcnt(0) on every path (expcnt and lgkmcnt left open): no load of this phase
@@ -782,9 +789,9 @@ __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(
 #pragma unroll
                         for (int r = 0; r < 4; ++r) {
                             const int rlo = tm * 16 + 4 * r;
-                            const int row = tm * 16 + MF::row(lk, r);
+                            const int row = tm * 16 + MO::row(lk, r);
                             if (rlo < PP && (rlo + 3 < PP || row < PP))
-                                V[off[v][tm][r]] = prev[v][tm][r] + acc[tm][v][r];
+                                V[off[v][tm][r]] = (VT)(prev[v][tm][r] + acc[tm][v][r]);
                         }
                 }
             }
@@ -802,20 +809,22 @@ __global__ __launch_bounds__(256, 3) void stiffness_sf_mfma2_kernel(
                         // rows tm 16 + lk + 4 r, lk in [0, 4): whole groups of four are kept
                         // or dropped at compile time
                         const int rlo = tm * 16 + 4 * r;
-                        const int row = tm * 16 + MF::row(lk, r);
+                        const int row = tm * 16 + MO::row(lk, r);
                         if (rlo < PP && (rlo + 3 < PP || row < PP)) {
                             const int ab = row / P1, bb = row - ab * P1;
                             const int ta = (ab * P1 + ae_) * P1 + az, tb = (bb * P1 + be_) * P1 + bz_;
                             // one store through a selected address (the 0.0 + value form is
                             // what the gather would have written for an interior row, and
                             // leaves a block entry's sums unchanged)
-                            double *dst = Kd + ((int64_t)ta * N3 + tb);
-                            if (V) {
-                                const int32_t ro = sRow[ta];
-                                const int32_t rk = sRank[tb];
-                                dst = ro >= 0 ? V + ((int64_t)ro + 3 * rk + pk) : dst;
+                            CT *dst = Kd + ((int64_t)ta * N3 + tb);
+                            if constexpr (F64) {
+                                if (V) {
+                                    const int32_t ro = sRow[ta];
+                                    const int32_t rk = sRank[tb];
+                                    dst = ro >= 0 ? V + ((int64_t)ro + 3 * rk + pk) : dst;
+                                }
                             }
-                            *dst = 0.0 + acc[tm][v][r];
+                            *dst = CT(0) + acc[tm][v][r];
                         }
                     }
             }

@@ -49,6 +49,54 @@ def test_fp32_tangent_error(cuda, case):
     assert int(st[0].item()) == 0
 
 
+@pytest.mark.parametrize("p", [2, 3, 4])
+def test_fp32_factorised_tangent(cuda, p):
+    """The FP32 tangent of tensor-product hexes runs the sum-factorised kernel in FP32
+    (FP32 MFMA, FP32 accumulation, blocks gathered into the FP64 CSR): within the FP32 bar
+    of the FP64 oracle and of the dense FP32 contraction (FL_TUNE_STIFF_SF = 1), with T
+    bitwise the FP64 path's, and it leaves the C-tensor records the multigrid reads."""
+    import torch
+    from cases import cube_mesh, function_space
+    from florence_amd import _lib
+    from florence_amd.assembly import DeviceMeshData, assemble_device
+    from florence_amd.materials import MooneyRivlin
+    if p == 4:
+        m, fs, mat = cube_hex_p4(2)
+    else:
+        m = cube_mesh("hex", p, 2, jitter=0.05, seed=4)
+        fs = function_space(m, p)
+        mat = MooneyRivlin(3, mu1=0.5, mu2=0.5, lamb=20.0)
+    x = _deformed(np.asarray(m.points))
+    dm = DeviceMeshData(m, fs)
+    assert dm.tensor_basis is not None
+    lib = _lib.load()
+    assert lib.fl_df_records_form(_lib.PRECISIONS["fp32"], _lib.ptr(dm.tensor_basis), dm.npe, dm.ngauss) == \
+        lib.fl_df_records_form(_lib.PRECISIONS["fp64"], _lib.ptr(dm.tensor_basis), dm.npe, dm.ngauss)
+    V64, T64, _ = assemble_device(dm, mat, x)
+    V64, T64 = V64.clone(), T64.clone()
+    Vsf, Tsf, st = assemble_device(dm, mat, x, precision="fp32")
+    Vsf, Tsf = Vsf.clone(), Tsf.clone()
+    assert int(st[0].item()) == 0
+    try:
+        lib.fl_set_tuning(_lib.FL_TUNE_STIFF_SF, 1)
+        Vd, _, _ = assemble_device(dm, mat, x, precision="fp32")
+        Vd = Vd.clone()
+    finally:
+        lib.fl_set_tuning(_lib.FL_TUNE_STIFF_SF, 0)
+    torch.cuda.synchronize()
+    indptr, indices = O.sparsity_pattern(m.elements, m.nnode)
+    Vr, _ = O.assemble_csr(m.points, m.elements, x, fs.Jm, fs.AllGauss, mat, indptr, indices)
+    s = np.abs(Vr).max()
+    e_sf = np.abs(Vsf.cpu().numpy() - Vr).max() / s
+    e_d = np.abs(Vd.cpu().numpy() - Vr).max() / s
+    e_sd = np.abs(Vsf.cpu().numpy() - Vd.cpu().numpy()).max() / s
+    print("hex p{}: |K32sf-K|/|K| {:.2e}  |K32dense-K|/|K| {:.2e}  |sf-dense| {:.2e}".format(p, e_sf, e_d, e_sd),
+          flush=True)
+    assert e_sf <= 1e-5 and e_sd <= 1e-5
+    assert e_sf > 1e-12          # the contraction really ran in FP32
+    assert torch.equal(Tsf, T64)
+
+
 @pytest.mark.parametrize("precision,bar", [("fp32", 1e-7), ("bf16", 1e-6)])
 def test_mixed_newton_tet_p2_cube(cuda, precision, bar):
     from florence_amd.boundary_condition import BoundaryCondition
@@ -77,21 +125,35 @@ def test_mixed_newton_tet_p2_cube(cuda, precision, bar):
 
 
 def test_csr32_storage_and_solve(cuda):
-    """FL_PREC_CSR32: the FP32-stored values are the FP64 sums of the FP32-tangent path
-    rounded once (bitwise), T is unchanged, and block-Jacobi PCG on them solves the same
-    system to the FP32 level (solution within 1e-5 of the FP64-valued solve)."""
+    """FL_PREC_CSR32: on the block path the FP32-stored values are the FP64 sums of the
+    FP32-tangent path rounded once (bitwise); on the direct scatter (each element's FP32
+    contribution added in colour order) the FP32 CSR is within FP32 rounding of the
+    FP64-summed one, whose sums equal the block path's to FP64 rounding; T is unchanged;
+    block-Jacobi PCG on the FP32 values solves the same system to the FP32 level
+    (solution within 1e-5 of the FP64-valued solve)."""
     import torch
     from florence_amd.assembly import DeviceMeshData, assemble_device
     from florence_amd.solver import LinearSolver, spmv
     m, fs, mat = cube_hex_p4(2)
     x = _deformed(np.asarray(m.points))
     dm = DeviceMeshData(m, fs)
+    assert dm.scatter is not None
+    Vb64, Tb64, _ = assemble_device(dm, mat, x, precision="fp32", direct=False)
+    Vb64, Tb64 = Vb64.clone(), Tb64.clone()
+    Vb32, Tb32, _ = assemble_device(dm, mat, x, precision="fp32", csr="fp32", direct=False)
+    Vb32 = Vb32.clone()
     V64, T64, _ = assemble_device(dm, mat, x, precision="fp32")
     V64, T64 = V64.clone(), T64.clone()
     V32, T32, st = assemble_device(dm, mat, x, precision="fp32", csr="fp32")
     torch.cuda.synchronize()
-    assert V32.dtype == torch.float32
-    assert torch.equal(V32, V64.to(torch.float32)) and torch.equal(T32, T64)
+    assert V32.dtype == torch.float32 and Vb32.dtype == torch.float32
+    assert torch.equal(Vb32, Vb64.to(torch.float32)) and torch.equal(Tb32, Tb64)
+    s = float(V64.abs().max())
+    assert float((V64 - Vb64).abs().max()) <= 1e-13 * s
+    e32 = float((V32.to(torch.float64) - V64).abs().max()) / s
+    print("FP32 CSR on the direct scatter: |V32 - V64| / |V64| = {:.2e}".format(e32))
+    assert e32 <= 4.8e-7      # at most 8 FP32 roundings of a sum (8 x 2^-24)
+    assert torch.equal(T32, T64) and torch.equal(T64, Tb64)
     assert int(st[0].item()) == 0
     X = np.asarray(m.points)
     free = torch.as_tensor((np.abs(X[:, 0]) > 1e-10).repeat(3).astype(np.uint8), device="cuda")
