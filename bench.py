@@ -455,6 +455,40 @@ def newton_comparison(args, log):
     return out
 
 
+def config2_newton(n, linear, lin_rtol, log):
+    """The config-2 solve on one mesh: hex p=4 unit cube n^3 (jitter 0.03h, seed 0),
+    MR(0.5, 0.5, 20), 6^3 Gauss-Lobatto points, x=0 clamped, x=1 twisted 30 degrees, 10
+    increments, each to |R_free| <= 1e-8 R0 -- three modes of the lockstep driver:
+    the plain FP64 path (FP64 tangent and CSR, the reference's predictor start) with
+    fixed linear tolerance (lin_rtol) and with the adaptive one, and the fast path (FP32
+    tangent, FP32-stored CSR, extrapolated start, adaptive linear tolerance)."""
+    import torch
+    from florence_amd.learned import LockstepNewton, TwistStretchFamily
+    fam = TwistStretchFamily(n=n, nincr=10, theta_range=(30.0, 30.0), stretch_range=(1.0, 1.0))
+    modes, ref = {}, None
+    for prec, start, forcing, csr in (("fp64", "previous", "fixed", "fp64"), ("fp64", "previous", "adaptive", "fp64"),
+                                      ("fp32", "extrap", "adaptive", "fp32")):
+        ln = LockstepNewton(fam, [0], precision=prec, forcing=forcing, csr=csr, lin_rtol=lin_rtol, linear=linear)
+        log("lockstep {} {} {} csr {}: {} dofs, nnz {}".format(prec, start, forcing, csr, 3 * ln.dm.nnode, ln.dm.nnz))
+        with no_gc():
+            r = ln.solve(start, record=True, log=log)
+        final = r["trajectory"][-1]
+        ref = final if ref is None else ref
+        key = "{}+{}+{}".format(prec, start, forcing) + ("+csr32" if csr == "fp32" else "")
+        modes[key] = {"wall_s": round(r["wall_s"], 3), "linear_s": round(r["linear_s"], 3),
+                      "assembly_s": round(r["assembly_s"], 3), "residual_s": round(r["residual_s"], 3),
+                      "newton_iterations": r["lockstep_iterations"], "pcg_iterations_total":
+                      int(sum(r["pcg_iterations"])), "linear_iterations": list(r["pcg_iterations"]),
+                      "failed": r["failed"],
+                      "max_rel_diff_vs_plain": float(np.abs(final - ref).max() / np.abs(ref).max())}
+        log("{}: wall {:.3f}s, Newton {}, linear iterations {}".format(key, modes[key]["wall_s"],
+                                                                      modes[key]["newton_iterations"],
+                                                                      modes[key]["pcg_iterations_total"]))
+        del ln
+        torch.cuda.empty_cache()
+    return modes
+
+
 def bench_newton(args):
     """The twisted-cube solve: hex p=4 unit cube n^3 (jitter 0.03h, seed 0), MR(0.5, 0.5,
     20), 6^3 Gauss-Lobatto points, x=0 clamped, x=1 twisted 30 degrees, 10 increments.
@@ -467,7 +501,6 @@ def bench_newton(args):
       linear solves to 1e-10) against the fast path (FP32 tangent, FP32-stored CSR,
       extrapolated start, adaptive linear tolerance)."""
     import torch
-    from florence_amd.learned import LockstepNewton, TwistStretchFamily
 
     log = lambda m: print(m, file=sys.stderr, flush=True)
     out_modes = {}
@@ -508,27 +541,7 @@ def bench_newton(args):
             log("{}: {}".format(key, out_modes[key]))
             del solver, res
             torch.cuda.empty_cache()
-    fam = TwistStretchFamily(n=args.n, nincr=10, theta_range=(30.0, 30.0), stretch_range=(1.0, 1.0))
-    ref = None
-    for prec, start, forcing, csr in (("fp64", "previous", "fixed", "fp64"), ("fp64", "previous", "adaptive", "fp64"),
-                                      ("fp32", "extrap", "adaptive", "fp32")):
-        ln = LockstepNewton(fam, [0], precision=prec, forcing=forcing, csr=csr, lin_rtol=args.lin_rtol,
-                            linear=args.linear)
-        log("lockstep {} {} {} csr {}: {} dofs, nnz {}".format(prec, start, forcing, csr, 3 * ln.dm.nnode, ln.dm.nnz))
-        with no_gc():
-            r = ln.solve(start, record=True, log=log)
-        final = r["trajectory"][-1]
-        ref = final if ref is None else ref
-        key = "{}+{}+{}".format(prec, start, forcing) + ("+csr32" if csr == "fp32" else "")
-        out_modes[key] = {"wall_s": round(r["wall_s"], 3), "linear_s": round(r["linear_s"], 3),
-                          "assembly_s": round(r["assembly_s"], 3), "residual_s": round(r["residual_s"], 3),
-                          "newton_iterations": r["lockstep_iterations"], "pcg_iterations_total":
-                          int(sum(r["pcg_iterations"])), "linear_iterations": list(r["pcg_iterations"]),
-                          "failed": r["failed"],
-                          "max_rel_diff_vs_plain": float(np.abs(final - ref).max() / np.abs(ref).max())}
-        log("{}: {}".format(key, out_modes[key]))
-        del ln
-        torch.cuda.empty_cache()
+    out_modes.update(config2_newton(args.n, args.linear, args.lin_rtol, log))
     plain, fast = out_modes["fp64+previous+adaptive"]["wall_s"], out_modes["fp32+extrap+adaptive+csr32"]["wall_s"]
     out = {"metric": "Newton wall time to FP64 residual 1e-8 (twisted hex p=4 cube)", "value": fast, "unit": "s",
            "n_gpus": 1, "steps": 1, "warmup": 0, "higher_is_better": False, "vs_baseline": None, "dtype": "fp32",
@@ -580,6 +593,8 @@ def main():
                                                              "(calibrates PMC byte counters)")
     ap.add_argument("--newton-members", type=int, default=16,
                     help="assembly: held-out members of the Newton comparison in the default line (0 = skip)")
+    ap.add_argument("--newton-n", type=int, default=24,
+                    help="assembly: cells per edge of the config-2 Newton solve in the default line (0 = skip)")
     ap.add_argument("--nmesh", type=int, default=256)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16"])
@@ -729,26 +744,30 @@ def main():
     # accumulation) and BF16 (dense BF16 MFMA) element contractions, blocks gathered into
     # the FP64 CSR; kinematics, stress and T in FP64
     precision_forms = {}
-    for prec in ("fp32", "bf16"):
-        assemble_device(dm, mat, x, V, T, precision=prec)
+    V32 = torch.empty(dm.nnz, dtype=torch.float32, device="cuda")
+    for prec, csr in (("fp32", "fp64"), ("fp32", "fp32"), ("bf16", "fp64")):
+        Vp = V32 if csr == "fp32" else V
+        assemble_device(dm, mat, x, Vp, T, precision=prec, csr=csr)
         torch.cuda.synchronize()
         _lib.profile_enable(True)
         q0, q1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         with no_gc():
             q0.record()
             for _ in range(3):
-                assemble_device(dm, mat, x, V, T, precision=prec)
+                assemble_device(dm, mat, x, Vp, T, precision=prec, csr=csr)
             q1.record()
             torch.cuda.synchronize()
         pcalls = _lib.profile_read()
         _lib.profile_enable(False)
         pph = np.array([c[1] for c in pcalls if c[0] == 1]).mean(axis=0)
-        precision_forms[prec] = {"ms_per_step": round(q0.elapsed_time(q1) / 3, 4),
+        key = prec + ("+csr32" if csr == "fp32" else "")
+        precision_forms[key] = {"ms_per_step": round(q0.elapsed_time(q1) / 3, 4),
                                  "elements_per_s": round(dm.nelem / (q0.elapsed_time(q1) / 3e3), 1),
                                  "phase_ms": {k: round(float(v), 4) for k, v in
                                               zip(("qp", "stiffness", "traction", "gather"), pph)}}
-        print("{} tangent: {:.3f} ms per assembly".format(prec, precision_forms[prec]["ms_per_step"]),
+        print("{} tangent: {:.3f} ms per assembly".format(key, precision_forms[key]["ms_per_step"]),
               file=sys.stderr, flush=True)
+    del V32
     # the practical read+write rate of this GPU: a 4 GiB device-to-device copy, same
     # clock (HIP events), for comparison with the HBM-bound phases
     del V
@@ -792,7 +811,20 @@ def main():
     ncolour = dm.ncolour
     del dm, x, T, Tres
     torch.cuda.empty_cache()
-    newton = newton_comparison(args, lambda m: print(m, file=sys.stderr, flush=True)) if args.newton_members else None
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    newton = newton_comparison(args, log) if args.newton_members else None
+    newton_c2 = None
+    if args.newton_n > 0:
+        modes = config2_newton(args.newton_n, "mg", 1e-10, log)
+        plain_fixed = modes["fp64+previous+fixed"]["wall_s"]
+        plain_adapt = modes["fp64+previous+adaptive"]["wall_s"]
+        fast = modes["fp32+extrap+adaptive+csr32"]["wall_s"]
+        newton_c2 = {"workload": "unit cube {0}^3 hex p=4 (jitter 0.03h), 6^3 GLL, MR(0.5,0.5,20), x=0 clamped, x=1 "
+                                 "twisted 30 deg, 10 increments, |R_free| <= 1e-8 R0, p-multigrid flexible CG "
+                                 "(fixed: linear rtol 1e-10; adaptive: forcing terms)".format(args.newton_n),
+                     "plain_fp64_fixed_s": plain_fixed, "plain_fp64_adaptive_s": plain_adapt, "mixed_fp32_s": fast,
+                     "speedup_equal_policy": round(plain_adapt / fast, 3),
+                     "speedup_vs_plain_fixed": round(plain_fixed / fast, 3), "modes": modes}
 
     out = {
         "metric": "K+residual assembly elements/s (DF, hex p=4, FP64)",
@@ -820,6 +852,7 @@ def main():
         "precision_forms": precision_forms,
         "residual_only": residual_only,
         "newton": newton,
+        "newton_config2": newton_c2,
         "status": status,
     }
     print(json.dumps(out), flush=True)
