@@ -11,8 +11,12 @@
 //             operator P^T K P fl_mg_galerkin forms from the fine point records;
 //   coarsest  a fixed number of block-Jacobi CG steps (no host synchronisation).
 // Smoother: Chebyshev polynomial of block-Jacobi (3x3 node blocks) on
-// [lmax / range, lmax], lmax of D^-1 A estimated per solve by power iteration; the same
-// polynomial before and after the coarse correction keeps the cycle symmetric.
+// [lmax / range, lmax], lmax of D^-1 A estimated per solve by power iteration (warm
+// started from the previous solve's vector when the caller says the operator is the
+// previous one's successor); the same polynomial before and after the coarse correction
+// keeps the cycle symmetric.  The fine level's smoother may read an FP32 copy of an
+// FP64 CSR (half the bytes of the six fine-level products of a V-cycle); the outer
+// iteration always applies the level's own operator.
 // Transfers are the nodal interpolation P (fine node <- coarse nodes of an element
 // containing it) and its transpose, both as per-node CSR lists summed in a fixed order.
 // Every reduction is two-pass in a fixed order: a solve is bitwise reproducible.
@@ -26,7 +30,6 @@
 namespace fl {
 namespace {
 
-constexpr int FL_MG_POWER_STEPS = 12;        // power iterations per eigenvalue estimate
 constexpr double FL_MG_LMAX_SAFETY = 1.15;   // the estimate approaches lmax from below
 
 __device__ inline void bj_apply(const double *__restrict__ P, int64_t a, const double (&r)[3], double (&z)[3]) {
@@ -341,15 +344,17 @@ __global__ __launch_bounds__(SV_T) void fcg_update_kernel(int64_t n, const doubl
 struct LevelWork {
     double *P;
     double *x, *r, *d, *d2, *b, *t1, *t2;
+    double *pv;   // the power iteration's vector, kept from call to call (warm starts)
     double lmax;
 };
 
-constexpr int NVEC = 7;
+constexpr int NVEC = 8;
 
 struct Mg {
     const FlMgLevel *lv;
     int nl;
     int v32;
+    const float *V0s;   // optional FP32 copy of level 0's values for its smoother (v32 = 0)
     int degree;
     double range;
     int coarse_its;
@@ -367,12 +372,20 @@ static const VT *lvals(const FlMgLevel &L) {
 static unsigned nb_sp(int64_t nnode) { return (unsigned)((nnode + SP_W - 1) / SP_W); }
 static unsigned nb_v(int64_t n) { return (unsigned)((n + SV_T - 1) / SV_T); }
 
-// q = A_l d, with per-block partials of d.q when part is given
-static int level_spmv(const Mg &m, int l, const double *d, double *q, double *part) {
+// the smoother's view of level l's values: level 0 reads the FP32 copy when one is given
+static bool smooth_f32(const Mg &m, int l) { return l == 0 && (m.v32 || m.V0s != nullptr); }
+static const float *smooth_v32(const Mg &m, int l) {
+    return m.v32 ? lvals<float>(m.lv[l]) : m.V0s;
+}
+
+// q = A_l d, with per-block partials of d.q when part is given; smoother: with the
+// values the smoother reads (the spectrum estimate), else the level's own operator
+static int level_spmv(const Mg &m, int l, const double *d, double *q, double *part, bool smoother = false) {
     const FlMgLevel &L = m.lv[l];
-    if (l == 0 && m.v32)
+    if (smoother ? smooth_f32(m, l) : (l == 0 && m.v32))
         spmv_kernel<float><<<nb_sp(L.nnode), 64 * SP_W, 0, m.st>>>(L.indptr, L.node_nbr_ptr, L.node_nbrs,
-                                                                    lvals<float>(L), L.nnode, L.free_mask, d, q, part);
+                                                                    smoother ? smooth_v32(m, l) : lvals<float>(L),
+                                                                    L.nnode, L.free_mask, d, q, part);
     else
         spmv_kernel<double><<<nb_sp(L.nnode), 64 * SP_W, 0, m.st>>>(L.indptr, L.node_nbr_ptr, L.node_nbrs,
                                                                      lvals<double>(L), L.nnode, L.free_mask, d, q,
@@ -385,9 +398,9 @@ static int level_spmv(const Mg &m, int l, const double *d, double *q, double *pa
 static int level_cheb_step(const Mg &m, int l, const double *d_old, double *d_new, double *x, double c1, double c2) {
     const FlMgLevel &L = m.lv[l];
     const LevelWork &w = m.w[l];
-    if (l == 0 && m.v32)
+    if (smooth_f32(m, l))
         cheb_step_kernel<float><<<nb_sp(L.nnode), 64 * SP_W, 0, m.st>>>(L.indptr, L.node_nbr_ptr, L.node_nbrs,
-                                                                         lvals<float>(L), L.nnode, L.free_mask, w.P,
+                                                                         smooth_v32(m, l), L.nnode, L.free_mask, w.P,
                                                                          d_old, d_new, w.r, x, c1, c2);
     else
         cheb_step_kernel<double><<<nb_sp(L.nnode), 64 * SP_W, 0, m.st>>>(L.indptr, L.node_nbr_ptr, L.node_nbrs,
@@ -473,17 +486,21 @@ static int vcycle(Mg &m, int l, const double *rhs) {
     return chebyshev(m, l, nullptr, &dl);
 }
 
-// largest eigenvalue of D^-1 A on level l by power iteration (device-side normalisation,
-// one host read at the end), times a safety factor
-static int estimate_lmax(Mg &m, int l, int steps) {
+// largest eigenvalue of D^-1 A on level l (A as the smoother sees it) by power iteration
+// (device-side normalisation, one host read at the end), times a safety factor; warm:
+// from the vector the previous call left (the previous Newton step's operator), else
+// from a fixed pseudo-random start
+static int estimate_lmax(Mg &m, int l, int steps, bool warm) {
     LevelWork &w = m.w[l];
     const FlMgLevel &L = m.lv[l];
     const int64_t n = 3 * L.nnode;
-    double *v = w.d, *y = w.d2, *z = w.r, *zz = m.s + 16;
-    power_start_kernel<<<nb_v(n), SV_T, 0, m.st>>>(n, L.free_mask, v);
-    FL_LAUNCH_CHECK();
+    double *v = w.pv, *y = w.d2, *z = w.r, *zz = m.s + 16;
+    if (!warm) {
+        power_start_kernel<<<nb_v(n), SV_T, 0, m.st>>>(n, L.free_mask, v);
+        FL_LAUNCH_CHECK();
+    }
     for (int it = 0; it < steps; ++it) {
-        int rc = level_spmv(m, l, v, y, nullptr);
+        int rc = level_spmv(m, l, v, y, nullptr, true);
         if (rc) return rc;
         bj_norm_kernel<<<nb_v(L.nnode), SV_T, 0, m.st>>>(L.nnode, w.P, y, z, m.part);
         sum1_kernel<<<1, SV_T, 0, m.st>>>(m.part, nb_v(L.nnode), zz);
@@ -506,7 +523,8 @@ static size_t carve_mg(Carve &cv, Mg &m, const FlMgLevel *lv, int nl) {
     for (int l = 0; l < nl; ++l) {
         const int64_t nn = lv[l].nnode;
         m.w[l].P = cv.take<double>(9 * (size_t)nn);
-        double **vec[NVEC] = {&m.w[l].x, &m.w[l].r, &m.w[l].d, &m.w[l].d2, &m.w[l].b, &m.w[l].t1, &m.w[l].t2};
+        double **vec[NVEC] = {&m.w[l].x, &m.w[l].r, &m.w[l].d, &m.w[l].d2, &m.w[l].b, &m.w[l].t1, &m.w[l].t2,
+                              &m.w[l].pv};
         for (int k = 0; k < NVEC; ++k) *vec[k] = cv.take<double>(3 * (size_t)nn);
         const int64_t pp = nb_sp(nn) + 2 * (int64_t)nb_v(3 * nn) + 8;
         if (pp > maxpart) maxpart = pp;
@@ -534,11 +552,13 @@ size_t fl_mg_workspace_size(const int64_t *nnodes, int32_t nlevels) {
     return carve_mg(cv, m, lv, nlevels);
 }
 
-int fl_mg_pcg(const FlMgLevel *levels, int32_t nlevels, int32_t v32, const double *b, double *x, double rtol,
-              int32_t maxit, int32_t check_every, int32_t cheb_degree, double cheb_range, int32_t coarse_iterations,
-              void *workspace, size_t workspace_bytes, int32_t *iterations, double *rel_residual, void *stream) {
+int fl_mg_pcg(const FlMgLevel *levels, int32_t nlevels, int32_t v32, const float *smoother_V32, const double *b,
+              double *x, double rtol, int32_t maxit, int32_t check_every, int32_t cheb_degree, double cheb_range,
+              int32_t coarse_iterations, int32_t power_steps, int32_t power_warm, void *workspace,
+              size_t workspace_bytes, int32_t *iterations, double *rel_residual, void *stream) {
     if (!levels || nlevels < 2 || nlevels > FL_MG_MAX_LEVELS || !b || !x || maxit <= 0 || !iterations ||
-        !rel_residual || cheb_degree < 1 || cheb_degree > 16 || !(cheb_range > 1.0) || coarse_iterations < 1) {
+        !rel_residual || cheb_degree < 1 || cheb_degree > 16 || !(cheb_range > 1.0) || coarse_iterations < 1 ||
+        power_steps < 1 || power_steps > 64) {
         set_error("fl_mg_pcg: invalid argument");
         return FL_E_ARG;
     }
@@ -558,6 +578,7 @@ int fl_mg_pcg(const FlMgLevel *levels, int32_t nlevels, int32_t v32, const doubl
     m.lv = levels;
     m.nl = nlevels;
     m.v32 = v32 ? 1 : 0;
+    m.V0s = v32 ? nullptr : smoother_V32;
     m.degree = cheb_degree;
     m.range = cheb_range;
     m.coarse_its = coarse_iterations;
@@ -590,7 +611,7 @@ int fl_mg_pcg(const FlMgLevel *levels, int32_t nlevels, int32_t v32, const doubl
         FL_LAUNCH_CHECK();
     }
     for (int l = 0; l + 1 < nlevels; ++l) {
-        const int rc = estimate_lmax(m, l, FL_MG_POWER_STEPS);
+        const int rc = estimate_lmax(m, l, power_steps, power_warm != 0);
         if (rc) return rc;
     }
     int32_t hbad = 0;

@@ -250,11 +250,15 @@ class MultigridSolver(object):
     """Newton-step linear solver: p-multigrid preconditioned flexible CG (fl_mg_pcg).
 
     cheb_degree / cheb_range: Chebyshev smoother on [lmax/range, lmax]; coarse_iterations:
-    block-Jacobi CG steps on the coarsest level.  ``rtol``, ``maxit``, ``check_every`` and
-    ``iterations`` as LinearSolver."""
+    block-Jacobi CG steps on the coarsest level; power_steps: power iterations for lmax on
+    a solver's first solve (cold), warm_power_steps on later ones (started from the
+    previous solve's vector: the operator is the next Newton step's); fp32_smoother: an
+    FP64 CSR's fine-level smoother reads an FP32 copy of it (the outer iteration keeps the
+    FP64 operator, so the solve meets rtol on it).  ``rtol``, ``maxit``, ``check_every``
+    and ``iterations`` as LinearSolver."""
 
     def __init__(self, rtol=1e-12, maxit=2000, check_every=1, cheb_degree=3, cheb_range=20.0,
-                 coarse_iterations=30):
+                 coarse_iterations=30, power_steps=12, warm_power_steps=3, fp32_smoother=True):
         self.linear_solver = "pmg"
         self.rtol = rtol
         self.maxit = maxit
@@ -262,9 +266,14 @@ class MultigridSolver(object):
         self.cheb_degree = cheb_degree
         self.cheb_range = cheb_range
         self.coarse_iterations = coarse_iterations
+        self.power_steps = power_steps
+        self.warm_power_steps = warm_power_steps
+        self.fp32_smoother = fp32_smoother
         self.iterations = []
         self._ws = None
         self._h = None
+        self._v32 = None
+        self._warm_key = None
 
     def hierarchy(self, dm):
         if self._h is None or self._h[0] is not dm:
@@ -306,13 +315,26 @@ class MultigridSolver(object):
         need = int(lib.fl_mg_workspace_size(nn, nl))
         if self._ws is None or self._ws.numel() < need:
             self._ws = torch.empty(need, dtype=torch.uint8, device=dm.device)
+        v32 = V.dtype == torch.float32
+        smooth = None
+        if not v32 and self.fp32_smoother:
+            if self._v32 is None or self._v32.numel() != V.numel() or self._v32.device != V.device:
+                self._v32 = torch.empty(V.numel(), dtype=torch.float32, device=V.device)
+            self._v32.copy_(V)
+            smooth = self._v32
+        # the power vectors in the workspace belong to this hierarchy and workspace
+        key = (id(h), self._ws.data_ptr(), tuple(nn), smooth is not None or v32)
+        warm = self._warm_key == key
         it = ctypes.c_int32(0)
         rel = ctypes.c_double(0.0)
-        _lib.check(lib.fl_mg_pcg(levels, nl, 1 if V.dtype == torch.float32 else 0, _lib.ptr(b), _lib.ptr(x),
+        _lib.check(lib.fl_mg_pcg(levels, nl, 1 if v32 else 0, _lib.ptr(smooth), _lib.ptr(b), _lib.ptr(x),
                                  float(self.rtol), int(self.maxit), int(self.check_every), int(self.cheb_degree),
-                                 float(self.cheb_range), int(self.coarse_iterations), _lib.ptr(self._ws),
-                                 self._ws.numel(), ctypes.byref(it), ctypes.byref(rel), _lib.stream_ptr()),
+                                 float(self.cheb_range), int(self.coarse_iterations),
+                                 int(self.warm_power_steps if warm else self.power_steps), 1 if warm else 0,
+                                 _lib.ptr(self._ws), self._ws.numel(), ctypes.byref(it), ctypes.byref(rel),
+                                 _lib.stream_ptr()),
                    "fl_mg_pcg")
+        self._warm_key = key
         self.iterations.append(int(it.value))
         if not (rel.value <= self.rtol) and it.value >= self.maxit:
             raise _lib.FlorenceAMDError("multigrid CG did not reach rtol={} in {} iterations (rel={})".format(

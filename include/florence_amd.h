@@ -323,7 +323,7 @@ int fl_df_residual_noalloc(int32_t material, const double *params, const int32_t
  * nodal spaces on the mesh's elements (level 0 = the assembled CSR, coarser levels =
  * the p=2 lattice or the vertices of each element), Chebyshev-smoothed (degree
  * cheb_degree on [lmax/cheb_range, lmax] of the block-Jacobi-scaled operator, lmax by
- * power iteration per call), with coarse_iterations block-Jacobi CG steps on the
+ * power iteration per call, below), with coarse_iterations block-Jacobi CG steps on the
  * coarsest level.  Stopping, b, x, iterations and rel_residual as fl_pcg.
  *
  * A coarse level's operator is the Galerkin product P^T K P of the fine tangent;
@@ -367,10 +367,17 @@ typedef struct {
     const int32_t *R_idx;
     const double *R_w;
 } FlMgLevel;
+/* smoother_V32_dev (optional, v32 = 0): level 0's values rounded to FP32 on the same
+ * pattern, read by the fine level's smoother and its spectrum estimate (the outer
+ * iteration applies level 0's own FP64 values); NULL: the smoother reads level 0's.
+ * power_steps: power iterations per level for the smoother's lmax (1..64); power_warm:
+ * nonzero = start them from the vectors the previous call on this workspace left (same
+ * levels; the previous Newton step's operator), 0 = from a fixed pseudo-random start. */
 size_t fl_mg_workspace_size(const int64_t *nnodes, int32_t nlevels);
-int fl_mg_pcg(const FlMgLevel *levels, int32_t nlevels, int32_t v32, const double *b_dev, double *x_dev,
-              double rtol, int32_t maxit, int32_t check_every, int32_t cheb_degree, double cheb_range,
-              int32_t coarse_iterations, void *workspace_dev, size_t workspace_bytes, int32_t *iterations,
+int fl_mg_pcg(const FlMgLevel *levels, int32_t nlevels, int32_t v32, const float *smoother_V32_dev,
+              const double *b_dev, double *x_dev, double rtol, int32_t maxit, int32_t check_every,
+              int32_t cheb_degree, double cheb_range, int32_t coarse_iterations, int32_t power_steps,
+              int32_t power_warm, void *workspace_dev, size_t workspace_bytes, int32_t *iterations,
               double *rel_residual, void *stream);
 
 /* ------------------------------------------------------------- profiling
