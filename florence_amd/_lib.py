@@ -20,7 +20,7 @@ FL_S_PATTERN = 4
 FL_GATHER_BUCKETS = 6
 FL_TUNE_STIFF_MINW = 1
 FL_TUNE_STIFF_VARIANT_BASE = 2
-STIFF_VARIANTS = {0: 1, 1: 2, 2: 1, 3: 2, 4: 3}   # padded node-count class -> number of tile variants
+STIFF_VARIANTS = {0: 1, 1: 4, 2: 3, 3: 5, 4: 3}   # padded node-count class -> number of tile variants
 FL_E_ARG = -1
 FL_E_HIP = -2
 FL_E_SPACE = -3
@@ -79,6 +79,7 @@ SIGNATURES = {
     "fl_df_residual_noalloc": (ctypes.c_int, [_i32, _PD, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i64, _i32,
                                               _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "fl_material_table_stride": (_i32, [_i32]),
+    "fl_stiffness_variant_count": (_i32, [_i32]),
     "fl_df_records_form": (_i32, [_i32, _vp, _i32, _i32]),
     "fl_mg_galerkin_workspace_size": (_sz, [_i64, _i32, _i64]),
     "fl_mg_galerkin": (ctypes.c_int, [_vp, _sz, _i32, _i64, _i32, _i32, _vp, _i32, _vp, _i64, _vp, _vp, _vp, _vp,

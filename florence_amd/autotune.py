@@ -1,8 +1,9 @@
 """Autotuner of the stiffness kernel per (element type, p, points, precision).
 
 For every group of a BatchedAssembly it times each tile-shape variant the library
-offers for the group's padded node count (FL_TUNE_STIFF_VARIANT_BASE keys) and, for
-FP64 tensor-product hexes, the sum-factorised kernel (FL_TUNE_STIFF_SF) with its two
+offers for the group's padded node count (FL_TUNE_STIFF_VARIANT_BASE keys: 1 to 5 tile
+shapes per class, fl_stiffness_variant_count) and, for FP64 and FP32 tensor-product
+hexes, the sum-factorised kernel (FL_TUNE_STIFF_SF) with its two
 data paths (element blocks + gather, or the colour-ordered direct scatter), using the
 library's own per-phase HIP events summed over the whole assembly, and keeps the
 fastest.  Results are cached as JSON keyed "hex-p2-G27-fp64" -> {"variant": v,
@@ -48,7 +49,7 @@ def _label(sf, v, direct):
 def tune_batch(batched, repeats=3, cache=None, force=False):
     """Pick each group's stiffness kernel and data path (mutates batched.groups[*]
     ['variant'], ['sum_factorised'] and ['direct']); returns the cache.  Candidates:
-    every tile variant of the dense contraction and, for FP64 tensor-product hexes, the
+    every tile variant of the dense contraction and, for FP64/FP32 tensor-product hexes, the
     sum-factorised kernel with element blocks + gather and, where the group has a colour
     plan, with the direct scatter.  Timed: the sum of the assembly's phases."""
     import torch
@@ -59,7 +60,7 @@ def tune_batch(batched, repeats=3, cache=None, force=False):
         npc = _np_class(g["dm"].npe)
         nvar = _lib.STIFF_VARIANTS[npc] if batched.precision != "bf16" else 1
         cands = [(False, v, False) for v in range(nvar)]
-        if g["dm"].tensor_basis is not None and batched.precision == "fp64":
+        if g["dm"].tensor_basis is not None and batched.precision in ("fp64", "fp32"):
             cands.append((True, 0, False))
             if batched.direct and g["dm"].scatter is not None:
                 cands.append((True, 0, True))

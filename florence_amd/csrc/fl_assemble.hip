@@ -781,21 +781,36 @@ static int launch_stiffness_t(int NP, const double *JmT, const double *qp, int n
     const dim3 grid((unsigned)groups);
     // tile-shape variants per padded node count (fl_set_tuning key
     // FL_TUNE_STIFF_VARIANT_BASE + class, value = variant + 1; 0 = default variant 0)
+    // <MT, NT, WGM, WGN>: MT x NT tiles of 16 x 16 per wave, WGM x WGN waves per block
+#define FL_DENSE(NPV, MT_, NT_, WGM_, WGN_) \
+    stiffness_kernel<CT, KT, NPV, MT_, NT_, WGM_, WGN_><<<grid, 64 * WGM_ * WGN_, 0, st>>>(JmT, qp, npe, G, nelem, Ke)
     switch (NP) {
-        case 16: stiffness_kernel<CT, KT, 16, 1, 1, 1, 1><<<grid, 64, 0, st>>>(JmT, qp, npe, G, nelem, Ke); break;
+        case 16: FL_DENSE(16, 1, 1, 1, 1); break;
         case 32:
-            if (tuning_value(TUNE_STIFF_VARIANT_BASE + 1, 1) == 2)
-                stiffness_kernel<CT, KT, 32, 1, 1, 2, 2><<<grid, 256, 0, st>>>(JmT, qp, npe, G, nelem, Ke);
-            else
-                stiffness_kernel<CT, KT, 32, 2, 2, 1, 1><<<grid, 64, 0, st>>>(JmT, qp, npe, G, nelem, Ke);
+            switch (tuning_value(TUNE_STIFF_VARIANT_BASE + 1, 1)) {
+                case 2: FL_DENSE(32, 1, 1, 2, 2); break;
+                case 3: FL_DENSE(32, 2, 1, 1, 2); break;
+                case 4: FL_DENSE(32, 1, 2, 2, 1); break;
+                default: FL_DENSE(32, 2, 2, 1, 1); break;
+            }
             break;
-        case 48: stiffness_kernel<CT, KT, 48, 3, 3, 1, 1><<<grid, 64, 0, st>>>(JmT, qp, npe, G, nelem, Ke); break;
+        case 48:
+            switch (tuning_value(TUNE_STIFF_VARIANT_BASE + 2, 1)) {
+                case 2: FL_DENSE(48, 1, 3, 3, 1); break;
+                case 3: FL_DENSE(48, 3, 1, 1, 3); break;
+                default: FL_DENSE(48, 3, 3, 1, 1); break;
+            }
+            break;
         case 64:
-            if (tuning_value(TUNE_STIFF_VARIANT_BASE + 3, 1) == 2)
-                stiffness_kernel<CT, KT, 64, 4, 4, 1, 1><<<grid, 64, 0, st>>>(JmT, qp, npe, G, nelem, Ke);
-            else
-                stiffness_kernel<CT, KT, 64, 2, 2, 2, 2><<<grid, 256, 0, st>>>(JmT, qp, npe, G, nelem, Ke);
+            switch (tuning_value(TUNE_STIFF_VARIANT_BASE + 3, 1)) {
+                case 2: FL_DENSE(64, 4, 4, 1, 1); break;
+                case 3: FL_DENSE(64, 4, 2, 1, 2); break;
+                case 4: FL_DENSE(64, 2, 4, 2, 1); break;
+                case 5: FL_DENSE(64, 2, 1, 2, 4); break;
+                default: FL_DENSE(64, 2, 2, 2, 2); break;
+            }
             break;
+#undef FL_DENSE
         case 128:
             // default: registers capped for two waves per SIMD (measured faster on MI355X)
             if (tuning_value(TUNE_STIFF_VARIANT_BASE + 4, 1) == 2 || tuning_value(TUNE_STIFF_MINW, 2) == 1) {
@@ -1646,6 +1661,11 @@ int fl_mg_galerkin(const void *df_workspace, size_t df_workspace_bytes, int32_t 
         if (rc) return rc;
     }
     return 0;
+}
+
+int32_t fl_stiffness_variant_count(int32_t np_class) {
+    static const int32_t counts[5] = {1, 4, 3, 5, 3};
+    return np_class >= 0 && np_class < 5 ? counts[np_class] : 0;
 }
 
 int32_t fl_material_table_stride(int32_t material) {

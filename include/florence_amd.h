@@ -395,11 +395,16 @@ int fl_profile_enable(int on);
  * stiffness kernel is compiled for (1 or 2).  value 0 restores the default. */
 #define FL_TUNE_STIFF_MINW 1
 /* keys FL_TUNE_STIFF_VARIANT_BASE + c, c = padded node-count class (16, 32, 48, 64, 128
- * nodes -> 0..4): value v+1 selects tile-shape variant v of the FP64/FP32 stiffness
- * kernel (32: v1 = 2x2 waves of 16x16 tiles; 64: v1 = one wave of 4x4 tiles;
- * 128: v1 = one wave per SIMD, all pairs square; v2 = off-diagonal pairs on 1x4 waves
- * of 8x2 tiles).  Used by the autotuner. */
+ * nodes -> 0..4): value v+1 selects tile-shape variant v of the dense FP64/FP32
+ * stiffness kernel, written (MT x NT tiles of 16 x 16 per wave, WGM x WGN waves):
+ * 32: v0 2x2/1x1, v1 1x1/2x2, v2 2x1/1x2, v3 1x2/2x1; 48: v0 3x3/1x1, v1 1x3/3x1,
+ * v2 3x1/1x3; 64: v0 2x2/2x2, v1 4x4/1x1, v2 4x2/1x2, v3 2x4/2x1, v4 2x1/2x4;
+ * 128: v0 default (off-diagonal pairs 4x4/2x2, then the diagonal triangles), v1 one
+ * wave per SIMD, all pairs square, v2 off-diagonal pairs 8x2/1x4.  Used by the
+ * autotuner; fl_stiffness_variant_count(c) gives the number of variants of class c
+ * (0 for an unknown class).  No reference counterpart. */
 #define FL_TUNE_STIFF_VARIANT_BASE 2
+int32_t fl_stiffness_variant_count(int32_t np_class);
 /* key 8: gather form -- bit 0: nodes of several elements add one element's rows at a
  * time instead of loading eight elements' rows first; bit 1: nodes inside one element
  * go through the general kernel instead of the single-element one.  Same V and T. */

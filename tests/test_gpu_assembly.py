@@ -148,6 +148,41 @@ def test_hex_p4_tile_variants(cuda, variant):
         lib.fl_set_tuning(_lib.FL_TUNE_STIFF_SF, 0)
 
 
+_CLASS_CASES = [("tet", 3, 1, v) for v in range(4)] + [("tet", 4, 2, v) for v in range(3)] + \
+    [("hex", 3, 3, v) for v in range(5)]
+
+
+@pytest.mark.parametrize("et,p,npc,variant", _CLASS_CASES,
+                         ids=["{}{}-class{}-v{}".format(*c) for c in _CLASS_CASES])
+def test_dense_tile_variants(cuda, et, p, npc, variant):
+    """Every tile-shape variant of the dense contraction for the 32-, 48- and 64-node
+    classes (tet p=3, tet p=4, hex p=3 with the factorised path off) meets the oracle bar,
+    in FP64, and in FP32 within the FP32 bar of the FP64 oracle."""
+    import torch
+    from florence_amd import _lib
+    from florence_amd.assembly import DeviceMeshData, assemble_device
+    lib = _lib.load()
+    assert lib.fl_stiffness_variant_count(npc) == _lib.STIFF_VARIANTS[npc] > variant
+    key = _lib.FL_TUNE_STIFF_VARIANT_BASE + npc
+    _lib.check(lib.fl_set_tuning(key, variant + 1), "fl_set_tuning")
+    _lib.check(lib.fl_set_tuning(_lib.FL_TUNE_STIFF_SF, 1), "fl_set_tuning")
+    try:
+        m = cube_mesh(et, p, 2 if et == "hex" else 1, jitter=0.05, seed=11)
+        fs = function_space(m, p)
+        mat = MooneyRivlin(3, mu1=0.5, mu2=0.5, lamb=20.0)
+        x = deformed(m.points, 0.05, seed=12)
+        _check(m, fs, mat, x, "{} p{} dense variant {}".format(et, p, variant))
+        dm = DeviceMeshData(m, fs)
+        V32, _, st = assemble_device(dm, mat, x, precision="fp32")
+        torch.cuda.synchronize()
+        V64, _, _ = assemble_device(dm, mat, x)
+        e32 = float((V32 - V64).abs().max() / V64.abs().max())
+        assert e32 <= 1e-5 and int(st[0]) == 0, e32
+    finally:
+        lib.fl_set_tuning(key, 0)
+        lib.fl_set_tuning(_lib.FL_TUNE_STIFF_SF, 0)
+
+
 @pytest.mark.parametrize("p", [1, 2, 3, 4])
 def test_sum_factorised_equals_dense(cuda, p):
     """Tensor-product hexes: the sum-factorised point, block and traction kernels (default

@@ -51,6 +51,10 @@ def test_host_only_queries():
     assert lib.fl_pattern_scatter_table_bytes(8, 125) == 8 * 5 * 8 * 64 * 8 * 2
     assert lib.fl_pattern_scatter_table_bytes(8, 27) == 8 * 3 * 2 * 64 * 4 * 2
     assert lib.fl_pattern_scatter_table_bytes(8, 10) == 0     # not a hex of p1^3 nodes
+    # tile-shape variants of the dense stiffness per padded node-count class, as the
+    # autotuner enumerates them
+    assert [lib.fl_stiffness_variant_count(c) for c in range(6)] == \
+        [_lib.STIFF_VARIANTS[c] for c in range(5)] + [0]
     # per-element material rows: 4 doubles, 8 for the anisotropic law (its fibre direction)
     assert [lib.fl_material_table_stride(m) for m in range(7)] == [0, 4, 4, 4, 4, 8, 0]
     # the rows the host puts in such a table (the batch groups) have the library's stride
