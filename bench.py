@@ -595,6 +595,9 @@ def main():
                     help="assembly: held-out members of the Newton comparison in the default line (0 = skip)")
     ap.add_argument("--newton-n", type=int, default=24,
                     help="assembly: cells per edge of the config-2 Newton solve in the default line (0 = skip)")
+    ap.add_argument("--numbering", default="first", choices=["first", "reference"],
+                    help="assembly: global node numbering of the p=4 mesh (first appearance, or the reference's "
+                         "GetHighOrderMesh numbering, built on the device)")
     ap.add_argument("--nmesh", type=int, default=256)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16"])
@@ -627,7 +630,7 @@ def main():
         _lib.check(_lib.load().fl_set_tuning(_lib.FL_TUNE_STIFF_VARIANT_BASE + 4, args.stiffness_variant + 1),
                    "fl_set_tuning")
     t_setup = time.perf_counter()
-    mesh, fs, mat = cube_hex_p4(args.n)
+    mesh, fs, mat = cube_hex_p4(args.n, numbering=args.numbering)
     x_host = twisted_state(mesh, 15.0)
     dm = DeviceMeshData(mesh, fs, build_indices=False)
     x = torch.as_tensor(x_host, device="cuda")
@@ -841,7 +844,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "unit cube 24^3 hex p=4 (13824 elements, 125 nodes/element), 6^3 Gauss-Lobatto "
                                "points, Mooney-Rivlin(0.5,0.5,20), twisted state, K+T into reference CSR "
-                               "(nnz {})".format(sizes["nnz"]), **sizes},
+                               "(nnz {})".format(sizes["nnz"]), "numbering": args.numbering, **sizes},
         "roofline": stiff_roof if dominant_is_stiffness else gather_roof,
         "roofline_other_phase": gather_roof if dominant_is_stiffness else stiff_roof,
         "phase_ms": {"qp": round(float(phase_ms[0]), 4), "stiffness": round(float(phase_ms[1]), 4),

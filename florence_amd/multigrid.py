@@ -257,8 +257,8 @@ class MultigridSolver(object):
     FP64 operator, so the solve meets rtol on it).  ``rtol``, ``maxit``, ``check_every``
     and ``iterations`` as LinearSolver."""
 
-    def __init__(self, rtol=1e-12, maxit=2000, check_every=1, cheb_degree=3, cheb_range=20.0,
-                 coarse_iterations=30, power_steps=12, warm_power_steps=3, fp32_smoother=True):
+    def __init__(self, rtol=1e-12, maxit=2000, check_every=1, cheb_degree=3, cheb_range=30.0,
+                 coarse_iterations=60, power_steps=12, warm_power_steps=3, fp32_smoother=True):
         self.linear_solver = "pmg"
         self.rtol = rtol
         self.maxit = maxit

@@ -27,13 +27,15 @@ from .quadrature import QuadratureRule
 __all__ = ["cube_mesh", "cube_tet_p2", "cube_hex_p4", "clamp_and_pull", "clamp_and_twist", "twist_field"]
 
 
-def cube_mesh(element_type, p, n, jitter=0.0, seed=0, spacing="node"):
-    """Unit cube with n^3 linear cells (hexes, or 6 tets each), raised to degree p;
+def cube_mesh(element_type, p, n, jitter=0.0, seed=0, spacing="node", numbering="first"):
+    """Unit cube with n^3 linear cells (hexes, or 6 tets each), raised to degree p
+    (node numbering: "first" appearance, or the reference's, see Mesh.GetHighOrderMesh);
     ``jitter`` is the half-width of the uniform noise in units of h: the node spacing
-    1/(n p) (``spacing="node"``) or the cell edge 1/n (``spacing="cell"``)."""
+    1/(n p) (``spacing="node"``) or the cell edge 1/n (``spacing="cell"``), drawn per node
+    in node order."""
     m = Mesh()
     m.Cube(n=n, element_type=element_type)
-    m.GetHighOrderMesh(p=p)
+    m.GetHighOrderMesh(p=p, numbering=numbering)
     if jitter:
         h = 1.0 / n if spacing == "cell" else 1.0 / (n * p)
         m.JitterInteriorNodes(jitter * h, seed=seed, lower=(0, 0, 0), upper=(1, 1, 1))
@@ -46,8 +48,8 @@ def cube_tet_p2(n=8, seed=0):
     return m, fs, NeoHookean(3, mu=1.0, lamb=10.0)
 
 
-def cube_hex_p4(n=24, seed=0):
-    m = cube_mesh("hex", 4, n, jitter=0.03, seed=seed)
+def cube_hex_p4(n=24, seed=0, numbering="first"):
+    m = cube_mesh("hex", 4, n, jitter=0.03, seed=seed, numbering=numbering)
     fs = FunctionSpace(m, QuadratureRule(qtype="lobatto", norder=6, mesh_type="hex"), p=4)
     return m, fs, MooneyRivlin(3, mu1=0.5, mu2=0.5, lamb=20.0)
 

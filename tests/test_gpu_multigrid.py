@@ -178,3 +178,31 @@ def test_multigrid_newton_matches_oracle(cuda, precision):
         assert max(errs) <= 1e-10
     else:
         assert max(errs) <= 1e-8
+
+
+def test_femsolver_default_linear_solver(cuda):
+    """FEMSolver without a linear solver takes the multigrid for hexes of p >= 2 and
+    block-Jacobi PCG for tets, and either reproduces the oracle's Newton solve (config 2
+    at 2^3 hexes: iterations and every increment within 1e-10)."""
+    from florence_amd.boundary_condition import BoundaryCondition
+    from florence_amd.multigrid import MultigridSolver
+    from florence_amd.assembly import DeviceMeshData
+    from florence_amd.solver import FEMSolver, LinearSolver, default_linear_solver
+    ref = np.load(os.path.join(GOLD, "c2_newton_n2.npz"))
+    m, fs, mat = cube_hex_p4(2)
+
+    class Form(object):
+        ndim = nvar = 3
+        fields = "mechanics"
+        function_spaces = (fs, None, None)
+
+    bc = BoundaryCondition()
+    bc.SetDirichletCriteria(lambda mm: clamp_and_twist(mm, 30.0), m)
+    solver = FEMSolver(optimise=True, number_of_load_increments=10, newton_raphson_tolerance=1e-8)
+    res = solver.Solve(formulation=Form(), mesh=m, material=mat, boundary_condition=bc)
+    assert isinstance(solver.linear_solver, MultigridSolver)
+    assert list(res.newton_iterations) == ref["iterations"].tolist()
+    disp = np.moveaxis(res.TotalDisp.cpu().numpy(), 2, 0)
+    assert max(np.abs(disp[k] - ref["disp"][k]).max() / np.abs(ref["disp"][k]).max() for k in range(10)) <= 1e-10
+    mt = cube_mesh("tet", 2, 1)
+    assert isinstance(default_linear_solver(DeviceMeshData(mt, function_space(mt, 2))), LinearSolver)
