@@ -212,11 +212,30 @@ class FEMSolver(object):
             if self.newton_raphson_failed_to_converge:
                 TotalDisp = TotalDisp[:, :, :Increment]
                 break
+            self._save_increment(TotalDisp, Increment)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         self.assembly_time = ta
         return NewtonResult(TotalDisp, self.NRConvergence, iters, list(self.linear_solver.iterations), wall, ta,
                             self.newton_raphson_failed_to_converge)
+
+    def _save_increment(self, TotalDisp, Increment):
+        """save_incremental_solution (FEMSolver.py:1613-1627): every
+        incremental_solution_save_frequency-th increment's displacement to
+        <incremental_solution_filename>_<Increment>.mat as {'solution': (nnode, 3)}."""
+        if not getattr(self, "save_incremental_solution", False):
+            return
+        if Increment % int(getattr(self, "incremental_solution_save_frequency", 50)) != 0:
+            return
+        filename = getattr(self, "incremental_solution_filename", None)
+        if filename is None:
+            raise ValueError("No file name provided to save incremental solution")
+        if ".mat" in filename:
+            filename = filename.split(".")[0]
+        from scipy.io import savemat
+        # the extension written out: scipy only appends it when the bare name cannot be opened
+        savemat(filename + "_" + str(Increment) + ".mat", {"solution": TotalDisp[:, :, Increment].cpu().numpy()},
+                do_compression=True)
 
     def _newton(self, Increment, dm, material, free, free_f, Eulerx, V, T, Residual, NodalForces, dU, rhs, assemble,
                 tol):
