@@ -85,7 +85,8 @@ SIGNATURES = {
     "fl_mg_galerkin": (ctypes.c_int, [_vp, _sz, _i32, _i64, _i32, _i32, _vp, _i32, _vp, _i64, _vp, _vp, _vp, _vp,
                                       _vp, _P32, _P32, _vp, _vp, _vp, _vp, _sz, _vp, _vp]),
     "fl_mg_workspace_size": (_sz, [_vp, _i32]),
-    "fl_mg_pcg": (ctypes.c_int, [_vp, _i32, _i32, _vp, _vp, _vp, ctypes.c_double, _i32, _i32, _i32,
+    "fl_mf_apply": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "fl_mg_pcg": (ctypes.c_int, [_vp, _i32, _i32, _vp, _vp, _vp, _vp, ctypes.c_double, _i32, _i32, _i32,
                                  ctypes.c_double, _i32, _i32, _i32, _vp, _sz, _P32, ctypes.POINTER(ctypes.c_double),
                                  _vp]),
     "fl_mesh_refine_workspace_size": (_sz, [_i64, _i64, _i32, _i32]),

@@ -430,6 +430,7 @@ __global__ __launch_bounds__(256) void qp_kernel(MatParams mp0, const double *__
 #include "fl_stiffness_bf16.hpp"
 #include "fl_stiffness_sf.hpp"
 #include "fl_mg_galerkin.hpp"
+#include "fl_mg_mf.hpp"
 namespace fl {
 
 // ------------------------------------------------------------------ phase 3
@@ -1174,6 +1175,72 @@ static bool check_table(int material, const int32_t *elem_mat, const double *mat
         return false;
     }
     return true;
+}
+
+int mf_prepare(const FlMgMatrixFree *in, MfOp *op) {
+    if (!in || !op || !in->df_workspace || !in->tensor_basis || !in->elements || !in->node_elem_ptr ||
+        !in->node_elem || !in->element_vectors || in->nelem <= 0 || in->nnode <= 0 ||
+        in->nelem * (int64_t)in->npe >= INT32_MAX) {
+        set_error("matrix-free operator: incomplete description");
+        return FL_E_ARG;
+    }
+    if (in->records != FL_RECORDS_C) {
+        set_error("matrix-free operator: needs the C point records of the factorised hex path");
+        return FL_E_ARG;
+    }
+    if (!sf_shape(in->npe, in->ngauss)) {
+        set_error("matrix-free operator: %d nodes and %d points are not a tensor-product hex rule", in->npe,
+                  in->ngauss);
+        return FL_E_ARG;
+    }
+    size_t need = 0;
+    DFWork fw = carve_df(const_cast<void *>(in->df_workspace), in->df_workspace_bytes, in->ngauss, in->nelem,
+                         in->npe, true, &need, FL_PREC_FP64, false);
+    if (!fw.JmT) {
+        set_error("matrix-free operator: the workspace (%zu bytes) holds no point records for these sizes",
+                  in->df_workspace_bytes);
+        return FL_E_SPACE;
+    }
+    int p1 = 1, q = 1;
+    while (p1 * p1 * p1 < in->npe) ++p1;
+    while (q * q * q < in->ngauss) ++q;
+    op->qp = fw.qp;
+    op->basis = static_cast<const int32_t *>(in->tensor_basis);
+    op->elements = in->elements;
+    op->nelem = in->nelem;
+    op->nnode = in->nnode;
+    op->npe = in->npe;
+    op->G = in->ngauss;
+    op->p1 = p1;
+    op->q = q;
+    op->node_elem_ptr = in->node_elem_ptr;
+    op->node_elem = in->node_elem;
+    op->ye = in->element_vectors;
+    return 0;
+}
+
+int mf_element_apply(const MfOp &op, const uint8_t *free_mask, const double *x, hipStream_t st) {
+    const int64_t chunk = (op.nelem + 7) / 8;
+    const unsigned grid = (unsigned)(8 * chunk);
+#define FL_MF_LAUNCH(P, QQ)                                                                                        \
+    mf_element_kernel<P, QQ><<<grid, 256, 0, st>>>(op.qp, op.basis, op.elements, op.nelem, free_mask, x, op.ye)
+    if (op.p1 == 2 && op.q == 2)
+        FL_MF_LAUNCH(2, 2);
+    else if (op.p1 == 3 && op.q == 3)
+        FL_MF_LAUNCH(3, 3);
+    else if (op.p1 == 4 && op.q == 4)
+        FL_MF_LAUNCH(4, 4);
+    else if (op.p1 == 5 && op.q == 5)
+        FL_MF_LAUNCH(5, 5);
+    else if (op.p1 == 5 && op.q == 6)
+        FL_MF_LAUNCH(5, 6);
+    else {
+        set_error("matrix-free operator: no kernel for %d^3 nodes and %d^3 points", op.p1, op.q);
+        return FL_E_ARG;
+    }
+#undef FL_MF_LAUNCH
+    FL_LAUNCH_CHECK();
+    return 0;
 }
 
 }  // namespace fl

@@ -69,6 +69,23 @@ __host__ __device__ inline int64_t sf_scatter_index(int p1, int64_t e, int az, i
     return ((((e * p1 + az) * sf_col_tiles(p1) + s) * 64 + lane) * sf_row_tiles(p1) + tm) * 4 + r;
 }
 
+// The assembled tangent in matrix-free form (fl_mf_apply, the multigrid's fine-level
+// smoother): the element products run where the point-record layout lives
+// (fl_assemble.hip), the per-node sums with the solver (fl_mg.hip).
+struct MfOp {
+    const double *qp;          // point records of the assembly
+    const int32_t *basis;      // tensor basis table
+    const int32_t *elements;
+    int64_t nelem, nnode;
+    int npe, G, p1, q;
+    const int32_t *node_elem_ptr, *node_elem;
+    double *ye;                // element vectors [nelem][npe][3], local node order
+};
+// Checks a FlMgMatrixFree and resolves its records (FL_E_* with the error set).
+int mf_prepare(const FlMgMatrixFree *in, MfOp *op);
+// ye_e = K_e x_e for every element (x read as 0 on fixed dofs).
+int mf_element_apply(const MfOp &op, const uint8_t *free_mask, const double *x, hipStream_t st);
+
 // Exclusive prefix sum of int64 values in place (n entries, last entry = total), device.
 int exclusive_scan_i64(int64_t *data, int64_t n, int64_t *scratch, size_t scratch_count, hipStream_t st);
 size_t scan_scratch_count(int64_t n);

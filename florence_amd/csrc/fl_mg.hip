@@ -122,6 +122,52 @@ __global__ __launch_bounds__(64 * SP_W) void cheb_step_kernel(const int32_t *__r
     }
 }
 
+// The matrix-free product's node sums: q = sum of the element vectors ye of node a's
+// elements (node_elem: element * npe + local, ascending element order), 0 on fixed dofs.
+// With q_out: q_out = q.  Otherwise as cheb_step_kernel: r -= q and, with d_new, the
+// Chebyshev update.
+__global__ __launch_bounds__(SV_T) void mf_node_kernel(int64_t nnode, const int32_t *__restrict__ node_elem_ptr,
+                                                       const int32_t *__restrict__ node_elem,
+                                                       const double *__restrict__ ye,
+                                                       const uint8_t *__restrict__ free_mask,
+                                                       const double *__restrict__ P, const double *__restrict__ d_old,
+                                                       double *__restrict__ d_new, double *__restrict__ r,
+                                                       double *__restrict__ x, double *__restrict__ q_out, double c1,
+                                                       double c2) {
+    const int64_t a = (int64_t)blockIdx.x * SV_T + threadIdx.x;
+    if (a >= nnode) return;
+    double q[3] = {0.0, 0.0, 0.0};
+    for (int32_t j = node_elem_ptr[a]; j < node_elem_ptr[a + 1]; ++j) {
+        const double *v = ye + 3 * (int64_t)node_elem[j];
+        q[0] += v[0];
+        q[1] += v[1];
+        q[2] += v[2];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        if (!free_mask[3 * a + i]) q[i] = 0.0;
+    if (q_out) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) q_out[3 * a + i] = q[i];
+        return;
+    }
+    double rv[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        rv[i] = r[3 * a + i] - q[i];
+        r[3 * a + i] = rv[i];
+    }
+    if (!d_new) return;
+    double z[3];
+    bj_apply(P, a, rv, z);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const double dn = c1 * d_old[3 * a + i] + c2 * z[i];
+        d_new[3 * a + i] = dn;
+        x[3 * a + i] += dn;
+    }
+}
+
 // coarse b[c] = sum_j R_w[j] r_fine[R_idx[j]], masked by the coarse free dofs
 __global__ __launch_bounds__(SV_T) void restrict_kernel(int64_t nnode_c, const int32_t *__restrict__ R_ptr,
                                                         const int32_t *__restrict__ R_idx,
@@ -355,6 +401,8 @@ struct Mg {
     int nl;
     int v32;
     const float *V0s;   // optional FP32 copy of level 0's values for its smoother (v32 = 0)
+    bool use_mf;        // level 0's smoother applies the matrix-free operator mf
+    MfOp mf;
     int degree;
     double range;
     int coarse_its;
@@ -373,7 +421,20 @@ static unsigned nb_sp(int64_t nnode) { return (unsigned)((nnode + SP_W - 1) / SP
 static unsigned nb_v(int64_t n) { return (unsigned)((n + SV_T - 1) / SV_T); }
 
 // the smoother's view of level l's values: level 0 reads the FP32 copy when one is given
-static bool smooth_f32(const Mg &m, int l) { return l == 0 && (m.v32 || m.V0s != nullptr); }
+static bool smooth_f32(const Mg &m, int l) { return l == 0 && !m.use_mf && (m.v32 || m.V0s != nullptr); }
+
+// the matrix-free product on level 0: element vectors, then node_kernel's sums (q_out, or
+// the Chebyshev step of mf_node_kernel)
+static int mf_product(const Mg &m, const double *d, double *q_out, const double *d_old, double *d_new, double *x,
+                      double c1, double c2) {
+    const FlMgLevel &L = m.lv[0];
+    const int rc = mf_element_apply(m.mf, L.free_mask, d, m.st);
+    if (rc) return rc;
+    mf_node_kernel<<<nb_v(L.nnode), SV_T, 0, m.st>>>(L.nnode, m.mf.node_elem_ptr, m.mf.node_elem, m.mf.ye,
+                                                     L.free_mask, m.w[0].P, d_old, d_new, m.w[0].r, x, q_out, c1, c2);
+    FL_LAUNCH_CHECK();
+    return 0;
+}
 static const float *smooth_v32(const Mg &m, int l) {
     return m.v32 ? lvals<float>(m.lv[l]) : m.V0s;
 }
@@ -382,6 +443,7 @@ static const float *smooth_v32(const Mg &m, int l) {
 // values the smoother reads (the spectrum estimate), else the level's own operator
 static int level_spmv(const Mg &m, int l, const double *d, double *q, double *part, bool smoother = false) {
     const FlMgLevel &L = m.lv[l];
+    if (smoother && l == 0 && m.use_mf && !part) return mf_product(m, d, q, nullptr, nullptr, nullptr, 0.0, 0.0);
     if (smoother ? smooth_f32(m, l) : (l == 0 && m.v32))
         spmv_kernel<float><<<nb_sp(L.nnode), 64 * SP_W, 0, m.st>>>(L.indptr, L.node_nbr_ptr, L.node_nbrs,
                                                                     smoother ? smooth_v32(m, l) : lvals<float>(L),
@@ -398,6 +460,7 @@ static int level_spmv(const Mg &m, int l, const double *d, double *q, double *pa
 static int level_cheb_step(const Mg &m, int l, const double *d_old, double *d_new, double *x, double c1, double c2) {
     const FlMgLevel &L = m.lv[l];
     const LevelWork &w = m.w[l];
+    if (l == 0 && m.use_mf) return mf_product(m, d_old, nullptr, d_old, d_new, x, c1, c2);
     if (smooth_f32(m, l))
         cheb_step_kernel<float><<<nb_sp(L.nnode), 64 * SP_W, 0, m.st>>>(L.indptr, L.node_nbr_ptr, L.node_nbrs,
                                                                          smooth_v32(m, l), L.nnode, L.free_mask, w.P,
@@ -552,10 +615,28 @@ size_t fl_mg_workspace_size(const int64_t *nnodes, int32_t nlevels) {
     return carve_mg(cv, m, lv, nlevels);
 }
 
-int fl_mg_pcg(const FlMgLevel *levels, int32_t nlevels, int32_t v32, const float *smoother_V32, const double *b,
-              double *x, double rtol, int32_t maxit, int32_t check_every, int32_t cheb_degree, double cheb_range,
-              int32_t coarse_iterations, int32_t power_steps, int32_t power_warm, void *workspace,
-              size_t workspace_bytes, int32_t *iterations, double *rel_residual, void *stream) {
+int fl_mf_apply(const FlMgMatrixFree *op, const uint8_t *free_mask, const double *x, double *y, void *stream) {
+    if (!free_mask || !x || !y) {
+        set_error("fl_mf_apply: invalid argument");
+        return FL_E_ARG;
+    }
+    MfOp mo;
+    int rc = mf_prepare(op, &mo);
+    if (rc) return rc;
+    hipStream_t st = as_stream(stream);
+    rc = mf_element_apply(mo, free_mask, x, st);
+    if (rc) return rc;
+    mf_node_kernel<<<nb_v(mo.nnode), SV_T, 0, st>>>(mo.nnode, mo.node_elem_ptr, mo.node_elem, mo.ye, free_mask,
+                                                    nullptr, nullptr, nullptr, nullptr, nullptr, y, 0.0, 0.0);
+    FL_LAUNCH_CHECK();
+    return 0;
+}
+
+int fl_mg_pcg(const FlMgLevel *levels, int32_t nlevels, int32_t v32, const float *smoother_V32,
+              const FlMgMatrixFree *fine_mf, const double *b, double *x, double rtol, int32_t maxit,
+              int32_t check_every, int32_t cheb_degree, double cheb_range, int32_t coarse_iterations,
+              int32_t power_steps, int32_t power_warm, void *workspace, size_t workspace_bytes, int32_t *iterations,
+              double *rel_residual, void *stream) {
     if (!levels || nlevels < 2 || nlevels > FL_MG_MAX_LEVELS || !b || !x || maxit <= 0 || !iterations ||
         !rel_residual || cheb_degree < 1 || cheb_degree > 16 || !(cheb_range > 1.0) || coarse_iterations < 1 ||
         power_steps < 1 || power_steps > 64) {
@@ -579,6 +660,16 @@ int fl_mg_pcg(const FlMgLevel *levels, int32_t nlevels, int32_t v32, const float
     m.nl = nlevels;
     m.v32 = v32 ? 1 : 0;
     m.V0s = v32 ? nullptr : smoother_V32;
+    m.use_mf = fine_mf != nullptr;
+    if (fine_mf) {
+        const int rc = mf_prepare(fine_mf, &m.mf);
+        if (rc) return rc;
+        if (m.mf.nnode != levels[0].nnode) {
+            set_error("fl_mg_pcg: the matrix-free operator has %lld nodes, level 0 %lld", (long long)m.mf.nnode,
+                      (long long)levels[0].nnode);
+            return FL_E_ARG;
+        }
+    }
     m.degree = cheb_degree;
     m.range = cheb_range;
     m.coarse_its = coarse_iterations;

@@ -1,0 +1,180 @@
+// The assembled tangent applied element by element from the point records, for the
+// multigrid's fine-level smoother (included by fl_assemble.hip, which owns the record
+// layout; the per-node sums are in fl_mg.hip).
+//
+// For a tensor-product hex with the factorised path's records (the C tensors, detJ and
+// the point weight included),
+//     y_{a,i} = sum_q sum_m dN_a/dxi_m(q) S_{im}(q),
+//     S_{im}  = sum_{k,n} C^{ik}_{mn}(q) G_{kn}(q),   G_{kn} = sum_b dN_b/dxi_n(q) x_{b,k},
+// with dN_b/dxi_n = prod_d T_d^{[d == n]}[b_d][q_d] (T^0 = l, T^1 = l'; d = zeta, eta,
+// beta; tensor node (ab P1 + ae) P1 + az, tensor point (qb Q + qe) Q + qz).  The gradients
+// are three one-direction contractions,
+//   U1[k][sb][qb][ae,az]  = sum_ab T_b^sb[ab][qb] x_k[ab,ae,az]                    sb = 0, 1
+//   U2[k][n][qb][qe][az]  = sum_ae T_e^se[ae][qe] U1[k][sb][qb][ae,az]   n: (sb, se) = (0,0) (0,1) (1,0)
+//   G[k][n][qb,qe,qz]     = sum_az T_z^{[n = 0]}[az][qz] U2[k][n][qb][qe][az]
+// and the transposed sum runs them backwards:
+//   W[i][m][qb][qe][az]   = sum_qz T_z^{[m = 0]}[az][qz] S[i][m][qb,qe,qz]
+//   Z[i][sb][qb][ae,az]   = sum_qe (sb = 0: T_e^0 W[i][0] + T_e^1 W[i][1]; sb = 1: T_e^0 W[i][2])
+//   y[i][ab,ae,az]        = sum_qb T_b^0[ab][qb] Z[i][0] + T_b^1[ab][qb] Z[i][1]
+// About 124 kflop per p = 4 element, and the 54 C values per point read once (K_e itself
+// is 140,625 doubles).  One workgroup per element, each XCD taking a contiguous range of
+// elements (its L2 then holds the x of neighbouring elements); two LDS buffers alternate
+// between the stages.
+#pragma once
+
+namespace fl {
+
+template <int P1, int Q>
+struct MfLayout {
+    static constexpr int N3 = P1 * P1 * P1, Q3 = Q * Q * Q, PP = P1 * P1, QQ = Q * Q;
+    static constexpr int mx(int a, int b) { return a > b ? a : b; }
+    // buffer A: x (3 N3), U2 (9 QQ P1), S (9 Q3), Z (6 Q PP); buffer B: U1 (6 Q PP), G (9 Q3), W (9 QQ P1)
+    static constexpr int BUF = mx(mx(3 * N3, 9 * QQ * P1), mx(9 * Q3, 6 * Q * PP));
+};
+
+template <int P1, int Q>
+__global__ __launch_bounds__(256) void mf_element_kernel(const double *__restrict__ qp,
+                                                         const int32_t *__restrict__ basis,
+                                                         const int32_t *__restrict__ elements, int64_t nelem,
+                                                         const uint8_t *__restrict__ free_mask,
+                                                         const double *__restrict__ x, double *__restrict__ ye) {
+    typedef SfLayout<P1, Q> L;
+    typedef MfLayout<P1, Q> M;
+    constexpr int N3 = M::N3, Q3 = M::Q3, PP = M::PP, QQ = M::QQ, NT = 256;
+    __shared__ double sF[3][2][P1][Q];
+    __shared__ double sA[M::BUF];
+    __shared__ double sB[M::BUF];
+    __shared__ int32_t sLoc[N3];   // local node of tensor node t
+    __shared__ int16_t sT[Q3];     // tensor point of record point g
+
+    const int64_t chunk = (nelem + 7) / 8;
+    const int64_t k8 = blockIdx.x >> 3;
+    const int64_t e = (int64_t)(blockIdx.x & 7) * chunk + k8;
+    if (k8 >= chunk || e >= nelem) return;
+    const int tid = threadIdx.x;
+    const int32_t *g_of_t = basis + 4;
+    const int32_t *loc = basis + 4 + Q3;
+    const double *f1d = reinterpret_cast<const double *>(basis + L::INTS);
+    const int32_t *conn = elements + e * N3;
+    for (int t = tid; t < 6 * P1 * Q; t += NT) (&sF[0][0][0][0])[t] = f1d[t];
+    for (int t = tid; t < Q3; t += NT) sT[min(max(g_of_t[t], 0), Q3 - 1)] = (int16_t)t;
+    for (int t = tid; t < N3; t += NT) {
+        const int l = min(max(loc[t], 0), N3 - 1);
+        sLoc[t] = l;
+        const int64_t a = conn[l];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) sA[k * N3 + t] = free_mask[3 * a + k] ? x[3 * a + k] : 0.0;
+    }
+    __syncthreads();
+    // beta: U1 -> B
+    for (int t = tid; t < 6 * Q * PP; t += NT) {
+        const int r = t % PP, qb = (t / PP) % Q, ks = t / (PP * Q);
+        const int k = ks >> 1, sb = ks & 1;
+        const double *xk = sA + k * N3 + r;
+        double s = 0.0;
+#pragma unroll
+        for (int ab = 0; ab < P1; ++ab) s += sF[2][sb][ab][qb] * xk[ab * PP];
+        sB[t] = s;
+    }
+    __syncthreads();
+    // eta: U2 -> A
+    for (int t = tid; t < 9 * QQ * P1; t += NT) {
+        const int az = t % P1, qe = (t / P1) % Q, qb = (t / (P1 * Q)) % Q, kn = t / (P1 * QQ);
+        const int k = kn / 3, n = kn - 3 * k;
+        const int sb = n == 2, se = n == 1;
+        const double *u = sB + ((k * 2 + sb) * Q + qb) * PP + az;
+        double s = 0.0;
+#pragma unroll
+        for (int ae = 0; ae < P1; ++ae) s += sF[1][se][ae][qe] * u[ae * P1];
+        sA[t] = s;
+    }
+    __syncthreads();
+    // zeta: G -> B
+    for (int t = tid; t < 9 * Q3; t += NT) {
+        const int qz = t % Q, qbe = (t / Q) % QQ, kn = t / Q3;
+        const int n = kn % 3;
+        const double *u = sA + (kn * QQ + qbe) * P1;
+        double s = 0.0;
+#pragma unroll
+        for (int az = 0; az < P1; ++az) s += sF[0][n == 0][az][qz] * u[az];
+        sB[t] = s;
+    }
+    __syncthreads();
+    // per point: S_im = sum_kn C^{ik}_mn G_kn -> A.  The record holds C^{ik} of the six
+    // pairs i <= k (0,0 1,1 2,2 0,1 0,2 1,2); C^{ki}_mn = C^{ik}_nm.
+    const double *rec = qp + e * (int64_t)QS * Q3;
+    for (int g = tid; g < Q3; g += NT) {
+        const int t = min(max((int)sT[g], 0), Q3 - 1);
+        double gk[3][3], S[3][3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int n = 0; n < 3; ++n) {
+                gk[k][n] = sB[(3 * k + n) * Q3 + t];
+                S[k][n] = 0.0;
+            }
+#pragma unroll
+        for (int p = 0; p < 6; ++p) {
+            const int i = p < 3 ? p : (p == 5 ? 1 : 0), k = p < 3 ? p : (p == 3 ? 1 : 2);
+            double c[9];
+#pragma unroll
+            for (int mn = 0; mn < 9; ++mn) c[mn] = rec[(int64_t)(Q_D + 9 * p + mn) * Q3 + g];
+#pragma unroll
+            for (int m = 0; m < 3; ++m)
+                S[i][m] += c[3 * m] * gk[k][0] + c[3 * m + 1] * gk[k][1] + c[3 * m + 2] * gk[k][2];
+            if (i != k) {
+#pragma unroll
+                for (int m = 0; m < 3; ++m) S[k][m] += c[m] * gk[i][0] + c[3 + m] * gk[i][1] + c[6 + m] * gk[i][2];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int m = 0; m < 3; ++m) sA[(3 * i + m) * Q3 + t] = S[i][m];
+    }
+    __syncthreads();
+    // zeta transposed: W -> B
+    for (int t = tid; t < 9 * QQ * P1; t += NT) {
+        const int az = t % P1, qbe = (t / P1) % QQ, im = t / (P1 * QQ);
+        const int m = im % 3;
+        const double *sv = sA + im * Q3 + qbe * Q;
+        double s = 0.0;
+#pragma unroll
+        for (int qz = 0; qz < Q; ++qz) s += sF[0][m == 0][az][qz] * sv[qz];
+        sB[t] = s;
+    }
+    __syncthreads();
+    // eta transposed: Z -> A
+    for (int t = tid; t < 6 * Q * PP; t += NT) {
+        const int r = t % PP, qb = (t / PP) % Q, is = t / (PP * Q);
+        const int i = is >> 1, sb = is & 1;
+        const int ae = r / P1, az = r - ae * P1;
+        double s = 0.0;
+        if (sb == 0) {
+            const double *w0 = sB + ((i * 3) * QQ + qb * Q) * P1 + az;
+            const double *w1 = sB + ((i * 3 + 1) * QQ + qb * Q) * P1 + az;
+#pragma unroll
+            for (int qe = 0; qe < Q; ++qe) s += sF[1][0][ae][qe] * w0[qe * P1] + sF[1][1][ae][qe] * w1[qe * P1];
+        } else {
+            const double *w2 = sB + ((i * 3 + 2) * QQ + qb * Q) * P1 + az;
+#pragma unroll
+            for (int qe = 0; qe < Q; ++qe) s += sF[1][0][ae][qe] * w2[qe * P1];
+        }
+        sA[t] = s;
+    }
+    __syncthreads();
+    // beta transposed: y, in local node order
+    double *yo = ye + e * (int64_t)N3 * 3;
+    for (int t = tid; t < 3 * N3; t += NT) {
+        const int i = t / N3, tn = t - i * N3;
+        const int ab = tn / PP, r = tn - ab * PP;
+        const double *z0 = sA + (i * 2 * Q) * PP + r;
+        const double *z1 = sA + ((i * 2 + 1) * Q) * PP + r;
+        double s = 0.0;
+#pragma unroll
+        for (int qb = 0; qb < Q; ++qb) s += sF[2][0][ab][qb] * z0[qb * PP] + sF[2][1][ab][qb] * z1[qb * PP];
+        yo[3 * sLoc[tn] + i] = s;
+    }
+}
+
+}  // namespace fl

@@ -31,7 +31,8 @@ import numpy as np
 from . import _lib
 from .bases import FeketePointsTet, GaussLobattoPointsHex, HexBases, TetBases
 
-__all__ = ["MultigridSolver", "PMultigridHierarchy", "hierarchy_tables", "local_node_coordinates", "coarse_degrees"]
+__all__ = ["MultigridSolver", "PMultigridHierarchy", "hierarchy_tables", "local_node_coordinates", "coarse_degrees",
+           "matrix_free_operator", "mf_apply"]
 
 
 class FlMgLevel(ctypes.Structure):
@@ -39,6 +40,45 @@ class FlMgLevel(ctypes.Structure):
                 ("node_nbrs", ctypes.c_void_p), ("V", ctypes.c_void_p), ("free_mask", ctypes.c_void_p),
                 ("P_ptr", ctypes.c_void_p), ("P_idx", ctypes.c_void_p), ("P_w", ctypes.c_void_p),
                 ("R_ptr", ctypes.c_void_p), ("R_idx", ctypes.c_void_p), ("R_w", ctypes.c_void_p)]
+
+
+class FlMgMatrixFree(ctypes.Structure):
+    _fields_ = [("df_workspace", ctypes.c_void_p), ("df_workspace_bytes", ctypes.c_size_t),
+                ("records", ctypes.c_int32), ("tensor_basis", ctypes.c_void_p), ("elements", ctypes.c_void_p),
+                ("nelem", ctypes.c_int64), ("npe", ctypes.c_int32), ("ngauss", ctypes.c_int32),
+                ("nnode", ctypes.c_int64), ("node_elem_ptr", ctypes.c_void_p), ("node_elem", ctypes.c_void_p),
+                ("element_vectors", ctypes.c_void_p)]
+
+
+def matrix_free_operator(dm, precision="fp64", element_vectors=None):
+    """The last assembly's tangent on ``dm`` in the matrix-free form of fl_mf_apply
+    (tensor-product hexes whose assembly left C point records), or None where that form
+    does not exist.  Returns (descriptor, element-vector buffer)."""
+    import torch
+    lib = _lib.load()
+    if dm.tensor_basis is None:
+        return None, None
+    rec = int(lib.fl_df_records_form(_lib.PRECISIONS[precision], _lib.ptr(dm.tensor_basis), dm.npe, dm.ngauss))
+    if rec != _lib.FL_RECORDS_C:
+        return None, None
+    n = dm.nelem * 3 * dm.npe
+    if element_vectors is None or element_vectors.numel() < n or element_vectors.device != dm.device:
+        element_vectors = torch.empty(n, dtype=torch.float64, device=dm.device)
+    ws = dm.workspace()
+    op = FlMgMatrixFree(ws.data_ptr(), ws.numel(), rec, dm.tensor_basis.data_ptr(), dm.elements.data_ptr(),
+                        dm.nelem, dm.npe, dm.ngauss, dm.nnode, dm.node_elem_ptr.data_ptr(), dm.node_elem.data_ptr(),
+                        element_vectors.data_ptr())
+    return op, element_vectors
+
+
+def mf_apply(dm, free_mask, x, y, precision="fp64"):
+    """y = M K M x of the last assembly on ``dm`` without its CSR (fl_mf_apply)."""
+    op, buf = matrix_free_operator(dm, precision)
+    if op is None:
+        raise _lib.FlorenceAMDError("no matrix-free form for this mesh and precision")
+    _lib.check(_lib.load().fl_mf_apply(ctypes.byref(op), _lib.ptr(free_mask), _lib.ptr(x), _lib.ptr(y),
+                                       _lib.stream_ptr()), "fl_mf_apply")
+    return y
 
 
 def local_node_coordinates(element_type, npe):
@@ -254,11 +294,14 @@ class MultigridSolver(object):
     a solver's first solve (cold), warm_power_steps on later ones (started from the
     previous solve's vector: the operator is the next Newton step's); fp32_smoother: an
     FP64 CSR's fine-level smoother reads an FP32 copy of it (the outer iteration keeps the
-    FP64 operator, so the solve meets rtol on it).  ``rtol``, ``maxit``, ``check_every``
-    and ``iterations`` as LinearSolver."""
+    FP64 operator, so the solve meets rtol on it); matrix_free: where the assembly left C
+    point records (tensor-product hexes), the fine-level smoother applies the tangent
+    from them element by element (fl_mf_apply: 54 values per point instead of the CSR's
+    values, no FP32 copy) -- the outer iteration still applies the assembled CSR.
+    ``rtol``, ``maxit``, ``check_every`` and ``iterations`` as LinearSolver."""
 
     def __init__(self, rtol=1e-12, maxit=2000, check_every=1, cheb_degree=3, cheb_range=30.0,
-                 coarse_iterations=60, power_steps=12, warm_power_steps=3, fp32_smoother=True):
+                 coarse_iterations=60, power_steps=12, warm_power_steps=3, fp32_smoother=True, matrix_free=True):
         self.linear_solver = "pmg"
         self.rtol = rtol
         self.maxit = maxit
@@ -269,6 +312,8 @@ class MultigridSolver(object):
         self.power_steps = power_steps
         self.warm_power_steps = warm_power_steps
         self.fp32_smoother = fp32_smoother
+        self.matrix_free = matrix_free
+        self._ye = None
         self.iterations = []
         self._ws = None
         self._h = None
@@ -316,18 +361,22 @@ class MultigridSolver(object):
         if self._ws is None or self._ws.numel() < need:
             self._ws = torch.empty(need, dtype=torch.uint8, device=dm.device)
         v32 = V.dtype == torch.float32
+        mf = None
+        if self.matrix_free:
+            mf, self._ye = matrix_free_operator(dm, precision, self._ye)
         smooth = None
-        if not v32 and self.fp32_smoother:
+        if mf is None and not v32 and self.fp32_smoother:
             if self._v32 is None or self._v32.numel() != V.numel() or self._v32.device != V.device:
                 self._v32 = torch.empty(V.numel(), dtype=torch.float32, device=V.device)
             self._v32.copy_(V)
             smooth = self._v32
         # the power vectors in the workspace belong to this hierarchy and workspace
-        key = (id(h), self._ws.data_ptr(), tuple(nn), smooth is not None or v32)
+        key = (id(h), self._ws.data_ptr(), tuple(nn), smooth is not None or v32, mf is not None)
         warm = self._warm_key == key
         it = ctypes.c_int32(0)
         rel = ctypes.c_double(0.0)
-        _lib.check(lib.fl_mg_pcg(levels, nl, 1 if v32 else 0, _lib.ptr(smooth), _lib.ptr(b), _lib.ptr(x),
+        _lib.check(lib.fl_mg_pcg(levels, nl, 1 if v32 else 0, _lib.ptr(smooth),
+                                 None if mf is None else ctypes.byref(mf), _lib.ptr(b), _lib.ptr(x),
                                  float(self.rtol), int(self.maxit), int(self.check_every), int(self.cheb_degree),
                                  float(self.cheb_range), int(self.coarse_iterations),
                                  int(self.warm_power_steps if warm else self.power_steps), 1 if warm else 0,

@@ -367,18 +367,49 @@ typedef struct {
     const int32_t *R_idx;
     const double *R_w;
 } FlMgLevel;
+/* The assembled tangent applied without its CSR (no reference counterpart): for a
+ * tensor-product hex (npe = p1^3, ngauss = q^3, its tensor basis table; records form
+ * FL_RECORDS_C), K x = sum_e K_e x_e with
+ *   K_e^{ik}_{ab} = sum_g sum_{m,n} dN_a/dxi_m(g) C^{ik}_{mn}(g) dN_b/dxi_n(g)
+ * evaluated by sum factorisation from the C tensors the last fl_df_assemble_* call on
+ * df_workspace left in its point records: the per-point tangent that call contracted
+ * into V, so y is V's product to rounding.  Element vectors go to element_vectors
+ * (double[nelem * 3 * npe]); each node sums its elements' entries in ascending element
+ * order (node_elem_ptr / node_elem of fl_pattern_graph), so a product is bitwise
+ * reproducible.  Masked as the operator of fl_csr_spmv: x is read as 0 on fixed dofs
+ * (free_mask 0) and y is 0 there. */
+typedef struct {
+    const void *df_workspace;
+    size_t df_workspace_bytes;
+    int32_t records;
+    const void *tensor_basis;
+    const int32_t *elements;
+    int64_t nelem;
+    int32_t npe;
+    int32_t ngauss;
+    int64_t nnode;
+    const int32_t *node_elem_ptr;
+    const int32_t *node_elem;
+    double *element_vectors;
+} FlMgMatrixFree;
+int fl_mf_apply(const FlMgMatrixFree *op, const uint8_t *free_mask_dev, const double *x_dev, double *y_dev,
+                void *stream);
+
 /* smoother_V32_dev (optional, v32 = 0): level 0's values rounded to FP32 on the same
  * pattern, read by the fine level's smoother and its spectrum estimate (the outer
  * iteration applies level 0's own FP64 values); NULL: the smoother reads level 0's.
+ * fine_matrix_free (optional): the fine level's smoother and spectrum estimate apply
+ * level 0's operator in the matrix-free form above instead (smoother_V32_dev is then
+ * not read); the outer iteration and the block-Jacobi blocks keep level 0's CSR.
  * power_steps: power iterations per level for the smoother's lmax (1..64); power_warm:
  * nonzero = start them from the vectors the previous call on this workspace left (same
  * levels; the previous Newton step's operator), 0 = from a fixed pseudo-random start. */
 size_t fl_mg_workspace_size(const int64_t *nnodes, int32_t nlevels);
 int fl_mg_pcg(const FlMgLevel *levels, int32_t nlevels, int32_t v32, const float *smoother_V32_dev,
-              const double *b_dev, double *x_dev, double rtol, int32_t maxit, int32_t check_every,
-              int32_t cheb_degree, double cheb_range, int32_t coarse_iterations, int32_t power_steps,
-              int32_t power_warm, void *workspace_dev, size_t workspace_bytes, int32_t *iterations,
-              double *rel_residual, void *stream);
+              const FlMgMatrixFree *fine_matrix_free, const double *b_dev, double *x_dev, double rtol,
+              int32_t maxit, int32_t check_every, int32_t cheb_degree, double cheb_range,
+              int32_t coarse_iterations, int32_t power_steps, int32_t power_warm, void *workspace_dev,
+              size_t workspace_bytes, int32_t *iterations, double *rel_residual, void *stream);
 
 /* ------------------------------------------------------------- profiling
  * No reference counterpart (the reference only records wall time,
