@@ -82,7 +82,7 @@ SIGNATURES = {
     "fl_stiffness_variant_count": (_i32, [_i32]),
     "fl_df_records_form": (_i32, [_i32, _vp, _i32, _i32]),
     "fl_mg_galerkin_workspace_size": (_sz, [_i64, _i32, _i64]),
-    "fl_mg_galerkin": (ctypes.c_int, [_vp, _sz, _i32, _i64, _i32, _i32, _vp, _i32, _vp, _i64, _vp, _vp, _vp, _vp,
+    "fl_mg_galerkin": (ctypes.c_int, [_vp, _sz, _i32, _i64, _i32, _i32, _vp, _vp, _i32, _vp, _i64, _vp, _vp, _vp, _vp,
                                       _vp, _P32, _P32, _vp, _vp, _vp, _vp, _sz, _vp, _vp]),
     "fl_mg_workspace_size": (_sz, [_vp, _i32]),
     "fl_mf_apply": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),

@@ -878,6 +878,39 @@ static int launch_stiffness_sf(int npe, int G, const double *qp, const void *bas
     return FL_E_ARG;
 }
 
+// Galerkin operators of hex coarse levels (the p=2 lattice or the vertices of the fine
+// hexes): the factorised FP64 stiffness of the coarse space over the fine rule's C
+// records, every block stored in the coarse table's tensor node order.
+static bool galerkin_sf_shape(int npe_c, int G) {
+    return (npe_c == 27 && (G == 216 || G == 125)) || (npe_c == 8 && (G == 216 || G == 125 || G == 64 || G == 27));
+}
+
+static int launch_galerkin_sf(int npe_c, int G, const double *qp, const int32_t *basis_c, int64_t nelem, double *Kc,
+                              hipStream_t st) {
+    const int64_t groups = (nelem + 7) / 8 * 8 * 9;
+    if (groups > (int64_t)INT32_MAX) {
+        set_error("too many elements for one launch");
+        return FL_E_LIMIT;
+    }
+    const dim3 grid((unsigned)groups);
+#define FL_GSF_CASE(P, QQ)                                                                                   \
+    if (npe_c == P * P * P && G == QQ * QQ * QQ) {                                                         \
+        stiffness_sf_mfma2_kernel<P, QQ, false, double, double>                                            \
+            <<<grid, 256, 0, st>>>(qp, basis_c, nelem, Kc, nullptr, nullptr);                               \
+        FL_LAUNCH_CHECK();                                                                                 \
+        return 0;                                                                                          \
+    }
+    FL_GSF_CASE(3, 6)
+    FL_GSF_CASE(3, 5)
+    FL_GSF_CASE(2, 6)
+    FL_GSF_CASE(2, 5)
+    FL_GSF_CASE(2, 4)
+    FL_GSF_CASE(2, 3)
+#undef FL_GSF_CASE
+    set_error("no factorised Galerkin kernel for %d coarse nodes and %d points", npe_c, G);
+    return FL_E_ARG;
+}
+
 // Sum-factorised FP32 stiffness blocks (FP32 MFMA, FP32 accumulation) of tensor-product
 // hexes, stored in tensor node order for the gather (no interior-row fusion).
 static int launch_stiffness_sf32(int npe, int G, const double *qp, const void *basis, int64_t nelem, float *Ke,
@@ -1216,6 +1249,14 @@ int mf_prepare(const FlMgMatrixFree *in, MfOp *op) {
     op->node_elem_ptr = in->node_elem_ptr;
     op->node_elem = in->node_elem;
     op->ye = in->element_vectors;
+    op->c32 = nullptr;
+    return 0;
+}
+
+int mf_round_records(const MfOp &op, float *c32, hipStream_t st) {
+    const int64_t n = op.nelem * 54 * (int64_t)op.G;
+    mf_round_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(op.qp, op.G, n, c32);
+    FL_LAUNCH_CHECK();
     return 0;
 }
 
@@ -1223,7 +1264,14 @@ int mf_element_apply(const MfOp &op, const uint8_t *free_mask, const double *x, 
     const int64_t chunk = (op.nelem + 7) / 8;
     const unsigned grid = (unsigned)(8 * chunk);
 #define FL_MF_LAUNCH(P, QQ)                                                                                        \
-    mf_element_kernel<P, QQ><<<grid, 256, 0, st>>>(op.qp, op.basis, op.elements, op.nelem, free_mask, x, op.ye)
+    do {                                                                                                           \
+        if (op.c32)                                                                                                \
+            mf_element_kernel<P, QQ, true><<<grid, 256, 0, st>>>(op.qp, op.c32, op.basis, op.elements, op.nelem,    \
+                                                                 free_mask, x, op.ye);                             \
+        else                                                                                                       \
+            mf_element_kernel<P, QQ, false><<<grid, 256, 0, st>>>(op.qp, op.c32, op.basis, op.elements, op.nelem,   \
+                                                                  free_mask, x, op.ye);                            \
+    } while (0)
     if (op.p1 == 2 && op.q == 2)
         FL_MF_LAUNCH(2, 2);
     else if (op.p1 == 3 && op.q == 3)
@@ -1657,11 +1705,13 @@ size_t fl_mg_galerkin_workspace_size(int64_t nelem, int32_t npe_c, int64_t nnode
     cv.take<double>((size_t)nelem * 9 * npe_c * npe_c);
     cv.take<double>((size_t)nelem * 3 * npe_c);
     cv.take<double>((size_t)3 * nnode_c);
+    cv.take<int32_t>((size_t)nelem * npe_c);
     return cv.off + 256;
 }
 
 int fl_mg_galerkin(const void *df_workspace, size_t df_workspace_bytes, int32_t G, int64_t nelem, int32_t npe,
-                   int32_t records, const double *JmCt, int32_t npe_c, const int32_t *elements_c, int64_t nnode_c,
+                   int32_t records, const double *JmCt, const void *tensor_basis_c, int32_t npe_c,
+                   const int32_t *elements_c, int64_t nnode_c,
                    const int32_t *node_elem_ptr_c, const int32_t *node_elem_c, const int32_t *node_nbr_ptr_c,
                    const int32_t *node_nbrs_c, const int32_t *node_order_c, const int32_t *bucket_offsets_c,
                    const int32_t *bucket_degree_c, const int32_t *indptr_c, const int16_t *ranks_c, double *V_c,
@@ -1694,6 +1744,7 @@ int fl_mg_galerkin(const void *df_workspace, size_t df_workspace_bytes, int32_t 
     double *Kc = cv.take<double>((size_t)nelem * 9 * npe_c * npe_c);
     double *te = cv.take<double>((size_t)nelem * 3 * npe_c);
     double *Tdum = cv.take<double>((size_t)3 * nnode_c);
+    int32_t *slot = cv.take<int32_t>((size_t)nelem * npe_c);
     if (!cv.ok) {
         set_error("fl_mg_galerkin: workspace too small (%zu < %zu)", workspace_bytes, cv.off + 256);
         return FL_E_SPACE;
@@ -1710,12 +1761,26 @@ int fl_mg_galerkin(const void *df_workspace, size_t df_workspace_bytes, int32_t 
     zero_kernel<<<(unsigned)((nte + 255) / 256), 256, 0, st>>>(te, nte);
     FL_LAUNCH_CHECK();
     const int form_c = records == FL_RECORDS_C ? 1 : 0;
-    if (npe_c == 4)
+    // a tensor-product coarse space over C records: the factorised stiffness kernel with
+    // the coarse level's basis table (blocks in its tensor node order, as the fine blocks)
+    const int32_t *kperm = nullptr, *kinv = nullptr, *eslot = nullptr;
+    if (tensor_basis_c && form_c && galerkin_sf_shape(npe_c, G)) {
+        const int32_t *tb = static_cast<const int32_t *>(tensor_basis_c);
+        const int rc = launch_galerkin_sf(npe_c, G, fw.qp, tb, nelem, Kc, st);
+        if (rc) return rc;
+        kperm = tb + 4 + G + npe_c;   // tensor_of_l
+        kinv = tb + 4 + G;            // local_of_t
+        const int64_t ne = nelem * npe_c;
+        entry_slot_kernel<<<(unsigned)((ne + 255) / 256), 256, 0, st>>>(node_elem_c, ne, npe_c, kperm, slot);
+        FL_LAUNCH_CHECK();
+        eslot = slot;
+    } else if (npe_c == 4) {
         mg_galerkin_kernel<4><<<(unsigned)nelem, 256, 0, st>>>(fw.qp, G, form_c, JmCt, nelem, Kc);
-    else if (npe_c == 8)
+    } else if (npe_c == 8) {
         mg_galerkin_kernel<8><<<(unsigned)nelem, 256, 0, st>>>(fw.qp, G, form_c, JmCt, nelem, Kc);
-    else
+    } else {
         mg_galerkin_kernel<27><<<(unsigned)nelem, 256, 0, st>>>(fw.qp, G, form_c, JmCt, nelem, Kc);
+    }
     FL_LAUNCH_CHECK();
     // the coarse CSR: owner-computes rows summed in ascending element order, as the fine gather
     for (int b = 0; b < FL_GATHER_BUCKETS; ++b) {
@@ -1723,8 +1788,8 @@ int fl_mg_galerkin(const void *df_workspace, size_t df_workspace_bytes, int32_t 
         if (nb <= 0) continue;
         const int rc = launch_gather<double, double>(b, nb, elements_c, npe_c, node_order_c + bucket_offsets_c[b],
                                                      node_elem_ptr_c, node_elem_c, node_nbr_ptr_c, node_nbrs_c,
-                                                     indptr_c, Kc, te, bucket_degree_c[b], ranks_c, nullptr, nullptr,
-                                                     nullptr, V_c, Tdum, status, st);
+                                                     indptr_c, Kc, te, bucket_degree_c[b], ranks_c, kperm, kinv,
+                                                     eslot, V_c, Tdum, status, st);
         if (rc) return rc;
     }
     return 0;

@@ -80,11 +80,14 @@ struct MfOp {
     int npe, G, p1, q;
     const int32_t *node_elem_ptr, *node_elem;
     double *ye;                // element vectors [nelem][npe][3], local node order
+    const float *c32;          // optional FP32 copy of the C tensors [nelem][54][G]
 };
 // Checks a FlMgMatrixFree and resolves its records (FL_E_* with the error set).
 int mf_prepare(const FlMgMatrixFree *in, MfOp *op);
-// ye_e = K_e x_e for every element (x read as 0 on fixed dofs).
+// ye_e = K_e x_e for every element (x read as 0 on fixed dofs), from op.c32 when set.
 int mf_element_apply(const MfOp &op, const uint8_t *free_mask, const double *x, hipStream_t st);
+// c32[e][f][g] = (float) C field f of point g of element e (the records' 54 C values).
+int mf_round_records(const MfOp &op, float *c32, hipStream_t st);
 
 // Exclusive prefix sum of int64 values in place (n entries, last entry = total), device.
 int exclusive_scan_i64(int64_t *data, int64_t n, int64_t *scratch, size_t scratch_count, hipStream_t st);

@@ -20,6 +20,7 @@
 // Transfers are the nodal interpolation P (fine node <- coarse nodes of an element
 // containing it) and its transpose, both as per-node CSR lists summed in a fixed order.
 // Every reduction is two-pass in a fixed order: a solve is bitwise reproducible.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <vector>
@@ -165,6 +166,128 @@ __global__ __launch_bounds__(SV_T) void mf_node_kernel(int64_t nnode, const int3
         const double dn = c1 * d_old[3 * a + i] + c2 * z[i];
         d_new[3 * a + i] = dn;
         x[3 * a + i] += dn;
+    }
+}
+
+// The coarsest level's block-Jacobi CG in two launches per step.  Update partials U_k =
+// per-block (r.z, r.r) after k updates (U_0 from cg_start_kernel) rotate through three
+// buffers; each block re-sums the partials it needs in the same fixed order, so every
+// block holds the same alpha and beta and no single-block reduction launch is needed.
+//   step k, part 1 (cc_spmv_kernel): beta = sum U_k[rz] / sum U_{k-1}[rz] (0 at k = 0),
+//     p_new = z + beta p_old on the fly, q = A p_new, partials of p_new.q;
+//   step k, part 2 (cc_update_kernel): alpha = sum U_k[rz] / sum(p.q), x += alpha p,
+//     r -= alpha q, z = P r, partials U_{k+1}.
+template <typename VT>
+__global__ __launch_bounds__(64 * SP_W) void cc_spmv_kernel(const int32_t *__restrict__ indptr,
+                                                            const int32_t *__restrict__ nbr_ptr,
+                                                            const int32_t *__restrict__ nbrs,
+                                                            const VT *__restrict__ V, int64_t nnode,
+                                                            const uint8_t *__restrict__ free_mask,
+                                                            const double *__restrict__ z,
+                                                            const double *__restrict__ p_old, double *__restrict__ p_new,
+                                                            double *__restrict__ q, double *__restrict__ spart,
+                                                            const double *__restrict__ ucur,
+                                                            const double *__restrict__ uprev, int64_t nu) {
+    constexpr int NT = 64 * SP_W;
+    __shared__ double sh0[NT], sh1[NT];
+    double beta = 0.0;
+    if (uprev) {
+        double rz, rr, rzo, rro;
+        block_sum2<NT>(ucur, nu, 2, rz, rr, sh0, sh1);
+        __syncthreads();
+        block_sum2<NT>(uprev, nu, 2, rzo, rro, sh0, sh1);
+        __syncthreads();
+        beta = rzo != 0.0 ? rz / rzo : 0.0;
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t a = (int64_t)blockIdx.x * SP_W + wave;
+    double d = 0.0;
+    if (a < nnode) {
+        const bool all_fixed = !free_mask[3 * a] && !free_mask[3 * a + 1] && !free_mask[3 * a + 2];
+        const int32_t p0 = nbr_ptr[a], deg = all_fixed ? 0 : nbr_ptr[a + 1] - p0;
+        const VT *r0 = V + indptr[3 * a];
+        const VT *r1 = V + indptr[3 * a + 1];
+        const VT *r2 = V + indptr[3 * a + 2];
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        for (int r = lane; r < deg; r += 64) {
+            // z and p_old are +0 on fixed dofs (the block inverse has identity rows there
+            // and r starts masked; p_new is set to 0 there)
+            const int64_t b = nbrs[p0 + r];
+            const double x0 = z[3 * b] + beta * p_old[3 * b];
+            const double x1 = z[3 * b + 1] + beta * p_old[3 * b + 1];
+            const double x2 = z[3 * b + 2] + beta * p_old[3 * b + 2];
+            s0 += (double)r0[3 * r] * x0 + (double)r0[3 * r + 1] * x1 + (double)r0[3 * r + 2] * x2;
+            s1 += (double)r1[3 * r] * x0 + (double)r1[3 * r + 1] * x1 + (double)r1[3 * r + 2] * x2;
+            s2 += (double)r2[3 * r] * x0 + (double)r2[3 * r + 1] * x1 + (double)r2[3 * r + 2] * x2;
+        }
+        s0 = wave_sum(s0);
+        s1 = wave_sum(s1);
+        s2 = wave_sum(s2);
+        if (lane == 0) {
+            const double out[3] = {s0, s1, s2};
+            for (int i = 0; i < 3; ++i) {
+                const int64_t row = 3 * a + i;
+                const bool fixed = !free_mask[row];
+                const double pn = fixed ? 0.0 : z[row] + beta * p_old[row];
+                p_new[row] = pn;
+                const double v = fixed ? 0.0 : out[i];
+                q[row] = v;
+                d += v * pn;
+            }
+        }
+    }
+    if (lane == 0) sh0[wave] = d;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < SP_W; ++w) t += sh0[w];
+        spart[blockIdx.x] = t;
+    }
+}
+
+__global__ __launch_bounds__(SV_T) void cc_update_kernel(int64_t nnode, const double *__restrict__ P,
+                                                         const double *__restrict__ p, const double *__restrict__ q,
+                                                         double *__restrict__ x, double *__restrict__ r,
+                                                         double *__restrict__ z, const double *__restrict__ spart,
+                                                         int64_t ns, const double *__restrict__ ucur, int64_t nu,
+                                                         double *__restrict__ unext) {
+    __shared__ double sh0[SV_T], sh1[SV_T];
+    double pq, unused, rzc, rrc;
+    block_sum2<SV_T>(spart, ns, 1, pq, unused, sh0, sh1);
+    __syncthreads();
+    block_sum2<SV_T>(ucur, nu, 2, rzc, rrc, sh0, sh1);
+    __syncthreads();
+    const double alpha = pq != 0.0 ? rzc / pq : 0.0;
+    const int64_t a = (int64_t)blockIdx.x * SV_T + threadIdx.x;
+    double rz = 0.0, rr = 0.0;
+    if (a < nnode) {
+        double rv[3];
+        for (int i = 0; i < 3; ++i) {
+            const int64_t k = 3 * a + i;
+            x[k] += alpha * p[k];
+            rv[i] = r[k] - alpha * q[k];
+            r[k] = rv[i];
+        }
+        for (int i = 0; i < 3; ++i) {
+            const double zi = P[9 * a + 3 * i] * rv[0] + P[9 * a + 3 * i + 1] * rv[1] + P[9 * a + 3 * i + 2] * rv[2];
+            z[3 * a + i] = zi;
+            rz += rv[i] * zi;
+            rr += rv[i] * rv[i];
+        }
+    }
+    sh0[threadIdx.x] = rz;
+    sh1[threadIdx.x] = rr;
+    __syncthreads();
+    for (int off = SV_T / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            sh0[threadIdx.x] += sh0[threadIdx.x + off];
+            sh1[threadIdx.x] += sh1[threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        unext[2 * blockIdx.x] = sh0[0];
+        unext[2 * blockIdx.x + 1] = sh1[0];
     }
 }
 
@@ -499,27 +622,27 @@ static int chebyshev(Mg &m, int l, const double *b, double **d_last) {
     return 0;
 }
 
-// fixed-step block-Jacobi CG from x = 0 on the coarsest level: x ~ A^-1 b
+// fixed-step block-Jacobi CG from x = 0 on the coarsest level: x ~ A^-1 b (two
+// launches per step: cc_spmv_kernel, cc_update_kernel)
 static int coarse_solve(Mg &m, int l) {
     const FlMgLevel &L = m.lv[l];
     LevelWork &w = m.w[l];
     const int64_t nnode = L.nnode;
-    double *s = m.s + 8;
+    const int64_t ns = nb_sp(nnode), nu = nb_v(nnode);
     double *z = w.d, *q = w.t2;
     double *pbuf[2] = {w.d2, w.t1};
-    zero_small_kernel<<<1, 64, 0, m.st>>>(s, 8);
-    cg_start_kernel<<<nb_v(nnode), SV_T, 0, m.st>>>(nnode, w.P, w.b, L.free_mask, w.x, w.r, z, pbuf[0], m.part);
-    reduce2_kernel<<<1, SV_T, 0, m.st>>>(m.part, nb_v(nnode), s + 0, s + 6);
+    double *spart = m.part;
+    double *ubuf[3] = {m.part + ns, m.part + ns + 2 * nu, m.part + ns + 4 * nu};
+    cg_start_kernel<<<(unsigned)nu, SV_T, 0, m.st>>>(nnode, w.P, w.b, L.free_mask, w.x, w.r, z, pbuf[0], ubuf[0]);
     FL_LAUNCH_CHECK();
     for (int it = 0; it < m.coarse_its; ++it) {
         double *p_old = pbuf[it & 1], *p_new = pbuf[(it + 1) & 1];
-        pcg_spmv_kernel<double><<<nb_sp(nnode), 64 * SP_W, 0, m.st>>>(L.indptr, L.node_nbr_ptr, L.node_nbrs,
-                                                                      lvals<double>(L), nnode, L.free_mask, z, p_old,
-                                                                      p_new, q, m.part, s);
-        pcg_alpha_kernel<<<1, SV_T, 0, m.st>>>(m.part, nb_sp(nnode), s);
-        pcg_update_kernel<<<nb_v(nnode), SV_T, 0, m.st>>>(nnode, s, w.P, p_new, q, w.x, w.r, z,
-                                                          m.part + nb_sp(nnode));
-        pcg_beta_kernel<<<1, SV_T, 0, m.st>>>(m.part + nb_sp(nnode), nb_v(nnode), s);
+        const double *ucur = ubuf[it % 3], *uprev = it ? ubuf[(it + 2) % 3] : nullptr;
+        cc_spmv_kernel<double><<<(unsigned)ns, 64 * SP_W, 0, m.st>>>(L.indptr, L.node_nbr_ptr, L.node_nbrs,
+                                                                     lvals<double>(L), nnode, L.free_mask, z, p_old,
+                                                                     p_new, q, spart, ucur, uprev, nu);
+        cc_update_kernel<<<(unsigned)nu, SV_T, 0, m.st>>>(nnode, w.P, p_new, q, w.x, w.r, z, spart, ns, ucur, nu,
+                                                          ubuf[(it + 1) % 3]);
         FL_LAUNCH_CHECK();
     }
     return 0;
@@ -589,7 +712,9 @@ static size_t carve_mg(Carve &cv, Mg &m, const FlMgLevel *lv, int nl) {
         double **vec[NVEC] = {&m.w[l].x, &m.w[l].r, &m.w[l].d, &m.w[l].d2, &m.w[l].b, &m.w[l].t1, &m.w[l].t2,
                               &m.w[l].pv};
         for (int k = 0; k < NVEC; ++k) *vec[k] = cv.take<double>(3 * (size_t)nn);
-        const int64_t pp = nb_sp(nn) + 2 * (int64_t)nb_v(3 * nn) + 8;
+        // the outer iteration's partials, or the coarse CG's (p.q per SpMV block and three
+        // buffers of (r.z, r.r) per update block)
+        const int64_t pp = std::max<int64_t>(nb_sp(nn) + 2 * (int64_t)nb_v(3 * nn), nb_sp(nn) + 6 * (int64_t)nb_v(nn)) + 8;
         if (pp > maxpart) maxpart = pp;
     }
     m.part = cv.take<double>((size_t)maxpart);
@@ -662,8 +787,13 @@ int fl_mg_pcg(const FlMgLevel *levels, int32_t nlevels, int32_t v32, const float
     m.V0s = v32 ? nullptr : smoother_V32;
     m.use_mf = fine_mf != nullptr;
     if (fine_mf) {
-        const int rc = mf_prepare(fine_mf, &m.mf);
+        int rc = mf_prepare(fine_mf, &m.mf);
         if (rc) return rc;
+        if (fine_mf->records32) {
+            rc = mf_round_records(m.mf, fine_mf->records32, m.st);
+            if (rc) return rc;
+            m.mf.c32 = fine_mf->records32;
+        }
         if (m.mf.nnode != levels[0].nnode) {
             set_error("fl_mg_pcg: the matrix-free operator has %lld nodes, level 0 %lld", (long long)m.mf.nnode,
                       (long long)levels[0].nnode);

@@ -32,8 +32,12 @@ struct MfLayout {
     static constexpr int BUF = mx(mx(3 * N3, 9 * QQ * P1), mx(9 * Q3, 6 * Q * PP));
 };
 
-template <int P1, int Q>
+// C32: the C tensors rounded to FP32 (c32 [e][54][Q3]) instead of the records; each
+// thread's 54 values are loaded into registers when the workgroup starts, so their latency
+// overlaps the gradient stages.
+template <int P1, int Q, bool C32>
 __global__ __launch_bounds__(256) void mf_element_kernel(const double *__restrict__ qp,
+                                                         const float *__restrict__ c32,
                                                          const int32_t *__restrict__ basis,
                                                          const int32_t *__restrict__ elements, int64_t nelem,
                                                          const uint8_t *__restrict__ free_mask,
@@ -41,6 +45,7 @@ __global__ __launch_bounds__(256) void mf_element_kernel(const double *__restric
     typedef SfLayout<P1, Q> L;
     typedef MfLayout<P1, Q> M;
     constexpr int N3 = M::N3, Q3 = M::Q3, PP = M::PP, QQ = M::QQ, NT = 256;
+    static_assert(!C32 || Q3 <= NT, "one point per thread");
     __shared__ double sF[3][2][P1][Q];
     __shared__ double sA[M::BUF];
     __shared__ double sB[M::BUF];
@@ -56,6 +61,12 @@ __global__ __launch_bounds__(256) void mf_element_kernel(const double *__restric
     const int32_t *loc = basis + 4 + Q3;
     const double *f1d = reinterpret_cast<const double *>(basis + L::INTS);
     const int32_t *conn = elements + e * N3;
+    float cr[C32 ? 54 : 1];
+    if constexpr (C32) {
+        const float *r32 = c32 + e * (int64_t)54 * Q3 + min(tid, Q3 - 1);
+#pragma unroll
+        for (int f = 0; f < 54; ++f) cr[f] = r32[f * Q3];
+    }
     for (int t = tid; t < 6 * P1 * Q; t += NT) (&sF[0][0][0][0])[t] = f1d[t];
     for (int t = tid; t < Q3; t += NT) sT[min(max(g_of_t[t], 0), Q3 - 1)] = (int16_t)t;
     for (int t = tid; t < N3; t += NT) {
@@ -102,7 +113,7 @@ __global__ __launch_bounds__(256) void mf_element_kernel(const double *__restric
     __syncthreads();
     // per point: S_im = sum_kn C^{ik}_mn G_kn -> A.  The record holds C^{ik} of the six
     // pairs i <= k (0,0 1,1 2,2 0,1 0,2 1,2); C^{ki}_mn = C^{ik}_nm.
-    const double *rec = qp + e * (int64_t)QS * Q3;
+    const double *rec = qp + e * (int64_t)QS * Q3 + (int64_t)Q_D * Q3;
     for (int g = tid; g < Q3; g += NT) {
         const int t = min(max((int)sT[g], 0), Q3 - 1);
         double gk[3][3], S[3][3];
@@ -118,7 +129,12 @@ __global__ __launch_bounds__(256) void mf_element_kernel(const double *__restric
             const int i = p < 3 ? p : (p == 5 ? 1 : 0), k = p < 3 ? p : (p == 3 ? 1 : 2);
             double c[9];
 #pragma unroll
-            for (int mn = 0; mn < 9; ++mn) c[mn] = rec[(int64_t)(Q_D + 9 * p + mn) * Q3 + g];
+            for (int mn = 0; mn < 9; ++mn) {
+                if constexpr (C32)
+                    c[mn] = (double)cr[9 * p + mn];
+                else
+                    c[mn] = rec[(int64_t)(9 * p + mn) * Q3 + g];
+            }
 #pragma unroll
             for (int m = 0; m < 3; ++m)
                 S[i][m] += c[3 * m] * gk[k][0] + c[3 * m + 1] * gk[k][1] + c[3 * m + 2] * gk[k][2];
@@ -175,6 +191,16 @@ __global__ __launch_bounds__(256) void mf_element_kernel(const double *__restric
         for (int qb = 0; qb < Q; ++qb) s += sF[2][0][ab][qb] * z0[qb * PP] + sF[2][1][ab][qb] * z1[qb * PP];
         yo[3 * sLoc[tn] + i] = s;
     }
+}
+
+// c32[e][f][g] = (float) records' C field f at point g (f < 54)
+__global__ __launch_bounds__(256) void mf_round_kernel(const double *__restrict__ qp, int G, int64_t n,
+                                                       float *__restrict__ c32) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int64_t per = (int64_t)54 * G;
+    const int64_t e = i / per, r = i - e * per;
+    c32[i] = (float)qp[e * (int64_t)QS * G + (int64_t)Q_D * G + r];
 }
 
 }  // namespace fl

@@ -47,13 +47,14 @@ class FlMgMatrixFree(ctypes.Structure):
                 ("records", ctypes.c_int32), ("tensor_basis", ctypes.c_void_p), ("elements", ctypes.c_void_p),
                 ("nelem", ctypes.c_int64), ("npe", ctypes.c_int32), ("ngauss", ctypes.c_int32),
                 ("nnode", ctypes.c_int64), ("node_elem_ptr", ctypes.c_void_p), ("node_elem", ctypes.c_void_p),
-                ("element_vectors", ctypes.c_void_p)]
+                ("element_vectors", ctypes.c_void_p), ("records32", ctypes.c_void_p)]
 
 
-def matrix_free_operator(dm, precision="fp64", element_vectors=None):
+def matrix_free_operator(dm, precision="fp64", element_vectors=None, records32=None):
     """The last assembly's tangent on ``dm`` in the matrix-free form of fl_mf_apply
     (tensor-product hexes whose assembly left C point records), or None where that form
-    does not exist.  Returns (descriptor, element-vector buffer)."""
+    does not exist.  Returns (descriptor, element-vector buffer).  ``records32``: a float32
+    buffer of nelem * 54 * ngauss for fl_mg_pcg's FP32 copy of the C tensors."""
     import torch
     lib = _lib.load()
     if dm.tensor_basis is None:
@@ -67,7 +68,7 @@ def matrix_free_operator(dm, precision="fp64", element_vectors=None):
     ws = dm.workspace()
     op = FlMgMatrixFree(ws.data_ptr(), ws.numel(), rec, dm.tensor_basis.data_ptr(), dm.elements.data_ptr(),
                         dm.nelem, dm.npe, dm.ngauss, dm.nnode, dm.node_elem_ptr.data_ptr(), dm.node_elem.data_ptr(),
-                        element_vectors.data_ptr())
+                        element_vectors.data_ptr(), None if records32 is None else records32.data_ptr())
     return op, element_vectors
 
 
@@ -138,6 +139,56 @@ def coarse_basis(element_type, pc, nodes, points):
     return V @ coef, np.einsum("dpm,ma->dpa", G, coef)
 
 
+def coarse_tensor_basis(pc, cxi, points, JmCt):
+    """fl_tensor_basis table (uint8 numpy) of a hex coarse level of degree pc whose local
+    nodes sit at ``cxi`` (nc, 3) on the lattice {-1, 0, 1} or {-1, 1}, under the fine
+    tensor-product rule ``points``: the 1D Lagrange bases through the lattice at the
+    rule's 1D points, the fine g_of_t and the level's local/tensor node maps; None when
+    the rule is not a tensor product.  The factorisation is checked against ``JmCt``
+    (ngauss, nc, 3), the level's gradients at the points."""
+    z = np.asarray(points, dtype=np.float64)
+    G = z.shape[0]
+    u = [np.unique(np.round(z[:, d], 13)) for d in range(3)]
+    Q = len(u[0])
+    if any(len(v) != Q for v in u) or Q ** 3 != G:
+        return None
+    qi = np.stack([np.searchsorted(u[d], np.round(z[:, d], 13)) for d in range(3)], axis=1)
+    g_of_t = np.full(G, -1, dtype=np.int32)
+    g_of_t[(qi[:, 2] * Q + qi[:, 1]) * Q + qi[:, 0]] = np.arange(G, dtype=np.int32)
+    if (g_of_t < 0).any():
+        return None
+    P1 = pc + 1
+    lat = np.linspace(-1.0, 1.0, P1)
+    f1d = np.zeros((3, 2, P1, Q))
+    for d in range(3):
+        x = u[d]
+        for a in range(P1):
+            others = [b for b in range(P1) if b != a]
+            f1d[d, 0, a] = np.prod([(x - lat[b]) / (lat[a] - lat[b]) for b in others], axis=0)
+            f1d[d, 1, a] = sum(np.prod([(x - lat[b]) / (lat[a] - lat[b]) for b in others if b != c] + [np.ones(Q)],
+                                       axis=0) / (lat[a] - lat[c]) for c in others)
+    idx = np.rint((np.asarray(cxi) + 1.0) * 0.5 * pc).astype(np.int64)
+    if np.abs(lat[idx] - cxi).max() > 1e-12:
+        return None
+    nc = idx.shape[0]
+    tensor_of_l = ((idx[:, 2] * P1 + idx[:, 1]) * P1 + idx[:, 0]).astype(np.int32)
+    if nc != P1 ** 3 or not np.array_equal(np.sort(tensor_of_l), np.arange(nc)):
+        return None
+    local_of_t = np.empty(nc, dtype=np.int32)
+    local_of_t[tensor_of_l] = np.arange(nc, dtype=np.int32)
+    for m in range(3):
+        rec = np.ones((nc, G))
+        for d in range(3):
+            rec *= f1d[d, int(d == m)][idx[:, d]][:, qi[:, d]]
+        ref = np.asarray(JmCt)[:, :, m].T
+        if np.abs(rec - ref).max() > 1e-12 * max(1.0, np.abs(ref).max()):
+            return None
+    ints = [np.array([P1, Q, 0, 0], dtype=np.int32), g_of_t, local_of_t, tensor_of_l]
+    if G % 2:
+        ints.append(np.zeros(1, dtype=np.int32))
+    return np.concatenate([np.concatenate(ints).view(np.uint8), f1d.ravel().view(np.uint8)])
+
+
 def _coarse_locals(element_type, pc, xi):
     """Local fine nodes lying on the degree-pc lattice of the reference element."""
     if element_type == "hex":
@@ -183,7 +234,7 @@ class _Level(object):
     """A coarse level: connectivity, node ids in the fine mesh, device pattern and
     transfer tables to the next finer level."""
 
-    def __init__(self, device, degree, elements, fine_ids, JmCt, P, R):
+    def __init__(self, device, degree, elements, fine_ids, JmCt, P, R, tensor_basis=None):
         import torch
         lib = _lib.load()
         i32 = torch.int32
@@ -193,6 +244,7 @@ class _Level(object):
         self.fine_ids = torch.as_tensor(fine_ids.astype(np.int64), device=device)
         self.elements = torch.as_tensor(np.ascontiguousarray(elements, dtype=np.int32), device=device)
         self.JmCt = torch.as_tensor(np.ascontiguousarray(JmCt), device=device)
+        self.tensor_basis = None if tensor_basis is None else torch.as_tensor(tensor_basis, device=device)
         as32 = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.int32), device=device)
         asf = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device=device)
         self.P_ptr, self.P_idx, self.P_w = as32(P[0]), as32(P[1]), asf(P[2])
@@ -245,8 +297,9 @@ def hierarchy_tables(element_type, elements, nnode, quadrature_points):
     one dict per level, coarsest last, with its degree, connectivity ``elements``
     (coarse node ids), ``fine_ids`` (the fine-mesh node of each coarse node), ``JmCt``
     (ngauss, npe_c, 3) -- the coarse basis gradients at the fine quadrature points --
-    and the prolongation ``P`` = (ptr, idx, w) over the next finer level's nodes with
-    its transpose ``R``."""
+    ``tensor_basis`` (hexes under a tensor-product rule: the level's fl_tensor_basis
+    table, else None) and the prolongation ``P`` = (ptr, idx, w) over the next finer
+    level's nodes with its transpose ``R``."""
     el = np.asarray(elements, dtype=np.int64)
     npe = el.shape[1]
     xi = local_node_coordinates(element_type, npe)
@@ -267,9 +320,10 @@ def hierarchy_tables(element_type, elements, nnode, quadrature_points):
         _, G = coarse_basis(element_type, pc, cxi, z)
         W, _ = coarse_basis(element_type, pc, cxi, fine_xi)              # (npe_finer, nc)
         P = _csr_from_first(fine_el, W, cel, fine_n)
-        out.append({"degree": pc, "elements": cel, "fine_ids": fine_ids[ids_in_fine],
-                    "JmCt": np.ascontiguousarray(np.transpose(G, (1, 2, 0))), "P": P,
-                    "R": _transpose(P[0], P[1], P[2], ids_in_fine.size)})
+        JmCt = np.ascontiguousarray(np.transpose(G, (1, 2, 0)))
+        tb = coarse_tensor_basis(pc, cxi, z, JmCt) if element_type == "hex" else None
+        out.append({"degree": pc, "elements": cel, "fine_ids": fine_ids[ids_in_fine], "JmCt": JmCt,
+                    "tensor_basis": tb, "P": P, "R": _transpose(P[0], P[1], P[2], ids_in_fine.size)})
         fine_el, fine_xi, fine_n, fine_ids = cel, cxi, ids_in_fine.size, fine_ids[ids_in_fine]
     return out
 
@@ -282,8 +336,8 @@ class PMultigridHierarchy(object):
         if z.shape[0] != dm.ngauss:
             raise _lib.FlorenceAMDError("quadrature points do not match the mesh data")
         tabs = hierarchy_tables(element_type, dm.elements.cpu().numpy(), dm.nnode, z)
-        self.levels = [_Level(dm.device, t["degree"], t["elements"], t["fine_ids"], t["JmCt"], t["P"], t["R"])
-                       for t in tabs]
+        self.levels = [_Level(dm.device, t["degree"], t["elements"], t["fine_ids"], t["JmCt"], t["P"], t["R"],
+                              t["tensor_basis"]) for t in tabs]
 
 
 class MultigridSolver(object):
@@ -297,10 +351,11 @@ class MultigridSolver(object):
     FP64 operator, so the solve meets rtol on it); matrix_free: where the assembly left C
     point records (tensor-product hexes), the fine-level smoother applies the tangent
     from them element by element (fl_mf_apply: 54 values per point instead of the CSR's
-    values, no FP32 copy) -- the outer iteration still applies the assembled CSR.
+    values; with fp32_smoother those rounded to FP32 once per solve) -- the outer
+    iteration still applies the assembled CSR.
     ``rtol``, ``maxit``, ``check_every`` and ``iterations`` as LinearSolver."""
 
-    def __init__(self, rtol=1e-12, maxit=2000, check_every=1, cheb_degree=3, cheb_range=30.0,
+    def __init__(self, rtol=1e-12, maxit=2000, check_every=1, cheb_degree=5, cheb_range=60.0,
                  coarse_iterations=60, power_steps=12, warm_power_steps=3, fp32_smoother=True, matrix_free=True):
         self.linear_solver = "pmg"
         self.rtol = rtol
@@ -313,7 +368,9 @@ class MultigridSolver(object):
         self.warm_power_steps = warm_power_steps
         self.fp32_smoother = fp32_smoother
         self.matrix_free = matrix_free
+        self.factorised_galerkin = True
         self._ye = None
+        self._c32 = None
         self.iterations = []
         self._ws = None
         self._h = None
@@ -334,7 +391,9 @@ class MultigridSolver(object):
         rec = int(lib.fl_df_records_form(_lib.PRECISIONS[precision], _lib.ptr(dm.tensor_basis), dm.npe, dm.ngauss))
         P_ = _lib.ptr
         for lv in h.levels:
-            _lib.check(lib.fl_mg_galerkin(P_(ws), ws.numel(), dm.ngauss, dm.nelem, dm.npe, rec, P_(lv.JmCt), lv.npe,
+            tb = lv.tensor_basis if self.factorised_galerkin else None
+            _lib.check(lib.fl_mg_galerkin(P_(ws), ws.numel(), dm.ngauss, dm.nelem, dm.npe, rec, P_(lv.JmCt), P_(tb),
+                                          lv.npe,
                                           P_(lv.elements), lv.nnode, P_(lv.node_elem_ptr), P_(lv.node_elem),
                                           P_(lv.node_nbr_ptr), P_(lv.node_nbrs), P_(lv.node_order),
                                           lv.bucket_offsets, lv.bucket_degree, P_(lv.indptr), P_(lv.ranks),
@@ -363,7 +422,12 @@ class MultigridSolver(object):
         v32 = V.dtype == torch.float32
         mf = None
         if self.matrix_free:
-            mf, self._ye = matrix_free_operator(dm, precision, self._ye)
+            n32 = dm.nelem * 54 * dm.ngauss
+            if self.fp32_smoother and (self._c32 is None or self._c32.numel() < n32 or
+                                       self._c32.device != dm.device):
+                self._c32 = torch.empty(n32, dtype=torch.float32, device=dm.device)
+            mf, self._ye = matrix_free_operator(dm, precision, self._ye,
+                                                self._c32 if self.fp32_smoother else None)
         smooth = None
         if mf is None and not v32 and self.fp32_smoother:
             if self._v32 is None or self._v32.numel() != V.numel() or self._v32.device != V.device:

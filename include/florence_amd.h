@@ -334,13 +334,19 @@ int fl_df_residual_noalloc(int32_t material, const double *params, const int32_t
  * at the fine quadrature points; the coarse mesh (elements_c [nelem][npe_c], npe_c =
  * 4, 8 or 27) and its pattern, gather plan and optional rank table are built by the
  * fl_pattern_* calls above.  workspace: fl_mg_galerkin_workspace_size() bytes.
+ * tensor_basis_c (optional): the coarse level's own fl_tensor_basis table (a hex level:
+ * its degree's 1D Lagrange bases through the lattice {-1, 0, 1} or {-1, 1} at the fine
+ * rule's 1D points, g_of_t of the fine table, its local/tensor node maps); with C
+ * records the operator is then formed by the factorised stiffness kernel of that space
+ * (27 or 8 coarse nodes over 6^3, 5^3, 4^3 or 3^3 points) instead of point by point.
  */
 #define FL_RECORDS_D 0   /* dxi/dx and the D blocks (dense point kernel) */
 #define FL_RECORDS_C 1   /* the C tensors of the factorised FP64 path */
 int32_t fl_df_records_form(int32_t precision, const void *tensor_basis_dev, int32_t npe, int32_t ngauss);
 size_t fl_mg_galerkin_workspace_size(int64_t nelem, int32_t npe_c, int64_t nnode_c);
 int fl_mg_galerkin(const void *df_workspace_dev, size_t df_workspace_bytes, int32_t ngauss, int64_t nelem,
-                   int32_t npe, int32_t records, const double *JmCt_dev, int32_t npe_c, const int32_t *elements_c_dev,
+                   int32_t npe, int32_t records, const double *JmCt_dev, const void *tensor_basis_c_dev,
+                   int32_t npe_c, const int32_t *elements_c_dev,
                    int64_t nnode_c, const int32_t *node_elem_ptr_c_dev, const int32_t *node_elem_c_dev,
                    const int32_t *node_nbr_ptr_c_dev, const int32_t *node_nbrs_c_dev, const int32_t *node_order_c_dev,
                    const int32_t *bucket_offsets_c, const int32_t *bucket_degree_c, const int32_t *indptr_c_dev,
@@ -391,6 +397,7 @@ typedef struct {
     const int32_t *node_elem_ptr;
     const int32_t *node_elem;
     double *element_vectors;
+    float *records32;                /* optional, float[nelem * 54 * ngauss]: see fl_mg_pcg */
 } FlMgMatrixFree;
 int fl_mf_apply(const FlMgMatrixFree *op, const uint8_t *free_mask_dev, const double *x_dev, double *y_dev,
                 void *stream);
@@ -400,7 +407,10 @@ int fl_mf_apply(const FlMgMatrixFree *op, const uint8_t *free_mask_dev, const do
  * iteration applies level 0's own FP64 values); NULL: the smoother reads level 0's.
  * fine_matrix_free (optional): the fine level's smoother and spectrum estimate apply
  * level 0's operator in the matrix-free form above instead (smoother_V32_dev is then
- * not read); the outer iteration and the block-Jacobi blocks keep level 0's CSR.
+ * not read); the outer iteration and the block-Jacobi blocks keep level 0's CSR.  With
+ * its records32 given, the call first rounds the C tensors of the records to FP32 there
+ * and the smoother reads those (half the bytes; fl_mf_apply always reads the FP64
+ * records).
  * power_steps: power iterations per level for the smoother's lmax (1..64); power_warm:
  * nonzero = start them from the vectors the previous call on this workspace left (same
  * levels; the previous Newton step's operator), 0 = from a fixed pseudo-random start. */

@@ -2,8 +2,9 @@
 
 * every coarse level's operator equals the Galerkin product P^T K P of the fine
   tangent computed with scipy from the assembled CSR and the host transfer tables, to
-  1e-12 relative, from both forms of point records (the factorised FP64 path's C
-  tensors, hex p=4; the dense path's dxi/dx and D, the FP32 hex tangent); on the
+  1e-12 relative, from both forms of point records (the factorised paths' C tensors,
+  hex p=2, 3, 4; the dense path's dxi/dx and D) and by both coarse kernels (the
+  factorised stiffness kernel of the coarse space, the per-point form); on the
   non-conforming 6-tet split (tet p=2) it equals the element-by-element product of the
   oracle's element matrices with each element's own p=2 -> p=1 interpolation;
 * a solve meets its tolerance on the masked operator (checked by a product with the
@@ -42,7 +43,7 @@ def _P(t, nfine):
     return sp.kron(P, sp.eye(3), format="csr")
 
 
-@pytest.mark.parametrize("case", ["hex4-fp64", "hex4-fp32", "tet2"])
+@pytest.mark.parametrize("case", ["hex4-fp64", "hex4-fp32", "hex4-pointwise", "hex3", "hex2", "tet2"])
 def test_galerkin_operators(cuda, case):
     import torch
     from florence_amd.assembly import DeviceMeshData, assemble_device
@@ -50,6 +51,13 @@ def test_galerkin_operators(cuda, case):
     if case.startswith("hex4"):
         m, fs, mat = cube_hex_p4(2)
         x = twisted_state(m, 12.0)
+    elif case.startswith("hex"):
+        from florence_amd.materials import MooneyRivlin
+        p = int(case[3])
+        m = cube_mesh("hex", p, 3, jitter=0.05, seed=p)
+        fs = function_space(m, p)
+        mat = MooneyRivlin(3, mu1=0.5, mu2=0.5, lamb=20.0)
+        x = deformed(m.points, 0.05, seed=40 + p)
     else:
         m = cube_mesh("tet", 2, 2, jitter=0.05, seed=2)
         fs = function_space(m, 2)
@@ -63,7 +71,12 @@ def test_galerkin_operators(cuda, case):
     if prec == "fp32":
         assemble_device(dm, mat, x, precision="fp32")   # the records the Galerkin step reads
     mg = MultigridSolver()
+    # hexes form the coarse operators with the factorised stiffness kernel of each coarse
+    # space; "pointwise" takes the per-point form the tets use
+    mg.factorised_galerkin = case != "hex4-pointwise"
     h = mg.galerkin(dm, prec)
+    if case.startswith("hex"):
+        assert all(lv.tensor_basis is not None for lv in h.levels)
     torch.cuda.synchronize()
     tabs = hierarchy_tables(m.element_type, m.elements, m.nnode, fs.quadrature.points)
     Kc, nf = K, m.nnode
