@@ -523,7 +523,7 @@ struct Mg {
     const FlMgLevel *lv;
     int nl;
     int v32;
-    const float *V0s;   // optional FP32 copy of level 0's values for its smoother (v32 = 0)
+    const float *Vs32[FL_MG_MAX_LEVELS];   // optional FP32 copies of the levels' values for their smoothers
     bool use_mf;        // level 0's smoother applies the matrix-free operator mf
     MfOp mf;
     int degree;
@@ -544,7 +544,10 @@ static unsigned nb_sp(int64_t nnode) { return (unsigned)((nnode + SP_W - 1) / SP
 static unsigned nb_v(int64_t n) { return (unsigned)((n + SV_T - 1) / SV_T); }
 
 // the smoother's view of level l's values: level 0 reads the FP32 copy when one is given
-static bool smooth_f32(const Mg &m, int l) { return l == 0 && !m.use_mf && (m.v32 || m.V0s != nullptr); }
+static bool smooth_f32(const Mg &m, int l) {
+    if (l == 0 && (m.use_mf || m.v32)) return !m.use_mf;
+    return m.Vs32[l] != nullptr;
+}
 
 // the matrix-free product on level 0: element vectors, then node_kernel's sums (q_out, or
 // the Chebyshev step of mf_node_kernel)
@@ -559,7 +562,7 @@ static int mf_product(const Mg &m, const double *d, double *q_out, const double 
     return 0;
 }
 static const float *smooth_v32(const Mg &m, int l) {
-    return m.v32 ? lvals<float>(m.lv[l]) : m.V0s;
+    return l == 0 && m.v32 ? lvals<float>(m.lv[l]) : m.Vs32[l];
 }
 
 // q = A_l d, with per-block partials of d.q when part is given; smoother: with the
@@ -757,7 +760,7 @@ int fl_mf_apply(const FlMgMatrixFree *op, const uint8_t *free_mask, const double
     return 0;
 }
 
-int fl_mg_pcg(const FlMgLevel *levels, int32_t nlevels, int32_t v32, const float *smoother_V32,
+int fl_mg_pcg(const FlMgLevel *levels, int32_t nlevels, int32_t v32, const float *const *smoother_V32,
               const FlMgMatrixFree *fine_mf, const double *b, double *x, double rtol, int32_t maxit,
               int32_t check_every, int32_t cheb_degree, double cheb_range, int32_t coarse_iterations,
               int32_t power_steps, int32_t power_warm, void *workspace, size_t workspace_bytes, int32_t *iterations,
@@ -784,7 +787,7 @@ int fl_mg_pcg(const FlMgLevel *levels, int32_t nlevels, int32_t v32, const float
     m.lv = levels;
     m.nl = nlevels;
     m.v32 = v32 ? 1 : 0;
-    m.V0s = v32 ? nullptr : smoother_V32;
+    for (int l = 0; l < nlevels; ++l) m.Vs32[l] = smoother_V32 && !(l == 0 && v32) ? smoother_V32[l] : nullptr;
     m.use_mf = fine_mf != nullptr;
     if (fine_mf) {
         int rc = mf_prepare(fine_mf, &m.mf);

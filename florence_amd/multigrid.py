@@ -428,18 +428,29 @@ class MultigridSolver(object):
                 self._c32 = torch.empty(n32, dtype=torch.float32, device=dm.device)
             mf, self._ye = matrix_free_operator(dm, precision, self._ye,
                                                 self._c32 if self.fp32_smoother else None)
+        # FP32 copies of the values the smoothers read: level 0 (unless matrix-free or
+        # FP32 already) and the intermediate levels (the coarsest one runs CG on its own)
         smooth = None
-        if mf is None and not v32 and self.fp32_smoother:
-            if self._v32 is None or self._v32.numel() != V.numel() or self._v32.device != V.device:
-                self._v32 = torch.empty(V.numel(), dtype=torch.float32, device=V.device)
-            self._v32.copy_(V)
-            smooth = self._v32
+        if self.fp32_smoother:
+            srcs = [None if (mf is not None or v32) else V] + [lv.V for lv in h.levels[:-1]] + [None]
+            if self._v32 is None or len(self._v32) != nl:
+                self._v32 = [None] * nl
+            ptrs = (ctypes.c_void_p * nl)()
+            for i, src in enumerate(srcs):
+                if src is None:
+                    continue
+                buf = self._v32[i]
+                if buf is None or buf.numel() != src.numel() or buf.device != src.device:
+                    buf = self._v32[i] = torch.empty(src.numel(), dtype=torch.float32, device=src.device)
+                buf.copy_(src)
+                ptrs[i] = buf.data_ptr()
+            smooth = ptrs
         # the power vectors in the workspace belong to this hierarchy and workspace
-        key = (id(h), self._ws.data_ptr(), tuple(nn), smooth is not None or v32, mf is not None)
+        key = (id(h), self._ws.data_ptr(), tuple(nn), bool(self.fp32_smoother), v32, mf is not None)
         warm = self._warm_key == key
         it = ctypes.c_int32(0)
         rel = ctypes.c_double(0.0)
-        _lib.check(lib.fl_mg_pcg(levels, nl, 1 if v32 else 0, _lib.ptr(smooth),
+        _lib.check(lib.fl_mg_pcg(levels, nl, 1 if v32 else 0, smooth,
                                  None if mf is None else ctypes.byref(mf), _lib.ptr(b), _lib.ptr(x),
                                  float(self.rtol), int(self.maxit), int(self.check_every), int(self.cheb_degree),
                                  float(self.cheb_range), int(self.coarse_iterations),

@@ -402,11 +402,13 @@ typedef struct {
 int fl_mf_apply(const FlMgMatrixFree *op, const uint8_t *free_mask_dev, const double *x_dev, double *y_dev,
                 void *stream);
 
-/* smoother_V32_dev (optional, v32 = 0): level 0's values rounded to FP32 on the same
- * pattern, read by the fine level's smoother and its spectrum estimate (the outer
- * iteration applies level 0's own FP64 values); NULL: the smoother reads level 0's.
+/* smoother_V32 (optional host array of nlevels device pointers): smoother_V32[l] =
+ * level l's values rounded to FP32 on the same pattern, read by level l's smoother and
+ * its spectrum estimate (the outer iteration, the block-Jacobi blocks and the coarsest
+ * level's CG apply the levels' own FP64 values); a NULL array or entry: the smoother
+ * reads the level's own values (level 0 of v32 = 1 is FP32 already).
  * fine_matrix_free (optional): the fine level's smoother and spectrum estimate apply
- * level 0's operator in the matrix-free form above instead (smoother_V32_dev is then
+ * level 0's operator in the matrix-free form above instead (smoother_V32[0] is then
  * not read); the outer iteration and the block-Jacobi blocks keep level 0's CSR.  With
  * its records32 given, the call first rounds the C tensors of the records to FP32 there
  * and the smoother reads those (half the bytes; fl_mf_apply always reads the FP64
@@ -415,7 +417,7 @@ int fl_mf_apply(const FlMgMatrixFree *op, const uint8_t *free_mask_dev, const do
  * nonzero = start them from the vectors the previous call on this workspace left (same
  * levels; the previous Newton step's operator), 0 = from a fixed pseudo-random start. */
 size_t fl_mg_workspace_size(const int64_t *nnodes, int32_t nlevels);
-int fl_mg_pcg(const FlMgLevel *levels, int32_t nlevels, int32_t v32, const float *smoother_V32_dev,
+int fl_mg_pcg(const FlMgLevel *levels, int32_t nlevels, int32_t v32, const float *const *smoother_V32,
               const FlMgMatrixFree *fine_matrix_free, const double *b_dev, double *x_dev, double rtol,
               int32_t maxit, int32_t check_every, int32_t cheb_degree, double cheb_range,
               int32_t coarse_iterations, int32_t power_steps, int32_t power_warm, void *workspace_dev,
