@@ -461,7 +461,9 @@ def config2_newton(n, linear, lin_rtol, log):
     increments, each to |R_free| <= 1e-8 R0 -- three modes of the lockstep driver:
     the plain FP64 path (FP64 tangent and CSR, the reference's predictor start) with
     fixed linear tolerance (lin_rtol) and with the adaptive one, and the fast path (FP32
-    tangent, FP32-stored CSR, extrapolated start, adaptive linear tolerance)."""
+    tangent, FP32-stored CSR, extrapolated start, adaptive linear tolerance).  The
+    multigrid's coarse levels are per-mesh set-up, like the sparsity pattern: built
+    before the timed solve, their time is mg_setup_s."""
     import torch
     from florence_amd.learned import LockstepNewton, TwistStretchFamily
     fam = TwistStretchFamily(n=n, nincr=10, theta_range=(30.0, 30.0), stretch_range=(1.0, 1.0))
@@ -477,6 +479,7 @@ def config2_newton(n, linear, lin_rtol, log):
         key = "{}+{}+{}".format(prec, start, forcing) + ("+csr32" if csr == "fp32" else "")
         modes[key] = {"wall_s": round(r["wall_s"], 3), "linear_s": round(r["linear_s"], 3),
                       "assembly_s": round(r["assembly_s"], 3), "residual_s": round(r["residual_s"], 3),
+                      "mg_setup_s": round(ln.setup_s, 3),
                       "newton_iterations": r["lockstep_iterations"], "pcg_iterations_total":
                       int(sum(r["pcg_iterations"])), "linear_iterations": list(r["pcg_iterations"]),
                       "failed": r["failed"],

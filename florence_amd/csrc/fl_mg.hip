@@ -169,6 +169,15 @@ __global__ __launch_bounds__(SV_T) void mf_node_kernel(int64_t nnode, const int3
     }
 }
 
+// Sum of part[stride * i + col], i < n, by one wave: lane l takes i = l, l + 64, ... in
+// order, then a butterfly; every wave of every block gets the same value.
+__device__ inline double wave_sum_partials(const double *__restrict__ part, int64_t n, int stride, int col,
+                                           int lane) {
+    double a = 0.0;
+    for (int64_t i = lane; i < n; i += 64) a += part[stride * i + col];
+    return wave_sum(a);
+}
+
 // The coarsest level's block-Jacobi CG in two launches per step.  Update partials U_k =
 // per-block (r.z, r.r) after k updates (U_0 from cg_start_kernel) rotate through three
 // buffers; each block re-sums the partials it needs in the same fixed order, so every
@@ -188,18 +197,13 @@ __global__ __launch_bounds__(64 * SP_W) void cc_spmv_kernel(const int32_t *__res
                                                             double *__restrict__ q, double *__restrict__ spart,
                                                             const double *__restrict__ ucur,
                                                             const double *__restrict__ uprev, int64_t nu) {
-    constexpr int NT = 64 * SP_W;
-    __shared__ double sh0[NT], sh1[NT];
+    __shared__ double sh0[SP_W];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     double beta = 0.0;
     if (uprev) {
-        double rz, rr, rzo, rro;
-        block_sum2<NT>(ucur, nu, 2, rz, rr, sh0, sh1);
-        __syncthreads();
-        block_sum2<NT>(uprev, nu, 2, rzo, rro, sh0, sh1);
-        __syncthreads();
+        const double rz = wave_sum_partials(ucur, nu, 2, 0, lane), rzo = wave_sum_partials(uprev, nu, 2, 0, lane);
         beta = rzo != 0.0 ? rz / rzo : 0.0;
     }
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t a = (int64_t)blockIdx.x * SP_W + wave;
     double d = 0.0;
     if (a < nnode) {
@@ -252,11 +256,8 @@ __global__ __launch_bounds__(SV_T) void cc_update_kernel(int64_t nnode, const do
                                                          int64_t ns, const double *__restrict__ ucur, int64_t nu,
                                                          double *__restrict__ unext) {
     __shared__ double sh0[SV_T], sh1[SV_T];
-    double pq, unused, rzc, rrc;
-    block_sum2<SV_T>(spart, ns, 1, pq, unused, sh0, sh1);
-    __syncthreads();
-    block_sum2<SV_T>(ucur, nu, 2, rzc, rrc, sh0, sh1);
-    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const double pq = wave_sum_partials(spart, ns, 1, 0, lane), rzc = wave_sum_partials(ucur, nu, 2, 0, lane);
     const double alpha = pq != 0.0 ? rzc / pq : 0.0;
     const int64_t a = (int64_t)blockIdx.x * SV_T + threadIdx.x;
     double rz = 0.0, rr = 0.0;
