@@ -74,6 +74,25 @@ def test_host_only_queries():
     assert rc == -3 and b"fixed maximum" in lib.fl_last_error()
 
 
+def test_matrix_free_rejections():
+    """fl_mf_apply refuses, on the host before any device call, records that are not the
+    factorised path's C tensors and elements that are not tensor-product hexes."""
+    from florence_amd.multigrid import FlMgMatrixFree
+    lib = _lib.load()
+    fake = 256   # never dereferenced: every case is refused before the first launch
+    def op(records, npe, ngauss):
+        return FlMgMatrixFree(fake, 1 << 30, records, fake, fake, 8, npe, ngauss, 125, fake, fake, fake, None)
+    y = ctypes.c_void_p(fake)
+    rc = lib.fl_mf_apply(ctypes.byref(op(_lib.FL_RECORDS_D, 125, 216)), y, y, y, None)
+    assert rc == _lib.FL_E_ARG and b"C point records" in lib.fl_last_error()
+    rc = lib.fl_mf_apply(ctypes.byref(op(_lib.FL_RECORDS_C, 10, 11)), y, y, y, None)
+    assert rc == _lib.FL_E_ARG and b"tensor-product hex" in lib.fl_last_error()
+    rc = lib.fl_mf_apply(None, y, y, y, None)
+    assert rc == _lib.FL_E_ARG
+    rc = lib.fl_mf_apply(ctypes.byref(op(_lib.FL_RECORDS_C, 125, 216)), None, y, y, None)
+    assert rc == _lib.FL_E_ARG
+
+
 def lib_noalloc_max():
     """FL_NOALLOC_SCRATCH_BYTES of include/florence_amd.h."""
     with open(os.path.join(ROOT, "include", "florence_amd.h")) as f:
