@@ -256,8 +256,12 @@ __global__ __launch_bounds__(SV_T) void cc_update_kernel(int64_t nnode, const do
                                                          int64_t ns, const double *__restrict__ ucur, int64_t nu,
                                                          double *__restrict__ unext) {
     __shared__ double sh0[SV_T], sh1[SV_T];
-    const int lane = threadIdx.x & 63;
-    const double pq = wave_sum_partials(spart, ns, 1, 0, lane), rzc = wave_sum_partials(ucur, nu, 2, 0, lane);
+    // p.q over the SpMV blocks (thousands): the whole workgroup, four loads in flight per
+    // thread; r.z over the update blocks (tens): one wave
+    double pq, unused;
+    block_sum2<SV_T>(spart, ns, 1, pq, unused, sh0, sh1);
+    __syncthreads();
+    const double rzc = wave_sum_partials(ucur, nu, 2, 0, threadIdx.x & 63);
     const double alpha = pq != 0.0 ? rzc / pq : 0.0;
     const int64_t a = (int64_t)blockIdx.x * SV_T + threadIdx.x;
     double rz = 0.0, rr = 0.0;
