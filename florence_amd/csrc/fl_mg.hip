@@ -186,6 +186,17 @@ __device__ inline double wave_sum_partials(const double *__restrict__ part, int6
 //     p_new = z + beta p_old on the fly, q = A p_new, partials of p_new.q;
 //   step k, part 2 (cc_update_kernel): alpha = sum U_k[rz] / sum(p.q), x += alpha p,
 //     r -= alpha q, z = P r, partials U_{k+1}.
+// The coarsest level is a vertex space (≈ 27 neighbours per node at most on hexes): half a
+// wave per node, CC_NB nodes per workgroup.
+constexpr int CC_LPN = 32;
+constexpr int CC_NB = 64 * SP_W / CC_LPN;
+
+__device__ inline double group_sum(double v) {
+#pragma unroll
+    for (int off = CC_LPN / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
 template <typename VT>
 __global__ __launch_bounds__(64 * SP_W) void cc_spmv_kernel(const int32_t *__restrict__ indptr,
                                                             const int32_t *__restrict__ nbr_ptr,
@@ -197,14 +208,14 @@ __global__ __launch_bounds__(64 * SP_W) void cc_spmv_kernel(const int32_t *__res
                                                             double *__restrict__ q, double *__restrict__ spart,
                                                             const double *__restrict__ ucur,
                                                             const double *__restrict__ uprev, int64_t nu) {
-    __shared__ double sh0[SP_W];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __shared__ double sh0[CC_NB];
+    const int lane = threadIdx.x & 63, grp = threadIdx.x / CC_LPN, gl = threadIdx.x % CC_LPN;
     double beta = 0.0;
     if (uprev) {
         const double rz = wave_sum_partials(ucur, nu, 2, 0, lane), rzo = wave_sum_partials(uprev, nu, 2, 0, lane);
         beta = rzo != 0.0 ? rz / rzo : 0.0;
     }
-    const int64_t a = (int64_t)blockIdx.x * SP_W + wave;
+    const int64_t a = (int64_t)blockIdx.x * CC_NB + grp;
     double d = 0.0;
     if (a < nnode) {
         const bool all_fixed = !free_mask[3 * a] && !free_mask[3 * a + 1] && !free_mask[3 * a + 2];
@@ -213,7 +224,7 @@ __global__ __launch_bounds__(64 * SP_W) void cc_spmv_kernel(const int32_t *__res
         const VT *r1 = V + indptr[3 * a + 1];
         const VT *r2 = V + indptr[3 * a + 2];
         double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-        for (int r = lane; r < deg; r += 64) {
+        for (int r = gl; r < deg; r += CC_LPN) {
             // z and p_old are +0 on fixed dofs (the block inverse has identity rows there
             // and r starts masked; p_new is set to 0 there)
             const int64_t b = nbrs[p0 + r];
@@ -224,10 +235,10 @@ __global__ __launch_bounds__(64 * SP_W) void cc_spmv_kernel(const int32_t *__res
             s1 += (double)r1[3 * r] * x0 + (double)r1[3 * r + 1] * x1 + (double)r1[3 * r + 2] * x2;
             s2 += (double)r2[3 * r] * x0 + (double)r2[3 * r + 1] * x1 + (double)r2[3 * r + 2] * x2;
         }
-        s0 = wave_sum(s0);
-        s1 = wave_sum(s1);
-        s2 = wave_sum(s2);
-        if (lane == 0) {
+        s0 = group_sum(s0);
+        s1 = group_sum(s1);
+        s2 = group_sum(s2);
+        if (gl == 0) {
             const double out[3] = {s0, s1, s2};
             for (int i = 0; i < 3; ++i) {
                 const int64_t row = 3 * a + i;
@@ -240,11 +251,11 @@ __global__ __launch_bounds__(64 * SP_W) void cc_spmv_kernel(const int32_t *__res
             }
         }
     }
-    if (lane == 0) sh0[wave] = d;
+    if (gl == 0) sh0[grp] = d;
     __syncthreads();
     if (threadIdx.x == 0) {
         double t = 0.0;
-        for (int w = 0; w < SP_W; ++w) t += sh0[w];
+        for (int w = 0; w < CC_NB; ++w) t += sh0[w];
         spart[blockIdx.x] = t;
     }
 }
@@ -636,7 +647,7 @@ static int coarse_solve(Mg &m, int l) {
     const FlMgLevel &L = m.lv[l];
     LevelWork &w = m.w[l];
     const int64_t nnode = L.nnode;
-    const int64_t ns = nb_sp(nnode), nu = nb_v(nnode);
+    const int64_t ns = (nnode + CC_NB - 1) / CC_NB, nu = nb_v(nnode);
     double *z = w.d, *q = w.t2;
     double *pbuf[2] = {w.d2, w.t1};
     double *spart = m.part;
