@@ -307,25 +307,32 @@ __global__ __launch_bounds__(SV_T) void cc_update_kernel(int64_t nnode, const do
     }
 }
 
-// coarse b[c] = sum_j R_w[j] r_fine[R_idx[j]], masked by the coarse free dofs
-__global__ __launch_bounds__(SV_T) void restrict_kernel(int64_t nnode_c, const int32_t *__restrict__ R_ptr,
-                                                        const int32_t *__restrict__ R_idx,
-                                                        const double *__restrict__ R_w,
-                                                        const uint8_t *__restrict__ free_c,
-                                                        const double *__restrict__ r, double *__restrict__ bc) {
-    const int64_t c = (int64_t)blockIdx.x * SV_T + threadIdx.x;
+// coarse b[c] = sum_j R_w[j] r_fine[R_idx[j]], masked by the coarse free dofs; one wave
+// per coarse node (its list holds tens of fine nodes), lanes striding over the list
+__global__ __launch_bounds__(64 * SP_W) void restrict_kernel(int64_t nnode_c, const int32_t *__restrict__ R_ptr,
+                                                             const int32_t *__restrict__ R_idx,
+                                                             const double *__restrict__ R_w,
+                                                             const uint8_t *__restrict__ free_c,
+                                                             const double *__restrict__ r, double *__restrict__ bc) {
+    const int lane = threadIdx.x & 63;
+    const int64_t c = (int64_t)blockIdx.x * SP_W + (threadIdx.x >> 6);
     if (c >= nnode_c) return;
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (int32_t j = R_ptr[c]; j < R_ptr[c + 1]; ++j) {
+    for (int32_t j = R_ptr[c] + lane; j < R_ptr[c + 1]; j += 64) {
         const int64_t f = R_idx[j];
         const double w = R_w[j];
         s0 += w * r[3 * f];
         s1 += w * r[3 * f + 1];
         s2 += w * r[3 * f + 2];
     }
-    bc[3 * c] = free_c[3 * c] ? s0 : 0.0;
-    bc[3 * c + 1] = free_c[3 * c + 1] ? s1 : 0.0;
-    bc[3 * c + 2] = free_c[3 * c + 2] ? s2 : 0.0;
+    s0 = wave_sum(s0);
+    s1 = wave_sum(s1);
+    s2 = wave_sum(s2);
+    if (lane == 0) {
+        bc[3 * c] = free_c[3 * c] ? s0 : 0.0;
+        bc[3 * c + 1] = free_c[3 * c + 1] ? s1 : 0.0;
+        bc[3 * c + 2] = free_c[3 * c + 2] ? s2 : 0.0;
+    }
 }
 
 // d[f] = sum_j P_w[j] e_coarse[P_idx[j]] (masked by the fine free dofs); x += d
@@ -460,6 +467,19 @@ __global__ __launch_bounds__(SV_T) void fcg_dots_kernel(int64_t n, const double 
         part[2 * blockIdx.x] = sh0[0];
         part[2 * blockIdx.x + 1] = sh1[0];
     }
+}
+
+// out[b] = sum of part[b * PART_CHUNK, (b + 1) * PART_CHUNK) in a fixed order: the first
+// of two stages summing one partial per SpMV workgroup (a quarter of a million at 24^3)
+constexpr int PART_CHUNK = 1024;
+__global__ __launch_bounds__(SV_T) void part_sum_kernel(const double *__restrict__ part, int64_t n,
+                                                        double *__restrict__ out) {
+    __shared__ double sh0[SV_T], sh1[SV_T];
+    const int64_t lo = (int64_t)blockIdx.x * PART_CHUNK;
+    const int64_t cnt = n - lo < PART_CHUNK ? n - lo : PART_CHUNK;
+    double v, unused;
+    block_sum2<SV_T>(part + lo, cnt, 1, v, unused, sh0, sh1);
+    if (threadIdx.x == 0) out[blockIdx.x] = v;
 }
 
 // scalars s[]: 0 zr (current), 1 pq, 2 alpha, 3 beta, 4 rr, 5 zr_old
@@ -680,7 +700,8 @@ static int vcycle(Mg &m, int l, const double *rhs) {
     if (rc) return rc;
     const FlMgLevel &C = m.lv[l + 1];
     LevelWork &wc = m.w[l + 1];
-    restrict_kernel<<<nb_v(C.nnode), SV_T, 0, m.st>>>(C.nnode, C.R_ptr, C.R_idx, C.R_w, C.free_mask, w.r, wc.b);
+    restrict_kernel<<<nb_sp(C.nnode), 64 * SP_W, 0, m.st>>>(C.nnode, C.R_ptr, C.R_idx, C.R_w, C.free_mask, w.r,
+                                                             wc.b);
     FL_LAUNCH_CHECK();
     rc = vcycle(m, l + 1, nullptr);
     if (rc) return rc;
@@ -884,7 +905,9 @@ int fl_mg_pcg(const FlMgLevel *levels, int32_t nlevels, int32_t v32, const float
     while (it < maxit) {
         rc = level_spmv(m, 0, p, q, m.part);
         if (rc) return rc;
-        fcg_alpha_kernel<<<1, SV_T, 0, st>>>(m.part, nbs0, s);
+        const unsigned nch = (unsigned)((nbs0 + PART_CHUNK - 1) / PART_CHUNK);
+        part_sum_kernel<<<nch, SV_T, 0, st>>>(m.part, nbs0, m.part + nbs0);
+        fcg_alpha_kernel<<<1, SV_T, 0, st>>>(m.part + nbs0, nch, s);
         fcg_update_kernel<<<nbv0, SV_T, 0, st>>>(n0, s, p, q, x, rO, rOld, m.part);
         sum1_kernel<<<1, SV_T, 0, st>>>(m.part, nbv0, s + 4);
         FL_LAUNCH_CHECK();
