@@ -19,7 +19,8 @@
 // About 124 kflop per p = 4 element, and the 54 C values per point read once (K_e itself
 // is 140,625 doubles).  One workgroup per element, each XCD taking a contiguous range of
 // elements (its L2 then holds the x of neighbouring elements); two LDS buffers alternate
-// between the stages.
+// between the stages, and each stage reads an operand from LDS once per pencil, not once
+// per multiply-add.
 #pragma once
 
 namespace fl {
@@ -77,38 +78,60 @@ __global__ __launch_bounds__(256) void mf_element_kernel(const double *__restric
         for (int k = 0; k < 3; ++k) sA[k * N3 + t] = free_mask[3 * a + k] ? x[3 * a + k] : 0.0;
     }
     __syncthreads();
-    // beta: U1 -> B
-    for (int t = tid; t < 6 * Q * PP; t += NT) {
-        const int r = t % PP, qb = (t / PP) % Q, ks = t / (PP * Q);
-        const int k = ks >> 1, sb = ks & 1;
-        const double *xk = sA + k * N3 + r;
-        double s = 0.0;
+    // Each stage runs over "pencils": a thread loads the P1 (or Q) values along the
+    // contracted direction into registers once and writes every output of its pencil.
+    // beta: U1 -> B, pencils (k, sb, ae az)
+    for (int t = tid; t < 6 * PP; t += NT) {
+        const int r = t % PP, kb = t / PP;
+        const int k = kb >> 1, sb = kb & 1;
+        double xv[P1];
 #pragma unroll
-        for (int ab = 0; ab < P1; ++ab) s += sF[2][sb][ab][qb] * xk[ab * PP];
-        sB[t] = s;
+        for (int ab = 0; ab < P1; ++ab) xv[ab] = sA[k * N3 + ab * PP + r];
+        double *o = sB + kb * Q * PP + r;
+#pragma unroll
+        for (int qb = 0; qb < Q; ++qb) {
+            double s = 0.0;
+#pragma unroll
+            for (int ab = 0; ab < P1; ++ab) s += sF[2][sb][ab][qb] * xv[ab];
+            o[qb * PP] = s;
+        }
     }
     __syncthreads();
-    // eta: U2 -> A
-    for (int t = tid; t < 9 * QQ * P1; t += NT) {
-        const int az = t % P1, qe = (t / P1) % Q, qb = (t / (P1 * Q)) % Q, kn = t / (P1 * QQ);
+    // eta: U2 -> A, pencils (k n, qb, az)
+    for (int t = tid; t < 9 * Q * P1; t += NT) {
+        const int az = t % P1, qb = (t / P1) % Q, kn = t / (P1 * Q);
         const int k = kn / 3, n = kn - 3 * k;
         const int sb = n == 2, se = n == 1;
         const double *u = sB + ((k * 2 + sb) * Q + qb) * PP + az;
-        double s = 0.0;
+        double uv[P1];
 #pragma unroll
-        for (int ae = 0; ae < P1; ++ae) s += sF[1][se][ae][qe] * u[ae * P1];
-        sA[t] = s;
+        for (int ae = 0; ae < P1; ++ae) uv[ae] = u[ae * P1];
+        double *o = sA + (kn * Q + qb) * Q * P1 + az;
+#pragma unroll
+        for (int qe = 0; qe < Q; ++qe) {
+            double s = 0.0;
+#pragma unroll
+            for (int ae = 0; ae < P1; ++ae) s += sF[1][se][ae][qe] * uv[ae];
+            o[qe * P1] = s;
+        }
     }
     __syncthreads();
-    // zeta: G -> B
-    for (int t = tid; t < 9 * Q3; t += NT) {
-        const int qz = t % Q, qbe = (t / Q) % QQ, kn = t / Q3;
+    // zeta: G -> B, pencils (k n, qb qe)
+    for (int t = tid; t < 9 * QQ; t += NT) {
+        const int qbe = t % QQ, kn = t / QQ;
         const int n = kn % 3;
         const double *u = sA + (kn * QQ + qbe) * P1;
-        double s = 0.0;
+        double uv[P1];
 #pragma unroll
-        for (int az = 0; az < P1; ++az) s += sF[0][n == 0][az][qz] * u[az];
-        sB[t] = s;
+        for (int az = 0; az < P1; ++az) uv[az] = u[az];
+        double *o = sB + kn * Q3 + qbe * Q;
+#pragma unroll
+        for (int qz = 0; qz < Q; ++qz) {
+            double s = 0.0;
+#pragma unroll
+            for (int az = 0; az < P1; ++az) s += sF[0][n == 0][az][qz] * uv[az];
+            o[qz] = s;
+        }
     }
     __syncthreads();
     // per point: S_im = sum_kn C^{ik}_mn G_kn -> A.  The record holds C^{ik} of the six
@@ -149,47 +172,77 @@ __global__ __launch_bounds__(256) void mf_element_kernel(const double *__restric
             for (int m = 0; m < 3; ++m) sA[(3 * i + m) * Q3 + t] = S[i][m];
     }
     __syncthreads();
-    // zeta transposed: W -> B
-    for (int t = tid; t < 9 * QQ * P1; t += NT) {
-        const int az = t % P1, qbe = (t / P1) % QQ, im = t / (P1 * QQ);
+    // zeta transposed: W -> B, pencils (i m, qb qe)
+    for (int t = tid; t < 9 * QQ; t += NT) {
+        const int qbe = t % QQ, im = t / QQ;
         const int m = im % 3;
         const double *sv = sA + im * Q3 + qbe * Q;
-        double s = 0.0;
+        double v[Q];
 #pragma unroll
-        for (int qz = 0; qz < Q; ++qz) s += sF[0][m == 0][az][qz] * sv[qz];
-        sB[t] = s;
+        for (int qz = 0; qz < Q; ++qz) v[qz] = sv[qz];
+        double *o = sB + (im * QQ + qbe) * P1;
+#pragma unroll
+        for (int az = 0; az < P1; ++az) {
+            double s = 0.0;
+#pragma unroll
+            for (int qz = 0; qz < Q; ++qz) s += sF[0][m == 0][az][qz] * v[qz];
+            o[az] = s;
+        }
     }
     __syncthreads();
-    // eta transposed: Z -> A
-    for (int t = tid; t < 6 * Q * PP; t += NT) {
-        const int r = t % PP, qb = (t / PP) % Q, is = t / (PP * Q);
+    // eta transposed: Z -> A, pencils (i, sb, qb, az)
+    for (int t = tid; t < 6 * Q * P1; t += NT) {
+        const int az = t % P1, qb = (t / P1) % Q, is = t / (P1 * Q);
         const int i = is >> 1, sb = is & 1;
-        const int ae = r / P1, az = r - ae * P1;
-        double s = 0.0;
+        double *o = sA + ((i * 2 + sb) * Q + qb) * PP + az;
         if (sb == 0) {
             const double *w0 = sB + ((i * 3) * QQ + qb * Q) * P1 + az;
             const double *w1 = sB + ((i * 3 + 1) * QQ + qb * Q) * P1 + az;
+            double a0[Q], a1[Q];
 #pragma unroll
-            for (int qe = 0; qe < Q; ++qe) s += sF[1][0][ae][qe] * w0[qe * P1] + sF[1][1][ae][qe] * w1[qe * P1];
+            for (int qe = 0; qe < Q; ++qe) {
+                a0[qe] = w0[qe * P1];
+                a1[qe] = w1[qe * P1];
+            }
+#pragma unroll
+            for (int ae = 0; ae < P1; ++ae) {
+                double s = 0.0;
+#pragma unroll
+                for (int qe = 0; qe < Q; ++qe) s += sF[1][0][ae][qe] * a0[qe] + sF[1][1][ae][qe] * a1[qe];
+                o[ae * P1] = s;
+            }
         } else {
             const double *w2 = sB + ((i * 3 + 2) * QQ + qb * Q) * P1 + az;
+            double a2[Q];
 #pragma unroll
-            for (int qe = 0; qe < Q; ++qe) s += sF[1][0][ae][qe] * w2[qe * P1];
+            for (int qe = 0; qe < Q; ++qe) a2[qe] = w2[qe * P1];
+#pragma unroll
+            for (int ae = 0; ae < P1; ++ae) {
+                double s = 0.0;
+#pragma unroll
+                for (int qe = 0; qe < Q; ++qe) s += sF[1][0][ae][qe] * a2[qe];
+                o[ae * P1] = s;
+            }
         }
-        sA[t] = s;
     }
     __syncthreads();
-    // beta transposed: y, in local node order
+    // beta transposed: y, in local node order, pencils (i, ae az)
     double *yo = ye + e * (int64_t)N3 * 3;
-    for (int t = tid; t < 3 * N3; t += NT) {
-        const int i = t / N3, tn = t - i * N3;
-        const int ab = tn / PP, r = tn - ab * PP;
-        const double *z0 = sA + (i * 2 * Q) * PP + r;
-        const double *z1 = sA + ((i * 2 + 1) * Q) * PP + r;
-        double s = 0.0;
+    for (int t = tid; t < 3 * PP; t += NT) {
+        const int r = t % PP, i = t / PP;
+        double z0[Q], z1[Q];
 #pragma unroll
-        for (int qb = 0; qb < Q; ++qb) s += sF[2][0][ab][qb] * z0[qb * PP] + sF[2][1][ab][qb] * z1[qb * PP];
-        yo[3 * sLoc[tn] + i] = s;
+        for (int qb = 0; qb < Q; ++qb) {
+            z0[qb] = sA[((i * 2) * Q + qb) * PP + r];
+            z1[qb] = sA[((i * 2 + 1) * Q + qb) * PP + r];
+        }
+#pragma unroll
+        for (int ab = 0; ab < P1; ++ab) {
+            double s = 0.0;
+#pragma unroll
+            for (int qb = 0; qb < Q; ++qb) s += sF[2][0][ab][qb] * z0[qb] + sF[2][1][ab][qb] * z1[qb];
+            yo[3 * sLoc[ab * PP + r] + i] = s;
+        }
     }
 }
 
