@@ -606,14 +606,15 @@ static const float *smooth_v32(const Mg &m, int l) {
 static int level_spmv(const Mg &m, int l, const double *d, double *q, double *part, bool smoother = false) {
     const FlMgLevel &L = m.lv[l];
     if (smoother && l == 0 && m.use_mf && !part) return mf_product(m, d, q, nullptr, nullptr, nullptr, 0.0, 0.0);
+    // d is +0 on fixed dofs here (the cycle's and the outer iteration's vectors are)
     if (smoother ? smooth_f32(m, l) : (l == 0 && m.v32))
-        spmv_kernel<float><<<nb_sp(L.nnode), 64 * SP_W, 0, m.st>>>(L.indptr, L.node_nbr_ptr, L.node_nbrs,
-                                                                    smoother ? smooth_v32(m, l) : lvals<float>(L),
-                                                                    L.nnode, L.free_mask, d, q, part);
+        spmv_kernel<float, false><<<nb_sp(L.nnode), 64 * SP_W, 0, m.st>>>(
+            L.indptr, L.node_nbr_ptr, L.node_nbrs, smoother ? smooth_v32(m, l) : lvals<float>(L), L.nnode,
+            L.free_mask, d, q, part);
     else
-        spmv_kernel<double><<<nb_sp(L.nnode), 64 * SP_W, 0, m.st>>>(L.indptr, L.node_nbr_ptr, L.node_nbrs,
-                                                                     lvals<double>(L), L.nnode, L.free_mask, d, q,
-                                                                     part);
+        spmv_kernel<double, false><<<nb_sp(L.nnode), 64 * SP_W, 0, m.st>>>(L.indptr, L.node_nbr_ptr, L.node_nbrs,
+                                                                            lvals<double>(L), L.nnode, L.free_mask,
+                                                                            d, q, part);
     FL_LAUNCH_CHECK();
     return 0;
 }

@@ -20,7 +20,9 @@ __device__ inline double wave_sum(double v) {
 
 // y = A x (masked when free != nullptr).  One wave per node a, its three rows share the
 // neighbour list; lanes stride over neighbours.  Optional: partial of dot(x, y).
-template <typename VT>
+// MASK_X = false: the caller guarantees x is +0 on fixed dofs (the Krylov vectors of the
+// solvers are), so the neighbours' mask bytes are not read.
+template <typename VT, bool MASK_X = true>
 __global__ __launch_bounds__(64 * SP_W) void spmv_kernel(const int32_t *__restrict__ indptr,
                                                          const int32_t *__restrict__ nbr_ptr,
                                                          const int32_t *__restrict__ nbrs,
@@ -43,7 +45,7 @@ __global__ __launch_bounds__(64 * SP_W) void spmv_kernel(const int32_t *__restri
         for (int r = lane; r < deg; r += 64) {
             const int64_t b = nbrs[p0 + r];
             double x0 = x[3 * b], x1 = x[3 * b + 1], x2 = x[3 * b + 2];
-            if (free_mask) {
+            if (MASK_X && free_mask) {
                 x0 = free_mask[3 * b] ? x0 : 0.0;
                 x1 = free_mask[3 * b + 1] ? x1 : 0.0;
                 x2 = free_mask[3 * b + 2] ? x2 : 0.0;
