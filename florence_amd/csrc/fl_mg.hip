@@ -38,9 +38,17 @@ __device__ inline void bj_apply(const double *__restrict__ P, int64_t a, const d
     for (int i = 0; i < 3; ++i) z[i] = P[9 * a + 3 * i] * r[0] + P[9 * a + 3 * i + 1] * r[1] + P[9 * a + 3 * i + 2] * r[2];
 }
 
-// q = A d for the three rows of node a (one wave; d is +0 on fixed dofs, fixed rows
-// give q = d = 0); returns in lane 0.
-template <typename VT>
+// sum over a group of LPN consecutive lanes (LPN a power of two <= 64), in every lane
+template <int LPN>
+__device__ inline double group_sum(double v) {
+#pragma unroll
+    for (int off = LPN / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// q = A d for the three rows of node a (a group of LPN lanes; d is +0 on fixed dofs,
+// fixed rows give q = d = 0); returns in the group's lane 0.
+template <typename VT, int LPN>
 __device__ inline void node_spmv(const int32_t *__restrict__ indptr, const int32_t *__restrict__ nbr_ptr,
                                  const int32_t *__restrict__ nbrs, const VT *__restrict__ V,
                                  const uint8_t *__restrict__ free_mask, const double *__restrict__ d, int64_t a,
@@ -51,16 +59,16 @@ __device__ inline void node_spmv(const int32_t *__restrict__ indptr, const int32
     const VT *r1 = V + indptr[3 * a + 1];
     const VT *r2 = V + indptr[3 * a + 2];
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (int r = lane; r < deg; r += 64) {
+    for (int r = lane; r < deg; r += LPN) {
         const int64_t b = nbrs[p0 + r];
         const double x0 = d[3 * b], x1 = d[3 * b + 1], x2 = d[3 * b + 2];
         s0 += (double)r0[3 * r] * x0 + (double)r0[3 * r + 1] * x1 + (double)r0[3 * r + 2] * x2;
         s1 += (double)r1[3 * r] * x0 + (double)r1[3 * r + 1] * x1 + (double)r1[3 * r + 2] * x2;
         s2 += (double)r2[3 * r] * x0 + (double)r2[3 * r + 1] * x1 + (double)r2[3 * r + 2] * x2;
     }
-    q[0] = wave_sum(s0);
-    q[1] = wave_sum(s1);
-    q[2] = wave_sum(s2);
+    q[0] = group_sum<LPN>(s0);
+    q[1] = group_sum<LPN>(s1);
+    q[2] = group_sum<LPN>(s2);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
         if (!free_mask[3 * a + i]) q[i] = 0.0;
@@ -89,8 +97,9 @@ __global__ __launch_bounds__(SV_T) void cheb_init_kernel(int64_t nnode, const do
 }
 
 // One Chebyshev step: r -= A d_old; d_new = c1 d_old + c2 D^-1 r; x += d_new.
-// c2 = 0 and x = d_new = nullptr: only the residual update r -= A d.
-template <typename VT>
+// c2 = 0 and x = d_new = nullptr: only the residual update r -= A d.  LPN lanes per node
+// (64 on the fine level; 32 on the coarse levels, whose nodes have fewer neighbours).
+template <typename VT, int LPN>
 __global__ __launch_bounds__(64 * SP_W) void cheb_step_kernel(const int32_t *__restrict__ indptr,
                                                               const int32_t *__restrict__ nbr_ptr,
                                                               const int32_t *__restrict__ nbrs,
@@ -100,11 +109,11 @@ __global__ __launch_bounds__(64 * SP_W) void cheb_step_kernel(const int32_t *__r
                                                               const double *__restrict__ d_old,
                                                               double *__restrict__ d_new, double *__restrict__ r,
                                                               double *__restrict__ x, double c1, double c2) {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int64_t a = (int64_t)blockIdx.x * SP_W + wave;
+    const int grp = threadIdx.x / LPN, lane = threadIdx.x % LPN;
+    const int64_t a = (int64_t)blockIdx.x * (64 * SP_W / LPN) + grp;
     if (a >= nnode) return;
     double q[3];
-    node_spmv<VT>(indptr, nbr_ptr, nbrs, V, free_mask, d_old, a, lane, q);
+    node_spmv<VT, LPN>(indptr, nbr_ptr, nbrs, V, free_mask, d_old, a, lane, q);
     if (lane != 0) return;
     double rv[3];
 #pragma unroll
@@ -191,11 +200,6 @@ __device__ inline double wave_sum_partials(const double *__restrict__ part, int6
 constexpr int CC_LPN = 32;
 constexpr int CC_NB = 64 * SP_W / CC_LPN;
 
-__device__ inline double group_sum(double v) {
-#pragma unroll
-    for (int off = CC_LPN / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
 
 template <typename VT>
 __global__ __launch_bounds__(64 * SP_W) void cc_spmv_kernel(const int32_t *__restrict__ indptr,
@@ -235,9 +239,9 @@ __global__ __launch_bounds__(64 * SP_W) void cc_spmv_kernel(const int32_t *__res
             s1 += (double)r1[3 * r] * x0 + (double)r1[3 * r + 1] * x1 + (double)r1[3 * r + 2] * x2;
             s2 += (double)r2[3 * r] * x0 + (double)r2[3 * r + 1] * x1 + (double)r2[3 * r + 2] * x2;
         }
-        s0 = group_sum(s0);
-        s1 = group_sum(s1);
-        s2 = group_sum(s2);
+        s0 = group_sum<CC_LPN>(s0);
+        s1 = group_sum<CC_LPN>(s1);
+        s2 = group_sum<CC_LPN>(s2);
         if (gl == 0) {
             const double out[3] = {s0, s1, s2};
             for (int i = 0; i < 3; ++i) {
@@ -624,14 +628,25 @@ static int level_cheb_step(const Mg &m, int l, const double *d_old, double *d_ne
     const FlMgLevel &L = m.lv[l];
     const LevelWork &w = m.w[l];
     if (l == 0 && m.use_mf) return mf_product(m, d_old, nullptr, d_old, d_new, x, c1, c2);
-    if (smooth_f32(m, l))
-        cheb_step_kernel<float><<<nb_sp(L.nnode), 64 * SP_W, 0, m.st>>>(L.indptr, L.node_nbr_ptr, L.node_nbrs,
-                                                                         smooth_v32(m, l), L.nnode, L.free_mask, w.P,
-                                                                         d_old, d_new, w.r, x, c1, c2);
+    const float *v32 = smooth_f32(m, l) ? smooth_v32(m, l) : nullptr;
+    const double *v64 = lvals<double>(L);
+    const unsigned nb64 = nb_sp(L.nnode), nb32 = (unsigned)((L.nnode + 2 * SP_W - 1) / (2 * SP_W));
+    if (l == 0 && v32)
+        cheb_step_kernel<float, 64><<<nb64, 64 * SP_W, 0, m.st>>>(L.indptr, L.node_nbr_ptr, L.node_nbrs, v32,
+                                                                   L.nnode, L.free_mask, w.P, d_old, d_new, w.r, x,
+                                                                   c1, c2);
+    else if (l == 0)
+        cheb_step_kernel<double, 64><<<nb64, 64 * SP_W, 0, m.st>>>(L.indptr, L.node_nbr_ptr, L.node_nbrs, v64,
+                                                                    L.nnode, L.free_mask, w.P, d_old, d_new, w.r, x,
+                                                                    c1, c2);
+    else if (v32)
+        cheb_step_kernel<float, 32><<<nb32, 64 * SP_W, 0, m.st>>>(L.indptr, L.node_nbr_ptr, L.node_nbrs, v32,
+                                                                   L.nnode, L.free_mask, w.P, d_old, d_new, w.r, x,
+                                                                   c1, c2);
     else
-        cheb_step_kernel<double><<<nb_sp(L.nnode), 64 * SP_W, 0, m.st>>>(L.indptr, L.node_nbr_ptr, L.node_nbrs,
-                                                                          lvals<double>(L), L.nnode, L.free_mask, w.P,
-                                                                          d_old, d_new, w.r, x, c1, c2);
+        cheb_step_kernel<double, 32><<<nb32, 64 * SP_W, 0, m.st>>>(L.indptr, L.node_nbr_ptr, L.node_nbrs, v64,
+                                                                    L.nnode, L.free_mask, w.P, d_old, d_new, w.r, x,
+                                                                    c1, c2);
     FL_LAUNCH_CHECK();
     return 0;
 }
