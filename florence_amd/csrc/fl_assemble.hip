@@ -1250,6 +1250,14 @@ int mf_prepare(const FlMgMatrixFree *in, MfOp *op) {
     op->node_elem = in->node_elem;
     op->ye = in->element_vectors;
     op->c32 = nullptr;
+    op->c16 = nullptr;
+    return 0;
+}
+
+int mf_round_records16(const MfOp &op, uint32_t *c16, hipStream_t st) {
+    const int64_t n = op.nelem * 27 * (int64_t)op.G;
+    mf_round16_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(op.qp, op.G, n, c16);
+    FL_LAUNCH_CHECK();
     return 0;
 }
 
@@ -1265,12 +1273,15 @@ int mf_element_apply(const MfOp &op, const uint8_t *free_mask, const double *x, 
     const unsigned grid = (unsigned)(8 * chunk);
 #define FL_MF_LAUNCH(P, QQ)                                                                                        \
     do {                                                                                                           \
-        if (op.c32)                                                                                                \
-            mf_element_kernel<P, QQ, true><<<grid, 256, 0, st>>>(op.qp, op.c32, op.basis, op.elements, op.nelem,    \
-                                                                 free_mask, x, op.ye);                             \
+        if (op.c16)                                                                                                \
+            mf_element_kernel<P, QQ, 2><<<grid, 256, 0, st>>>(op.qp, op.c32, op.c16, op.basis, op.elements,         \
+                                                              op.nelem, free_mask, x, op.ye);                      \
+        else if (op.c32)                                                                                           \
+            mf_element_kernel<P, QQ, 1><<<grid, 256, 0, st>>>(op.qp, op.c32, op.c16, op.basis, op.elements,         \
+                                                              op.nelem, free_mask, x, op.ye);                      \
         else                                                                                                       \
-            mf_element_kernel<P, QQ, false><<<grid, 256, 0, st>>>(op.qp, op.c32, op.basis, op.elements, op.nelem,   \
-                                                                  free_mask, x, op.ye);                            \
+            mf_element_kernel<P, QQ, 0><<<grid, 256, 0, st>>>(op.qp, op.c32, op.c16, op.basis, op.elements,         \
+                                                              op.nelem, free_mask, x, op.ye);                      \
     } while (0)
     if (op.p1 == 2 && op.q == 2)
         FL_MF_LAUNCH(2, 2);

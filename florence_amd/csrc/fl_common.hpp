@@ -81,6 +81,7 @@ struct MfOp {
     const int32_t *node_elem_ptr, *node_elem;
     double *ye;                // element vectors [nelem][npe][3], local node order
     const float *c32;          // optional FP32 copy of the C tensors [nelem][54][G]
+    const uint32_t *c16;       // optional BF16 copy, two per word [nelem][27][G] (fields 2j, 2j+1)
 };
 // Checks a FlMgMatrixFree and resolves its records (FL_E_* with the error set).
 int mf_prepare(const FlMgMatrixFree *in, MfOp *op);
@@ -88,6 +89,7 @@ int mf_prepare(const FlMgMatrixFree *in, MfOp *op);
 int mf_element_apply(const MfOp &op, const uint8_t *free_mask, const double *x, hipStream_t st);
 // c32[e][f][g] = (float) C field f of point g of element e (the records' 54 C values).
 int mf_round_records(const MfOp &op, float *c32, hipStream_t st);
+int mf_round_records16(const MfOp &op, uint32_t *c16, hipStream_t st);
 
 // Exclusive prefix sum of int64 values in place (n entries, last entry = total), device.
 int exclusive_scan_i64(int64_t *data, int64_t n, int64_t *scratch, size_t scratch_count, hipStream_t st);

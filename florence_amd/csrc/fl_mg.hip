@@ -845,7 +845,11 @@ int fl_mg_pcg(const FlMgLevel *levels, int32_t nlevels, int32_t v32, const float
     if (fine_mf) {
         int rc = mf_prepare(fine_mf, &m.mf);
         if (rc) return rc;
-        if (fine_mf->records32) {
+        if (fine_mf->records16) {
+            rc = mf_round_records16(m.mf, fine_mf->records16, m.st);
+            if (rc) return rc;
+            m.mf.c16 = fine_mf->records16;
+        } else if (fine_mf->records32) {
             rc = mf_round_records(m.mf, fine_mf->records32, m.st);
             if (rc) return rc;
             m.mf.c32 = fine_mf->records32;

@@ -36,9 +36,10 @@ struct MfLayout {
 // C32: the C tensors rounded to FP32 (c32 [e][54][Q3]) instead of the records; each
 // thread's 54 values are loaded into registers when the workgroup starts, so their latency
 // overlaps the gradient stages.
-template <int P1, int Q, bool C32>
+template <int P1, int Q, int CK>
 __global__ __launch_bounds__(256) void mf_element_kernel(const double *__restrict__ qp,
                                                          const float *__restrict__ c32,
+                                                         const uint32_t *__restrict__ c16,
                                                          const int32_t *__restrict__ basis,
                                                          const int32_t *__restrict__ elements, int64_t nelem,
                                                          const uint8_t *__restrict__ free_mask,
@@ -46,6 +47,8 @@ __global__ __launch_bounds__(256) void mf_element_kernel(const double *__restric
     typedef SfLayout<P1, Q> L;
     typedef MfLayout<P1, Q> M;
     constexpr int N3 = M::N3, Q3 = M::Q3, PP = M::PP, QQ = M::QQ, NT = 256;
+    // CK: 0 the FP64 records, 1 their FP32 copy, 2 their BF16 copy
+    constexpr bool C32 = CK != 0;
     static_assert(!C32 || Q3 <= NT, "one point per thread");
     __shared__ double sF[3][2][P1][Q];
     __shared__ double sA[M::BUF];
@@ -63,10 +66,20 @@ __global__ __launch_bounds__(256) void mf_element_kernel(const double *__restric
     const double *f1d = reinterpret_cast<const double *>(basis + L::INTS);
     const int32_t *conn = elements + e * N3;
     float cr[C32 ? 54 : 1];
-    if constexpr (C32) {
+    if constexpr (CK == 1) {
         const float *r32 = c32 + e * (int64_t)54 * Q3 + min(tid, Q3 - 1);
 #pragma unroll
         for (int f = 0; f < 54; ++f) cr[f] = r32[f * Q3];
+    } else if constexpr (CK == 2) {
+        const uint32_t *r16 = c16 + e * (int64_t)27 * Q3 + min(tid, Q3 - 1);
+        uint32_t w[27];
+#pragma unroll
+        for (int j = 0; j < 27; ++j) w[j] = r16[j * Q3];
+#pragma unroll
+        for (int j = 0; j < 27; ++j) {
+            cr[2 * j] = __uint_as_float(w[j] << 16);
+            cr[2 * j + 1] = __uint_as_float(w[j] & 0xFFFF0000u);
+        }
     }
     for (int t = tid; t < 6 * P1 * Q; t += NT) (&sF[0][0][0][0])[t] = f1d[t];
     for (int t = tid; t < Q3; t += NT) sT[min(max(g_of_t[t], 0), Q3 - 1)] = (int16_t)t;
@@ -254,6 +267,22 @@ __global__ __launch_bounds__(256) void mf_round_kernel(const double *__restrict_
     const int64_t per = (int64_t)54 * G;
     const int64_t e = i / per, r = i - e * per;
     c32[i] = (float)qp[e * (int64_t)QS * G + (int64_t)Q_D * G + r];
+}
+
+// c16[e][j][g] = bf16(C field 2j) | bf16(C field 2j+1) << 16, rounded to nearest even
+__device__ inline uint32_t to_bf16_bits(double v) {
+    const uint32_t u = __float_as_uint((float)v);
+    return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+}
+__global__ __launch_bounds__(256) void mf_round16_kernel(const double *__restrict__ qp, int G, int64_t n,
+                                                         uint32_t *__restrict__ c16) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int64_t per = (int64_t)27 * G;
+    const int64_t e = i / per, r = i - e * per;
+    const int64_t j = r / G, g = r - j * G;
+    const double *rec = qp + e * (int64_t)QS * G + (int64_t)Q_D * G;
+    c16[i] = to_bf16_bits(rec[(2 * j) * G + g]) | (to_bf16_bits(rec[(2 * j + 1) * G + g]) << 16);
 }
 
 }  // namespace fl

@@ -47,10 +47,11 @@ class FlMgMatrixFree(ctypes.Structure):
                 ("records", ctypes.c_int32), ("tensor_basis", ctypes.c_void_p), ("elements", ctypes.c_void_p),
                 ("nelem", ctypes.c_int64), ("npe", ctypes.c_int32), ("ngauss", ctypes.c_int32),
                 ("nnode", ctypes.c_int64), ("node_elem_ptr", ctypes.c_void_p), ("node_elem", ctypes.c_void_p),
-                ("element_vectors", ctypes.c_void_p), ("records32", ctypes.c_void_p)]
+                ("element_vectors", ctypes.c_void_p), ("records32", ctypes.c_void_p),
+                ("records16", ctypes.c_void_p)]
 
 
-def matrix_free_operator(dm, precision="fp64", element_vectors=None, records32=None):
+def matrix_free_operator(dm, precision="fp64", element_vectors=None, records32=None, records16=None):
     """The last assembly's tangent on ``dm`` in the matrix-free form of fl_mf_apply
     (tensor-product hexes whose assembly left C point records), or None where that form
     does not exist.  Returns (descriptor, element-vector buffer).  ``records32``: a float32
@@ -68,7 +69,8 @@ def matrix_free_operator(dm, precision="fp64", element_vectors=None, records32=N
     ws = dm.workspace()
     op = FlMgMatrixFree(ws.data_ptr(), ws.numel(), rec, dm.tensor_basis.data_ptr(), dm.elements.data_ptr(),
                         dm.nelem, dm.npe, dm.ngauss, dm.nnode, dm.node_elem_ptr.data_ptr(), dm.node_elem.data_ptr(),
-                        element_vectors.data_ptr(), None if records32 is None else records32.data_ptr())
+                        element_vectors.data_ptr(), None if records32 is None else records32.data_ptr(),
+                        None if records16 is None else records16.data_ptr())
     return op, element_vectors
 
 
@@ -351,12 +353,14 @@ class MultigridSolver(object):
     FP64 operator, so the solve meets rtol on it); matrix_free: where the assembly left C
     point records (tensor-product hexes), the fine-level smoother applies the tangent
     from them element by element (fl_mf_apply: 54 values per point instead of the CSR's
-    values; with fp32_smoother those rounded to FP32 once per solve) -- the outer
-    iteration still applies the assembled CSR.
+    values) -- the outer iteration still applies the assembled CSR; smoother_records:
+    the precision of the copy of those values the smoother reads, made once per solve
+    ("bf16": a quarter of the bytes, "fp32", or "fp64": the records themselves).
     ``rtol``, ``maxit``, ``check_every`` and ``iterations`` as LinearSolver."""
 
     def __init__(self, rtol=1e-12, maxit=2000, check_every=1, cheb_degree=5, cheb_range=60.0,
-                 coarse_iterations=60, power_steps=12, warm_power_steps=3, fp32_smoother=True, matrix_free=True):
+                 coarse_iterations=60, power_steps=12, warm_power_steps=3, fp32_smoother=True, matrix_free=True,
+                 smoother_records="bf16"):
         self.linear_solver = "pmg"
         self.rtol = rtol
         self.maxit = maxit
@@ -369,6 +373,9 @@ class MultigridSolver(object):
         self.fp32_smoother = fp32_smoother
         self.matrix_free = matrix_free
         self.factorised_galerkin = True
+        if smoother_records not in ("bf16", "fp32", "fp64"):
+            raise ValueError("smoother_records must be bf16, fp32 or fp64")
+        self.smoother_records = smoother_records
         self._ye = None
         self._c32 = None
         self.iterations = []
@@ -422,12 +429,17 @@ class MultigridSolver(object):
         v32 = V.dtype == torch.float32
         mf = None
         if self.matrix_free:
-            n32 = dm.nelem * 54 * dm.ngauss
-            if self.fp32_smoother and (self._c32 is None or self._c32.numel() < n32 or
-                                       self._c32.device != dm.device):
-                self._c32 = torch.empty(n32, dtype=torch.float32, device=dm.device)
-            mf, self._ye = matrix_free_operator(dm, precision, self._ye,
-                                                self._c32 if self.fp32_smoother else None)
+            # the smoother's copy of the C values: BF16 (27 words per point) or FP32 (54)
+            c32 = c16 = None
+            if self.smoother_records in ("bf16", "fp32"):
+                n32 = dm.nelem * (27 if self.smoother_records == "bf16" else 54) * dm.ngauss
+                if self._c32 is None or self._c32.numel() < n32 or self._c32.device != dm.device:
+                    self._c32 = torch.empty(n32, dtype=torch.float32, device=dm.device)
+                if self.smoother_records == "bf16":
+                    c16 = self._c32
+                else:
+                    c32 = self._c32
+            mf, self._ye = matrix_free_operator(dm, precision, self._ye, c32, c16)
         # FP32 copies of the values the smoothers read: level 0 (unless matrix-free or
         # FP32 already) and the intermediate levels (the coarsest one runs CG on its own)
         smooth = None
@@ -446,7 +458,8 @@ class MultigridSolver(object):
                 ptrs[i] = buf.data_ptr()
             smooth = ptrs
         # the power vectors in the workspace belong to this hierarchy and workspace
-        key = (id(h), self._ws.data_ptr(), tuple(nn), bool(self.fp32_smoother), v32, mf is not None)
+        key = (id(h), self._ws.data_ptr(), tuple(nn), bool(self.fp32_smoother), v32, mf is not None,
+               self.smoother_records)
         warm = self._warm_key == key
         it = ctypes.c_int32(0)
         rel = ctypes.c_double(0.0)

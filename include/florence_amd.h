@@ -398,6 +398,7 @@ typedef struct {
     const int32_t *node_elem;
     double *element_vectors;
     float *records32;                /* optional, float[nelem * 54 * ngauss]: see fl_mg_pcg */
+    uint32_t *records16;             /* optional, uint32[nelem * 27 * ngauss]: see fl_mg_pcg */
 } FlMgMatrixFree;
 int fl_mf_apply(const FlMgMatrixFree *op, const uint8_t *free_mask_dev, const double *x_dev, double *y_dev,
                 void *stream);
@@ -411,8 +412,9 @@ int fl_mf_apply(const FlMgMatrixFree *op, const uint8_t *free_mask_dev, const do
  * level 0's operator in the matrix-free form above instead (smoother_V32[0] is then
  * not read); the outer iteration and the block-Jacobi blocks keep level 0's CSR.  With
  * its records32 given, the call first rounds the C tensors of the records to FP32 there
- * and the smoother reads those (half the bytes; fl_mf_apply always reads the FP64
- * records).
+ * and the smoother reads those (half the bytes); with records16 given instead, to BF16
+ * (round to nearest even), two values per word (a quarter of the bytes).  fl_mf_apply
+ * always reads the FP64 records.
  * power_steps: power iterations per level for the smoother's lmax (1..64); power_warm:
  * nonzero = start them from the vectors the previous call on this workspace left (same
  * levels; the previous Newton step's operator), 0 = from a fixed pseudo-random start. */

@@ -209,8 +209,10 @@ def test_matrix_free_operator(cuda, case):
     assert np.array_equal(a, y2.cpu().numpy())
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32", "fp64-csr"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "fp64-csr", "fp64-mf32", "fp64-mf64"])
 def test_multigrid_newton_matches_oracle(cuda, precision):
+    """fp64: the default smoother (matrix-free from BF16 copies of the C values);
+    -mf32 / -mf64: from FP32 copies or the FP64 records; -csr: on the CSR values."""
     from florence_amd.boundary_condition import BoundaryCondition
     from florence_amd.multigrid import MultigridSolver
     from florence_amd.solver import FEMSolver
@@ -226,7 +228,10 @@ def test_multigrid_newton_matches_oracle(cuda, precision):
     bc.SetDirichletCriteria(lambda mm: clamp_and_twist(mm, 30.0), m)
     kw = {"tangent_precision": "fp32", "csr_precision": "fp32"} if precision == "fp32" else {}
     solver = FEMSolver(optimise=True, number_of_load_increments=10, newton_raphson_tolerance=1e-8,
-                       linear_solver=MultigridSolver(rtol=1e-12, matrix_free=precision != "fp64-csr"), **kw)
+                       linear_solver=MultigridSolver(rtol=1e-12, matrix_free=precision != "fp64-csr",
+                                                     smoother_records={"fp64-mf32": "fp32",
+                                                                       "fp64-mf64": "fp64"}.get(precision, "bf16")),
+                       **kw)
     res = solver.Solve(formulation=Form(), mesh=m, material=mat, boundary_condition=bc)
     assert not res.newton_raphson_failed_to_converge
     disp = np.moveaxis(res.TotalDisp.cpu().numpy(), 2, 0)
