@@ -208,27 +208,36 @@ def _coarse_locals(element_type, pc, xi):
 
 def _csr_from_first(elements, weights, coarse_elements, n_fine):
     """Prolongation CSR over fine nodes: fine node f takes the weights of the first
-    element containing it, W[local, :] over that element's coarse nodes."""
+    element containing it, W[local, :] over that element's coarse nodes (the nonzero
+    ones, in coarse local order).  Built per local index: every fine node whose first
+    element holds it at local position l gets row l of W."""
     flat = elements.ravel()
     _, first = np.unique(flat, return_index=True)
     if first.size != n_fine:
         raise _lib.FlorenceAMDError("a node belongs to no element")
     e, l = np.divmod(first, elements.shape[1])
-    W = weights[l]                               # (n_fine, nc)
-    cols = coarse_elements[e]                    # (n_fine, nc)
-    keep = np.abs(W) > 1e-14
-    counts = keep.sum(axis=1)
+    keep = np.abs(weights) > 1e-14                # (npe, nc)
+    counts = keep.sum(axis=1)[l]
     ptr = np.zeros(n_fine + 1, dtype=np.int64)
     np.cumsum(counts, out=ptr[1:])
-    return ptr, cols[keep].astype(np.int64), W[keep]
+    idx = np.empty(int(ptr[-1]), dtype=np.int64)
+    w = np.empty(int(ptr[-1]), dtype=np.float64)
+    for loc in np.unique(l):
+        nodes = np.nonzero(l == loc)[0]
+        nz = np.nonzero(keep[loc])[0]
+        pos = ptr[nodes][:, None] + np.arange(nz.size)[None, :]
+        idx[pos] = coarse_elements[e[nodes]][:, nz]
+        w[pos] = weights[loc, nz][None, :]
+    return ptr, idx, w
 
 
 def _transpose(ptr, idx, w, n_coarse):
+    """The transpose of a CSR over fine nodes as a CSR over coarse nodes, each coarse
+    node's list in ascending fine node order."""
     rows = np.repeat(np.arange(ptr.size - 1), np.diff(ptr))
-    order = np.lexsort((rows, idx))              # by coarse node, then fine node
-    cidx = idx[order]
+    order = np.argsort(idx, kind="stable")        # rows ascend within each coarse node
     rptr = np.zeros(n_coarse + 1, dtype=np.int64)
-    np.cumsum(np.bincount(cidx, minlength=n_coarse), out=rptr[1:])
+    np.cumsum(np.bincount(idx, minlength=n_coarse), out=rptr[1:])
     return rptr, rows[order], w[order]
 
 
