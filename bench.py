@@ -152,8 +152,8 @@ def cpu_baseline(mesh, fs, mat, x, nsample, max_threads=16, one_core_sample=192)
     """Time oracle/liboracle_df.so (C restatement of _GlobalAssemblyDF_) on bounded
     samples of the mesh: on one core (the reference kernel is serial) over the first
     ``one_core_sample`` elements, and on every core of this process's affinity set (at
-    most ``max_threads``: a GPU box gives one GPU's process 16 cores, whatever the
-    machine has) over the first ``nsample`` elements split into equal chunks, each thread
+    most ``max_threads``: the CPU share of one GPU on an 8-GPU MI355X node is 16 cores,
+    whatever the machine reports) over the first ``nsample`` elements split into equal chunks, each thread
     assembling its chunk into the chunk's own CSR pattern (the library call releases the
     GIL and keeps no global state) -- the reference's element-partitioned pool model.
     value = elements / wall time of the parallel section."""
@@ -455,8 +455,8 @@ def newton_comparison(args, log):
     return out
 
 
-def config2_newton(n, linear, lin_rtol, log):
-    """The config-2 solve on one mesh: hex p=4 unit cube n^3 (jitter 0.03h, seed 0),
+def twisted_cube_newton(n, linear, lin_rtol, log):
+    """The twisted-cube solve on one mesh: hex p=4 unit cube n^3 (jitter 0.03h, seed 0),
     MR(0.5, 0.5, 20), 6^3 Gauss-Lobatto points, x=0 clamped, x=1 twisted 30 degrees, 10
     increments, each to |R_free| <= 1e-8 R0 -- three modes of the lockstep driver:
     the plain FP64 path (FP64 tangent and CSR, the reference's predictor start) with
@@ -544,7 +544,7 @@ def bench_newton(args):
             log("{}: {}".format(key, out_modes[key]))
             del solver, res
             torch.cuda.empty_cache()
-    out_modes.update(config2_newton(args.n, args.linear, args.lin_rtol, log))
+    out_modes.update(twisted_cube_newton(args.n, args.linear, args.lin_rtol, log))
     plain, fast = out_modes["fp64+previous+adaptive"]["wall_s"], out_modes["fp32+extrap+adaptive+csr32"]["wall_s"]
     out = {"metric": "Newton wall time to FP64 residual 1e-8 (twisted hex p=4 cube)", "value": fast, "unit": "s",
            "n_gpus": 1, "steps": 1, "warmup": 0, "higher_is_better": False, "vs_baseline": None, "dtype": "fp32",
@@ -597,7 +597,8 @@ def main():
     ap.add_argument("--newton-members", type=int, default=16,
                     help="assembly: held-out members of the Newton comparison in the default line (0 = skip)")
     ap.add_argument("--newton-n", type=int, default=24,
-                    help="assembly: cells per edge of the config-2 Newton solve in the default line (0 = skip)")
+                    help="assembly: cells per edge of the twisted hex p=4 cube's Newton solve in the default line "
+                         "(0 = skip)")
     ap.add_argument("--numbering", default="first", choices=["first", "reference"],
                     help="assembly: global node numbering of the p=4 mesh (first appearance, or the reference's "
                          "GetHighOrderMesh numbering, built on the device)")
@@ -821,7 +822,7 @@ def main():
     newton = newton_comparison(args, log) if args.newton_members else None
     newton_c2 = None
     if args.newton_n > 0:
-        modes = config2_newton(args.newton_n, "mg", 1e-10, log)
+        modes = twisted_cube_newton(args.newton_n, "mg", 1e-10, log)
         plain_fixed = modes["fp64+previous+fixed"]["wall_s"]
         plain_adapt = modes["fp64+previous+adaptive"]["wall_s"]
         fast = modes["fp32+extrap+adaptive+csr32"]["wall_s"]
@@ -858,7 +859,7 @@ def main():
         "precision_forms": precision_forms,
         "residual_only": residual_only,
         "newton": newton,
-        "newton_config2": newton_c2,
+        "newton_twisted_cube": newton_c2,
         "status": status,
     }
     print(json.dumps(out), flush=True)

@@ -1,5 +1,5 @@
 """Synthetic workloads of the assembly path: seeded meshes, rules, materials and
-boundary conditions for the benchmark configurations.
+boundary conditions of the benchmark workloads.
 
 * ``cube_tet_p2``  -- unit cube, n^3 cubes split into 6 tets, p=2 (10 nodes), the
   degree-4 (11 point) tetrahedral rule, interior nodes jittered by U(-0.05h, 0.05h)

@@ -2,9 +2,9 @@
 (oracle/newton.py: C restatement of the assembly + scipy sparse direct solve, the
 reference's StaticSolver / NewtonRaphson semantics, FEMSolver.py:841-1044):
 
-* c1_newton.npz -- the tet p=2 cube (unit cube, 8^3 x 6 tets, p=2, 11-point rule,
+* newton_tet_p2_cube8.npz -- the tet p=2 cube (unit cube, 8^3 x 6 tets, p=2, 11-point rule,
   Neo-Hookean(1, 10), x=0 clamped, x=1 u_x = 0.2, 5 increments, tolerance 1e-8);
-* c2_newton_n2.npz -- the twisted hex p=4 cube at n=2 (2^3 hexes, p=4, 6^3
+* newton_hex_p4_twist_cube2.npz -- the twisted hex p=4 cube at n=2 (2^3 hexes, p=4, 6^3
   Gauss-Lobatto points, Mooney-Rivlin(0.5, 0.5, 20), x=0 clamped, x=1 twisted 30
   degrees about the x axis, 10 increments, tolerance 1e-8).
 
@@ -46,9 +46,9 @@ def solve(name, m, fs, mat, flags, nincr):
 
 def main():
     m, fs, mat = cube_tet_p2(8)
-    solve("c1_newton.npz", m, fs, mat, clamp_and_pull(m), 5)
+    solve("newton_tet_p2_cube8.npz", m, fs, mat, clamp_and_pull(m), 5)
     m, fs, mat = cube_hex_p4(2)
-    solve("c2_newton_n2.npz", m, fs, mat, clamp_and_twist(m, 30.0), 10)
+    solve("newton_hex_p4_twist_cube2.npz", m, fs, mat, clamp_and_twist(m, 30.0), 10)
 
 
 if __name__ == "__main__":

@@ -9,9 +9,9 @@
   oracle's element matrices with each element's own p=2 -> p=1 interpolation;
 * a solve meets its tolerance on the masked operator (checked by a product with the
   CSR), agrees with the block-Jacobi CG solution, and needs at least 5x fewer
-  iterations on config 2's twisted hex p=4 cube;
-* FEMSolver with it reproduces the oracle's Newton solve of config 2 at n=2
-  (tests/golden/c2_newton_n2.npz): iteration counts and every increment's
+  iterations on the twisted hex p=4 cube;
+* FEMSolver with it reproduces the oracle's Newton solve of the twisted hex p=4 cube at n=2
+  (tests/golden/newton_hex_p4_twist_cube2.npz): iteration counts and every increment's
   displacement within 1e-10, with the fine-level smoother matrix-free (the default)
   and on the CSR;
 * the matrix-free tangent (fl_mf_apply, the fine smoother's operator) equals the masked
@@ -216,7 +216,7 @@ def test_multigrid_newton_matches_oracle(cuda, precision):
     from florence_amd.boundary_condition import BoundaryCondition
     from florence_amd.multigrid import MultigridSolver
     from florence_amd.solver import FEMSolver
-    ref = np.load(os.path.join(GOLD, "c2_newton_n2.npz"))
+    ref = np.load(os.path.join(GOLD, "newton_hex_p4_twist_cube2.npz"))
     m, fs, mat = cube_hex_p4(2)
 
     class Form(object):
@@ -248,13 +248,13 @@ def test_multigrid_newton_matches_oracle(cuda, precision):
 
 def test_femsolver_default_linear_solver(cuda):
     """FEMSolver without a linear solver takes the multigrid for hexes of p >= 2 and
-    block-Jacobi PCG for tets, and either reproduces the oracle's Newton solve (config 2
+    block-Jacobi PCG for tets, and either reproduces the oracle's Newton solve (the twisted cube
     at 2^3 hexes: iterations and every increment within 1e-10)."""
     from florence_amd.boundary_condition import BoundaryCondition
     from florence_amd.multigrid import MultigridSolver
     from florence_amd.assembly import DeviceMeshData
     from florence_amd.solver import FEMSolver, LinearSolver, default_linear_solver
-    ref = np.load(os.path.join(GOLD, "c2_newton_n2.npz"))
+    ref = np.load(os.path.join(GOLD, "newton_hex_p4_twist_cube2.npz"))
     m, fs, mat = cube_hex_p4(2)
 
     class Form(object):

@@ -3,7 +3,7 @@ solves of the same workloads (tests/golden/*.npz, written by
 tests/golden/make_newton_fixture.py and pinned by tests/test_oracle_newton.py):
 
 * the tet p=2 cube (5 increments of u_x = 0.2, Neo-Hookean);
-* config 2's twisted hex p=4 cube at n=2 (10 increments of a 30 degree twist,
+* the twisted hex p=4 cube at n=2 (10 increments of a 30 degree twist,
   Mooney-Rivlin, 6^3 Gauss-Lobatto points), FP64 tangent and FP32 tangent (the
   inexact-Jacobian precision form, FP32-stored CSR values in the Krylov solve).
 
@@ -62,7 +62,7 @@ def _compare(res, ref, nincr, label, bar, exact_iterations=True):
 
 
 def test_tet_p2_cube_newton_matches_oracle(cuda):
-    ref = np.load(os.path.join(GOLD, "c1_newton.npz"))
+    ref = np.load(os.path.join(GOLD, "newton_tet_p2_cube8.npz"))
     m, fs, mat = cube_tet_p2(8)
     assert np.array_equal(np.asarray(m.points), ref["points"]), "mesh generator changed; regenerate the fixture"
     _compare(_solve(m, fs, mat, clamp_and_pull, 5), ref, 5, "tet p2 cube, 5 increments", 1e-10)
@@ -70,7 +70,7 @@ def test_tet_p2_cube_newton_matches_oracle(cuda):
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_twisted_hex_p4_newton_matches_oracle(cuda, precision):
-    ref = np.load(os.path.join(GOLD, "c2_newton_n2.npz"))
+    ref = np.load(os.path.join(GOLD, "newton_hex_p4_twist_cube2.npz"))
     m, fs, mat = cube_hex_p4(2)
     assert np.array_equal(np.asarray(m.points), ref["points"]), "mesh generator changed; regenerate the fixture"
     kw = {} if precision == "fp64" else {"tangent_precision": "fp32", "csr_precision": "fp32"}

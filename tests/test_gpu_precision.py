@@ -4,7 +4,7 @@ Checks: the FP32 tangent's error against the FP64 oracle is at FP32 level (state
 bar 1e-5 relative, infinity norm); the residual T, computed in FP64 by the same
 kernels, is bitwise the FP64 path's; the inexact-Jacobian Newton on the tet p=2 cube
 still reaches the 1e-8 relative residual and its displacement agrees with the FP64
-oracle's (tests/golden/c1_newton.npz) to 1e-7 (FP32 tangent) and 1e-6 (BF16 tangent:
+oracle's (tests/golden/newton_tet_p2_cube8.npz) to 1e-7 (FP32 tangent) and 1e-6 (BF16 tangent:
 linear convergence, the last iterate's residual decides how close it lands)."""
 import os
 
@@ -15,7 +15,7 @@ from florence_amd.workloads import cube_hex_p4, cube_tet_p2
 from oracle import df_numpy as O
 
 pytestmark = pytest.mark.gpu
-GOLD = os.path.join(os.path.dirname(__file__), "golden", "c1_newton.npz")
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "newton_tet_p2_cube8.npz")
 
 
 def _deformed(X, amp=0.04):

@@ -34,8 +34,8 @@ GOLD = os.path.join(os.path.dirname(__file__), "golden")
 TOL = 1e-8
 
 CASES = {
-    "c1_newton.npz": (lambda: cube_tet_p2(8), clamp_and_pull, 5),
-    "c2_newton_n2.npz": (lambda: cube_hex_p4(2), lambda m: clamp_and_twist(m, 30.0), 10),
+    "newton_tet_p2_cube8.npz": (lambda: cube_tet_p2(8), clamp_and_pull, 5),
+    "newton_hex_p4_twist_cube2.npz": (lambda: cube_hex_p4(2), lambda m: clamp_and_twist(m, 30.0), 10),
 }
 
 
