@@ -89,6 +89,18 @@ PMC_SOURCE = ("profiles/" + PMC_SUMMARY + ": rocprofv3 --pmc FETCH_SIZE, then WR
               "on this workload (copy-calibrated, KiB per dispatch summed over one assembly's launches)")
 
 
+def copy_calibration(torch, mib):
+    """One plain device-to-device copy of mib MiB: the known-size dispatch that scales the
+    PMC byte counters of a profiled run."""
+    a = torch.ones(mib << 17, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print("copy calibration: {} bytes read, {} bytes written".format(a.numel() * 8, a.numel() * 8),
+          file=sys.stderr, flush=True)
+
+
 def device_copy_gbs(torch, nbytes, reps=10):
     """Read+write rate of a plain device-to-device copy of nbytes (GB/s, bytes read plus
     bytes written), timed with HIP events."""
@@ -455,7 +467,11 @@ def newton_comparison(args, log):
     return out
 
 
-def twisted_cube_newton(n, linear, lin_rtol, log):
+NEWTON_MODES = {"fixed": ("fp64", "previous", "fixed", "fp64"), "adaptive": ("fp64", "previous", "adaptive", "fp64"),
+                "fast": ("fp32", "extrap", "adaptive", "fp32")}
+
+
+def twisted_cube_newton(n, linear, lin_rtol, log, which=("fixed", "adaptive", "fast")):
     """The twisted-cube solve on one mesh: hex p=4 unit cube n^3 (jitter 0.03h, seed 0),
     MR(0.5, 0.5, 20), 6^3 Gauss-Lobatto points, x=0 clamped, x=1 twisted 30 degrees, 10
     increments, each to |R_free| <= 1e-8 R0 -- three modes of the lockstep driver:
@@ -468,8 +484,7 @@ def twisted_cube_newton(n, linear, lin_rtol, log):
     from florence_amd.learned import LockstepNewton, TwistStretchFamily
     fam = TwistStretchFamily(n=n, nincr=10, theta_range=(30.0, 30.0), stretch_range=(1.0, 1.0))
     modes, ref = {}, None
-    for prec, start, forcing, csr in (("fp64", "previous", "fixed", "fp64"), ("fp64", "previous", "adaptive", "fp64"),
-                                      ("fp32", "extrap", "adaptive", "fp32")):
+    for prec, start, forcing, csr in [NEWTON_MODES[k] for k in which]:
         ln = LockstepNewton(fam, [0], precision=prec, forcing=forcing, csr=csr, lin_rtol=lin_rtol, linear=linear)
         log("lockstep {} {} {} csr {}: {} dofs, nnz {}".format(prec, start, forcing, csr, 3 * ln.dm.nnode, ln.dm.nnz))
         with no_gc():
@@ -544,11 +559,15 @@ def bench_newton(args):
             log("{}: {}".format(key, out_modes[key]))
             del solver, res
             torch.cuda.empty_cache()
-    out_modes.update(twisted_cube_newton(args.n, args.linear, args.lin_rtol, log))
-    plain, fast = out_modes["fp64+previous+adaptive"]["wall_s"], out_modes["fp32+extrap+adaptive+csr32"]["wall_s"]
+    if args.copy_mib:
+        copy_calibration(torch, args.copy_mib)
+    modes = tuple(args.modes.split(","))
+    out_modes.update(twisted_cube_newton(args.n, args.linear, args.lin_rtol, log, modes))
+    plain = out_modes.get("fp64+previous+adaptive", {}).get("wall_s")
+    fast = out_modes.get("fp32+extrap+adaptive+csr32", {}).get("wall_s")
     out = {"metric": "Newton wall time to FP64 residual 1e-8 (twisted hex p=4 cube)", "value": fast, "unit": "s",
            "n_gpus": 1, "steps": 1, "warmup": 0, "higher_is_better": False, "vs_baseline": None, "dtype": "fp32",
-           "data": "synthetic", "speedup_vs_plain_fp64": round(plain / fast, 3),
+           "data": "synthetic", "speedup_vs_plain_fp64": round(plain / fast, 3) if plain and fast else None,
            "config": {"workload": "unit cube {0}^3 hex p=4 (jitter 0.03h), 6^3 GLL, MR(0.5,0.5,20), x=0 clamped, "
                                   "x=1 twisted 30 deg, 10 increments, {1}".format(
                                       args.n, "p-multigrid flexible CG" if args.linear == "mg" else
@@ -592,8 +611,10 @@ def main():
     ap.add_argument("--femsolver", action="store_true", help="newton: also run FEMSolver (reference semantics)")
     ap.add_argument("--linear", default="mg", choices=["mg", "bj"],
                     help="newton: linear solver of the Newton steps (p-multigrid or block-Jacobi CG)")
-    ap.add_argument("--copy-mib", type=int, default=0, help="assembly: run one D2D copy of this size first "
+    ap.add_argument("--copy-mib", type=int, default=0, help="assembly, newton: run one D2D copy of this size first "
                                                              "(calibrates PMC byte counters)")
+    ap.add_argument("--modes", default="fixed,adaptive,fast",
+                    help="newton: which solves of the twisted cube to run (fixed, adaptive, fast)")
     ap.add_argument("--newton-members", type=int, default=16,
                     help="assembly: held-out members of the Newton comparison in the default line (0 = skip)")
     ap.add_argument("--newton-n", type=int, default=24,
@@ -646,15 +667,7 @@ def main():
         t_setup, dm.nelem, dm.nnode, dm.nnz, dm.max_degree), file=sys.stderr, flush=True)
 
     if args.copy_mib:
-        # a plain device-to-device copy of known size: the scale of the PMC byte counters
-        a = torch.ones(args.copy_mib << 17, dtype=torch.float64, device="cuda")
-        b = torch.empty_like(a)
-        torch.cuda.synchronize()
-        b.copy_(a)
-        torch.cuda.synchronize()
-        print("copy calibration: {} bytes read, {} bytes written".format(a.numel() * 8, a.numel() * 8),
-              file=sys.stderr, flush=True)
-        del a, b
+        copy_calibration(torch, args.copy_mib)
     direct = not args.no_direct
     for _ in range(args.warmup):
         assemble_device(dm, mat, x, V, T, direct=direct)
