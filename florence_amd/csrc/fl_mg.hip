@@ -14,12 +14,14 @@
 // [lmax / range, lmax], lmax of D^-1 A estimated per solve by power iteration (warm
 // started from the previous solve's vector when the caller says the operator is the
 // previous one's successor); the same polynomial before and after the coarse correction
-// keeps the cycle symmetric.  The fine level's smoother may read an FP32 copy of an
-// FP64 CSR (half the bytes of the six fine-level products of a V-cycle); the outer
-// iteration always applies the level's own operator.
+// keeps the cycle symmetric.  The fine level's smoother applies the tangent either
+// matrix-free from the assembly's C point records (fl_mf_apply's operator, optionally
+// from an FP32 or BF16 copy of them) or from the CSR values (optionally an FP32 copy);
+// the smoothers of the intermediate levels may read FP32 copies of their values; the
+// outer iteration always applies level 0's own CSR.
 // Transfers are the nodal interpolation P (fine node <- coarse nodes of an element
 // containing it) and its transpose, both as per-node CSR lists summed in a fixed order.
-// Every reduction is two-pass in a fixed order: a solve is bitwise reproducible.
+// Every reduction is in a fixed order: a solve is bitwise reproducible.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
