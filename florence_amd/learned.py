@@ -24,6 +24,8 @@ Start modes of LockstepNewton.solve:
   "previous"  the reference's: u_{k-1} + Dirichlet increment, first step with the
               previous tangent on the predictor residual K[:, out] du_D;
   "extrap"    2 u_{k-1} - u_{k-2} (u_0 = u_{-1} = 0), full Newton from there;
+  "extrap2"   3 u_{k-1} - 3 u_{k-2} + u_{k-3} from the third increment on (the quadratic
+              through the last three converged states; "extrap" before);
   "learned"   the MLP's prediction u_L, the extrapolation u_E, or the point
               u_E + a* (u_L - u_E) that minimises the linearised residual along the
               predicted correction -- whichever has the smallest FP64 residual, per
@@ -184,7 +186,7 @@ class LockstepNewton(object):
         choices and (record=True) the converged displacements u_0..u_nincr on the host.
         ``log``: optional callable given one line per increment."""
         import torch
-        if start not in ("previous", "extrap", "learned"):
+        if start not in ("previous", "extrap", "extrap2", "learned"):
             raise ValueError(start)
         if start == "learned" and predictor is None:
             raise ValueError("start='learned' needs a predictor")
@@ -193,7 +195,7 @@ class LockstepNewton(object):
         nincr = self.family.nincr
         fixed_f = 1.0 - self.free_f
         z = lambda: torch.zeros(n3, dtype=torch.float64, device=dev)
-        u_prev, u_prev2, u, du, rhs, R, RL, uL = z(), z(), z(), z(), z(), z(), z(), z()
+        u_prev, u_prev2, u_prev3, u, du, rhs, R, RL, uL = z(), z(), z(), z(), z(), z(), z(), z(), z()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         t_lin = t_asm = t_res = t_pred = 0.0
@@ -230,7 +232,10 @@ class LockstepNewton(object):
                 conv = torch.zeros(self.nmember, dtype=torch.bool, device=dev)
                 r_cur = float((self._norms(rhs) / R0).max())
             else:
-                uE = (2.0 * u_prev - u_prev2) * self.free_f + ubc
+                if start == "extrap2" and k >= 3:
+                    uE = (3.0 * u_prev - 3.0 * u_prev2 + u_prev3) * self.free_f + ubc
+                else:
+                    uE = (2.0 * u_prev - u_prev2) * self.free_f + ubc
                 t_res += timed(self._residual, uE, R)
                 nE = self._norms(R)
                 if start == "learned":
@@ -266,7 +271,7 @@ class LockstepNewton(object):
                 if bool(conv.all()):
                     lock_iters[k - 1] = 0
                     history.append([r_cur])
-                    u_prev2, u_prev = u_prev, u.clone()
+                    u_prev3, u_prev2, u_prev = u_prev2, u_prev, u.clone()
                     if record:
                         traj.append(u.cpu().numpy())
                     continue
@@ -315,7 +320,7 @@ class LockstepNewton(object):
             # in the reference's mode V now holds K at the converged state, which the
             # next increment's predictor uses; the warm-started modes assemble at
             # their start instead
-            u_prev2, u_prev = u_prev, u.clone()
+            u_prev3, u_prev2, u_prev = u_prev2, u_prev, u.clone()
             if record:
                 traj.append(u.cpu().numpy())
             if failed:
