@@ -428,7 +428,8 @@ def newton_comparison(args, log):
     U[0.8,1.5], 30 increments) solved in lockstep to |R_free,i| <= 1e-8 R0_i per member.
     Plain FP64 path: FP64 tangent and CSR, the reference's predictor start.  Fast path:
     FP32 tangent, FP32-stored CSR, the committed learned warm start; beside it the same
-    fast path started from the quadratic extrapolation (no network).  All arms use the
+    fast path started from the polynomial extrapolation (no network: per member the
+    linear, quadratic or cubic one, whichever has the smallest FP64 residual).  All arms use the
     SAME linear solver and the same tolerance policy, for every policy listed."""
     import torch
     from florence_amd.learned import LockstepNewton, TwistStretchFamily, WarmStartMLP
@@ -436,13 +437,14 @@ def newton_comparison(args, log):
     net = WarmStartMLP.load(WEIGHTS)
     seeds = range(10 ** 6, 10 ** 6 + args.newton_members)
     arms = (("plain_fp64", "fp64", "previous", "fp64"), ("mixed_learned", "fp32", "learned", "fp32"),
-            ("mixed_quadratic", "fp32", "extrap2", "fp32"))
+            ("mixed_polynomial", "fp32", "poly", "fp32"))
     out = {"workload": "{} held-out members (seeds 1e6..) of the learned family: unit cubes 2^3 hex p=4, jitter "
                        "0.03h, MR(0.5,0.5,20), 6^3 GLL, x=0 clamped, x=1 twist U[0,60] deg + stretch U[0.8,1.5], "
                        "30 increments, lockstep, |R_free,i| <= 1e-8 R0_i".format(args.newton_members),
            "plain_fp64": "FP64 tangent, FP64 CSR, reference predictor start",
            "mixed_learned": "FP32 tangent, FP32-stored CSR, learned warm start (FP64 residual arbitrates)",
-           "mixed_quadratic": "FP32 tangent, FP32-stored CSR, quadratic extrapolation of the last three states",
+           "mixed_polynomial": "FP32 tangent, FP32-stored CSR, polynomial extrapolation of the last converged "
+                               "states (order 1-3 per member, FP64 residual picks)",
            "policies": {}}
     for linear, forcing in (("mg", "fixed"), ("mg", "adaptive"), ("bj", "adaptive")):
         pol = {}
@@ -459,23 +461,23 @@ def newton_comparison(args, log):
                          "final": r["trajectory"][-1]}
             del ln
             torch.cuda.empty_cache()
-        a, b, c = pol["plain_fp64"], pol["mixed_learned"], pol["mixed_quadratic"]
+        a, b, c = pol["plain_fp64"], pol["mixed_learned"], pol["mixed_polynomial"]
         fa, fb, fc = a.pop("final"), b.pop("final"), c.pop("final")
         key = "{}+{}".format("multigrid" if linear == "mg" else "block_jacobi", forcing)
         pol["speedup"] = round(a["wall_s"] / b["wall_s"], 3)
-        pol["speedup_quadratic"] = round(a["wall_s"] / c["wall_s"], 3)
+        pol["speedup_polynomial"] = round(a["wall_s"] / c["wall_s"], 3)
         pol["max_rel_diff_displacement"] = max(float(np.abs(fb - fa).max() / np.abs(fa).max()),
                                                float(np.abs(fc - fa).max() / np.abs(fa).max()))
         out["policies"][key] = pol
-        log("newton {}: plain {:.3f}s ({} its), mixed+learned {:.3f}s ({} its), {:.2f}x; mixed+quadratic "
+        log("newton {}: plain {:.3f}s ({} its), mixed+learned {:.3f}s ({} its), {:.2f}x; mixed+polynomial "
             "{:.3f}s ({} its), {:.2f}x".format(key, a["wall_s"], a["newton_iterations"], b["wall_s"],
                                                b["newton_iterations"], pol["speedup"], c["wall_s"],
-                                               c["newton_iterations"], pol["speedup_quadratic"]))
+                                               c["newton_iterations"], pol["speedup_polynomial"]))
     return out
 
 
 NEWTON_MODES = {"fixed": ("fp64", "previous", "fixed", "fp64"), "adaptive": ("fp64", "previous", "adaptive", "fp64"),
-                "fast": ("fp32", "extrap2", "adaptive", "fp32")}
+                "fast": ("fp32", "poly", "adaptive", "fp32")}
 
 
 def twisted_cube_newton(n, linear, lin_rtol, log, which=("fixed", "adaptive", "fast")):
@@ -484,8 +486,9 @@ def twisted_cube_newton(n, linear, lin_rtol, log, which=("fixed", "adaptive", "f
     increments, each to |R_free| <= 1e-8 R0 -- three modes of the lockstep driver:
     the plain FP64 path (FP64 tangent and CSR, the reference's predictor start) with
     fixed linear tolerance (lin_rtol) and with the adaptive one, and the fast path (FP32
-    tangent, FP32-stored CSR, the quadratic extrapolation of the last three converged
-    states as start, adaptive linear tolerance).  The
+    tangent, FP32-stored CSR, the polynomial extrapolation of the last converged states
+    as start -- per member the order, 1 to 3, with the smallest FP64 residual -- adaptive
+    linear tolerance).  The
     multigrid's coarse levels are per-mesh set-up, like the sparsity pattern: built
     before the timed solve, their time is mg_setup_s."""
     import torch
@@ -572,7 +575,7 @@ def bench_newton(args):
     modes = tuple(args.modes.split(","))
     out_modes.update(twisted_cube_newton(args.n, args.linear, args.lin_rtol, log, modes))
     plain = out_modes.get("fp64+previous+adaptive", {}).get("wall_s")
-    fast = out_modes.get("fp32+extrap2+adaptive+csr32", {}).get("wall_s")
+    fast = out_modes.get("fp32+poly+adaptive+csr32", {}).get("wall_s")
     out = {"metric": "Newton wall time to FP64 residual 1e-8 (twisted hex p=4 cube)", "value": fast, "unit": "s",
            "n_gpus": 1, "steps": 1, "warmup": 0, "higher_is_better": False, "vs_baseline": None, "dtype": "fp32",
            "data": "synthetic", "speedup_vs_plain_fp64": round(plain / fast, 3) if plain and fast else None,
@@ -846,11 +849,11 @@ def main():
         modes = twisted_cube_newton(args.newton_n, "mg", 1e-10, log)
         plain_fixed = modes["fp64+previous+fixed"]["wall_s"]
         plain_adapt = modes["fp64+previous+adaptive"]["wall_s"]
-        fast = modes["fp32+extrap2+adaptive+csr32"]["wall_s"]
+        fast = modes["fp32+poly+adaptive+csr32"]["wall_s"]
         newton_c2 = {"workload": "unit cube {0}^3 hex p=4 (jitter 0.03h), 6^3 GLL, MR(0.5,0.5,20), x=0 clamped, x=1 "
                                  "twisted 30 deg, 10 increments, |R_free| <= 1e-8 R0, p-multigrid flexible CG "
                                  "(fixed: linear rtol 1e-10; adaptive: forcing terms)".format(args.newton_n),
-                     "mixed": "FP32 tangent, FP32-stored CSR, quadratic-extrapolation start, adaptive",
+                     "mixed": "FP32 tangent, FP32-stored CSR, polynomial-extrapolation start, adaptive",
                      "plain_fp64_fixed_s": plain_fixed, "plain_fp64_adaptive_s": plain_adapt, "mixed_fp32_s": fast,
                      "speedup_equal_policy": round(plain_adapt / fast, 3),
                      "speedup_vs_plain_fixed": round(plain_fixed / fast, 3), "modes": modes}

@@ -26,7 +26,11 @@ Start modes of LockstepNewton.solve:
   "extrap"    2 u_{k-1} - u_{k-2} (u_0 = u_{-1} = 0), full Newton from there;
   "extrap2"   3 u_{k-1} - 3 u_{k-2} + u_{k-3} from the third increment on (the quadratic
               through the last three converged states; "extrap" before);
-  "learned"   the MLP's prediction u_L, the extrapolation u_E, or the point
+  "extrap3"   4 u_{k-1} - 6 u_{k-2} + 4 u_{k-3} - u_{k-4} from the fourth increment on
+              (the cubic through the last four; the lower orders before);
+  "poly"      per member, whichever of the linear, quadratic and cubic extrapolations
+              (those the increment has states for) has the smallest FP64 residual;
+  "learned"   the MLP's prediction u_L, the "poly" start u_E, or the point
               u_E + a* (u_L - u_E) that minimises the linearised residual along the
               predicted correction -- whichever has the smallest FP64 residual, per
               member (u_E is the fallback).
@@ -186,7 +190,7 @@ class LockstepNewton(object):
         choices and (record=True) the converged displacements u_0..u_nincr on the host.
         ``log``: optional callable given one line per increment."""
         import torch
-        if start not in ("previous", "extrap", "extrap2", "learned"):
+        if start not in ("previous", "extrap", "extrap2", "extrap3", "poly", "learned"):
             raise ValueError(start)
         if start == "learned" and predictor is None:
             raise ValueError("start='learned' needs a predictor")
@@ -195,7 +199,12 @@ class LockstepNewton(object):
         nincr = self.family.nincr
         fixed_f = 1.0 - self.free_f
         z = lambda: torch.zeros(n3, dtype=torch.float64, device=dev)
-        u_prev, u_prev2, u_prev3, u, du, rhs, R, RL, uL = z(), z(), z(), z(), z(), z(), z(), z(), z()
+        u_prev, u_prev2, u_prev3, u_prev4, u, du, rhs, R, RL, uL = (z() for _ in range(10))
+        # order of the polynomial extrapolation through the last converged states (the
+        # load factors are equally spaced, u_0 = 0 at load 0 included)
+        order = {"extrap2": 2, "extrap3": 3, "poly": 3, "learned": 3}.get(start, 1)
+        # "poly" and "learned": the FP64 residual picks the order per member
+        pick_order = start in ("poly", "learned")
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         t_lin = t_asm = t_res = t_pred = 0.0
@@ -232,12 +241,23 @@ class LockstepNewton(object):
                 conv = torch.zeros(self.nmember, dtype=torch.bool, device=dev)
                 r_cur = float((self._norms(rhs) / R0).max())
             else:
-                if start == "extrap2" and k >= 3:
-                    uE = (3.0 * u_prev - 3.0 * u_prev2 + u_prev3) * self.free_f + ubc
-                else:
-                    uE = (2.0 * u_prev - u_prev2) * self.free_f + ubc
+                extrap = [lambda: (2.0 * u_prev - u_prev2) * self.free_f + ubc,
+                          lambda: (3.0 * u_prev - 3.0 * u_prev2 + u_prev3) * self.free_f + ubc,
+                          lambda: (4.0 * u_prev - 6.0 * u_prev2 + 4.0 * u_prev3 - u_prev4) * self.free_f + ubc]
+                q = min(order, k - 1)
+                uE = extrap[q - 1]()
                 t_res += timed(self._residual, uE, R)
                 nE = self._norms(R)
+                if pick_order:
+                    # the lower orders, kept per member where their residual is smaller
+                    for qq in range(q - 1, 0, -1):
+                        uq, Rq = extrap[qq - 1](), torch.empty_like(R)
+                        t_res += timed(self._residual, uq, Rq)
+                        nq = self._norms(Rq)
+                        better = nq < nE
+                        uE = torch.where(self._per_dof(better), uq, uE)
+                        R = torch.where(self._per_dof(better), Rq, R)
+                        nE = torch.where(better, nq, nE)
                 if start == "learned":
                     t_pred += timed(predictor.predict, self.X, u_prev, u_prev2, lam, self.free, ubc, uL)
                     t_res += timed(self._residual, uL, RL)
@@ -271,7 +291,7 @@ class LockstepNewton(object):
                 if bool(conv.all()):
                     lock_iters[k - 1] = 0
                     history.append([r_cur])
-                    u_prev3, u_prev2, u_prev = u_prev2, u_prev, u.clone()
+                    u_prev4, u_prev3, u_prev2, u_prev = u_prev3, u_prev2, u_prev, u.clone()
                     if record:
                         traj.append(u.cpu().numpy())
                     continue
@@ -320,7 +340,7 @@ class LockstepNewton(object):
             # in the reference's mode V now holds K at the converged state, which the
             # next increment's predictor uses; the warm-started modes assemble at
             # their start instead
-            u_prev3, u_prev2, u_prev = u_prev2, u_prev, u.clone()
+            u_prev4, u_prev3, u_prev2, u_prev = u_prev3, u_prev2, u_prev, u.clone()
             if record:
                 traj.append(u.cpu().numpy())
             if failed:

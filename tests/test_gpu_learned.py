@@ -13,9 +13,11 @@
   extrapolated start it falls back to, and every start mode converges to the same
   displacements (1e-8 relative; 1e-7 with inexact solves).  (On 48 members most
   increments take two iterations either way; the iteration gain shows on the harder
-  members, where the extrapolated start needs a third.)  The quadratic extrapolation of
-  the last three states (no network) is run beside it and reaches the same state; its
-  start residuals are printed (on this family it starts below the predictor).
+  members, where the extrapolated start needs a third.)  The quadratic and cubic
+  extrapolations of the last states (no network) run beside it and reach the same state;
+  the "poly" start, per member the order whose FP64 residual is smallest, starts at most
+  at the quadratic's residual, and the learned start, which falls back to "poly", at most
+  at poly's (1e-3 slack: the modes' converged histories differ at the 1e-9 level).
 """
 import os
 
@@ -72,7 +74,8 @@ def test_committed_predictor_gains(cuda):
     net = WarmStartMLP.load(WEIGHTS)
     fam = _family(n=2, nincr=30)
     held = LockstepNewton(fam, range(2 * 10 ** 6, 2 * 10 ** 6 + 48))
-    res = {s: held.solve(s, predictor=net, record=True) for s in ("previous", "extrap", "learned", "extrap2")}
+    modes = ("previous", "extrap", "learned", "extrap2", "extrap3", "poly")
+    res = {s: held.solve(s, predictor=net, record=True) for s in modes}
     for s, r in res.items():
         assert not r["failed"], s
         print(s, "lockstep", sum(r["lockstep_iterations"]), "member mean", r["member_iterations"].sum(0).mean(),
@@ -80,10 +83,12 @@ def test_committed_predictor_gains(cuda):
               np.mean(r["start_residual"]) if r["start_residual"] else None, "chose learned",
               sum(r["chose_learned"]), "wall {:.2f}s".format(r["wall_s"]))
     ref = res["previous"]["trajectory"][-1]
-    for s in ("extrap", "learned", "extrap2"):
+    for s in modes[1:]:
         assert np.abs(res[s]["trajectory"][-1] - ref).max() <= 1e-8 * np.abs(ref).max()
     sE, sL = np.mean(res["extrap"]["start_residual"]), np.mean(res["learned"]["start_residual"])
     assert sL < 0.5 * sE
+    s2, sP = np.mean(res["extrap2"]["start_residual"]), np.mean(res["poly"]["start_residual"])
+    assert sP <= 1.001 * s2 and sL <= 1.001 * sP
     assert sum(res["learned"]["lockstep_iterations"]) <= sum(res["extrap"]["lockstep_iterations"])
     assert res["learned"]["member_iterations"].sum() <= res["extrap"]["member_iterations"].sum()
     # the inexact-Newton mode reaches the same state
