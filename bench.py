@@ -375,9 +375,10 @@ def bench_learned(args):
     # (tangent precision, start, linear-solve forcing, CSR storage); the first is the
     # plain FP64 path
     modes = [("fp64", "previous", "fixed", "fp64"), ("fp64", "extrap", "fixed", "fp64"),
-             ("fp64", "learned", "fixed", "fp64"), ("fp64", "extrap", "adaptive", "fp64"),
-             ("fp64", "learned", "adaptive", "fp64"), ("fp32", "extrap", "adaptive", "fp64"),
-             ("fp32", "learned", "adaptive", "fp64"), ("fp32", "extrap", "adaptive", "fp32"),
+             ("fp64", "poly", "fixed", "fp64"), ("fp64", "learned", "fixed", "fp64"),
+             ("fp64", "extrap", "adaptive", "fp64"), ("fp64", "poly", "adaptive", "fp64"),
+             ("fp64", "learned", "adaptive", "fp64"), ("fp32", "poly", "adaptive", "fp64"),
+             ("fp32", "learned", "adaptive", "fp64"), ("fp32", "poly", "adaptive", "fp32"),
              ("fp32", "learned", "adaptive", "fp32"), ("bf16", "learned", "adaptive", "fp32")]
     results, ref = {}, None
     for prec, start, forcing, csr in modes:
