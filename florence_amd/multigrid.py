@@ -367,7 +367,7 @@ class MultigridSolver(object):
     ("bf16": a quarter of the bytes, "fp32", or "fp64": the records themselves).
     ``rtol``, ``maxit``, ``check_every`` and ``iterations`` as LinearSolver."""
 
-    def __init__(self, rtol=1e-12, maxit=2000, check_every=1, cheb_degree=5, cheb_range=60.0,
+    def __init__(self, rtol=1e-12, maxit=2000, check_every=1, cheb_degree=7, cheb_range=60.0,
                  coarse_iterations=60, power_steps=12, warm_power_steps=3, fp32_smoother=True, matrix_free=True,
                  smoother_records="bf16"):
         self.linear_solver = "pmg"

@@ -128,7 +128,7 @@ class FEMSolver(object):
         self.user_defined_stop_func = user_defined_stop_func
         self.report_log_level = report_log_level
         # None: chosen in Solve -- the p-multigrid flexible CG for hexes of p >= 2 (nested
-        # coarse spaces; 14-16 iterations per Newton step on the 24^3 twisted hex p=4 cube), else
+        # coarse spaces; 12-16 iterations per Newton step on the 24^3 twisted hex p=4 cube), else
         # block-Jacobi PCG; both to rtol 1e-12, where the reference solves directly
         self.linear_solver = linear_solver
         self.warm_start = warm_start
