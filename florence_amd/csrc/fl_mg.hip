@@ -34,6 +34,7 @@ namespace fl {
 namespace {
 
 constexpr double FL_MG_LMAX_SAFETY = 1.15;   // the estimate approaches lmax from below
+constexpr int FL_MG_COLD_POWER_STEPS = 12;   // power steps of a restart from the random vector
 
 __device__ inline void bj_apply(const double *__restrict__ P, int64_t a, const double (&r)[3], double (&z)[3]) {
 #pragma unroll
@@ -366,16 +367,20 @@ __global__ __launch_bounds__(SV_T) void prolong_kernel(int64_t nnode, const int3
     }
 }
 
-// deterministic pseudo-random start of the power iteration, masked
+// start of the power iteration, zero on fixed dofs: a deterministic pseudo-random
+// vector, or (warm) the vector already in v restricted to this call's free dofs -- on a
+// fixed dof the smoother's operator is the identity, so a component left there would
+// pull the estimate towards 1
 __global__ __launch_bounds__(SV_T) void power_start_kernel(int64_t n, const uint8_t *__restrict__ free_mask,
-                                                           double *__restrict__ v) {
+                                                           double *__restrict__ v, int warm) {
     const int64_t i = (int64_t)blockIdx.x * SV_T + threadIdx.x;
     if (i >= n) return;
     uint64_t h = (uint64_t)i * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
     h ^= h >> 31;
     h *= 0xBF58476D1CE4E5B9ull;
     h ^= h >> 29;
-    v[i] = free_mask[i] ? 0.5 + (double)(h >> 11) * (1.0 / 9007199254740992.0) : 0.0;
+    const double start = warm ? v[i] : 0.5 + (double)(h >> 11) * (1.0 / 9007199254740992.0);
+    v[i] = free_mask[i] ? start : 0.0;
 }
 
 // z = D^-1 y with per-block partials of z.z
@@ -733,16 +738,17 @@ static int vcycle(Mg &m, int l, const double *rhs) {
 // largest eigenvalue of D^-1 A on level l (A as the smoother sees it) by power iteration
 // (device-side normalisation, one host read at the end), times a safety factor; warm:
 // from the vector the previous call left (the previous Newton step's operator), else
-// from a fixed pseudo-random start
+// from a fixed pseudo-random start; either way restricted to this call's free dofs.  A
+// warm vector can be exactly zero there (a lockstep solve's converged members leave the
+// mask, and repeated power steps concentrate the vector on one member): the estimate
+// then starts again from the pseudo-random vector with FL_MG_COLD_POWER_STEPS steps.
 static int estimate_lmax(Mg &m, int l, int steps, bool warm) {
     LevelWork &w = m.w[l];
     const FlMgLevel &L = m.lv[l];
     const int64_t n = 3 * L.nnode;
     double *v = w.pv, *y = w.d2, *z = w.r, *zz = m.s + 16;
-    if (!warm) {
-        power_start_kernel<<<nb_v(n), SV_T, 0, m.st>>>(n, L.free_mask, v);
-        FL_LAUNCH_CHECK();
-    }
+    power_start_kernel<<<nb_v(n), SV_T, 0, m.st>>>(n, L.free_mask, v, warm ? 1 : 0);
+    FL_LAUNCH_CHECK();
     for (int it = 0; it < steps; ++it) {
         int rc = level_spmv(m, l, v, y, nullptr, true);
         if (rc) return rc;
@@ -754,6 +760,7 @@ static int estimate_lmax(Mg &m, int l, int steps, bool warm) {
     double h = 0.0;
     FL_HIP_CHECK(hipMemcpyAsync(&h, zz, sizeof(double), hipMemcpyDeviceToHost, m.st));
     FL_HIP_CHECK(hipStreamSynchronize(m.st));
+    if (warm && !(h > 0.0)) return estimate_lmax(m, l, std::max(steps, FL_MG_COLD_POWER_STEPS), false);
     if (!(h > 0.0) || !std::isfinite(h)) {
         set_error("fl_mg_pcg: eigenvalue estimate of level %d failed (%g)", l, h);
         return FL_E_ARG;

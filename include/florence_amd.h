@@ -417,7 +417,9 @@ int fl_mf_apply(const FlMgMatrixFree *op, const uint8_t *free_mask_dev, const do
  * always reads the FP64 records.
  * power_steps: power iterations per level for the smoother's lmax (1..64); power_warm:
  * nonzero = start them from the vectors the previous call on this workspace left (same
- * levels; the previous Newton step's operator), 0 = from a fixed pseudo-random start. */
+ * levels; the previous Newton step's operator), 0 = from a fixed pseudo-random start.  A
+ * warm vector that is zero on this call's free dofs (another free mask) makes that level
+ * start again from the pseudo-random vector with max(power_steps, 12) steps. */
 size_t fl_mg_workspace_size(const int64_t *nnodes, int32_t nlevels);
 int fl_mg_pcg(const FlMgLevel *levels, int32_t nlevels, int32_t v32, const float *const *smoother_V32,
               const FlMgMatrixFree *fine_matrix_free, const double *b_dev, double *x_dev, double rtol,

@@ -311,3 +311,42 @@ def test_femsolver_default_linear_solver(cuda):
     assert max(np.abs(disp[k] - ref["disp"][k]).max() / np.abs(ref["disp"][k]).max() for k in range(10)) <= 1e-10
     mt = cube_mesh("tet", 2, 1)
     assert isinstance(default_linear_solver(DeviceMeshData(mt, function_space(mt, 2))), LinearSolver)
+
+
+def test_warm_spectrum_estimate_after_mask_change(cuda):
+    """A lockstep solve holds converged members out of the linear solve.  After a solve
+    in which only member A is free, the smoother's warm power-iteration vector lives on
+    member A alone.  A warm solve in which only member B is free restricts it to B's
+    dofs, finds it zero and starts the estimate again from the pseudo-random vector.
+    Unrestricted, the identity rows of the held dofs pulled the estimate of lmax to about
+    1, below B's spectrum, and the Chebyshev smoother diverged.  Every solve meets its
+    tolerance, and the held member's update stays zero."""
+    import torch
+    from florence_amd.learned import LockstepNewton, TwistStretchFamily
+    fam = TwistStretchFamily(n=2, nincr=4)
+    ln = LockstepNewton(fam, [0, 1], forcing="fixed", linear="mg", lin_rtol=1e-10)
+    u = torch.zeros(3 * ln.dm.nnode, dtype=torch.float64, device="cuda")
+    ln._assemble(u)
+    for live in ([True, False], [False, True], [True, True]):
+        done = np.array([not v for v in live])
+        g = torch.Generator(device="cuda").manual_seed(3)
+        rhs = torch.randn(3 * ln.dm.nnode, dtype=torch.float64, device="cuda", generator=g) * ln.free_f
+        rhs = rhs * ln._per_dof(torch.as_tensor(live, device="cuda")).to(rhs.dtype)
+        du = torch.empty_like(rhs)
+        ln._solve(rhs, du, done)
+        r = rhs - spmv_masked(ln, du, live)
+        rel = float(r.norm() / rhs.norm())
+        print(live, "iterations", ln.lin.iterations[-1], "relative residual {:.1e}".format(rel), flush=True)
+        assert rel <= 1e-9
+        held = ~torch.as_tensor(live, device="cuda")
+        assert float((du * ln._per_dof(held).to(du.dtype)).abs().max()) == 0.0
+
+
+def spmv_masked(ln, x, live):
+    """K x on the free dofs of the live members (the operator the held solve applies)."""
+    import torch
+    from florence_amd.solver import spmv
+    mask = ln.free * ln._per_dof(torch.as_tensor(live, device="cuda")).to(ln.free.dtype)
+    y = torch.empty_like(x)
+    spmv(ln.dm, ln.V, x, y, mask)
+    return y * mask.to(y.dtype)
