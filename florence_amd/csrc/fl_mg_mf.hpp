@@ -19,8 +19,9 @@
 // About 124 kflop per p = 4 element, and the 54 C values per point read once (K_e itself
 // is 140,625 doubles).  One workgroup per element, each XCD taking a contiguous range of
 // elements (its L2 then holds the x of neighbouring elements); two LDS buffers alternate
-// between the stages, and each stage reads an operand from LDS once per pencil, not once
-// per multiply-add.
+// between the stages (the per-point stage works in place: each thread reads its point's
+// nine gradients before it writes the nine S values over them), and each stage reads an
+// operand from LDS once per pencil, not once per multiply-add.
 #pragma once
 
 namespace fl {
@@ -29,15 +30,17 @@ template <int P1, int Q>
 struct MfLayout {
     static constexpr int N3 = P1 * P1 * P1, Q3 = Q * Q * Q, PP = P1 * P1, QQ = Q * Q;
     static constexpr int mx(int a, int b) { return a > b ? a : b; }
-    // buffer A: x (3 N3), U2 (9 QQ P1), S (9 Q3), Z (6 Q PP); buffer B: U1 (6 Q PP), G (9 Q3), W (9 QQ P1)
-    static constexpr int BUF = mx(mx(3 * N3, 9 * QQ * P1), mx(9 * Q3, 6 * Q * PP));
+    // buffer A: x (3 N3), U2 (9 QQ P1), W (9 QQ P1); buffer B: U1 (6 Q PP), G then S (9 Q3), Z (6 Q PP)
+    static constexpr int BUFA = mx(3 * N3, 9 * QQ * P1);
+    static constexpr int BUFB = mx(6 * Q * PP, 9 * Q3);
 };
 
 // C32: the C tensors rounded to FP32 (c32 [e][54][Q3]) instead of the records; each
 // thread's 54 values are loaded into registers when the workgroup starts, so their latency
-// overlaps the gradient stages.
+// overlaps the gradient stages.  The BF16 form fits five workgroups per CU (LDS and
+// registers); the others four.
 template <int P1, int Q, int CK>
-__global__ __launch_bounds__(256) void mf_element_kernel(const double *__restrict__ qp,
+__global__ __launch_bounds__(256, CK == 2 ? 5 : 4) void mf_element_kernel(const double *__restrict__ qp,
                                                          const float *__restrict__ c32,
                                                          const uint32_t *__restrict__ c16,
                                                          const int32_t *__restrict__ basis,
@@ -51,8 +54,8 @@ __global__ __launch_bounds__(256) void mf_element_kernel(const double *__restric
     constexpr bool C32 = CK != 0;
     static_assert(!C32 || Q3 <= NT, "one point per thread");
     __shared__ double sF[3][2][P1][Q];
-    __shared__ double sA[M::BUF];
-    __shared__ double sB[M::BUF];
+    __shared__ double sA[M::BUFA];
+    __shared__ double sB[M::BUFB];
     __shared__ int32_t sLoc[N3];   // local node of tensor node t
     __shared__ int16_t sT[Q3];     // tensor point of record point g
 
@@ -147,7 +150,7 @@ __global__ __launch_bounds__(256) void mf_element_kernel(const double *__restric
         }
     }
     __syncthreads();
-    // per point: S_im = sum_kn C^{ik}_mn G_kn -> A.  The record holds C^{ik} of the six
+    // per point: S_im = sum_kn C^{ik}_mn G_kn, in place in B.  The record holds C^{ik} of the six
     // pairs i <= k (0,0 1,1 2,2 0,1 0,2 1,2); C^{ki}_mn = C^{ik}_nm.
     const double *rec = qp + e * (int64_t)QS * Q3 + (int64_t)Q_D * Q3;
     for (int g = tid; g < Q3; g += NT) {
@@ -182,18 +185,18 @@ __global__ __launch_bounds__(256) void mf_element_kernel(const double *__restric
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
-            for (int m = 0; m < 3; ++m) sA[(3 * i + m) * Q3 + t] = S[i][m];
+            for (int m = 0; m < 3; ++m) sB[(3 * i + m) * Q3 + t] = S[i][m];
     }
     __syncthreads();
-    // zeta transposed: W -> B, pencils (i m, qb qe)
+    // zeta transposed: W -> A, pencils (i m, qb qe)
     for (int t = tid; t < 9 * QQ; t += NT) {
         const int qbe = t % QQ, im = t / QQ;
         const int m = im % 3;
-        const double *sv = sA + im * Q3 + qbe * Q;
+        const double *sv = sB + im * Q3 + qbe * Q;
         double v[Q];
 #pragma unroll
         for (int qz = 0; qz < Q; ++qz) v[qz] = sv[qz];
-        double *o = sB + (im * QQ + qbe) * P1;
+        double *o = sA + (im * QQ + qbe) * P1;
 #pragma unroll
         for (int az = 0; az < P1; ++az) {
             double s = 0.0;
@@ -203,14 +206,14 @@ __global__ __launch_bounds__(256) void mf_element_kernel(const double *__restric
         }
     }
     __syncthreads();
-    // eta transposed: Z -> A, pencils (i, sb, qb, az)
+    // eta transposed: Z -> B, pencils (i, sb, qb, az)
     for (int t = tid; t < 6 * Q * P1; t += NT) {
         const int az = t % P1, qb = (t / P1) % Q, is = t / (P1 * Q);
         const int i = is >> 1, sb = is & 1;
-        double *o = sA + ((i * 2 + sb) * Q + qb) * PP + az;
+        double *o = sB + ((i * 2 + sb) * Q + qb) * PP + az;
         if (sb == 0) {
-            const double *w0 = sB + ((i * 3) * QQ + qb * Q) * P1 + az;
-            const double *w1 = sB + ((i * 3 + 1) * QQ + qb * Q) * P1 + az;
+            const double *w0 = sA + ((i * 3) * QQ + qb * Q) * P1 + az;
+            const double *w1 = sA + ((i * 3 + 1) * QQ + qb * Q) * P1 + az;
             double a0[Q], a1[Q];
 #pragma unroll
             for (int qe = 0; qe < Q; ++qe) {
@@ -225,7 +228,7 @@ __global__ __launch_bounds__(256) void mf_element_kernel(const double *__restric
                 o[ae * P1] = s;
             }
         } else {
-            const double *w2 = sB + ((i * 3 + 2) * QQ + qb * Q) * P1 + az;
+            const double *w2 = sA + ((i * 3 + 2) * QQ + qb * Q) * P1 + az;
             double a2[Q];
 #pragma unroll
             for (int qe = 0; qe < Q; ++qe) a2[qe] = w2[qe * P1];
@@ -246,8 +249,8 @@ __global__ __launch_bounds__(256) void mf_element_kernel(const double *__restric
         double z0[Q], z1[Q];
 #pragma unroll
         for (int qb = 0; qb < Q; ++qb) {
-            z0[qb] = sA[((i * 2) * Q + qb) * PP + r];
-            z1[qb] = sA[((i * 2 + 1) * Q + qb) * PP + r];
+            z0[qb] = sB[((i * 2) * Q + qb) * PP + r];
+            z1[qb] = sB[((i * 2 + 1) * Q + qb) * PP + r];
         }
 #pragma unroll
         for (int ab = 0; ab < P1; ++ab) {
