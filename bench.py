@@ -358,6 +358,7 @@ def bench_learned(args):
             del ln
         t_gen = time.perf_counter() - t0
         net = WarmStartMLP(seed=args.seed)
+        net.base_order = args.base_order
         F, T = build_dataset(net, fam, runs)
         del runs
         torch.cuda.synchronize()
@@ -617,6 +618,8 @@ def main():
     ap.add_argument("--traj-batch", type=int, default=250)
     ap.add_argument("--epochs", type=int, default=6)
     ap.add_argument("--heldout", type=int, default=64)
+    ap.add_argument("--base-order", type=int, default=1, choices=[1, 2, 3],
+                    help="learned workload: the extrapolation order the trained network corrects")
     ap.add_argument("--weights", default="", help="learned: load trained weights instead of training")
     ap.add_argument("--save-weights", default="")
     ap.add_argument("--lin-rtol", type=float, default=1e-10, help="newton: PCG relative tolerance")

@@ -259,7 +259,9 @@ class LockstepNewton(object):
                         R = torch.where(self._per_dof(better), Rq, R)
                         nE = torch.where(better, nq, nE)
                 if start == "learned":
-                    t_pred += timed(predictor.predict, self.X, u_prev, u_prev2, lam, self.free, ubc, uL)
+                    qb = min(predictor.base_order, k - 1)
+                    base = extrap[qb - 1]() if qb > 1 else None
+                    t_pred += timed(predictor.predict, self.X, u_prev, u_prev2, lam, self.free, ubc, uL, base)
                     t_res += timed(self._residual, uL, RL)
                     nL = self._norms(RL)
                     # step length along the predicted correction d = uL - uE: with R
@@ -367,6 +369,7 @@ class WarmStartMLP(object):
         lib = _lib.load()
         self.device = torch.device(device)
         self.nparam = int(lib.fl_mlp_param_count())
+        self.base_order = 1     # the extrapolation the network's output corrects
         self.P = torch.as_tensor(self.initial_params(seed), device=self.device)
         self.m = torch.zeros_like(self.P)
         self.v = torch.zeros_like(self.P)
@@ -410,14 +413,18 @@ class WarmStartMLP(object):
                    "fl_warmstart_features")
         return F
 
-    def predict(self, X, u1, u2, lam, free_u8, ubc, out):
-        """out = prediction of the next converged displacement (Dirichlet dofs = ubc)."""
+    def predict(self, X, u1, u2, lam, free_u8, ubc, out, base=None):
+        """out = prediction of the next converged displacement (Dirichlet dofs = ubc):
+        the network's correction added to its base, the linear extrapolation
+        2 u1 - u2, or ``base`` (the extrapolation of order base_order the network was
+        trained against; fl_warmstart_apply forms 2 a - b + correction, so a = b = base)."""
         lib = _lib.load()
         nn = X.numel() // 3
         b = self._buffers(nn)
         self.features(X, u1, u2, lam, b["F"])
         self.forward(b["F"], nn)
-        _lib.check(lib.fl_warmstart_apply(_lib.ptr(u1), _lib.ptr(u2), _lib.ptr(b["Y"]), nn, _lib.ptr(self.norm_out),
+        a1, a2 = (u1, u2) if base is None else (base, base)
+        _lib.check(lib.fl_warmstart_apply(_lib.ptr(a1), _lib.ptr(a2), _lib.ptr(b["Y"]), nn, _lib.ptr(self.norm_out),
                                           _lib.ptr(free_u8), _lib.ptr(ubc), _lib.ptr(out), _lib.stream_ptr()),
                    "fl_warmstart_apply")
         return out
@@ -444,7 +451,7 @@ class WarmStartMLP(object):
 
     def state(self):
         return {"P": self.P.cpu().numpy(), "norm_in": self.norm_in.cpu().numpy(),
-                "norm_out": self.norm_out.cpu().numpy()}
+                "norm_out": self.norm_out.cpu().numpy(), "base_order": np.int32(self.base_order)}
 
     def save(self, path, **meta):
         np.savez(path, **self.state(), **{k: np.asarray(v) for k, v in meta.items()})
@@ -457,11 +464,31 @@ class WarmStartMLP(object):
         net.P.copy_(torch.as_tensor(d["P"]))
         net.norm_in.copy_(torch.as_tensor(d["norm_in"]))
         net.norm_out.copy_(torch.as_tensor(d["norm_out"]))
+        net.base_order = int(d["base_order"]) if "base_order" in d.files else 1
         return net
 
 
 def _free_nodes(free_u8):
     return free_u8.view(-1, 3).bool().all(dim=1)
+
+
+# polynomial extrapolation to the next of equally spaced states u_{k-1}, u_{k-2}, ...
+EXTRAP_COEF = {1: (2.0, -1.0), 2: (3.0, -3.0, 1.0), 3: (4.0, -6.0, 4.0, -1.0)}
+
+
+def extrapolate(history, order):
+    """sum_j EXTRAP_COEF[order][j] history[j] (history: u_{k-1}, u_{k-2}, ...)."""
+    c = EXTRAP_COEF[order]
+    out = c[0] * history[0]
+    for j in range(1, len(c)):
+        out = out + c[j] * history[j]
+    return out
+
+
+def _base(U, k, order):
+    """The extrapolated start of increment k from the recorded states U[0..k-1]."""
+    q = min(order, k - 1)
+    return extrapolate([U[k - 1 - j] for j in range(q + 1)], q)
 
 
 def build_dataset(net, family, runs):
@@ -470,7 +497,8 @@ def build_dataset(net, family, runs):
     ``runs`` = [(X [3 nnode], free mask uint8 [3 nnode], trajectory (nincr + 1, 3 nnode)),
     ...] (X and the mask as device tensors, e.g. a LockstepNewton's ``X``/``free``).  For every
     increment k >= 2 and every node whose three dofs are free: inputs (X, u_{k-1},
-    u_{k-1} - u_{k-2}, k / nincr), target u_k - (2 u_{k-1} - u_{k-2}).  Sets the
+    u_{k-1} - u_{k-2}, k / nincr), target u_k minus the polynomial extrapolation of
+    order min(net.base_order, k - 1) (order 1: 2 u_{k-1} - u_{k-2}).  Sets the
     network's input/output standardisation from these rows and returns device
     (F [N, 32], T [N, 32]) with N rounded down to a multiple of 32."""
     import torch
@@ -489,7 +517,7 @@ def build_dataset(net, family, runs):
         for k in range(2, nincr + 1):
             u1, u2, uk = U[k - 1].view(-1, 3), U[k - 2].view(-1, 3), U[k].view(-1, 3)
             raw = torch.cat([X, u1, u1 - u2, torch.full_like(X[:, :1], k / float(nincr))], dim=1)[sel]
-            tgt = (uk - (2 * u1 - u2))[sel]
+            tgt = (uk - _base(U, k, net.base_order).view(-1, 3))[sel]
             raw_s += raw.sum(0)
             raw_q += (raw * raw).sum(0)
             tgt_s += tgt.sum(0)
@@ -512,8 +540,8 @@ def build_dataset(net, family, runs):
         Fk = torch.empty((nn, IP), dtype=torch.float32, device=dev)
         for k in range(2, nincr + 1):
             net.features(Xf, U[k - 1], U[k - 2], k / float(nincr), Fk)
-            u1, u2, uk = U[k - 1].view(-1, 3), U[k - 2].view(-1, 3), U[k].view(-1, 3)
-            tgt = ((uk - (2 * u1 - u2))[sel] - tmean) / tstd
+            uk = U[k].view(-1, 3)
+            tgt = ((uk - _base(U, k, net.base_order).view(-1, 3))[sel] - tmean) / tstd
             m = tgt.shape[0]
             F[o:o + m] = Fk[sel]
             T[o:o + m, :3] = tgt.to(torch.float32)
