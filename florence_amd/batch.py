@@ -84,11 +84,14 @@ class BatchedAssembly(object):
 
     Meshes of one (type, p) are concatenated into groups capped so that a group's CSR
     stays inside int32 indices (sum of element-block entries <= max_block_entries) and
-    its element-block workspace stays small; all groups share one workspace.
+    its element-block workspace stays small.  The groups are spread over ``branches``
+    streams (longest first onto the least-loaded branch), each with its own workspace,
+    so independent groups run concurrently: in the captured graph the branches fork from
+    the capture stream and join it again, and the batch stays one graph launch.
     """
 
     def __init__(self, batch, precision="fp64", tuning=None, device="cuda", max_block_entries=1.5e9,
-                 direct=True):
+                 direct=True, branches=4):
         import torch
         self.precision = precision
         self.direct = direct
@@ -107,8 +110,19 @@ class BatchedAssembly(object):
             for idx in self._split(allidx, max_block_entries):
                 self.groups.append(self._make_group((et, p), idx, fs, tuning))
         lib = _lib.load()
-        need = max(int(lib.fl_df_workspace_size(g["dm"].ngauss, g["dm"].nelem, g["dm"].npe)) for g in self.groups)
-        self.workspace = torch.empty(need, dtype=torch.uint8, device=self.device)
+        nb = max(1, min(int(branches), len(self.groups)))
+        load = [0.0] * nb
+        for g in sorted(self.groups, key=lambda g: -float(g["dm"].nelem) * g["dm"].npe ** 2):
+            b = int(np.argmin(load))
+            g["branch"] = b
+            load[b] += float(g["dm"].nelem) * g["dm"].npe ** 2
+        self.workspaces = []
+        for b in range(nb):
+            need = max(int(lib.fl_df_workspace_size(g["dm"].ngauss, g["dm"].nelem, g["dm"].npe))
+                       for g in self.groups if g["branch"] == b)
+            self.workspaces.append(torch.empty(need, dtype=torch.uint8, device=self.device))
+        self.workspace = self.workspaces[0]
+        self.streams = [torch.cuda.Stream(device=self.device) for _ in range(nb)]   # default priority
         self.graph = None
 
     def _split(self, idx, cap):
@@ -180,7 +194,7 @@ class BatchedAssembly(object):
         _lib.check(lib.fl_set_tuning(_lib.FL_TUNE_STIFF_SF, 2 if g.get("sum_factorised", True) else 1),
                    "fl_set_tuning")
         prm, _keep = _lib.double_array(g["params"])
-        ws = self.workspace
+        ws = self.workspaces[g.get("branch", 0)]
         args = (g["material"], prm, _lib.ptr(g["elem_mat"]), _lib.ptr(g["table"]), _lib.ptr(dm.points),
                 _lib.ptr(dm.elements), _lib.ptr(g["x"]), _lib.ptr(dm.Jm), _lib.ptr(dm.AllGauss), dm.ngauss,
                 dm.nelem, dm.npe, dm.nnode, 1, _lib.ptr(dm.node_elem_ptr), _lib.ptr(dm.node_elem),
@@ -197,9 +211,18 @@ class BatchedAssembly(object):
             _lib.check(lib.fl_df_assemble_mt(*(args + tail)), "fl_df_assemble_mt")
 
     def assemble(self):
-        """All groups, eagerly (one library call per group)."""
+        """All groups (one library call per group), each on its branch's stream; the
+        branches start after the current stream's earlier work and the current stream
+        waits for all of them."""
+        import torch
+        cur = torch.cuda.current_stream()
+        for st in self.streams:
+            st.wait_stream(cur)
         for g in self.groups:
-            self._launch_group(g)
+            with torch.cuda.stream(self.streams[g["branch"]]):
+                self._launch_group(g)
+        for st in self.streams:
+            cur.wait_stream(st)
 
     def capture(self):
         """Record the batch into one HIP graph (call assemble() once before, so every
